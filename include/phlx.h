@@ -1,0 +1,150 @@
+/*
+ * phlx.h -- C-ABI of the MI355X-native explicit finite-volume MHD update.
+ *
+ * The library replaces the device-side work of the reference's hydro_step
+ * path behind plain pointers and sizes (no C++ or torch types), so a
+ * Fortran caller binds it with ISO_C_BINDING and a Python caller with
+ * ctypes.  Every entry point returns 0 on success or a negative PHLX_E_*
+ * code; phlx_error_string() turns a code into text.  State lives on the GPU
+ * in structure-of-arrays form; the library never keeps host pointers.
+ * Calls on one context are ordered on that context's HIP stream; calls that
+ * return host scalars synchronise it.  One context per host thread.
+ *
+ * Host exchange layout ("interior SoA"): interior cells only, x1 fastest:
+ *   prim[v][k][j][i]   v = 0..nv-1 = rho, v1, v2, [v3], p, scalars
+ *   b_x1[k][j][i] with i in [0,nx1], b_x2 with j in [0,nx2], b_x3 k in [0,nx3]
+ * nv = sdims + 2 + nas (source/source.F90:472-477).
+ * Static equilibrium / gravity / potential / heating inputs are
+ * plane-parallel profiles indexed by the x2 cell (nx2 values) or x2 face
+ * (nx2+1 values), the form every supported problem uses.
+ */
+#ifndef PHLX_H
+#define PHLX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* error codes */
+#define PHLX_OK 0
+#define PHLX_E_HIP (-1)        /* a HIP runtime call failed */
+#define PHLX_E_CONFIG (-2)     /* unsupported or inconsistent configuration */
+#define PHLX_E_ARG (-3)        /* null pointer / bad size / bad index */
+#define PHLX_E_NOMEM (-4)      /* device allocation failed */
+#define PHLX_E_STATE (-5)      /* call out of order (e.g. step before init) */
+#define PHLX_E_NONFINITE (-6)  /* NaN / non-positive density or pressure found */
+#define PHLX_E_TABLE (-7)      /* EoS table missing or malformed */
+
+/* option values (names follow the reference's Makefile OPTS) */
+enum { PHLX_LIM2ND_REC = 0, PHLX_PPH_REC = 1 };
+enum { PHLX_HLLC_FLUX = 0, PHLX_LHLLC_FLUX = 1, PHLX_HLLD_FLUX = 2, PHLX_LHLLD_FLUX = 3 };
+enum { PHLX_BC_PERIODIC = 0, PHLX_BC_REFLECTIVE = 1, PHLX_BC_OUTFLOW = 2, PHLX_BC_DIODE = 3 };
+enum { PHLX_EOS_IDEAL = 0, PHLX_EOS_HELMHOLTZ = 1 };
+enum { PHLX_PREC_FP64 = 0, PHLX_PREC_MIXED = 1 };
+
+/* Run-time counterpart of the reference's compile-time OPTS subset
+ * (Makefile.config:16-23, Make.checks:27-305, source.F90:31-477). */
+typedef struct phlx_config {
+  int32_t sdims;            /* sdims_make: 2 or 3 */
+  int32_t nx1, nx2, nx3;    /* nx*_make (nx3 ignored for sdims=2) */
+  int32_t nas;              /* nas_make: advected scalars */
+  double x1l, x1u, x2l, x2u, x3l, x3u;  /* domain (initialize_simulation args) */
+  double gamma, mu;         /* gamma_ad, mu */
+  int32_t recon;            /* LIM2ND_REC / PPH_REC */
+  int32_t flux;             /* HLLC / LHLLC / HLLD / LHLLD (HLLD* imply USE_MHD) */
+  int32_t all_mach;         /* FLUX_IS_ALL_MACH */
+  int32_t rk_stages;        /* RK2_STEPPER (2) / RK3_STEPPER (3) */
+  double cfl, mach_cutoff, mach_ct, tmax;
+  int32_t bc[6];            /* X1L X1U X2L X2U X3L X3U */
+  int32_t use_wb, use_gravity, evolve_etot, use_edot;
+  int32_t eos, fasteos;     /* HELMHOLTZ_EOS / USE_FASTEOS */
+  int32_t precision;        /* PHLX_PREC_* */
+  int32_t device;           /* HIP device ordinal */
+  int32_t reserved[8];      /* must be zero */
+} phlx_config;
+
+typedef struct phlx_ctx phlx_ctx;
+
+const char *phlx_error_string(int code);
+int phlx_version(void);
+
+/* Allocation part of initialize_simulation (source/source.F90:1150-1881):
+ * validates cfg, allocates every device buffer, creates the context's
+ * stream. */
+int phlx_create(const phlx_config *cfg, phlx_ctx **out);
+/* finalize_simulation (source/source.F90:1889): frees device memory. */
+int phlx_destroy(phlx_ctx *ctx);
+int phlx_nvars(const phlx_ctx *ctx);
+
+/* Upload / download of the state held in the reference's `locgrid`
+ * (prim, eint, temp, b_x1..3; source/source.F90:839-1130) in interior-SoA
+ * layout.  Any pointer may be NULL to skip that field.  b_cc (download
+ * only) is 3 x N. */
+int phlx_upload_state(phlx_ctx *ctx, const double *prim, const double *eint,
+                      const double *temp, const double *b_x1, const double *b_x2,
+                      const double *b_x3);
+int phlx_download_state(phlx_ctx *ctx, double *prim, double *eint, double *temp,
+                        double *b_x1, double *b_x2, double *b_x3, double *b_cc);
+/* The reference's own array layout: prim(1:nv, lx1-ngc:ux1+ngc,
+ * lx2-ngc:ux2+ngc, lx3-ngc:ux3+ngc) in Fortran order (variable fastest,
+ * ghosts included; in 2D the x3 extent is 1).  Ghost values are ignored on
+ * upload and written as zeros on download. */
+int phlx_upload_prim_aos(phlx_ctx *ctx, const double *prim, int32_t ngc);
+int phlx_download_prim_aos(phlx_ctx *ctx, double *prim, int32_t ngc);
+/* Plane-parallel statics: eq_prim_cc / eq_prim_x2 (rho, p), grav(2,:),
+ * phi_cc / phi_x2, edot (source/source.F90:1415-1483). NULL = zeros. */
+int phlx_set_profiles(phlx_ctx *ctx, const double *eq_rho_cc, const double *eq_p_cc,
+                      const double *eq_rho_f, const double *eq_p_f,
+                      const double *grav_cc, const double *phi_cc,
+                      const double *phi_f, const double *edot_cc);
+
+/* Step-0 part of time_loop (source/source.F90:6012-6062, 6271-6409):
+ * b_cc from the face field, eint and T from (rho, p), then
+ * compute_hyperbolic_dt.  Sets time = 0, step = 0. */
+int phlx_init(phlx_ctx *ctx);
+/* hydro_step(mgrid, lgrid) (source/source.F90:6847-13701) with the current
+ * dt; ghost zones are produced on the device (no halo exchange). */
+int phlx_hydro_step(phlx_ctx *ctx);
+/* compute_hyperbolic_dt (source/source.F90:13707-13960): smax = max over
+ * zones of sum_d (|v_d| + c_f)/dx_d, dt = cfl/smax.  Outputs may be NULL
+ * (then the call does not synchronise). */
+int phlx_compute_dt(phlx_ctx *ctx, double *dt_out, double *smax_out);
+/* nsteps passes of the time_loop body (source/source.F90:6662-6720): clip
+ * dt to tmax, hydro_step, time += dt, compute_hyperbolic_dt, step += 1.
+ * Entirely device-resident and asynchronous: no host read-back per step. */
+int phlx_run(phlx_ctx *ctx, int32_t nsteps);
+int phlx_synchronize(phlx_ctx *ctx);
+int phlx_get_scalars(phlx_ctx *ctx, double *time, double *dt, double *smax,
+                     int64_t *step);
+int phlx_set_scalars(phlx_ctx *ctx, double time, double dt, double smax, int64_t step);
+
+/* Fine-grained form of hydro_step for callers that keep the RK stage loop
+ * (source/source.F90:7149-13699): stage_begin(irk) rebuilds state0 from
+ * prim/eint at irk == 0 (:7057-7147); sweep(dir) does the directional
+ * sweep dir = 0,1,2 (x1 :7960-9105, x2 :9107-10301, x3 :10303-11364);
+ * ct() the corner EMFs and face-B update (:11966-12437); rk_update() the
+ * sources, RK combine and cons->prim (:12606-13686).  (a1, a2, a3) is the
+ * stage's Shu-Osher row; a3 is multiplied by the device-resident dt. */
+int phlx_stage_begin(phlx_ctx *ctx, int32_t irk);
+int phlx_sweep(phlx_ctx *ctx, int32_t dir);
+int phlx_ct(phlx_ctx *ctx, int32_t irk, double a1, double a2, double a3);
+int phlx_rk_update(phlx_ctx *ctx, int32_t irk, double a1, double a2, double a3,
+                   int32_t last_stage);
+
+/* Diagnostics: max over cells of |sum_d (b_d(+) - b_d(-))/dx_d|. */
+int phlx_divb_max(phlx_ctx *ctx, double *out);
+/* Per-kernel device time accumulated while profiling is on (HIP events on
+ * the context stream).  kernel ids: 0 flux x1, 1 flux x2, 2 flux x3,
+ * 3 corner EMF, 4 face-B, 5 update, 6 misc. */
+int phlx_profile_enable(phlx_ctx *ctx, int32_t on);
+int phlx_profile_read(phlx_ctx *ctx, int32_t kernel, double *total_ms, int64_t *launches);
+/* Opaque HIP stream of the context (hipStream_t), for callers that want
+ * to order their own work after the library's. */
+void *phlx_stream(phlx_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,207 @@
+"""ctypes wrapper of the CPU oracle (oracle/phlx_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke()
+and by bench.py's cpu_baseline leg, always as the checker / baseline and
+never as the product path.  The product (phlegethon_amd) does not import
+this module.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from typing import Dict, Optional
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "_build", "libphlx_oracle.so")
+
+
+class OrcConfig(ctypes.Structure):
+    _fields_ = [
+        ("sdims", ctypes.c_int), ("nx1", ctypes.c_int), ("nx2", ctypes.c_int),
+        ("nx3", ctypes.c_int), ("nas", ctypes.c_int),
+        ("x1l", ctypes.c_double), ("x1u", ctypes.c_double), ("x2l", ctypes.c_double),
+        ("x2u", ctypes.c_double), ("x3l", ctypes.c_double), ("x3u", ctypes.c_double),
+        ("gamma", ctypes.c_double), ("mu", ctypes.c_double),
+        ("recon", ctypes.c_int), ("flux", ctypes.c_int), ("all_mach", ctypes.c_int),
+        ("rk_stages", ctypes.c_int),
+        ("cfl", ctypes.c_double), ("mach_cutoff", ctypes.c_double),
+        ("mach_ct", ctypes.c_double), ("tmax", ctypes.c_double),
+        ("bc", ctypes.c_int * 6),
+        ("use_wb", ctypes.c_int), ("use_gravity", ctypes.c_int),
+        ("evolve_etot", ctypes.c_int), ("use_edot", ctypes.c_int),
+        ("eos", ctypes.c_int), ("fasteos", ctypes.c_int),
+        ("nthreads", ctypes.c_int),
+    ]
+
+
+_lib = None
+
+
+def build() -> str:
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB
+
+
+def load() -> ctypes.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB):
+        build()
+    lib = ctypes.CDLL(LIB)
+    P = ctypes.c_void_p
+    lib.orc_create.restype = P
+    lib.orc_create.argtypes = [P]
+    lib.orc_destroy.argtypes = [P]
+    lib.orc_set_state.argtypes = [P] * 7
+    lib.orc_get_state.argtypes = [P] * 8
+    lib.orc_set_profiles.argtypes = [P] * 9
+    lib.orc_init.argtypes = [P]
+    lib.orc_run.argtypes = [P, ctypes.c_int]
+    lib.orc_hydro_step.argtypes = [P]
+    lib.orc_compute_dt.argtypes = [P]
+    lib.orc_get_scalar.restype = ctypes.c_double
+    lib.orc_get_scalar.argtypes = [P, ctypes.c_int]
+    lib.orc_set_scalar.argtypes = [P, ctypes.c_int, ctypes.c_double]
+    lib.orc_riemann_hydro.argtypes = [P, ctypes.c_int, P, P, P]
+    lib.orc_riemann_mhd.argtypes = [P, ctypes.c_int, P, P, P, P, P, P]
+    lib.orc_reconstruct.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, P, P, P]
+    _lib = lib
+    return lib
+
+
+def to_orc(cfg, nthreads: int = 0) -> OrcConfig:
+    c = OrcConfig()
+    for name, _ in OrcConfig._fields_:
+        if name in ("bc", "nthreads"):
+            continue
+        v = getattr(cfg, name)
+        setattr(c, name, int(v) if isinstance(v, bool) else v)
+    for i, v in enumerate(tuple(cfg.bc)):
+        c.bc[i] = int(v)
+    c.nthreads = nthreads
+    return c
+
+
+def _p(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class OracleRun:
+    """CPU restatement of time_loop + hydro_step for one problem."""
+
+    def __init__(self, problem, nthreads: int = 1):
+        self.lib = load()
+        self.cfg = problem.cfg
+        self._c = to_orc(problem.cfg, nthreads)
+        self.g = self.lib.orc_create(ctypes.byref(self._c))
+        if not self.g:
+            raise MemoryError("orc_create failed")
+        prim = np.ascontiguousarray(problem.prim, dtype=np.float64)
+        bs = [None, None, None]
+        if problem.b_faces is not None:
+            for d in range(self.cfg.sdims):
+                bs[d] = np.ascontiguousarray(problem.b_faces[d], dtype=np.float64)
+        self.lib.orc_set_state(self.g, _p(prim), None, None, _p(bs[0]), _p(bs[1]), _p(bs[2]))
+        names = ["eq_rho_cc", "eq_p_cc", "eq_rho_f", "eq_p_f", "grav_cc", "phi_cc",
+                 "phi_f", "edot_cc"]
+        arrs = [None if problem.profiles.get(n) is None else
+                np.ascontiguousarray(problem.profiles[n], dtype=np.float64) for n in names]
+        self._keep = arrs
+        self.lib.orc_set_profiles(self.g, *[_p(a) for a in arrs])
+
+    def close(self):
+        if self.g:
+            self.lib.orc_destroy(self.g)
+            self.g = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def init(self):
+        self.lib.orc_init(self.g)
+
+    def run(self, n: int):
+        self.lib.orc_run(self.g, int(n))
+
+    def hydro_step(self):
+        self.lib.orc_hydro_step(self.g)
+
+    def compute_dt(self):
+        self.lib.orc_compute_dt(self.g)
+        return self.scalar("dt"), self.scalar("smax")
+
+    def scalar(self, name: str) -> float:
+        return self.lib.orc_get_scalar(self.g, {"time": 0, "dt": 1, "smax": 2, "step": 3}[name])
+
+    def set_scalar(self, name: str, v: float):
+        self.lib.orc_set_scalar(self.g, {"time": 0, "dt": 1, "smax": 2, "step": 3}[name], v)
+
+    def state(self) -> Dict[str, np.ndarray]:
+        cfg = self.cfg
+        prim = np.empty((cfg.nvars,) + cfg.shape)
+        eint = np.empty(cfg.shape)
+        temp = np.empty(cfg.shape)
+        bcc = np.empty((3,) + cfg.shape)
+        bs = [np.empty(cfg.face_shape(d)) if cfg.mhd and d < cfg.sdims else None
+              for d in range(3)]
+        self.lib.orc_get_state(self.g, _p(prim), _p(eint), _p(temp), _p(bs[0]), _p(bs[1]),
+                               _p(bs[2]), _p(bcc))
+        out = {"prim": prim, "eint": eint, "temp": temp, "b_cc": bcc}
+        for d in range(3):
+            if bs[d] is not None:
+                out[f"b_x{d+1}"] = bs[d]
+        return out
+
+
+def riemann_hydro(cfg, qL: np.ndarray, qR: np.ndarray) -> np.ndarray:
+    lib = load()
+    c = to_orc(cfg)
+    qL = np.ascontiguousarray(qL, dtype=np.float64)
+    qR = np.ascontiguousarray(qR, dtype=np.float64)
+    out = np.zeros_like(qL)
+    lib.orc_riemann_hydro(ctypes.byref(c), qL.shape[0], _p(qL), _p(qR), _p(out))
+    return out
+
+
+def riemann_mhd(cfg, qL, qR, bL, bR):
+    lib = load()
+    c = to_orc(cfg)
+    qL = np.ascontiguousarray(qL, dtype=np.float64)
+    qR = np.ascontiguousarray(qR, dtype=np.float64)
+    bL = np.ascontiguousarray(bL, dtype=np.float64)
+    bR = np.ascontiguousarray(bR, dtype=np.float64)
+    f = np.zeros_like(qL)
+    fb = np.zeros_like(bL)
+    lib.orc_riemann_mhd(ctypes.byref(c), qL.shape[0], _p(qL), _p(qR), _p(bL), _p(bR),
+                        _p(f), _p(fb))
+    return f, fb
+
+
+def reconstruct(scheme: int, q: np.ndarray, limited_scalar: bool = False):
+    lib = load()
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    qm = np.full_like(q, np.nan)
+    qp = np.full_like(q, np.nan)
+    lib.orc_reconstruct(scheme, int(limited_scalar), q.size, _p(q), _p(qm), _p(qp))
+    return qm, qp
+
+
+def run_problem(problem, nsteps: int, nthreads: int = 1) -> Dict[str, np.ndarray]:
+    r = OracleRun(problem, nthreads)
+    r.init()
+    r.run(nsteps)
+    st = r.state()
+    st.update(time=r.scalar("time"), dt=r.scalar("dt"), smax=r.scalar("smax"),
+              step=int(r.scalar("step")))
+    r.close()
+    return st

@@ -1,0 +1,596 @@
+// phlx_api.hip -- the C-ABI (include/phlx.h): context, device buffers,
+// host exchange, stepping and profiling.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "../../include/phlx.h"
+#include "phlx_internal.hpp"
+
+using namespace phlx;
+
+struct ProfRec {
+  int kid;
+  hipEvent_t a, b;
+};
+
+struct phlx_ctx {
+  phlx_config cfg;
+  Grid g;
+  KernelTable kt;
+  hipStream_t stream = nullptr;
+  int device = 0;
+  int stages = 2;
+  double rk[3][3];
+  bool initialized = false;
+  std::vector<void *> allocs;
+  unsigned long long *scratch_bits = nullptr;
+  // profiling
+  bool prof = false;
+  std::vector<ProfRec> recs;
+  double prof_ms[KID_COUNT] = {0};
+  long long prof_n[KID_COUNT] = {0};
+};
+
+#define HIPCHK(x)                                                              \
+  do {                                                                         \
+    hipError_t e_ = (x);                                                       \
+    if (e_ != hipSuccess) {                                                    \
+      std::fprintf(stderr, "phlx: HIP error %s at %s:%d\n", hipGetErrorString(e_), \
+                   __FILE__, __LINE__);                                        \
+      return PHLX_E_HIP;                                                       \
+    }                                                                          \
+  } while (0)
+
+extern "C" const char *phlx_error_string(int code) {
+  switch (code) {
+    case PHLX_OK: return "ok";
+    case PHLX_E_HIP: return "HIP runtime error";
+    case PHLX_E_CONFIG: return "unsupported or inconsistent configuration";
+    case PHLX_E_ARG: return "invalid argument";
+    case PHLX_E_NOMEM: return "device allocation failed";
+    case PHLX_E_STATE: return "call out of order";
+    case PHLX_E_NONFINITE: return "non-finite or non-positive state";
+    case PHLX_E_TABLE: return "EoS table missing or malformed";
+  }
+  return "unknown error";
+}
+
+extern "C" int phlx_version(void) { return 100; }
+
+// Validation of the supported option subset, mirroring the reference's
+// Make.checks rules that apply (exactly one stepper/recon/flux, HLLD
+// requires MHD, periodic in pairs, FASTEOS requires a complex EoS).
+static int validate(const phlx_config *c) {
+  if (c->sdims != 2 && c->sdims != 3) return PHLX_E_CONFIG;
+  if (c->nx1 < 2 || c->nx2 < 2 || (c->sdims == 3 && c->nx3 < 2)) return PHLX_E_CONFIG;
+  if (c->recon != PHLX_LIM2ND_REC && c->recon != PHLX_PPH_REC) return PHLX_E_CONFIG;
+  if (c->flux < PHLX_HLLC_FLUX || c->flux > PHLX_LHLLD_FLUX) return PHLX_E_CONFIG;
+  if (c->rk_stages != 2 && c->rk_stages != 3) return PHLX_E_CONFIG;
+  if (c->nas != 0 && c->nas != 2) return PHLX_E_CONFIG;
+  const bool mhd = c->flux >= PHLX_HLLD_FLUX;
+  for (int d = 0; d < c->sdims; d++) {
+    const int lo = c->bc[2 * d], hi = c->bc[2 * d + 1];
+    if (lo < 0 || lo > 3 || hi < 0 || hi > 3) return PHLX_E_CONFIG;
+    if ((lo == PHLX_BC_PERIODIC) != (hi == PHLX_BC_PERIODIC)) return PHLX_E_CONFIG;
+    if (mhd && (lo == PHLX_BC_DIODE || hi == PHLX_BC_DIODE)) return PHLX_E_CONFIG;
+  }
+  if (c->eos != PHLX_EOS_IDEAL) return PHLX_E_CONFIG;  // Helmholtz: separate entry points
+  if (c->fasteos) return PHLX_E_CONFIG;
+  if (c->precision != PHLX_PREC_FP64) return PHLX_E_CONFIG;
+  if (c->cfl <= 0.0 || c->gamma <= 1.0 || c->mu <= 0.0) return PHLX_E_CONFIG;
+  for (int r = 0; r < 8; r++)
+    if (c->reserved[r] != 0) return PHLX_E_CONFIG;
+  return PHLX_OK;
+}
+
+template <typename T>
+static int dalloc(phlx_ctx *ctx, T **p, size_t n) {
+  void *q = nullptr;
+  if (hipMalloc(&q, n * sizeof(T) + 256) != hipSuccess) return PHLX_E_NOMEM;
+  if (hipMemsetAsync(q, 0, n * sizeof(T) + 256, ctx->stream) != hipSuccess) return PHLX_E_HIP;
+  ctx->allocs.push_back(q);
+  *p = (T *)q;
+  return PHLX_OK;
+}
+
+extern "C" int phlx_create(const phlx_config *cfg, phlx_ctx **out) {
+  if (!cfg || !out) return PHLX_E_ARG;
+  *out = nullptr;
+  int rc = validate(cfg);
+  if (rc) return rc;
+  phlx_ctx *ctx = new (std::nothrow) phlx_ctx();
+  if (!ctx) return PHLX_E_NOMEM;
+  ctx->cfg = *cfg;
+  ctx->device = cfg->device;
+  if (hipSetDevice(cfg->device) != hipSuccess) {
+    delete ctx;
+    return PHLX_E_HIP;
+  }
+  if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete ctx;
+    return PHLX_E_HIP;
+  }
+  Grid &g = ctx->g;
+  std::memset(&g, 0, sizeof(Grid));
+  g.sd = cfg->sdims;
+  g.nas = cfg->nas;
+  g.nv = cfg->sdims + 2 + cfg->nas;
+  g.ip = cfg->sdims + 1;
+  g.n1 = cfg->nx1;
+  g.n2 = cfg->nx2;
+  g.n3 = cfg->sdims == 3 ? cfg->nx3 : 1;
+  g.N = (long long)g.n1 * g.n2 * g.n3;
+  g.nf[0] = (long long)(g.n1 + 1) * g.n2 * g.n3;
+  g.nf[1] = (long long)g.n1 * (g.n2 + 1) * g.n3;
+  g.nf[2] = (long long)g.n1 * g.n2 * (g.n3 + 1);
+  for (int i = 0; i < 6; i++) g.bc[i] = cfg->bc[i];
+  // source/source.F90:1725-1734
+  const double dx1 = (cfg->x1u - cfg->x1l) / (double)cfg->nx1;
+  const double dx2 = (cfg->x2u - cfg->x2l) / (double)cfg->nx2;
+  const double dx3 = cfg->sdims == 3 ? (cfg->x3u - cfg->x3l) / (double)cfg->nx3 : 1.0;
+  g.inv_dx[0] = 1.0 / dx1;
+  g.inv_dx[1] = 1.0 / dx2;
+  g.inv_dx[2] = 1.0 / dx3;
+  g.gm = cfg->gamma;
+  g.gmm1 = g.gm - 1.0;
+  g.igmm1 = 1.0 / g.gmm1;
+  g.inv_mu = 1.0 / cfg->mu;
+  g.mach_cutoff = cfg->mach_cutoff;
+  g.mach_ct = cfg->mach_ct;
+  g.recon = cfg->recon;
+  g.flux = cfg->flux;
+  g.all_mach = cfg->all_mach;
+  g.wb = cfg->use_wb;
+  g.grav = cfg->use_gravity;
+  g.etot = cfg->evolve_etot;
+  g.edot = cfg->use_edot;
+  g.fasteos = cfg->fasteos;
+  g.mhd = cfg->flux >= PHLX_HLLD_FLUX;
+  ctx->stages = cfg->rk_stages;
+  // Shu-Osher rows (source/source.F90:7018-7048)
+  if (ctx->stages == 2) {
+    const double r[2][3] = {{1.0, 0.0, -1.0}, {0.5, 0.5, -0.5}};
+    for (int s = 0; s < 2; s++)
+      for (int c = 0; c < 3; c++) ctx->rk[s][c] = r[s][c];
+  } else {
+    ctx->rk[0][0] = 1.0; ctx->rk[0][1] = 0.0; ctx->rk[0][2] = -1.0;
+    ctx->rk[1][0] = 0.75; ctx->rk[1][1] = 0.25; ctx->rk[1][2] = -0.25;
+    ctx->rk[2][0] = 1.0 / 3.0; ctx->rk[2][1] = 2.0 / 3.0; ctx->rk[2][2] = -(2.0 / 3.0);
+  }
+  if (!select_kernels(g.sd, g.nas, g.mhd, ctx->kt)) {
+    phlx_destroy(ctx);
+    return PHLX_E_CONFIG;
+  }
+  const size_t N = (size_t)g.N, nv = (size_t)g.nv;
+  int e = 0;
+  e |= dalloc(ctx, &g.P, nv * N);
+  e |= dalloc(ctx, &g.U, nv * N);
+  e |= dalloc(ctx, &g.U0, nv * N);
+  e |= dalloc(ctx, &g.eint, N);
+  for (int d = 0; d < g.sd; d++) e |= dalloc(ctx, &g.F[d], nv * (size_t)g.nf[d]);
+  if (g.mhd) {
+    e |= dalloc(ctx, &g.bcc, 3 * N);
+    for (int d = 0; d < g.sd; d++) {
+      e |= dalloc(ctx, &g.bf[d], (size_t)g.nf[d]);
+      e |= dalloc(ctx, &g.b0f[d], (size_t)g.nf[d]);
+      e |= dalloc(ctx, &g.emf[d], 2 * (size_t)g.nf[d]);
+      e |= dalloc(ctx, &g.sgn[d], (size_t)g.nf[d]);
+    }
+    const size_t n1 = g.n1, n2 = g.n2, n3 = g.n3;
+    e |= dalloc(ctx, &g.cor[2], (n1 + 1) * (n2 + 1) * n3);
+    if (g.sd == 3) {
+      e |= dalloc(ctx, &g.cor[0], n1 * (n2 + 1) * (n3 + 1));
+      e |= dalloc(ctx, &g.cor[1], (n1 + 1) * n2 * (n3 + 1));
+    }
+  }
+  double *prof = nullptr;
+  const size_t ny = g.n2;
+  e |= dalloc(ctx, &prof, 8 * (ny + 1));
+  e |= dalloc(ctx, &g.sc, 1);
+  e |= dalloc(ctx, &ctx->scratch_bits, 1);
+  if (e) {
+    phlx_destroy(ctx);
+    return PHLX_E_NOMEM;
+  }
+  g.eq_rho_cc = prof + 0 * (ny + 1);
+  g.eq_p_cc = prof + 1 * (ny + 1);
+  g.eq_rho_f = prof + 2 * (ny + 1);
+  g.eq_p_f = prof + 3 * (ny + 1);
+  g.gravy = prof + 4 * (ny + 1);
+  g.phi_cc = prof + 5 * (ny + 1);
+  g.phi_f = prof + 6 * (ny + 1);
+  g.edot_cc = prof + 7 * (ny + 1);
+  Scalars s0;
+  std::memset(&s0, 0, sizeof(s0));
+  s0.tmax = cfg->tmax;
+  s0.cfl = cfg->cfl;
+  s0.active = 1;
+  if (hipMemcpyAsync(g.sc, &s0, sizeof(s0), hipMemcpyHostToDevice, ctx->stream) != hipSuccess ||
+      hipStreamSynchronize(ctx->stream) != hipSuccess) {
+    phlx_destroy(ctx);
+    return PHLX_E_HIP;
+  }
+  *out = ctx;
+  return PHLX_OK;
+}
+
+extern "C" int phlx_destroy(phlx_ctx *ctx) {
+  if (!ctx) return PHLX_E_ARG;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  for (auto &r : ctx->recs) {
+    (void)hipEventDestroy(r.a);
+    (void)hipEventDestroy(r.b);
+  }
+  for (void *p : ctx->allocs) (void)hipFree(p);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return PHLX_OK;
+}
+
+extern "C" int phlx_nvars(const phlx_ctx *ctx) { return ctx ? ctx->g.nv : PHLX_E_ARG; }
+extern "C" void *phlx_stream(phlx_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
+
+// ------------------------------------------------------------- host exchange
+
+static int copy_in(phlx_ctx *ctx, void *dst, const void *src, size_t bytes) {
+  HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+  return PHLX_OK;
+}
+static int copy_out(phlx_ctx *ctx, void *dst, const void *src, size_t bytes) {
+  HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  return PHLX_OK;
+}
+
+extern "C" int phlx_upload_state(phlx_ctx *ctx, const double *prim, const double *eint,
+                                 const double *temp, const double *b1, const double *b2,
+                                 const double *b3) {
+  if (!ctx) return PHLX_E_ARG;
+  HIPCHK(hipSetDevice(ctx->device));
+  Grid &g = ctx->g;
+  int rc = 0;
+  if (prim) rc |= copy_in(ctx, g.P, prim, sizeof(double) * g.nv * g.N);
+  if (eint) rc |= copy_in(ctx, g.eint, eint, sizeof(double) * g.N);
+  (void)temp;  // the ideal-gas EoS carries no temperature on the device
+  const double *bs[3] = {b1, b2, b3};
+  for (int d = 0; d < g.sd; d++)
+    if (bs[d]) {
+      if (!g.mhd) return PHLX_E_ARG;
+      rc |= copy_in(ctx, g.bf[d], bs[d], sizeof(double) * g.nf[d]);
+    }
+  if (rc) return PHLX_E_HIP;
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return PHLX_OK;
+}
+
+extern "C" int phlx_download_state(phlx_ctx *ctx, double *prim, double *eint, double *temp,
+                                   double *b1, double *b2, double *b3, double *bcc) {
+  if (!ctx) return PHLX_E_ARG;
+  HIPCHK(hipSetDevice(ctx->device));
+  Grid &g = ctx->g;
+  int rc = 0;
+  if (prim) rc |= copy_out(ctx, prim, g.P, sizeof(double) * g.nv * g.N);
+  if (eint) rc |= copy_out(ctx, eint, g.eint, sizeof(double) * g.N);
+  if (temp) {
+    // T = p / (rho R / mu) for the ideal gas (source/source.F90:13670-13674)
+    std::vector<double> p(2 * g.N);
+    rc |= copy_out(ctx, p.data(), g.P, sizeof(double) * g.N);
+    rc |= copy_out(ctx, p.data() + g.N, g.P + (size_t)g.ip * g.N, sizeof(double) * g.N);
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    for (long long c = 0; c < g.N; c++) temp[c] = p[g.N + c] / (p[c] * kRgas * g.inv_mu);
+  }
+  double *bs[3] = {b1, b2, b3};
+  for (int d = 0; d < g.sd; d++)
+    if (bs[d] && g.mhd) rc |= copy_out(ctx, bs[d], g.bf[d], sizeof(double) * g.nf[d]);
+  if (bcc) {
+    if (g.mhd) {
+      rc |= copy_out(ctx, bcc, g.bcc, sizeof(double) * g.sd * g.N);
+      if (g.sd == 2) std::memset(bcc + 2 * g.N, 0, sizeof(double) * g.N);
+    } else {
+      std::memset(bcc, 0, sizeof(double) * 3 * g.N);
+    }
+  }
+  if (rc) return PHLX_E_HIP;
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return PHLX_OK;
+}
+
+// Fortran AoS prim(1:nv, lx1-ngc:ux1+ngc, lx2-ngc:ux2+ngc, lx3-ngc:ux3+ngc)
+// <-> interior SoA (source/source.F90:1219-1227).
+static void aos_geom(const Grid &g, int ngc, long long &e1, long long &e2, long long &g3) {
+  e1 = g.n1 + 2 * ngc;
+  e2 = g.n2 + 2 * ngc;
+  g3 = (g.sd == 3) ? ngc : 0;
+}
+
+extern "C" int phlx_upload_prim_aos(phlx_ctx *ctx, const double *prim, int32_t ngc) {
+  if (!ctx || !prim || ngc < 0) return PHLX_E_ARG;
+  const Grid &g = ctx->g;
+  long long e1, e2, g3;
+  aos_geom(g, ngc, e1, e2, g3);
+  std::vector<double> soa((size_t)g.nv * g.N);
+  for (int k = 0; k < g.n3; k++)
+    for (int j = 0; j < g.n2; j++)
+      for (int i = 0; i < g.n1; i++) {
+        const long long a = (((k + g3) * e2 + (j + ngc)) * e1 + (i + ngc)) * g.nv;
+        const long long c = ((long long)k * g.n2 + j) * g.n1 + i;
+        for (int v = 0; v < g.nv; v++) soa[(size_t)v * g.N + c] = prim[a + v];
+      }
+  return phlx_upload_state(ctx, soa.data(), nullptr, nullptr, nullptr, nullptr, nullptr);
+}
+
+extern "C" int phlx_download_prim_aos(phlx_ctx *ctx, double *prim, int32_t ngc) {
+  if (!ctx || !prim || ngc < 0) return PHLX_E_ARG;
+  const Grid &g = ctx->g;
+  long long e1, e2, g3;
+  aos_geom(g, ngc, e1, e2, g3);
+  std::vector<double> soa((size_t)g.nv * g.N);
+  int rc = phlx_download_state(ctx, soa.data(), nullptr, nullptr, nullptr, nullptr, nullptr,
+                               nullptr);
+  if (rc) return rc;
+  const long long e3 = g.n3 + 2 * g3;
+  std::memset(prim, 0, sizeof(double) * g.nv * e1 * e2 * e3);
+  for (int k = 0; k < g.n3; k++)
+    for (int j = 0; j < g.n2; j++)
+      for (int i = 0; i < g.n1; i++) {
+        const long long a = (((k + g3) * e2 + (j + ngc)) * e1 + (i + ngc)) * g.nv;
+        const long long c = ((long long)k * g.n2 + j) * g.n1 + i;
+        for (int v = 0; v < g.nv; v++) prim[a + v] = soa[(size_t)v * g.N + c];
+      }
+  return PHLX_OK;
+}
+
+extern "C" int phlx_set_profiles(phlx_ctx *ctx, const double *eq_rho_cc, const double *eq_p_cc,
+                                 const double *eq_rho_f, const double *eq_p_f,
+                                 const double *grav_cc, const double *phi_cc,
+                                 const double *phi_f, const double *edot_cc) {
+  if (!ctx) return PHLX_E_ARG;
+  HIPCHK(hipSetDevice(ctx->device));
+  Grid &g = ctx->g;
+  const size_t ny = g.n2;
+  std::vector<double> h(8 * (ny + 1), 0.0);
+  const double *src[8] = {eq_rho_cc, eq_p_cc, eq_rho_f, eq_p_f, grav_cc, phi_cc, phi_f, edot_cc};
+  const size_t len[8] = {ny, ny, ny + 1, ny + 1, ny, ny, ny + 1, ny};
+  for (int a = 0; a < 8; a++)
+    if (src[a]) std::memcpy(h.data() + a * (ny + 1), src[a], len[a] * sizeof(double));
+  HIPCHK(hipMemcpyAsync((void *)g.eq_rho_cc, h.data(), h.size() * sizeof(double),
+                        hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return PHLX_OK;
+}
+
+// ------------------------------------------------------------- profiling
+
+static int kbegin(phlx_ctx *ctx, int kid, ProfRec &r) {
+  if (!ctx->prof) return 0;
+  r.kid = kid;
+  if (hipEventCreate(&r.a) != hipSuccess || hipEventCreate(&r.b) != hipSuccess) return -1;
+  (void)hipEventRecord(r.a, ctx->stream);
+  return 1;
+}
+static void kend(phlx_ctx *ctx, int on, ProfRec &r) {
+  if (on != 1) return;
+  (void)hipEventRecord(r.b, ctx->stream);
+  ctx->recs.push_back(r);
+}
+
+static int harvest(phlx_ctx *ctx) {
+  if (ctx->recs.empty()) return PHLX_OK;
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  for (auto &r : ctx->recs) {
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, r.a, r.b);
+    ctx->prof_ms[r.kid] += ms;
+    ctx->prof_n[r.kid] += 1;
+    (void)hipEventDestroy(r.a);
+    (void)hipEventDestroy(r.b);
+  }
+  ctx->recs.clear();
+  return PHLX_OK;
+}
+
+extern "C" int phlx_profile_enable(phlx_ctx *ctx, int32_t on) {
+  if (!ctx) return PHLX_E_ARG;
+  int rc = harvest(ctx);
+  if (rc) return rc;
+  ctx->prof = on != 0;
+  for (int i = 0; i < KID_COUNT; i++) {
+    ctx->prof_ms[i] = 0.0;
+    ctx->prof_n[i] = 0;
+  }
+  return PHLX_OK;
+}
+
+extern "C" int phlx_profile_read(phlx_ctx *ctx, int32_t kernel, double *total_ms,
+                                 int64_t *launches) {
+  if (!ctx || kernel < 0 || kernel >= KID_COUNT) return PHLX_E_ARG;
+  int rc = harvest(ctx);
+  if (rc) return rc;
+  if (total_ms) *total_ms = ctx->prof_ms[kernel];
+  if (launches) *launches = ctx->prof_n[kernel];
+  return PHLX_OK;
+}
+
+// ------------------------------------------------------------- stepping
+
+static int check_launch() {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    std::fprintf(stderr, "phlx: launch failed: %s\n", hipGetErrorString(e));
+    return PHLX_E_HIP;
+  }
+  return PHLX_OK;
+}
+
+extern "C" int phlx_stage_begin(phlx_ctx *ctx, int32_t irk) {
+  // state0 / b0 capture is fused into the first stage's update and face-B
+  // kernels; nothing to launch here beyond argument checks.
+  if (!ctx) return PHLX_E_ARG;
+  if (irk < 0 || irk >= ctx->stages) return PHLX_E_ARG;
+  if (!ctx->initialized) return PHLX_E_STATE;
+  return PHLX_OK;
+}
+
+extern "C" int phlx_sweep(phlx_ctx *ctx, int32_t dir) {
+  if (!ctx || dir < 0 || dir >= ctx->g.sd) return PHLX_E_ARG;
+  ProfRec r;
+  int on = kbegin(ctx, KID_FLUX1 + dir, r);
+  ctx->kt.flux(ctx->g, dir, ctx->stream);
+  kend(ctx, on, r);
+  return check_launch();
+}
+
+extern "C" int phlx_ct(phlx_ctx *ctx, int32_t irk, double a1, double a2, double a3) {
+  if (!ctx) return PHLX_E_ARG;
+  if (!ctx->g.mhd) return PHLX_OK;
+  ProfRec r;
+  int on = kbegin(ctx, KID_CORNER, r);
+  ctx->kt.corner(ctx->g, ctx->stream);
+  kend(ctx, on, r);
+  on = kbegin(ctx, KID_FACEB, r);
+  ctx->kt.faceb(ctx->g, irk == 0, a1, a2, a3, ctx->stream);
+  kend(ctx, on, r);
+  return check_launch();
+}
+
+extern "C" int phlx_rk_update(phlx_ctx *ctx, int32_t irk, double a1, double a2, double a3,
+                              int32_t last_stage) {
+  if (!ctx) return PHLX_E_ARG;
+  ProfRec r;
+  int on = kbegin(ctx, KID_UPDATE, r);
+  ctx->kt.update(ctx->g, irk == 0, last_stage != 0, a1, a2, a3, ctx->stream);
+  kend(ctx, on, r);
+  return check_launch();
+}
+
+static int do_hydro_step(phlx_ctx *ctx, bool with_cfl) {
+  for (int irk = 0; irk < ctx->stages; irk++) {
+    int rc = phlx_stage_begin(ctx, irk);
+    if (rc) return rc;
+    for (int d = 0; d < ctx->g.sd; d++) {
+      rc = phlx_sweep(ctx, d);
+      if (rc) return rc;
+    }
+    const double a1 = ctx->rk[irk][0], a2 = ctx->rk[irk][1], a3 = ctx->rk[irk][2];
+    rc = phlx_ct(ctx, irk, a1, a2, a3);
+    if (rc) return rc;
+    rc = phlx_rk_update(ctx, irk, a1, a2, a3, with_cfl && irk == ctx->stages - 1);
+    if (rc) return rc;
+  }
+  return PHLX_OK;
+}
+
+extern "C" int phlx_init(phlx_ctx *ctx) {
+  if (!ctx) return PHLX_E_ARG;
+  HIPCHK(hipSetDevice(ctx->device));
+  Scalars s0;
+  std::memset(&s0, 0, sizeof(s0));
+  s0.tmax = ctx->cfg.tmax;
+  s0.cfl = ctx->cfg.cfl;
+  s0.active = 1;
+  HIPCHK(hipMemcpyAsync(ctx->g.sc, &s0, sizeof(s0), hipMemcpyHostToDevice, ctx->stream));
+  ctx->kt.init(ctx->g, ctx->stream);
+  int rc = check_launch();
+  if (rc) return rc;
+  ctx->initialized = true;
+  return phlx_compute_dt(ctx, nullptr, nullptr);
+}
+
+extern "C" int phlx_compute_dt(phlx_ctx *ctx, double *dt_out, double *smax_out) {
+  if (!ctx) return PHLX_E_ARG;
+  if (!ctx->initialized) return PHLX_E_STATE;
+  launch_zero_bits(&ctx->g.sc->smax_bits, ctx->stream);
+  ctx->kt.cfl(ctx->g, ctx->stream);
+  launch_step_end(ctx->g.sc, 0, ctx->stream);
+  int rc = check_launch();
+  if (rc) return rc;
+  if (dt_out || smax_out) {
+    double t, dt, smax;
+    int64_t st;
+    rc = phlx_get_scalars(ctx, &t, &dt, &smax, &st);
+    if (rc) return rc;
+    if (dt_out) *dt_out = dt;
+    if (smax_out) *smax_out = smax;
+  }
+  return PHLX_OK;
+}
+
+extern "C" int phlx_hydro_step(phlx_ctx *ctx) {
+  if (!ctx) return PHLX_E_ARG;
+  if (!ctx->initialized) return PHLX_E_STATE;
+  HIPCHK(hipSetDevice(ctx->device));
+  return do_hydro_step(ctx, false);
+}
+
+extern "C" int phlx_run(phlx_ctx *ctx, int32_t nsteps) {
+  if (!ctx || nsteps < 0) return PHLX_E_ARG;
+  if (!ctx->initialized) return PHLX_E_STATE;
+  HIPCHK(hipSetDevice(ctx->device));
+  for (int s = 0; s < nsteps; s++) {
+    launch_step_begin(ctx->g.sc, ctx->stream);
+    int rc = do_hydro_step(ctx, true);
+    if (rc) return rc;
+    launch_step_end(ctx->g.sc, 1, ctx->stream);
+    rc = check_launch();
+    if (rc) return rc;
+  }
+  return PHLX_OK;
+}
+
+extern "C" int phlx_synchronize(phlx_ctx *ctx) {
+  if (!ctx) return PHLX_E_ARG;
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return PHLX_OK;
+}
+
+extern "C" int phlx_get_scalars(phlx_ctx *ctx, double *time, double *dt, double *smax,
+                                int64_t *step) {
+  if (!ctx) return PHLX_E_ARG;
+  Scalars s;
+  HIPCHK(hipMemcpyAsync(&s, ctx->g.sc, sizeof(s), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  if (time) *time = s.time;
+  if (dt) *dt = s.dt;
+  if (smax) *smax = s.smax;
+  if (step) *step = s.step;
+  return PHLX_OK;
+}
+
+extern "C" int phlx_set_scalars(phlx_ctx *ctx, double time, double dt, double smax,
+                                int64_t step) {
+  if (!ctx) return PHLX_E_ARG;
+  Scalars s;
+  HIPCHK(hipMemcpyAsync(&s, ctx->g.sc, sizeof(s), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  s.time = time;
+  s.dt = dt;
+  s.smax = smax;
+  s.step = step;
+  HIPCHK(hipMemcpyAsync(ctx->g.sc, &s, sizeof(s), hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return PHLX_OK;
+}
+
+extern "C" int phlx_divb_max(phlx_ctx *ctx, double *out) {
+  if (!ctx || !out) return PHLX_E_ARG;
+  if (!ctx->g.mhd) {
+    *out = 0.0;
+    return PHLX_OK;
+  }
+  launch_zero_bits(ctx->scratch_bits, ctx->stream);
+  ctx->kt.divb(ctx->g, ctx->scratch_bits, ctx->stream);
+  int rc = check_launch();
+  if (rc) return rc;
+  unsigned long long bits = 0;
+  HIPCHK(hipMemcpyAsync(&bits, ctx->scratch_bits, sizeof(bits), hipMemcpyDeviceToHost,
+                        ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  double v;
+  std::memcpy(&v, &bits, sizeof(v));
+  *out = v;
+  return PHLX_OK;
+}

@@ -1,0 +1,492 @@
+// phlx_device.hpp -- device-side physics of the explicit FV MHD update.
+//
+// State is structure-of-arrays in HBM with no stored ghost zones: a load
+// outside the interior is redirected through the boundary map of each
+// axis (periodic wrap or mirror) and the mirrored normal components are
+// sign-flipped, which reproduces the reference's halo copy + physical
+// boundary fill (source/source.F90:2361-2446, 7532-7956) value for value.
+//
+// Floating-point expressions keep the reference's evaluation order
+// (left-to-right, no FMA contraction: the library is compiled with
+// -ffp-contract=off) so the FP64 path agrees with a CPU run of the same
+// algorithm to round-off.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace phlx {
+
+constexpr double kRgas = 8.31446261815324e7;  // source/source.F90:605
+
+enum { BC_PERIODIC = 0, BC_REFLECTIVE = 1, BC_OUTFLOW = 2, BC_DIODE = 3 };
+enum { REC_LIM2ND = 0, REC_PPH = 1 };
+enum { FLUX_HLLC = 0, FLUX_LHLLC = 1, FLUX_HLLD = 2, FLUX_LHLLD = 3 };
+
+// Device-resident step control: dt/time/smax never round-trip to the host.
+struct Scalars {
+  double time, dt, smax, tmax, cfl;
+  unsigned long long smax_bits;  // running max of positive doubles as bits
+  long long step;
+  int active;  // 0 once time >= tmax: the step's kernels return at once
+  int pad;
+};
+
+struct Grid {
+  int n1, n2, n3, sd, nv, nas, ip;
+  long long N;       // interior cells
+  long long nf[3];   // faces per direction
+  int bc[6];
+  double inv_dx[3];
+  double gm, gmm1, igmm1, inv_mu, mach_cutoff, mach_ct;
+  int recon, flux, all_mach, wb, grav, etot, edot, fasteos, mhd;
+  double *P, *U, *U0, *eint, *bcc;
+  double *bf[3], *b0f[3];
+  double *F[3];
+  double *emf[3];          // two EMF components per face (see k_flux)
+  signed char *sgn[3];     // contact-upwind sign per face
+  double *cor[3];          // corner (edge) EMFs along x1, x2, x3
+  const double *eq_rho_cc, *eq_p_cc, *eq_rho_f, *eq_p_f, *gravy, *phi_cc, *phi_f,
+      *edot_cc;
+  Scalars *sc;
+};
+
+__device__ __forceinline__ long long cix(const Grid &g, int i, int j, int k) {
+  return ((long long)k * g.n2 + j) * g.n1 + i;
+}
+__device__ __forceinline__ long long fix(const Grid &g, int d, int i, int j, int k) {
+  if (d == 0) return ((long long)k * g.n2 + j) * (g.n1 + 1) + i;
+  if (d == 1) return ((long long)k * (g.n2 + 1) + j) * g.n1 + i;
+  return ((long long)k * g.n2 + j) * g.n1 + i;
+}
+// corner arrays: d = edge direction
+__device__ __forceinline__ long long kix(const Grid &g, int d, int i, int j, int k) {
+  if (d == 2) return ((long long)k * (g.n2 + 1) + j) * (g.n1 + 1) + i;
+  if (d == 0) return ((long long)k * (g.n2 + 1) + j) * g.n1 + i;
+  return ((long long)k * g.n2 + j) * (g.n1 + 1) + i;
+}
+
+// Map a cell-centred index along axis a into the interior.  side = -1/+1
+// when a physical (mirror) boundary was crossed, 0 otherwise.
+__device__ __forceinline__ int map_axis(const Grid &g, int a, int c, int n, int &side) {
+  side = 0;
+  if (c < 0) {
+    if (g.bc[2 * a] == BC_PERIODIC) return c + n;
+    side = -1;
+    return -1 - c;
+  }
+  if (c >= n) {
+    if (g.bc[2 * a + 1] == BC_PERIODIC) return c - n;
+    side = 1;
+    return 2 * n - 1 - c;
+  }
+  return c;
+}
+
+// Does the mirrored ghost flip the normal (axis a) velocity / field?
+// REFLECTIVE always; DIODE when the boundary cell flows inward
+// (source/source.F90:7556-7569).
+__device__ __forceinline__ bool flips(const Grid &g, int a, int side, long long cb) {
+  int bc = g.bc[2 * a + (side > 0)];
+  if (bc == BC_REFLECTIVE) return true;
+  if (bc == BC_DIODE) {
+    double v = g.P[(long long)(1 + a) * g.N + cb];
+    return side < 0 ? (v > 0.0) : (v < 0.0);
+  }
+  return false;
+}
+
+// Load primitive state (WB deviation when g.wb) and cell-centred B of a
+// possibly-ghost cell.
+template <int SD, int NV, bool MHD>
+__device__ __forceinline__ void load_cell(const Grid &g, int i, int j, int k, double *q,
+                                          double *b) {
+  int si, sj, sk = 0;
+  int mi = map_axis(g, 0, i, g.n1, si);
+  int mj = map_axis(g, 1, j, g.n2, sj);
+  int mk = (SD == 3) ? map_axis(g, 2, k, g.n3, sk) : k;
+  long long c = cix(g, mi, mj, mk);
+  const long long N = g.N;
+#pragma unroll
+  for (int v = 0; v < NV; v++) q[v] = g.P[v * N + c];
+  if (g.wb) {
+    q[0] = q[0] - g.eq_rho_cc[mj];
+    q[SD + 1] = q[SD + 1] - g.eq_p_cc[mj];
+  }
+  if (MHD) {
+#pragma unroll
+    for (int v = 0; v < SD; v++) b[v] = g.bcc[v * N + c];
+  }
+  if (si && flips(g, 0, si, cix(g, si < 0 ? 0 : g.n1 - 1, mj, mk))) {
+    q[1] = -q[1];
+    if (MHD) b[0] = -b[0];
+  }
+  if (sj && flips(g, 1, sj, cix(g, mi, sj < 0 ? 0 : g.n2 - 1, mk))) {
+    q[2] = -q[2];
+    if (MHD) b[1] = -b[1];
+  }
+  if (SD == 3 && sk && flips(g, 2, sk, cix(g, mi, mj, sk < 0 ? 0 : g.n3 - 1))) {
+    q[3] = -q[3];
+    if (MHD) b[2] = -b[2];
+  }
+}
+
+// ------------------------------------------------------------ reconstruction
+// Face values of one cell from (q-, q, q+): l = left edge, r = right edge.
+__device__ __forceinline__ void rec_lim2nd(double qm, double qc, double qp, double &l,
+                                           double &r) {
+  // harmonic-mean limited slope, source/source.F90:8141-8181
+  double sf = qp - qc, sb = qc - qm;
+  double prod = sf * sb, slope = 0.0;
+  if (prod > 0.0) slope = prod / (sf + sb);
+  double h = 0.5 * slope;
+  l = qc - h;
+  r = qc + h;
+}
+__device__ __forceinline__ void rec_pph(double qm, double qc, double qp, double &l,
+                                        double &r) {
+  // unlimited parabola through three cell means, source/source.F90:8183-8193
+  double c = (5.0 / 6.0) * qc;
+  l = (1.0 / 3.0) * qm + c - (1.0 / 6.0) * qp;
+  r = -(1.0 / 6.0) * qm + c + (1.0 / 3.0) * qp;
+}
+__device__ __forceinline__ void rec_pph_lim(double qm, double qc, double qp, double &l,
+                                            double &r) {
+  // parabola clipped to the neighbour range + extremum limiter for the
+  // advected scalars, source/source.F90:8194-8243
+  double c = (5.0 / 6.0) * qc;
+  double am = (1.0 / 3.0) * qm + c - (1.0 / 6.0) * qp;
+  double ap = -(1.0 / 6.0) * qm + c + (1.0 / 3.0) * qp;
+  double lo = fmin(qm, qc), hi = fmax(qm, qc);
+  am = am < lo ? lo : am;
+  am = am > hi ? hi : am;
+  lo = fmin(qc, qp);
+  hi = fmax(qc, qp);
+  ap = ap < lo ? lo : ap;
+  ap = ap > hi ? hi : ap;
+  am = am - qc;
+  ap = ap - qc;
+  if (ap * am >= 0.0) {
+    ap = 0.0;
+    am = 0.0;
+  } else {
+    if (fabs(ap) >= 2.0 * fabs(am)) ap = -2.0 * am;
+    if (fabs(am) >= 2.0 * fabs(ap)) am = -2.0 * ap;
+  }
+  l = am + qc;
+  r = ap + qc;
+}
+
+// ------------------------------------------------------------ Riemann: hydro
+// HLLC / LHLLC in the face frame (slot 1 = normal velocity).
+// source/source.F90:15611-16260.  g_e/g_c: FASTEOS gamma_e, gamma_c.
+template <int SD, int NAS>
+__device__ __forceinline__ void riemann_hydro(const Grid &g, const double *qL,
+                                              const double *qR, const double *gL,
+                                              const double *gR, double *F) {
+  constexpr int IE = SD + 1, IA = SD + 2, NV = SD + 2 + NAS;
+  const double rhoL = qL[0], irL = 1.0 / rhoL;
+  const double uL = qL[1], vL = qL[2], wL = (SD == 3) ? qL[3] : 0.0;
+  const double v2L = uL * uL + vL * vL + wL * wL;
+  const double pL = qL[IE];
+  double eL, cL, eR, cR;
+  if (g.fasteos) {
+    eL = pL / (gL[0] - 1.0);
+    cL = sqrt(gL[1] * pL * irL);
+  } else {
+    eL = g.igmm1 * pL;
+    cL = sqrt(g.gm * pL * irL);
+  }
+  const double EL = eL + 0.5 * rhoL * v2L;
+  const double rhoR = qR[0], irR = 1.0 / rhoR;
+  const double uR = qR[1], vR = qR[2], wR = (SD == 3) ? qR[3] : 0.0;
+  const double v2R = uR * uR + vR * vR + wR * wR;
+  const double pR = qR[IE];
+  if (g.fasteos) {
+    eR = pR / (gR[0] - 1.0);
+    cR = sqrt(gR[1] * pR * irR);
+  } else {
+    eR = g.igmm1 * pR;
+    cR = sqrt(g.gm * pR * irR);
+  }
+  const double ER = eR + 0.5 * rhoR * v2R;
+  const double cmax = fmax(cL, cR);
+  const double sL = fmin(uL, uR) - cmax;
+  const double sR = fmax(uL, uR) + cmax;
+  const double dL = sL - uL, dR = sR - uR;
+  const double us = (pR - pL + rhoL * uL * dL - rhoR * uR * dR) / (rhoL * dL - rhoR * dR);
+  const double rsL = rhoL * (dL / (sL - us));
+  const double EsL = rsL * (EL * irL + (us - uL) * (us + pL * irL / dL));
+  const double rsR = rhoR * (dR / (sR - us));
+  const double EsR = rsR * (ER * irR + (us - uR) * (us + pR * irR / dR));
+
+  if (g.flux == FLUX_LHLLC) {
+    // low-dissipation pressure, source/source.F90:16014-16030
+    double chi = fmin(1.0, fmax(sqrt(v2L) / cL, sqrt(v2R) / cR));
+    chi = fmax(g.mach_cutoff, chi);
+    const double phi = chi * (2.0 - chi);
+    const double rbar = 0.5 * (rhoL + rhoR), cbar = 0.5 * (cL + cR);
+    const double ps = 0.5 * (pL + pR) - 0.5 * phi * rbar * cbar * (uR - uL);
+    const double s = copysign(1.0, us);
+    const double hp = 0.5 * (1.0 + s), hm = 0.5 * (1.0 - s);
+    const double rs = hp * rsL + hm * rsR;
+    if (g.all_mach && sL > 0.0) {
+      F[0] = rhoL * uL;
+      F[1] = rhoL * uL * uL + pL;
+      F[2] = rhoL * uL * vL;
+      if (SD == 3) F[3] = rhoL * uL * wL;
+      F[IE] = (EL + pL) * uL;
+#pragma unroll
+      for (int v = IA; v < NV; v++) F[v] = rhoL * qL[v] * uL;
+    } else if (g.all_mach && sR < 0.0) {
+      F[0] = rhoR * uR;
+      F[1] = rhoR * uR * uR + pR;
+      F[2] = rhoR * uR * vR;
+      if (SD == 3) F[3] = rhoR * uR * wR;
+      F[IE] = (ER + pR) * uR;
+#pragma unroll
+      for (int v = IA; v < NV; v++) F[v] = rhoR * qR[v] * uR;
+    } else {
+      F[0] = rs * us;
+      F[1] = rs * us * us + ps;
+      F[2] = rs * us * (hp * vL + hm * vR);
+      if (SD == 3) F[3] = rs * us * (hp * wL + hm * wR);
+      F[IE] = (hp * EsL + hm * EsR + ps) * us;
+#pragma unroll
+      for (int v = IA; v < NV; v++) F[v] = rs * us * (hp * qL[v] + hm * qR[v]);
+    }
+    return;
+  }
+  // HLLC, source/source.F90:16124-16252
+  const bool left = us > 0.0;
+  const double rho = left ? rhoL : rhoR, u = left ? uL : uR, vv = left ? vL : vR,
+               ww = left ? wL : wR, E = left ? EL : ER, p = left ? pL : pR;
+  const double s = left ? sL : sR, rs = left ? rsL : rsR, Es = left ? EsL : EsR;
+  const double *q = left ? qL : qR;
+  double f[NV], U[NV];
+  U[0] = rho; U[1] = rho * u; U[2] = rho * vv;
+  if (SD == 3) U[3] = rho * ww;
+  U[IE] = E;
+  f[0] = rho * u; f[1] = rho * u * u + p; f[2] = rho * u * vv;
+  if (SD == 3) f[3] = rho * u * ww;
+  f[IE] = (E + p) * u;
+#pragma unroll
+  for (int v = IA; v < NV; v++) {
+    U[v] = rho * q[v];
+    f[v] = rho * u * q[v];
+  }
+  const bool outer = left ? (sL >= 0.0) : (sR < 0.0);
+  if (outer) {
+#pragma unroll
+    for (int v = 0; v < NV; v++) F[v] = f[v];
+  } else {
+    double Us[NV];
+    Us[0] = rs; Us[1] = rs * us; Us[2] = rs * vv;
+    if (SD == 3) Us[3] = rs * ww;
+    Us[IE] = Es;
+#pragma unroll
+    for (int v = IA; v < NV; v++) Us[v] = rs * q[v];
+#pragma unroll
+    for (int v = 0; v < NV; v++) F[v] = f[v] + s * (Us[v] - U[v]);
+  }
+}
+
+// -------------------------------------------------------------- Riemann: MHD
+// HLLD / LHLLD in the face frame; b = (b_n, b_t1, b_t2).  Returns the
+// induction fluxes Fb (Fb[0] = 0).  source/source.F90:13969-15024.
+template <int SD, int NAS>
+__device__ __forceinline__ void riemann_mhd(const Grid &g, const double *qL,
+                                            const double *qR, const double *bL,
+                                            const double *bR, const double *gL,
+                                            const double *gR, double *F, double *Fb) {
+  constexpr int IE = SD + 1, IA = SD + 2, NV = SD + 2 + NAS;
+  const double rhoL = qL[0], irL = 1.0 / rhoL;
+  const double uL = qL[1], vL = qL[2], wL = (SD == 3) ? qL[3] : 0.0;
+  const double v2L = uL * uL + vL * vL + wL * wL;
+  const double pL = qL[IE];
+  double e, c2L, c2R;
+  if (g.fasteos) {
+    e = pL / (gL[0] - 1.0);
+    c2L = gL[1] * pL * irL;
+  } else {
+    e = pL * g.igmm1;
+    c2L = g.gm * pL * irL;
+  }
+  double EL = e + 0.5 * rhoL * v2L;
+  const double byL = bL[1], bzL = (SD == 3) ? bL[2] : 0.0;
+  const double rhoR = qR[0], irR = 1.0 / rhoR;
+  const double uR = qR[1], vR = qR[2], wR = (SD == 3) ? qR[3] : 0.0;
+  const double v2R = uR * uR + vR * vR + wR * wR;
+  const double pR = qR[IE];
+  if (g.fasteos) {
+    e = pR / (gR[0] - 1.0);
+    c2R = gR[1] * pR * irR;
+  } else {
+    e = pR * g.igmm1;
+    c2R = g.gm * pR * irR;
+  }
+  double ER = e + 0.5 * rhoR * v2R;
+  const double byR = bR[1], bzR = (SD == 3) ? bR[2] : 0.0;
+
+  const double bx = 0.5 * (bL[0] + bR[0]), bx2 = bx * bx;
+  const double pbL = 0.5 * (bx2 + byL * byL + bzL * bzL);
+  const double pbR = 0.5 * (bx2 + byR * byR + bzR * bzR);
+  EL = EL + pbL;
+  ER = ER + pbR;
+  const double ptL = pL + pbL, ptR = pR + pbR;
+  const double a2L = 2.0 * pbL * irL, a2R = 2.0 * pbR * irR;
+  const double an2L = bx2 * irL, an2R = bx2 * irR;
+  double t = c2L + a2L;
+  const double cL = sqrt(0.5 * (c2L + a2L + sqrt(t * t - 4.0 * c2L * an2L)));
+  t = c2R + a2R;
+  const double cR = sqrt(0.5 * (c2R + a2R + sqrt(t * t - 4.0 * c2R * an2R)));
+  const double cmax = fmax(cL, cR);
+  const double sL = fmin(uL, uR) - cmax;
+  const double sR = fmax(uL, uR) + cmax;
+  const double dL = sL - uL, dR = sR - uR;
+  const double it1 = 1.0 / (dR * rhoR - dL * rhoL);
+  const double us = (dR * rhoR * uR - dL * rhoL * uL - ptR + ptL) * it1;
+  const double dmL = sL - us, dmR = sR - us;
+  double phi = 1.0;
+  if (g.flux == FLUX_LHLLD) {
+    // source/source.F90:14364-14384
+    t = v2L + a2L;
+    const double cu2L = 0.5 * (v2L + a2L + sqrt(t * t - 4.0 * v2L * an2L));
+    t = v2R + a2R;
+    const double cu2R = 0.5 * (v2R + a2R + sqrt(t * t - 4.0 * v2R * an2R));
+    const double mL = sqrt(cu2L) / cL, mR = sqrt(cu2R) / cR;
+    if (fmax(mL, mR) < 0.6) {
+      double chi = fmin(1.0, fmax(mL, mR));
+      chi = fmax(g.mach_cutoff, chi);
+      phi = chi * (2.0 - chi);
+    }
+  }
+  const double ps = (dR * rhoR * ptL - dL * rhoL * ptR - phi * rhoL * rhoR * dR * dL * (uL - uR)) * it1;
+  const double rsL = rhoL * dL / dmL;
+  const double rsR = rhoR * dR / dmR;
+  const double ia1 = 1.0 / (rhoL * dL * dmL - bx2);
+  const double vsL = vL - bx * byL * (us - uL) * ia1;
+  const double bysL = byL * (rhoL * dL * dL - bx2) * ia1;
+  const double wsL = wL - bx * bzL * (us - uL) * ia1;
+  const double bzsL = bzL * (rhoL * dL * dL - bx2) * ia1;
+  const double ia2 = 1.0 / (rhoR * dR * dmR - bx2);
+  const double vsR = vR - bx * byR * (us - uR) * ia2;
+  const double bysR = byR * (rhoR * dR * dR - bx2) * ia2;
+  const double wsR = wR - bx * bzR * (us - uR) * ia2;
+  const double bzsR = bzR * (rhoR * dR * dR - bx2) * ia2;
+  const double vbL = uL * bx + vL * byL + wL * bzL;
+  const double vbR = uR * bx + vR * byR + wR * bzR;
+  const double vbsL = us * bx + vsL * bysL + wsL * bzsL;
+  const double vbsR = us * bx + vsR * bysR + wsR * bzsR;
+  const double EsL = (dL * EL - ptL * uL + ps * us + bx * (vbL - vbsL)) / dmL;
+  const double EsR = (dR * ER - ptR * uR + ps * us + bx * (vbR - vbsR)) / dmR;
+  const double sqL = sqrt(rsL), sqR = sqrt(rsR);
+  const double abx = fabs(bx);
+  const double ssL = us - abx / sqL;
+  const double ssR = us + abx / sqR;
+  const double isq = 1.0 / (sqL + sqR);
+  const double sg = copysign(1.0, bx);
+  const double vss = (sqL * vsL + sqR * vsR + (bysR - bysL) * sg) * isq;
+  const double byss = (sqL * bysR + sqR * bysL + sqL * sqR * (vsR - vsL) * sg) * isq;
+  const double wss = (sqL * wsL + sqR * wsR + (bzsR - bzsL) * sg) * isq;
+  const double bzss = (sqL * bzsR + sqR * bzsL + sqL * sqR * (wsR - wsL) * sg) * isq;
+  const double vbss = us * bx + vss * byss + wss * bzss;
+  const double EssL = EsL - sqL * (vbsL - vbss) * sg;
+  const double EssR = EsR + sqR * (vbsR - vbss) * sg;
+
+  if (g.flux == FLUX_LHLLD) {
+    // one state selected by the wave fan, then the physical flux of it
+    double rho, u, v, w, E, by, bz, pt;
+    if (g.all_mach && sL > 0.0) {
+      rho = rhoL; u = uL; v = vL; w = wL; E = EL; by = byL; bz = bzL; pt = ptL;
+    } else if (ssL > 0.0) {
+      rho = rsL; u = us; v = vsL; w = wsL; E = EsL; by = bysL; bz = bzsL; pt = ps;
+    } else if (us > 0.0) {
+      rho = rsL; u = us; v = vss; w = wss; E = EssL; by = byss; bz = bzss; pt = ps;
+    } else if (ssR > 0.0) {
+      rho = rsR; u = us; v = vss; w = wss; E = EssR; by = byss; bz = bzss; pt = ps;
+    } else if (!g.all_mach || sR > 0.0) {
+      rho = rsR; u = us; v = vsR; w = wsR; E = EsR; by = bysR; bz = bzsR; pt = ps;
+    } else {
+      rho = rhoR; u = uR; v = vR; w = wR; E = ER; by = byR; bz = bzR; pt = ptR;
+    }
+    F[0] = rho * u;
+    F[1] = rho * u * u + pt - bx2;
+    F[2] = rho * u * v - bx * by;
+    if (SD == 3) F[3] = rho * u * w - bx * bz;
+    F[IE] = (E + pt) * u - bx * (bx * u + v * by + w * bz);
+    if (NAS > 0) {
+      const double s2 = copysign(1.0, u), hp = 0.5 * (1.0 + s2), hm = 0.5 * (1.0 - s2);
+#pragma unroll
+      for (int iv = IA; iv < NV; iv++) F[iv] = rho * u * (hp * qL[iv] + hm * qR[iv]);
+    }
+    Fb[0] = 0.0;
+    Fb[1] = u * by - bx * v;
+    Fb[2] = u * bz - bx * w;
+    return;
+  }
+  // HLLD: outer flux plus jumps across the outer and Alfven waves
+  const bool left = us > 0.0;
+  const double rho = left ? rhoL : rhoR, u = left ? uL : uR, v = left ? vL : vR,
+               w = left ? wL : wR, E = left ? EL : ER, pt = left ? ptL : ptR,
+               by = left ? byL : byR, bz = left ? bzL : bzR, vb = left ? vbL : vbR;
+  const double *q = left ? qL : qR;
+  double f[NV], U[NV], fb[3], Ub[3];
+  U[0] = rho; U[1] = rho * u; U[2] = rho * v;
+  if (SD == 3) U[3] = rho * w;
+  U[IE] = E;
+  f[0] = rho * u;
+  f[1] = rho * u * u + pt - bx2;
+  f[2] = rho * u * v - bx * by;
+  if (SD == 3) f[3] = rho * u * w - bx * bz;
+  f[IE] = (E + pt) * u - bx * vb;
+#pragma unroll
+  for (int iv = IA; iv < NV; iv++) {
+    U[iv] = rho * q[iv];
+    f[iv] = rho * u * q[iv];
+  }
+  Ub[0] = bx; Ub[1] = by; Ub[2] = bz;
+  fb[0] = 0.0;
+  fb[1] = u * by - bx * v;
+  fb[2] = u * bz - bx * w;
+  const double s = left ? sL : sR;
+  const bool outer = left ? (sL > 0.0) : (sR < 0.0);
+  if (outer) {
+#pragma unroll
+    for (int iv = 0; iv < NV; iv++) F[iv] = f[iv];
+#pragma unroll
+    for (int iv = 0; iv < 3; iv++) Fb[iv] = fb[iv];
+    return;
+  }
+  const double rs = left ? rsL : rsR, vs = left ? vsL : vsR, ws = left ? wsL : wsR,
+               Es = left ? EsL : EsR, bys = left ? bysL : bysR, bzs = left ? bzsL : bzsR;
+  double Us[NV], Ubs[3];
+  Us[0] = rs; Us[1] = rs * us; Us[2] = rs * vs;
+  if (SD == 3) Us[3] = rs * ws;
+  Us[IE] = Es;
+#pragma unroll
+  for (int iv = IA; iv < NV; iv++) Us[iv] = rs * q[iv];
+  Ubs[0] = bx; Ubs[1] = bys; Ubs[2] = bzs;
+  const bool single = left ? (ssL > 0.0) : (ssR < 0.0);
+  if (single) {
+#pragma unroll
+    for (int iv = 0; iv < NV; iv++) F[iv] = f[iv] + s * (Us[iv] - U[iv]);
+#pragma unroll
+    for (int iv = 0; iv < 3; iv++) Fb[iv] = fb[iv] + s * (Ubs[iv] - Ub[iv]);
+    return;
+  }
+  const double ss = left ? ssL : ssR, Ess = left ? EssL : EssR;
+  double Uss[NV], Ubss[3];
+  Uss[0] = rs; Uss[1] = rs * us; Uss[2] = rs * vss;
+  if (SD == 3) Uss[3] = rs * wss;
+  Uss[IE] = Ess;
+#pragma unroll
+  for (int iv = IA; iv < NV; iv++) Uss[iv] = rs * q[iv];
+  Ubss[0] = bx; Ubss[1] = byss; Ubss[2] = bzss;
+#pragma unroll
+  for (int iv = 0; iv < NV; iv++) F[iv] = f[iv] + s * (Us[iv] - U[iv]) + ss * (Uss[iv] - Us[iv]);
+#pragma unroll
+  for (int iv = 0; iv < 3; iv++) Fb[iv] = fb[iv] + s * (Ubs[iv] - Ub[iv]) + ss * (Ubss[iv] - Ubs[iv]);
+}
+
+}  // namespace phlx

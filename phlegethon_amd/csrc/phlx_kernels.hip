@@ -1,0 +1,598 @@
+// phlx_kernels.hip -- CDNA4 kernels of the explicit FV MHD update.
+//
+// One RK stage is
+//   k_flux<x1>, k_flux<x2>, k_flux<x3>   face states -> Riemann fluxes (+EMF, sign)
+//   k_corner                             CT-contact corner EMFs      (MHD)
+//   k_faceb                              face-B update (FP64)         (MHD)
+//   k_update                             flux divergence + sources + RK combine
+//                                        + cons->prim (+ CFL max at the last stage)
+// Ghost zones are never stored: loads outside the interior go through the
+// per-axis boundary map in phlx_device.hpp.  The stencil is not a
+// contraction, so there is no MFMA here; every kernel is HBM/VALU work.
+#include "phlx_internal.hpp"
+
+namespace phlx {
+
+// ---------------------------------------------------------------- flux
+// One thread per face along DIR.  Loads the four cells f-2..f+1, builds the
+// left state (right edge of cell f-1) and right state (left edge of cell
+// f), rotates into the face frame, solves, rotates back and applies the WB
+// flux correction and the ETOT potential flux.  MHD faces also store the
+// two tangential EMF components and the contact-upwind sign used by CT.
+template <int DIR, int SD, int NAS, bool MHD>
+__global__ void __launch_bounds__(256) k_flux(Grid g) {
+  constexpr int NV = SD + 2 + NAS, IE = SD + 1;
+  if (!g.sc->active) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = blockIdx.z;
+  const int e1 = g.n1 + (DIR == 0), e2 = g.n2 + (DIR == 1);
+  if (i >= e1 || j >= e2) return;
+
+  double q[4][NV], b[4][3];
+#pragma unroll
+  for (int s = 0; s < 4; s++) {
+    const int o = s - 2;
+    load_cell<SD, NV, MHD>(g, i + (DIR == 0 ? o : 0), j + (DIR == 1 ? o : 0),
+                           k + (DIR == 2 ? o : 0), q[s], b[s]);
+  }
+  double qL[NV], qR[NV], bL[3] = {0.0, 0.0, 0.0}, bR[3] = {0.0, 0.0, 0.0};
+  double l, r;
+#pragma unroll
+  for (int v = 0; v < NV; v++) {
+    if (g.recon == REC_LIM2ND) {
+      rec_lim2nd(q[0][v], q[1][v], q[2][v], l, qL[v]);
+      rec_lim2nd(q[1][v], q[2][v], q[3][v], qR[v], r);
+    } else if (v <= IE) {
+      rec_pph(q[0][v], q[1][v], q[2][v], l, qL[v]);
+      rec_pph(q[1][v], q[2][v], q[3][v], qR[v], r);
+    } else {
+      rec_pph_lim(q[0][v], q[1][v], q[2][v], l, qL[v]);
+      rec_pph_lim(q[1][v], q[2][v], q[3][v], qR[v], r);
+    }
+  }
+  const long long f = fix(g, DIR, i, j, k);
+  if (MHD) {
+#pragma unroll
+    for (int c = 0; c < SD; c++) {
+      if (c == DIR) continue;
+      if (g.recon == REC_LIM2ND) {
+        rec_lim2nd(b[0][c], b[1][c], b[2][c], l, bL[c]);
+        rec_lim2nd(b[1][c], b[2][c], b[3][c], bR[c], r);
+      } else {
+        rec_pph(b[0][c], b[1][c], b[2][c], l, bL[c]);
+        rec_pph(b[1][c], b[2][c], b[3][c], bR[c], r);
+      }
+    }
+    const double bn = g.bf[DIR][f];
+    bL[DIR] = bn;
+    bR[DIR] = bn;
+  }
+  // into the face frame: x2 (v1,v2)->(v2,-v1); x3 (v1,v2,v3)->(v3,v1,v2)
+  if (DIR == 1) {
+    double t = qL[1]; qL[1] = qL[2]; qL[2] = -t;
+    t = qR[1]; qR[1] = qR[2]; qR[2] = -t;
+    if (MHD) {
+      t = bL[0]; bL[0] = bL[1]; bL[1] = -t;
+      t = bR[0]; bR[0] = bR[1]; bR[1] = -t;
+    }
+  } else if (DIR == 2) {
+    double t = qL[1]; qL[1] = qL[3]; qL[3] = qL[2]; qL[2] = t;
+    t = qR[1]; qR[1] = qR[3]; qR[3] = qR[2]; qR[2] = t;
+    if (MHD) {
+      t = bL[0]; bL[0] = bL[2]; bL[2] = bL[1]; bL[1] = t;
+      t = bR[0]; bR[0] = bR[2]; bR[2] = bR[1]; bR[1] = t;
+    }
+  }
+  const int jy = j;  // x2 index of this face (x2-face index for x2, cell index otherwise)
+  if (g.wb) {
+    const double req = (DIR == 1) ? g.eq_rho_f[jy] : g.eq_rho_cc[jy];
+    const double peq = (DIR == 1) ? g.eq_p_f[jy] : g.eq_p_cc[jy];
+    qL[0] = qL[0] + req;
+    qR[0] = qR[0] + req;
+    qL[IE] = qL[IE] + peq;
+    qR[IE] = qR[IE] + peq;
+  }
+  double F[NV], Fb[3] = {0.0, 0.0, 0.0};
+  const double gdum[2] = {0.0, 0.0};
+  if (MHD)
+    riemann_mhd<SD, NAS>(g, qL, qR, bL, bR, gdum, gdum, F, Fb);
+  else
+    riemann_hydro<SD, NAS>(g, qL, qR, gdum, gdum, F);
+  // back to the grid frame
+  if (DIR == 1) {
+    double t = F[1]; F[1] = -F[2]; F[2] = t;
+    if (MHD) { t = Fb[0]; Fb[0] = -Fb[1]; Fb[1] = t; }
+  } else if (DIR == 2) {
+    double t = F[1]; F[1] = F[2]; F[2] = F[3]; F[3] = t;
+    if (MHD) { t = Fb[0]; Fb[0] = Fb[1]; Fb[1] = Fb[2]; Fb[2] = t; }
+  }
+  const long long nf = g.nf[DIR];
+  if (MHD) {
+    // E = -v x B from the induction fluxes (x1 :8926-8932, x2 :10113-10120,
+    // x3 :11255-11262); slots: x1 (E2,E3), x2 (E1,E3), x3 (E1,E2)
+    double e0, e1v;
+    if (DIR == 0) { e0 = Fb[2]; e1v = -Fb[1]; }
+    else if (DIR == 1) { e0 = -Fb[2]; e1v = Fb[0]; }
+    else { e0 = Fb[1]; e1v = -Fb[0]; }
+    g.emf[DIR][f] = e0;
+    g.emf[DIR][nf + f] = e1v;
+    const double den = 0.5 * (qL[0] + qR[0]) * g.sc->smax;
+    const double mach = F[0] / den;
+    signed char s = 0;
+    if (mach > g.mach_ct) s = 1;
+    if (mach < -g.mach_ct) s = -1;
+    g.sgn[DIR][f] = s;
+  }
+  if (g.wb) {
+    const double peq = (DIR == 1) ? g.eq_p_f[jy] : g.eq_p_cc[jy];
+    F[1 + DIR] = F[1 + DIR] - peq;
+  }
+  if (g.etot) {
+    const double phi = (DIR == 1) ? g.phi_f[jy] : g.phi_cc[jy];
+    F[IE] = F[IE] + F[0] * phi;
+  }
+#pragma unroll
+  for (int v = 0; v < NV; v++) g.F[DIR][v * nf + f] = F[v];
+}
+
+// ------------------------------------------------------------------ CT
+// Face-centred EMF component `comp` (0..2 = E1..E3) of the face array of
+// direction fd at a possibly-ghost transverse position; ghosts follow the
+// EMF halo copy and EMF boundary conditions (source/source.F90:11370-11964).
+__device__ __forceinline__ void face_emf(const Grid &g, int fd, int comp, int i, int j,
+                                         int k, double &e, double &s) {
+  int ii[3] = {i, j, k};
+  const int n[3] = {g.n1, g.n2, g.n3};
+  double fac = 1.0;
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    if (a == fd || a >= g.sd) continue;
+    int c = ii[a];
+    if (c < 0 || c >= n[a]) {
+      const int side = c < 0 ? 0 : 1;
+      if (g.bc[2 * a + side] == BC_PERIODIC) {
+        ii[a] = c < 0 ? c + n[a] : c - n[a];
+      } else {
+        ii[a] = c < 0 ? 0 : n[a] - 1;
+        if (g.bc[2 * a + side] == BC_REFLECTIVE && comp != a) fac = -fac;
+      }
+    }
+  }
+  const long long f = fix(g, fd, ii[0], ii[1], ii[2]);
+  // slot of component comp within the face array
+  int slot;
+  if (fd == 0) slot = (comp == 1) ? 0 : 1;
+  else if (fd == 1) slot = (comp == 0) ? 0 : 1;
+  else slot = (comp == 0) ? 0 : 1;
+  e = fac * g.emf[fd][slot * g.nf[fd] + f];
+  s = (double)g.sgn[fd][f];
+}
+
+// -(v x B)_comp of a (possibly ghost) cell from v and b_cc
+template <int SD>
+__device__ __forceinline__ double cell_emf(const Grid &g, int comp, int i, int j, int k) {
+  constexpr int NV = SD + 2;  // rho, velocities, p (only the velocities are used)
+  double q[NV], b[3] = {0.0, 0.0, 0.0};
+  load_cell<SD, NV, true>(g, i, j, k, q, b);
+  const double v1 = q[1], v2 = q[2], v3 = (SD == 3) ? q[3] : 0.0;
+  if (comp == 2) return -v1 * b[1] + v2 * b[0];
+  if (comp == 0) return v3 * b[1] - v2 * b[2];
+  return v1 * b[2] - v3 * b[0];
+}
+
+// Gardiner-Stone CT-contact corner EMF (source/source.F90:11966-12027)
+__device__ __forceinline__ double corner_emf(double eS, double eW, double eN, double eE,
+                                             double eSW, double eNW, double eNE,
+                                             double eSE, double sS, double sN, double sW,
+                                             double sE) {
+  const double dS = 0.5 * (1.0 + sS) * (eW - eSW) + 0.5 * (1.0 - sS) * (eE - eSE);
+  const double dN = 0.5 * (1.0 + sN) * (eNW - eW) + 0.5 * (1.0 - sN) * (eNE - eE);
+  const double dW = 0.5 * (1.0 + sW) * (eS - eSW) + 0.5 * (1.0 - sW) * (eN - eNW);
+  const double dE = 0.5 * (1.0 + sE) * (eSE - eS) + 0.5 * (1.0 - sE) * (eNE - eN);
+  return 0.25 * (eS + eW + eN + eE + dS - dN + dW - dE);
+}
+
+// One thread per grid node (i,j,k) in [0,n1]x[0,n2]x[0,n3]: the three edge
+// EMFs whose lower end is that node.
+template <int SD>
+__global__ void __launch_bounds__(256) k_corner(Grid g) {
+  if (!g.sc->active) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = blockIdx.z;
+  if (i > g.n1 || j > g.n2) return;
+  double eN, eE, eS, eW, sS, sN, sW, sE, t;
+  if (k < g.n3) {  // x3-edges (emfx3_cor)
+    face_emf(g, 0, 2, i, j, k, eN, sN);
+    face_emf(g, 1, 2, i, j, k, eE, sE);
+    face_emf(g, 0, 2, i, j - 1, k, eS, sS);
+    face_emf(g, 1, 2, i - 1, j, k, eW, sW);
+    const double eSW = cell_emf<SD>(g, 2, i - 1, j - 1, k);
+    const double eNW = cell_emf<SD>(g, 2, i - 1, j, k);
+    const double eNE = cell_emf<SD>(g, 2, i, j, k);
+    const double eSE = cell_emf<SD>(g, 2, i, j - 1, k);
+    g.cor[2][kix(g, 2, i, j, k)] = corner_emf(eS, eW, eN, eE, eSW, eNW, eNE, eSE, sS, sN, sW, sE);
+  }
+  if (SD == 3) {
+    if (i < g.n1) {  // x1-edges (emfx1_cor, source/source.F90:12050-12107)
+      face_emf(g, 1, 0, i, j, k, eN, sN);
+      face_emf(g, 2, 0, i, j, k, eE, sE);
+      face_emf(g, 1, 0, i, j, k - 1, eS, sS);
+      face_emf(g, 2, 0, i, j - 1, k, eW, sW);
+      const double eSW = cell_emf<SD>(g, 0, i, j - 1, k - 1);
+      const double eNW = cell_emf<SD>(g, 0, i, j - 1, k);
+      const double eNE = cell_emf<SD>(g, 0, i, j, k);
+      const double eSE = cell_emf<SD>(g, 0, i, j, k - 1);
+      g.cor[0][kix(g, 0, i, j, k)] = corner_emf(eS, eW, eN, eE, eSW, eNW, eNE, eSE, sS, sN, sW, sE);
+    }
+    if (j < g.n2) {  // x2-edges (emfx2_cor, source/source.F90:12128-12185)
+      face_emf(g, 2, 1, i, j, k, eN, sN);
+      face_emf(g, 0, 1, i, j, k, eE, sE);
+      face_emf(g, 2, 1, i - 1, j, k, eS, sS);
+      face_emf(g, 0, 1, i, j, k - 1, eW, sW);
+      const double eSW = cell_emf<SD>(g, 1, i - 1, j, k - 1);
+      const double eNW = cell_emf<SD>(g, 1, i, j, k - 1);
+      const double eNE = cell_emf<SD>(g, 1, i, j, k);
+      const double eSE = cell_emf<SD>(g, 1, i - 1, j, k);
+      g.cor[1][kix(g, 1, i, j, k)] = corner_emf(eS, eW, eN, eE, eSW, eNW, eNE, eSE, sS, sN, sW, sE);
+    }
+  }
+  (void)t;
+}
+
+// Face-B update b = a1*b0 + a2*b + a3*dt*curl(E) in FP64 (source/source.F90
+// :12208-12435); at the first stage the step-start field is kept in b0.
+template <int SD>
+__global__ void __launch_bounds__(256) k_faceb(Grid g, int first, double a1, double a2,
+                                               double c3) {
+  if (!g.sc->active) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = blockIdx.z;
+  if (i > g.n1 || j > g.n2) return;
+  const double a3 = c3 * g.sc->dt;
+  const double i1 = g.inv_dx[0], i2 = g.inv_dx[1], i3 = g.inv_dx[2];
+  const double *C1 = g.cor[0], *C2 = g.cor[1], *C3 = g.cor[2];
+  if (j < g.n2 && k < g.n3) {
+    double t;
+    if (SD == 3)
+      t = -(C2[kix(g, 1, i, j, k + 1)] - C2[kix(g, 1, i, j, k)]) * i3 +
+          (C3[kix(g, 2, i, j + 1, k)] - C3[kix(g, 2, i, j, k)]) * i2;
+    else
+      t = (C3[kix(g, 2, i, j + 1, k)] - C3[kix(g, 2, i, j, k)]) * i2;
+    const long long f = fix(g, 0, i, j, k);
+    const double b = g.bf[0][f];
+    double b0;
+    if (first) { b0 = b; g.b0f[0][f] = b; } else { b0 = g.b0f[0][f]; }
+    g.bf[0][f] = a1 * b0 + a2 * b + a3 * t;
+  }
+  if (i < g.n1 && k < g.n3) {
+    double t;
+    if (SD == 3)
+      t = (C1[kix(g, 0, i, j, k + 1)] - C1[kix(g, 0, i, j, k)]) * i3 +
+          (C3[kix(g, 2, i, j, k)] - C3[kix(g, 2, i + 1, j, k)]) * i1;
+    else
+      t = (C3[kix(g, 2, i, j, k)] - C3[kix(g, 2, i + 1, j, k)]) * i1;
+    const long long f = fix(g, 1, i, j, k);
+    const double b = g.bf[1][f];
+    double b0;
+    if (first) { b0 = b; g.b0f[1][f] = b; } else { b0 = g.b0f[1][f]; }
+    g.bf[1][f] = a1 * b0 + a2 * b + a3 * t;
+  }
+  if (SD == 3 && i < g.n1 && j < g.n2) {
+    const double t = (C2[kix(g, 1, i + 1, j, k)] - C2[kix(g, 1, i, j, k)]) * i1 -
+                     (C1[kix(g, 0, i, j + 1, k)] - C1[kix(g, 0, i, j, k)]) * i2;
+    const long long f = fix(g, 2, i, j, k);
+    const double b = g.bf[2][f];
+    double b0;
+    if (first) { b0 = b; g.b0f[2][f] = b; } else { b0 = g.b0f[2][f]; }
+    g.bf[2][f] = a1 * b0 + a2 * b + a3 * t;
+  }
+}
+
+// ----------------------------------------------------------- wave max
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+__device__ __forceinline__ void block_max_to(double s, unsigned long long *dst) {
+  __shared__ double part[16];
+  s = wave_max(s);
+  const int lane = threadIdx.x & 63, wid = (threadIdx.y * blockDim.x + threadIdx.x) >> 6;
+  const int nw = (blockDim.x * blockDim.y + 63) >> 6;
+  if (lane == 0) part[wid] = s;
+  __syncthreads();
+  if (threadIdx.x == 0 && threadIdx.y == 0) {
+    double m = part[0];
+    for (int w = 1; w < nw; w++) m = fmax(m, part[w]);
+    // positive doubles order like their bit patterns
+    atomicMax(dst, (unsigned long long)__double_as_longlong(m));
+  }
+}
+
+// Signal speed of one cell, summed over directions (source/source.F90
+// :13891-13944): (|v_d| + c_f) / dx_d with c_f^2 = c^2 + B^2/rho.
+template <int SD, bool MHD>
+__device__ __forceinline__ double cell_speed(const Grid &g, const double *p, const double *b) {
+  double c = sqrt(g.gm * p[SD + 1] / p[0]);
+  if (MHD) {
+    const double bz = (SD == 3) ? b[2] : 0.0;
+    const double ca2 = b[0] * b[0] + b[1] * b[1] + bz * bz;
+    c = sqrt(c * c + ca2 / p[0]);
+  }
+  const double s1 = (fabs(p[1]) + c) * g.inv_dx[0];
+  const double s2 = (fabs(p[2]) + c) * g.inv_dx[1];
+  if (SD == 3) {
+    const double s3 = (fabs(p[3]) + c) * g.inv_dx[2];
+    return s3 + s1 + s2;
+  }
+  return s1 + s2;
+}
+
+// ---------------------------------------------------------------- update
+// One thread per cell: state0 rebuild (first stage), flux divergence,
+// gravity / heating sources, RK combine, b_cc from the updated faces,
+// cons->prim.  At the last stage the cell's CFL signal speed joins the
+// device-wide max.
+template <int SD, int NAS, bool MHD>
+__global__ void __launch_bounds__(256) k_update(Grid g, int first, int last, double a1,
+                                                double a2, double c3) {
+  constexpr int NV = SD + 2 + NAS, IE = SD + 1, IA = SD + 2;
+  if (!g.sc->active) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = blockIdx.z;
+  const bool in = (i < g.n1) && (j < g.n2);
+  double smax_local = 0.0;
+  if (in) {
+    const long long N = g.N, c = cix(g, i, j, k);
+    double S[NV], S0[NV], R[NV], p[NV];
+#pragma unroll
+    for (int v = 0; v < NV; v++) p[v] = g.P[v * N + c];
+    if (first) {
+      // source/source.F90:7057-7117
+      const double rho = p[0], v1 = p[1], v2 = p[2], v3 = (SD == 3) ? p[3] : 0.0;
+      S0[0] = rho;
+      S0[1] = rho * v1;
+      S0[2] = rho * v2;
+      if (SD == 3) S0[3] = rho * v3;
+      double E = g.eint[c] + 0.5 * rho * (v1 * v1 + v2 * v2 + v3 * v3);
+      if (MHD) {
+        const double b1 = g.bcc[c], b2 = g.bcc[N + c], b3 = (SD == 3) ? g.bcc[2 * N + c] : 0.0;
+        E = E + 0.5 * (b1 * b1 + b2 * b2 + b3 * b3);
+      }
+      if (g.etot) E = E + rho * g.phi_cc[j];
+      S0[IE] = E;
+#pragma unroll
+      for (int v = IA; v < NV; v++) S0[v] = rho * p[v];
+#pragma unroll
+      for (int v = 0; v < NV; v++) {
+        S[v] = S0[v];
+        g.U0[v * N + c] = S0[v];
+      }
+    } else {
+#pragma unroll
+      for (int v = 0; v < NV; v++) {
+        S0[v] = g.U0[v * N + c];
+        S[v] = g.U[v * N + c];
+      }
+    }
+    // flux divergence: x1 overwrites, x2/x3 accumulate (source/source.F90
+    // :9036-9102, :10218-10298, :11322-11359)
+    {
+      const long long nf = g.nf[0];
+      const long long fm = fix(g, 0, i, j, k), fp = fm + 1;
+#pragma unroll
+      for (int v = 0; v < NV; v++) R[v] = (g.F[0][v * nf + fp] - g.F[0][v * nf + fm]) * g.inv_dx[0];
+    }
+    {
+      const long long nf = g.nf[1];
+      const long long fm = fix(g, 1, i, j, k), fp = fm + g.n1;
+#pragma unroll
+      for (int v = 0; v < NV; v++) R[v] = R[v] + (g.F[1][v * nf + fp] - g.F[1][v * nf + fm]) * g.inv_dx[1];
+    }
+    if (SD == 3) {
+      const long long nf = g.nf[2];
+      const long long fm = fix(g, 2, i, j, k), fp = fm + (long long)g.n1 * g.n2;
+#pragma unroll
+      for (int v = 0; v < NV; v++) R[v] = R[v] + (g.F[2][v * nf + fp] - g.F[2][v * nf + fm]) * g.inv_dx[2];
+    }
+    if (g.grav) {
+      // source/source.F90:12606-12644 on the WB deviation rho'
+      const double rho = g.wb ? (p[0] - g.eq_rho_cc[j]) : p[0];
+      const double gy = g.gravy[j];
+      R[2] = R[2] - rho * gy;
+      if (!g.etot) R[IE] = R[IE] - S[2] * gy;
+    }
+    if (g.edot) R[IE] = R[IE] - g.edot_cc[j];
+    const double a3 = c3 * g.sc->dt;
+#pragma unroll
+    for (int v = 0; v < NV; v++) S[v] = a1 * S0[v] + a2 * S[v] + a3 * R[v];
+    double b[3] = {0.0, 0.0, 0.0};
+    if (MHD) {
+      // source/source.F90:12439-12489
+      b[0] = 0.5 * (g.bf[0][fix(g, 0, i + 1, j, k)] + g.bf[0][fix(g, 0, i, j, k)]);
+      b[1] = 0.5 * (g.bf[1][fix(g, 1, i, j + 1, k)] + g.bf[1][fix(g, 1, i, j, k)]);
+      if (SD == 3) b[2] = 0.5 * (g.bf[2][fix(g, 2, i, j, k + 1)] + g.bf[2][fix(g, 2, i, j, k)]);
+#pragma unroll
+      for (int v = 0; v < SD; v++) g.bcc[v * N + c] = b[v];
+    }
+    // cons -> prim (source/source.F90:13533-13686), ideal-gas EoS
+    const double rho = S[0], m1 = S[1], m2 = S[2], m3 = (SD == 3) ? S[3] : 0.0;
+    const double ir = 1.0 / rho;
+    double eint = S[IE] - 0.5 * (m1 * m1 + m2 * m2 + m3 * m3) * ir;
+    if (MHD) eint = eint - 0.5 * (b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+    if (g.etot) eint = eint - rho * g.phi_cc[j];
+    p[0] = rho;
+    p[1] = m1 * ir;
+    p[2] = m2 * ir;
+    if (SD == 3) p[3] = m3 * ir;
+    p[IE] = g.gmm1 * eint;
+#pragma unroll
+    for (int v = IA; v < NV; v++) p[v] = S[v] * ir;
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+      g.P[v * N + c] = p[v];
+      g.U[v * N + c] = S[v];
+    }
+    g.eint[c] = eint;
+    if (last) smax_local = cell_speed<SD, MHD>(g, p, b);
+  }
+  if (last) block_max_to(smax_local, &g.sc->smax_bits);
+}
+
+// Standalone compute_hyperbolic_dt pass over the current prim / b_cc.
+template <int SD, bool MHD>
+__global__ void __launch_bounds__(256) k_cfl(Grid g) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = blockIdx.z;
+  double s = 0.0;
+  if (i < g.n1 && j < g.n2) {
+    const long long N = g.N, c = cix(g, i, j, k);
+    double p[SD + 2], b[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int v = 0; v < SD + 2; v++) p[v] = g.P[v * N + c];
+    if (MHD)
+#pragma unroll
+      for (int v = 0; v < SD; v++) b[v] = g.bcc[v * N + c];
+    s = cell_speed<SD, MHD>(g, p, b);
+  }
+  block_max_to(s, &g.sc->smax_bits);
+}
+
+// Step-0 initialisation: b_cc from faces and the ideal-gas eint
+// (source/source.F90:6012-6062, :6373-6376).
+template <int SD, bool MHD>
+__global__ void __launch_bounds__(256) k_init(Grid g) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = blockIdx.z;
+  if (i >= g.n1 || j >= g.n2) return;
+  const long long N = g.N, c = cix(g, i, j, k);
+  g.eint[c] = g.P[(SD + 1) * N + c] * g.igmm1;
+  if (MHD) {
+    g.bcc[c] = 0.5 * (g.bf[0][fix(g, 0, i + 1, j, k)] + g.bf[0][fix(g, 0, i, j, k)]);
+    g.bcc[N + c] = 0.5 * (g.bf[1][fix(g, 1, i, j + 1, k)] + g.bf[1][fix(g, 1, i, j, k)]);
+    if (SD == 3) g.bcc[2 * N + c] = 0.5 * (g.bf[2][fix(g, 2, i, j, k + 1)] + g.bf[2][fix(g, 2, i, j, k)]);
+  }
+}
+
+// max |div B| over cells, reduced like the CFL max.
+template <int SD>
+__global__ void __launch_bounds__(256) k_divb(Grid g, unsigned long long *out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = blockIdx.z;
+  double s = 0.0;
+  if (i < g.n1 && j < g.n2) {
+    double d = (g.bf[0][fix(g, 0, i + 1, j, k)] - g.bf[0][fix(g, 0, i, j, k)]) * g.inv_dx[0] +
+               (g.bf[1][fix(g, 1, i, j + 1, k)] - g.bf[1][fix(g, 1, i, j, k)]) * g.inv_dx[1];
+    if (SD == 3) d = d + (g.bf[2][fix(g, 2, i, j, k + 1)] - g.bf[2][fix(g, 2, i, j, k)]) * g.inv_dx[2];
+    s = fabs(d);
+  }
+  block_max_to(s, out);
+}
+
+// Step bracket kernels (one thread): time_loop's dt clip before the step
+// and time/dt/step update after it (source/source.F90:6662-6720).
+__global__ void k_step_begin(Scalars *sc) {
+  // the loop runs while time < tmax (source/source.F90:6525)
+  sc->active = (sc->time < sc->tmax) ? 1 : 0;
+  if (!sc->active) return;
+  if (sc->time + sc->dt > sc->tmax) sc->dt = sc->tmax - sc->time;
+  sc->smax_bits = 0ull;
+}
+__global__ void k_step_end(Scalars *sc, int advance) {
+  if (advance && !sc->active) return;
+  if (advance) sc->time = sc->time + sc->dt;
+  sc->smax = __longlong_as_double((long long)sc->smax_bits);
+  sc->dt = sc->cfl / sc->smax;
+  if (advance) sc->step = sc->step + 1;
+  sc->active = 1;
+}
+__global__ void k_zero_bits(unsigned long long *p) { *p = 0ull; }
+
+// -------------------------------------------------------------- launchers
+static dim3 grid_for(int e1, int e2, int e3, dim3 blk) {
+  return dim3((e1 + blk.x - 1) / blk.x, (e2 + blk.y - 1) / blk.y, e3);
+}
+
+template <int SD, int NAS, bool MHD>
+static void launch_flux(const Grid &g, int dir, hipStream_t st) {
+  const dim3 blk(64, 4, 1);
+  if (dir == 0)
+    k_flux<0, SD, NAS, MHD><<<grid_for(g.n1 + 1, g.n2, g.n3, blk), blk, 0, st>>>(g);
+  else if (dir == 1)
+    k_flux<1, SD, NAS, MHD><<<grid_for(g.n1, g.n2 + 1, g.n3, blk), blk, 0, st>>>(g);
+  else if (SD == 3)
+    k_flux<2, SD, NAS, MHD><<<grid_for(g.n1, g.n2, g.n3 + 1, blk), blk, 0, st>>>(g);
+}
+
+template <int SD, int NAS, bool MHD>
+struct Kern {
+  static void flux(const Grid &g, int dir, hipStream_t st) { launch_flux<SD, NAS, MHD>(g, dir, st); }
+  static void corner(const Grid &g, hipStream_t st) {
+    const dim3 blk(64, 4, 1);
+    k_corner<SD><<<grid_for(g.n1 + 1, g.n2 + 1, SD == 3 ? g.n3 + 1 : 1, blk), blk, 0, st>>>(g);
+  }
+  static void faceb(const Grid &g, int first, double a1, double a2, double c3, hipStream_t st) {
+    const dim3 blk(64, 4, 1);
+    k_faceb<SD><<<grid_for(g.n1 + 1, g.n2 + 1, SD == 3 ? g.n3 + 1 : 1, blk), blk, 0, st>>>(
+        g, first, a1, a2, c3);
+  }
+  static void update(const Grid &g, int first, int last, double a1, double a2, double c3,
+                     hipStream_t st) {
+    const dim3 blk(64, 4, 1);
+    k_update<SD, NAS, MHD><<<grid_for(g.n1, g.n2, g.n3, blk), blk, 0, st>>>(g, first, last, a1,
+                                                                           a2, c3);
+  }
+  static void cfl(const Grid &g, hipStream_t st) {
+    const dim3 blk(64, 4, 1);
+    k_cfl<SD, MHD><<<grid_for(g.n1, g.n2, g.n3, blk), blk, 0, st>>>(g);
+  }
+  static void init(const Grid &g, hipStream_t st) {
+    const dim3 blk(64, 4, 1);
+    k_init<SD, MHD><<<grid_for(g.n1, g.n2, g.n3, blk), blk, 0, st>>>(g);
+  }
+  static void divb(const Grid &g, unsigned long long *out, hipStream_t st) {
+    const dim3 blk(64, 4, 1);
+    k_divb<SD><<<grid_for(g.n1, g.n2, g.n3, blk), blk, 0, st>>>(g, out);
+  }
+};
+
+template <int SD, int NAS, bool MHD>
+static KernelTable make_table() {
+  KernelTable t;
+  t.flux = &Kern<SD, NAS, MHD>::flux;
+  t.corner = &Kern<SD, NAS, MHD>::corner;
+  t.faceb = &Kern<SD, NAS, MHD>::faceb;
+  t.update = &Kern<SD, NAS, MHD>::update;
+  t.cfl = &Kern<SD, NAS, MHD>::cfl;
+  t.init = &Kern<SD, NAS, MHD>::init;
+  t.divb = &Kern<SD, NAS, MHD>::divb;
+  return t;
+}
+
+bool select_kernels(int sd, int nas, bool mhd, KernelTable &t) {
+  if (sd == 2 && nas == 0 && !mhd) t = make_table<2, 0, false>();
+  else if (sd == 2 && nas == 0 && mhd) t = make_table<2, 0, true>();
+  else if (sd == 3 && nas == 0 && !mhd) t = make_table<3, 0, false>();
+  else if (sd == 3 && nas == 0 && mhd) t = make_table<3, 0, true>();
+  else if (sd == 2 && nas == 2 && !mhd) t = make_table<2, 2, false>();
+  else if (sd == 3 && nas == 2 && !mhd) t = make_table<3, 2, false>();
+  else if (sd == 3 && nas == 2 && mhd) t = make_table<3, 2, true>();
+  else return false;
+  return true;
+}
+
+void launch_step_begin(Scalars *sc, hipStream_t st) { k_step_begin<<<1, 1, 0, st>>>(sc); }
+void launch_step_end(Scalars *sc, int advance, hipStream_t st) {
+  k_step_end<<<1, 1, 0, st>>>(sc, advance);
+}
+void launch_zero_bits(unsigned long long *p, hipStream_t st) { k_zero_bits<<<1, 1, 0, st>>>(p); }
+
+}  // namespace phlx

@@ -1,0 +1,298 @@
+"""Initial-condition generators for the benchmark configurations and the
+validation problems.
+
+All random perturbations come from a counter-based generator
+(splitmix64 of seed + zone index), so a CPU run and a GPU run of the same
+problem start from identical bytes.  Geometry follows the reference's
+Cartesian-uniform grid (source/source.F90:16271-16371): cell centres at
+x_l + (i + 1/2) dx and nodes/faces at x_l + i dx (0-based i).
+
+Each generator returns a ``Problem`` carrying the configuration, the
+interior primitive state, the face-centred magnetic field (MHD) and the
+plane-parallel static profiles (equilibrium, gravity, potential, heating).
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional
+
+import numpy as np
+
+from .config import (BC_PERIODIC, BC_REFLECTIVE, BC_OUTFLOW, HLLC_FLUX, LHLLC_FLUX,
+                     HLLD_FLUX, LHLLD_FLUX, LIM2ND_REC, PPH_REC, PhlxConfig)
+
+SEED = 20261202
+_M64 = np.uint64(0xFFFFFFFFFFFFFFFF)
+
+
+def splitmix64(x: np.ndarray) -> np.ndarray:
+    """splitmix64 finaliser on uint64 counters (Steele, Lea & Flood 2014)."""
+    with np.errstate(over="ignore"):
+        z = (x + np.uint64(0x9E3779B97F4A7C15)) & _M64
+        z = ((z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)) & _M64
+        z = ((z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)) & _M64
+        return z ^ (z >> np.uint64(31))
+
+
+def uniform01(seed: int, stream: int, n: int) -> np.ndarray:
+    """n doubles in [0,1) from counters (seed, stream, zone index)."""
+    base = np.uint64((seed * 0x100000001B3 + stream * 0x9E3779B1) & 0xFFFFFFFFFFFFFFFF)
+    with np.errstate(over="ignore"):
+        ctr = (np.arange(n, dtype=np.uint64) + base) & _M64
+    bits = splitmix64(ctr)
+    return (bits >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+
+
+@dataclass
+class Problem:
+    name: str
+    cfg: PhlxConfig
+    prim: np.ndarray
+    b_faces: Optional[List[np.ndarray]] = None
+    profiles: Dict[str, np.ndarray] = field(default_factory=dict)
+
+    def copy(self) -> "Problem":
+        return Problem(self.name, self.cfg, self.prim.copy(),
+                       None if self.b_faces is None else
+                       [None if b is None else b.copy() for b in self.b_faces],
+                       {k: v.copy() for k, v in self.profiles.items()})
+
+
+def centres(cfg: PhlxConfig, d: int) -> np.ndarray:
+    lo = (cfg.x1l, cfg.x2l, cfg.x3l)[d]
+    n = (cfg.nx1, cfg.nx2, cfg.nx3)[d]
+    return lo + (np.arange(n, dtype=np.float64) + 0.5) * cfg.dx[d]
+
+
+def nodes(cfg: PhlxConfig, d: int) -> np.ndarray:
+    lo = (cfg.x1l, cfg.x2l, cfg.x3l)[d]
+    n = (cfg.nx1, cfg.nx2, cfg.nx3)[d]
+    return lo + np.arange(n + 1, dtype=np.float64) * cfg.dx[d]
+
+
+def mesh(cfg: PhlxConfig):
+    """Cell-centre coordinate arrays shaped (nz, ny, nx)."""
+    x = centres(cfg, 0)
+    y = centres(cfg, 1)
+    z = centres(cfg, 2) if cfg.sdims == 3 else np.array([0.0])
+    Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
+    return X, Y, Z
+
+
+# --------------------------------------------------------------------- C1
+def hot_bubble(n: int = 256, recon: int = LIM2ND_REC, flux: int = LHLLC_FLUX,
+               etot: bool = False, seed: int = SEED, amp: float = 1e-3,
+               noise: float = 1e-6, rk_stages: int = 2) -> Problem:
+    """2D hydrostatic polytrope (index 1.5, rho0 = p0 = 1, g = -1 in x2) with a
+    Gaussian entropy bump (amplitude amp, radius 0.1 at (0.5, 0.3)) and
+    seeded uniform velocity noise; periodic x1, reflecting x2, WB gravity.
+    Pattern of tests/hotbubble/app.F90:27-180 on the unit square."""
+    cfg = PhlxConfig(sdims=2, nx1=n, nx2=n, nx3=1, recon=recon, flux=flux,
+                     rk_stages=rk_stages, cfl=0.8, mach_cutoff=1e-4,
+                     bc=(BC_PERIODIC, BC_PERIODIC, BC_REFLECTIVE, BC_REFLECTIVE, 0, 0),
+                     use_wb=True, use_gravity=True, evolve_etot=etot).validate()
+    gm = cfg.gamma
+    yc = centres(cfg, 1)
+    yf = nodes(cfg, 1)
+    rho_cc = (1.0 - 0.4 * yc) ** 1.5
+    p_cc = rho_cc ** gm
+    rho_f = (1.0 - 0.4 * yf) ** 1.5
+    p_f = rho_f ** gm
+    X, Y, _ = mesh(cfg)
+    r2 = (X - 0.5) ** 2 + (Y - 0.3) ** 2
+    A = 1.0 + amp * np.exp(-r2 / 0.01)
+    p = np.broadcast_to(p_cc[None, :, None], X.shape).copy()
+    rho = (p / A) ** (1.0 / gm)
+    N = X.size
+    prim = np.empty((cfg.nvars,) + cfg.shape)
+    prim[0] = rho
+    prim[1] = (2.0 * uniform01(seed, 1, N) - 1.0).reshape(X.shape) * noise
+    prim[2] = (2.0 * uniform01(seed, 2, N) - 1.0).reshape(X.shape) * noise
+    prim[3] = p
+    prof = dict(eq_rho_cc=rho_cc, eq_p_cc=p_cc, eq_rho_f=rho_f, eq_p_f=p_f,
+                grav_cc=np.full(cfg.nx2, -1.0), phi_cc=yc.copy(), phi_f=yf.copy())
+    return Problem("hot_bubble", cfg, prim, None, prof)
+
+
+def hydrostatic_rest(n: int = 64, flux: int = LHLLC_FLUX, recon: int = LIM2ND_REC,
+                     etot: bool = False, sdims: int = 2) -> Problem:
+    """The C1 polytrope exactly at rest: the well-balanced scheme must keep it
+    bitwise (face p_eq cancels in the momentum flux, rho' = 0)."""
+    pb = hot_bubble(n=n, recon=recon, flux=flux, etot=etot, amp=0.0, noise=0.0)
+    if sdims == 3:
+        cfg = pb.cfg.with_(sdims=3, nx3=max(4, n // 8),
+                           bc=(BC_PERIODIC, BC_PERIODIC, BC_REFLECTIVE, BC_REFLECTIVE,
+                               BC_PERIODIC, BC_PERIODIC)).validate()
+        prim = np.empty((cfg.nvars,) + cfg.shape)
+        prim[0] = pb.prim[0][0][None]
+        prim[1:4] = 0.0
+        prim[4] = pb.prim[3][0][None]
+        return Problem("hydrostatic_rest_3d", cfg, prim, None, pb.profiles)
+    return Problem("hydrostatic_rest", pb.cfg, pb.prim, None, pb.profiles)
+
+
+# --------------------------------------------------------------------- C2
+def orszag_tang_3d(n: int = 256, flux: int = LHLLD_FLUX, recon: int = LIM2ND_REC,
+                   seed: int = SEED, noise: float = 1e-4, rk_stages: int = 2,
+                   shape: Optional[tuple] = None) -> Problem:
+    """3D Orszag-Tang on the periodic unit cube: rho = 25/(36 pi)(1 + noise U),
+    p = 5/(12 pi), v = (-sin 2pi y, sin 2pi x, 0.1 sin 2pi z); face B from the
+    nodal vector potential A_z = B0/(2pi) cos 2pi y + B0/(4pi) cos 4pi x,
+    B0 = 1/sqrt(4 pi) (tests/orszag-tang/app.F90:13-79), so div B = 0 to
+    round-off."""
+    nx, ny, nz = shape if shape is not None else (n, n, n)
+    cfg = PhlxConfig(sdims=3, nx1=nx, nx2=ny, nx3=nz, recon=recon, flux=flux,
+                     rk_stages=rk_stages, cfl=0.8, mach_cutoff=1e-5, mach_ct=1e-6,
+                     bc=(0, 0, 0, 0, 0, 0)).validate()
+    pi = math.pi
+    B0 = 1.0 / math.sqrt(4.0 * pi)
+    X, Y, Z = mesh(cfg)
+    prim = np.empty((cfg.nvars,) + cfg.shape)
+    u = uniform01(seed, 3, X.size).reshape(X.shape)
+    prim[0] = 25.0 / (36.0 * pi) * (1.0 + noise * (2.0 * u - 1.0))
+    prim[1] = -np.sin(2.0 * pi * Y)
+    prim[2] = np.sin(2.0 * pi * X)
+    prim[3] = 0.1 * np.sin(2.0 * pi * Z)
+    prim[4] = 5.0 / (12.0 * pi)
+    xn = nodes(cfg, 0)
+    yn = nodes(cfg, 1)
+    Az = (B0 / (2.0 * pi)) * np.cos(2.0 * pi * yn)[:, None] + \
+         (B0 / (4.0 * pi)) * np.cos(4.0 * pi * xn)[None, :]          # (ny+1, nx+1)
+    idx1, idx2 = 1.0 / cfg.dx[0], 1.0 / cfg.dx[1]
+    bx = (Az[1:, :] - Az[:-1, :]) * idx2                               # (ny, nx+1)
+    by = -(Az[:, 1:] - Az[:, :-1]) * idx1                              # (ny+1, nx)
+    b1 = np.broadcast_to(bx[None], cfg.face_shape(0)).copy()
+    b2 = np.broadcast_to(by[None], cfg.face_shape(1)).copy()
+    b3 = np.zeros(cfg.face_shape(2))
+    return Problem("orszag_tang_3d", cfg, prim, [b1, b2, b3], {})
+
+
+def orszag_tang_2d(n: int = 128, flux: int = LHLLD_FLUX, recon: int = LIM2ND_REC,
+                   rk_stages: int = 2) -> Problem:
+    """2D Orszag-Tang (tests/orszag-tang/app.F90) on the periodic unit square."""
+    cfg = PhlxConfig(sdims=2, nx1=n, nx2=n, recon=recon, flux=flux, rk_stages=rk_stages,
+                     cfl=0.8, mach_ct=1e-6).validate()
+    pi = math.pi
+    B0 = 1.0 / math.sqrt(4.0 * pi)
+    X, Y, _ = mesh(cfg)
+    prim = np.empty((cfg.nvars,) + cfg.shape)
+    prim[0] = 25.0 / (36.0 * pi)
+    prim[1] = -np.sin(2.0 * pi * Y)
+    prim[2] = np.sin(2.0 * pi * X)
+    prim[3] = 5.0 / (12.0 * pi)
+    xn, yn = nodes(cfg, 0), nodes(cfg, 1)
+    Az = (B0 / (2.0 * pi)) * np.cos(2.0 * pi * yn)[:, None] + \
+         (B0 / (4.0 * pi)) * np.cos(4.0 * pi * xn)[None, :]
+    bx = (Az[1:, :] - Az[:-1, :]) / cfg.dx[1]
+    by = -(Az[:, 1:] - Az[:, :-1]) / cfg.dx[0]
+    return Problem("orszag_tang_2d", cfg, prim,
+                   [bx[None].copy(), by[None].copy(), None], {})
+
+
+# ------------------------------------------------------ validation problems
+def sod_tube(n: int = 200, flux: int = HLLC_FLUX, recon: int = LIM2ND_REC,
+             ny: int = 4, axis: int = 0) -> Problem:
+    """Sod shock tube along x1 (or x2 with axis=1): (1, 0, 1) | (0.125, 0, 0.1),
+    gamma = 1.4, outflow ends, periodic across."""
+    nx1, nx2 = (n, ny) if axis == 0 else (ny, n)
+    bc = [BC_PERIODIC] * 6
+    bc[2 * axis] = bc[2 * axis + 1] = BC_OUTFLOW
+    cfg = PhlxConfig(sdims=2, nx1=nx1, nx2=nx2, gamma=1.4, recon=recon, flux=flux,
+                     cfl=0.8, mach_cutoff=1e-5, bc=tuple(bc)).validate()
+    X, Y, _ = mesh(cfg)
+    s = X if axis == 0 else Y
+    left = s < 0.5
+    prim = np.zeros((cfg.nvars,) + cfg.shape)
+    prim[0] = np.where(left, 1.0, 0.125)
+    prim[3] = np.where(left, 1.0, 0.1)
+    return Problem("sod", cfg, prim, None, {})
+
+
+def sound_wave(n: int = 64, amp: float = 1e-6, flux: int = HLLC_FLUX,
+               recon: int = LIM2ND_REC, rk_stages: int = 2) -> Problem:
+    """Right-going linear sound wave along x1 on a periodic box; rho0 = 1,
+    p0 = 1/gamma (c = 1).  After t = 1 the exact solution equals the IC."""
+    cfg = PhlxConfig(sdims=2, nx1=n, nx2=4, recon=recon, flux=flux, cfl=0.8,
+                     rk_stages=rk_stages, tmax=1.0).validate()
+    X, _, _ = mesh(cfg)
+    d = amp * np.sin(2.0 * math.pi * X)
+    prim = np.zeros((cfg.nvars,) + cfg.shape)
+    prim[0] = 1.0 + d
+    prim[1] = d
+    prim[3] = 1.0 / cfg.gamma + d
+    return Problem("sound_wave", cfg, prim, None, {})
+
+
+def alfven_wave(n: int = 32, flux: int = HLLD_FLUX, recon: int = LIM2ND_REC,
+                rk_stages: int = 2, amp: float = 0.1) -> Problem:
+    """Circularly polarised Alfven wave along x1 (exact nonlinear solution):
+    rho = 1, p = 0.1, B = (1, amp sin 2pi x, amp cos 2pi x),
+    v = (0, -amp sin, -amp cos); travels at v_A = 1, period 1."""
+    cfg = PhlxConfig(sdims=3, nx1=n, nx2=4, nx3=4, recon=recon, flux=flux, cfl=0.8,
+                     rk_stages=rk_stages, tmax=1.0).validate()
+    X, _, _ = mesh(cfg)
+    s = np.sin(2.0 * math.pi * X)
+    c = np.cos(2.0 * math.pi * X)
+    prim = np.zeros((cfg.nvars,) + cfg.shape)
+    prim[0] = 1.0
+    prim[2] = -amp * s
+    prim[3] = -amp * c
+    prim[4] = 0.1
+    b1 = np.ones(cfg.face_shape(0))
+    xc = centres(cfg, 0)
+    b2 = np.broadcast_to(amp * np.sin(2.0 * math.pi * xc)[None, None, :], cfg.face_shape(1)).copy()
+    b3 = np.broadcast_to(amp * np.cos(2.0 * math.pi * xc)[None, None, :], cfg.face_shape(2)).copy()
+    return Problem("alfven_wave", cfg, prim, [b1, b2, b3], {})
+
+
+def random_smooth(cfg: PhlxConfig, seed: int = SEED, amp: float = 0.2) -> Problem:
+    """Random smooth-ish state for parity tests (any dims / options): density
+    and pressure positive, velocities O(amp), divergence-free face field
+    from a random nodal vector potential when MHD."""
+    X, Y, Z = mesh(cfg)
+    N = X.size
+    sh = X.shape
+    u = lambda s: (2.0 * uniform01(seed, s, N) - 1.0).reshape(sh)
+    tp = 2.0 * math.pi
+    prim = np.zeros((cfg.nvars,) + cfg.shape)
+    prim[0] = 1.0 + 0.3 * np.sin(tp * X) * np.cos(tp * Y) + 0.05 * u(1)
+    for d in range(cfg.sdims):
+        prim[1 + d] = amp * (np.sin(tp * (Y if d == 0 else X) + d) + 0.2 * u(2 + d))
+    ip = cfg.sdims + 1
+    prim[ip] = 1.0 + 0.2 * np.cos(tp * (X + Y)) + 0.05 * u(7)
+    for s in range(cfg.sdims + 2, cfg.nvars):
+        prim[s] = 0.5 + 0.2 * u(10 + s)
+    bf = None
+    if cfg.mhd:
+        bf = _random_div_free(cfg, seed)
+    return Problem("random_smooth", cfg, prim, bf, {})
+
+
+def _random_div_free(cfg: PhlxConfig, seed: int, amp: float = 0.3):
+    """Face field b = curl A with A on cell edges (discrete div b == 0)."""
+    nz, ny, nx = cfg.shape
+    dx1, dx2, dx3 = cfg.dx
+    tp = 2.0 * math.pi
+    xn, yn = nodes(cfg, 0), nodes(cfg, 1)
+    xc, yc = centres(cfg, 0), centres(cfg, 1)
+    if cfg.sdims == 2:
+        # A_z on nodes, periodic-compatible Fourier modes
+        Az = amp * (np.cos(tp * yn)[:, None] / tp + 0.5 * np.sin(2 * tp * xn)[None, :] / tp
+                    + 0.2 * np.sin(tp * (xn[None, :] + 2 * yn[:, None])) / tp)
+        bx = (Az[1:, :] - Az[:-1, :]) / dx2
+        by = -(Az[:, 1:] - Az[:, :-1]) / dx1
+        return [bx[None].copy(), by[None].copy(), None]
+    zn, zc = nodes(cfg, 2), centres(cfg, 2)
+    # edge-centred potentials: Ax at (xc, yn, zn), Ay at (xn, yc, zn), Az at (xn, yn, zc)
+    Ax = amp * np.sin(tp * (yn[None, :, None] + zn[:, None, None])) * \
+        np.cos(tp * xc[None, None, :]) / tp
+    Ay = amp * np.cos(tp * (zn[:, None, None] - xn[None, None, :])) * \
+        (1 + 0 * yc[None, :, None]) / tp
+    Az = amp * (np.cos(tp * yn)[None, :, None] / tp + 0.5 * np.sin(2 * tp * xn)[None, None, :] / tp
+                + 0 * zc[:, None, None])
+    # b = curl A
+    b1 = (Az[:, 1:, :] - Az[:, :-1, :]) / dx2 - (Ay[1:, :, :] - Ay[:-1, :, :]) / dx3
+    b2 = (Ax[1:, :, :] - Ax[:-1, :, :]) / dx3 - (Az[:, :, 1:] - Az[:, :, :-1]) / dx1
+    b3 = (Ay[:, :, 1:] - Ay[:, :, :-1]) / dx1 - (Ax[:, 1:, :] - Ax[:, :-1, :]) / dx2
+    return [np.ascontiguousarray(b1), np.ascontiguousarray(b2), np.ascontiguousarray(b3)]

@@ -1,0 +1,217 @@
+"""Host-side mirror of the reference's hydro_step interface over the C-ABI.
+
+``Simulation`` plays the role of the reference's ``locgrid`` + ``time_loop``
+caller (source/source.F90:839-1130, :5694-6841): it uploads the initial
+state, runs ``init()`` (step-0 EoS and CFL), then ``run(nsteps)`` /
+``hydro_step()`` / ``compute_hyperbolic_dt()`` on the GPU, and downloads
+the state.  All numerics happen in libphlx.so (phlegethon_amd/csrc); if the
+library is missing this module raises instead of computing anything on
+the CPU.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Dict, Optional
+
+import numpy as np
+
+from .config import PhlxConfig
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_lib", "libphlx.so")
+
+_lib = None
+
+
+class PhlxError(RuntimeError):
+    def __init__(self, code: int, what: str):
+        msg = _lib.phlx_error_string(code).decode() if _lib is not None else str(code)
+        super().__init__(f"{what} failed: {msg} ({code})")
+        self.code = code
+
+
+def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
+    """Load libphlx.so (built by ``__graft_entry__.build()``).  Raises if it is
+    missing: there is no fallback implementation."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(
+            f"{path} not found: build the HIP library first (python -c "
+            "'import __graft_entry__ as g; g.build()')")
+    lib = ctypes.CDLL(path)
+    P, I, D = ctypes.c_void_p, ctypes.c_int32, ctypes.c_double
+    DP = ctypes.POINTER(ctypes.c_double)
+    lib.phlx_error_string.restype = ctypes.c_char_p
+    lib.phlx_error_string.argtypes = [ctypes.c_int]
+    lib.phlx_create.argtypes = [P, ctypes.POINTER(P)]
+    lib.phlx_destroy.argtypes = [P]
+    lib.phlx_nvars.argtypes = [P]
+    lib.phlx_upload_state.argtypes = [P] + [P] * 6
+    lib.phlx_download_state.argtypes = [P] + [P] * 7
+    lib.phlx_upload_prim_aos.argtypes = [P, P, I]
+    lib.phlx_download_prim_aos.argtypes = [P, P, I]
+    lib.phlx_set_profiles.argtypes = [P] + [P] * 8
+    lib.phlx_init.argtypes = [P]
+    lib.phlx_hydro_step.argtypes = [P]
+    lib.phlx_compute_dt.argtypes = [P, DP, DP]
+    lib.phlx_run.argtypes = [P, I]
+    lib.phlx_synchronize.argtypes = [P]
+    lib.phlx_get_scalars.argtypes = [P, DP, DP, DP, ctypes.POINTER(ctypes.c_int64)]
+    lib.phlx_set_scalars.argtypes = [P, D, D, D, ctypes.c_int64]
+    lib.phlx_stage_begin.argtypes = [P, I]
+    lib.phlx_sweep.argtypes = [P, I]
+    lib.phlx_ct.argtypes = [P, I, D, D, D]
+    lib.phlx_rk_update.argtypes = [P, I, D, D, D, I]
+    lib.phlx_divb_max.argtypes = [P, DP]
+    lib.phlx_profile_enable.argtypes = [P, I]
+    lib.phlx_profile_read.argtypes = [P, I, DP, ctypes.POINTER(ctypes.c_int64)]
+    lib.phlx_stream.argtypes = [P]
+    lib.phlx_stream.restype = P
+    _lib = lib
+    return lib
+
+
+def _ptr(a: Optional[np.ndarray]):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _chk(rc: int, what: str):
+    if rc != 0:
+        raise PhlxError(rc, what)
+
+
+KERNELS = {"flux_x1": 0, "flux_x2": 1, "flux_x3": 2, "corner": 3, "faceb": 4,
+           "update": 5, "misc": 6}
+
+
+class Simulation:
+    """Device-resident state of one problem (one context, one HIP stream)."""
+
+    def __init__(self, cfg: PhlxConfig):
+        self.cfg = cfg.validate()
+        self.lib = load_library()
+        self._ctx = ctypes.c_void_p()
+        c = self.cfg.to_c()
+        _chk(self.lib.phlx_create(ctypes.byref(c), ctypes.byref(self._ctx)), "phlx_create")
+
+    # -- lifecycle -------------------------------------------------------
+    def close(self):
+        if self._ctx:
+            self.lib.phlx_destroy(self._ctx)
+            self._ctx = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- state exchange ----------------------------------------------------
+    def upload(self, prim: np.ndarray, b_faces=None, eint: Optional[np.ndarray] = None):
+        cfg = self.cfg
+        prim = np.ascontiguousarray(prim, dtype=np.float64)
+        assert prim.shape == (cfg.nvars,) + cfg.shape, prim.shape
+        bs = [None, None, None]
+        if b_faces is not None:
+            for d in range(cfg.sdims):
+                bs[d] = np.ascontiguousarray(b_faces[d], dtype=np.float64)
+                assert bs[d].shape == cfg.face_shape(d), (d, bs[d].shape)
+        e = None if eint is None else np.ascontiguousarray(eint, dtype=np.float64)
+        _chk(self.lib.phlx_upload_state(self._ctx, _ptr(prim), _ptr(e), None,
+                                        _ptr(bs[0]), _ptr(bs[1]), _ptr(bs[2])),
+             "phlx_upload_state")
+
+    def set_profiles(self, **prof):
+        names = ["eq_rho_cc", "eq_p_cc", "eq_rho_f", "eq_p_f", "grav_cc", "phi_cc",
+                 "phi_f", "edot_cc"]
+        arrs = [None if prof.get(n) is None else
+                np.ascontiguousarray(prof[n], dtype=np.float64) for n in names]
+        _chk(self.lib.phlx_set_profiles(self._ctx, *[_ptr(a) for a in arrs]),
+             "phlx_set_profiles")
+
+    def download(self) -> Dict[str, np.ndarray]:
+        cfg = self.cfg
+        prim = np.empty((cfg.nvars,) + cfg.shape)
+        eint = np.empty(cfg.shape)
+        temp = np.empty(cfg.shape)
+        bcc = np.empty((3,) + cfg.shape)
+        bs = [np.empty(cfg.face_shape(d)) if cfg.mhd and d < cfg.sdims else None
+              for d in range(3)]
+        _chk(self.lib.phlx_download_state(self._ctx, _ptr(prim), _ptr(eint), _ptr(temp),
+                                          _ptr(bs[0]), _ptr(bs[1]), _ptr(bs[2]), _ptr(bcc)),
+             "phlx_download_state")
+        out = {"prim": prim, "eint": eint, "temp": temp, "b_cc": bcc}
+        for d in range(3):
+            if bs[d] is not None:
+                out[f"b_x{d+1}"] = bs[d]
+        return out
+
+    # -- the reference's entry points ------------------------------------
+    def init(self):
+        _chk(self.lib.phlx_init(self._ctx), "phlx_init")
+
+    def hydro_step(self):
+        _chk(self.lib.phlx_hydro_step(self._ctx), "phlx_hydro_step")
+
+    def compute_hyperbolic_dt(self):
+        dt, smax = ctypes.c_double(), ctypes.c_double()
+        _chk(self.lib.phlx_compute_dt(self._ctx, ctypes.byref(dt), ctypes.byref(smax)),
+             "phlx_compute_dt")
+        return dt.value, smax.value
+
+    def run(self, nsteps: int):
+        _chk(self.lib.phlx_run(self._ctx, int(nsteps)), "phlx_run")
+
+    def synchronize(self):
+        _chk(self.lib.phlx_synchronize(self._ctx), "phlx_synchronize")
+
+    def scalars(self):
+        t, dt, smax = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        st = ctypes.c_int64()
+        _chk(self.lib.phlx_get_scalars(self._ctx, ctypes.byref(t), ctypes.byref(dt),
+                                       ctypes.byref(smax), ctypes.byref(st)),
+             "phlx_get_scalars")
+        return {"time": t.value, "dt": dt.value, "smax": smax.value, "step": st.value}
+
+    def set_scalars(self, time: float, dt: float, smax: float, step: int):
+        _chk(self.lib.phlx_set_scalars(self._ctx, time, dt, smax, step), "phlx_set_scalars")
+
+    def divb_max(self) -> float:
+        v = ctypes.c_double()
+        _chk(self.lib.phlx_divb_max(self._ctx, ctypes.byref(v)), "phlx_divb_max")
+        return v.value
+
+    # -- fine-grained stage API ------------------------------------------
+    def stage_begin(self, irk):
+        _chk(self.lib.phlx_stage_begin(self._ctx, irk), "phlx_stage_begin")
+
+    def sweep(self, d):
+        _chk(self.lib.phlx_sweep(self._ctx, d), "phlx_sweep")
+
+    def ct(self, irk, a1, a2, a3):
+        _chk(self.lib.phlx_ct(self._ctx, irk, a1, a2, a3), "phlx_ct")
+
+    def rk_update(self, irk, a1, a2, a3, last):
+        _chk(self.lib.phlx_rk_update(self._ctx, irk, a1, a2, a3, int(last)), "phlx_rk_update")
+
+    # -- profiling -------------------------------------------------------
+    def profile(self, on: bool):
+        _chk(self.lib.phlx_profile_enable(self._ctx, int(on)), "phlx_profile_enable")
+
+    def profile_read(self, name: str):
+        ms, n = ctypes.c_double(), ctypes.c_int64()
+        _chk(self.lib.phlx_profile_read(self._ctx, KERNELS[name], ctypes.byref(ms),
+                                        ctypes.byref(n)), "phlx_profile_read")
+        return ms.value, n.value
