@@ -1,0 +1,201 @@
+#!/usr/bin/env python3
+"""Benchmark: zone-updates/s of the explicit FV MHD update on one MI355X.
+
+Workload (default): 3D Orszag-Tang ideal MHD with constrained transport,
+256^3 zones, FP64, LHLLD flux + LIM2ND reconstruction, RK2, CFL 0.8,
+periodic unit cube, seeded 1e-4 density noise.  One "step" is one full
+time step (both RK stages, CT, CFL update) over the whole grid; the metric
+follows the reference's "updated cells/s" = steps * zones / loop wall time
+(source/source.F90:6834-6835) with the state resident in HBM.
+
+Prints ONE JSON line on stdout (progress goes to stderr).  Includes
+  roofline      -- dominant kernel, algorithmic bytes / average launch time
+                   (HIP events on the library's stream) vs 8 TB/s HBM peak;
+  step_roofline -- whole step: B_zu * zone-updates/s vs the same peak;
+  cpu_baseline  -- the CPU FP64 restatement (oracle/) on the host cores over
+                   a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12  # B/s, MI355X HBM3E
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def kernel_bytes(cfg, name: str) -> float:
+    """Algorithmic HBM bytes of one launch: the compulsory reads of the
+    kernel's inputs (each value once) and writes of its outputs, FP64."""
+    nz, ny, nx = cfg.shape
+    N = nx * ny * nz
+    nv, sd = cfg.nvars, cfg.sdims
+    mhd = cfg.mhd
+    nf = [(nx + 1) * ny * nz, nx * (ny + 1) * nz, nx * ny * (nz + 1)]
+    cell_in = 8 * (nv + (sd if mhd else 0))           # prim (+ b_cc) per cell
+    if name.startswith("flux_x"):
+        d = int(name[-1]) - 1
+        per_face_out = 8 * nv + (8 * 2 + 1 if mhd else 0)  # fluxes (+2 EMFs + sign)
+        return N * cell_in + nf[d] * ((8 if mhd else 0) + per_face_out)
+    if name == "corner":   # 2 EMFs + sign per face in, v & b_cc per cell, 3 edge EMFs out
+        return sum(nf[:sd]) * 17 + N * 8 * (2 * sd) + 3 * N * 8
+    if name == "faceb":    # b, b0, 4 edge EMFs in; b (+b0 at stage 1) out
+        return sum(nf[:sd]) * 8 * 3 + 3 * N * 8
+    if name == "update":   # fluxes, prim, U, U0 (or eint, b_cc), faces in; prim, U, eint, b_cc out
+        rd = sum(nf[:sd]) * 8 * nv + N * 8 * (3 * nv) + (sum(nf[:sd]) * 8 if mhd else 0)
+        wr = N * 8 * (2 * nv + 1 + (sd if mhd else 0))
+        return rd + wr
+    return 0.0
+
+
+def b_zu(cfg) -> int:
+    """Algorithmic bytes per zone-update of the whole step, FP64, RK2:
+    8 * (5 n_q + 4 n_e) with n_q evolved fields (conserved + face B)."""
+    n_q = cfg.nvars + (cfg.sdims if cfg.mhd else 0)
+    n_e = 0
+    stages = cfg.rk_stages
+    if stages == 2:
+        return 8 * (5 * n_q + 4 * n_e)
+    return 8 * (8 * n_q + 6 * n_e)
+
+
+def cpu_baseline(pb, steps: int, threads: int):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    r = oracle.OracleRun(pb, threads)
+    r.init()
+    t0 = time.perf_counter()
+    r.run(steps)
+    dt = time.perf_counter() - t0
+    r.close()
+    zones = pb.cfg.nx1 * pb.cfg.nx2 * (pb.cfg.nx3 if pb.cfg.sdims == 3 else 1)
+    return zones * steps / dt, dt
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=256, help="zones per direction")
+    ap.add_argument("--profile-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; the multi-GPU "
+                          "brick decomposition is a design (not built): use --gpus 1"}))
+        sys.exit(2)
+
+    import numpy as np
+    from phlegethon_amd import problems
+    from phlegethon_amd.runtime import Simulation
+
+    n = args.n
+    log(f"building 3D Orszag-Tang {n}^3 initial data")
+    pb = problems.orszag_tang_3d(n=n)
+    cfg = pb.cfg
+    zones = n ** 3
+    sim = Simulation(cfg)
+    sim.upload(pb.prim, pb.b_faces)
+    sim.init()
+    log(f"warmup {args.warmup} steps")
+    sim.run(args.warmup)
+    sim.synchronize()
+    log(f"timing {args.steps} steps")
+    t0 = time.perf_counter()
+    sim.run(args.steps)
+    sim.synchronize()
+    t1 = time.perf_counter()
+    elapsed = t1 - t0
+    value = zones * args.steps / elapsed
+    ms_per_step = 1e3 * elapsed / args.steps
+    sc = sim.scalars()
+    divb = sim.divb_max()
+    log(f"{value:.4e} zone-updates/s, {ms_per_step:.3f} ms/step, t={sc['time']:.6f}, "
+        f"max|divB|={divb:.2e}")
+
+    # per-kernel device time (HIP events on the library stream)
+    sim.profile(True)
+    sim.run(args.profile_steps)
+    sim.synchronize()
+    kern = {}
+    for name in ("flux_x1", "flux_x2", "flux_x3", "corner", "faceb", "update"):
+        ms, cnt = sim.profile_read(name)
+        if cnt:
+            kern[name] = {"total_ms": ms, "launches": cnt, "avg_ms": ms / cnt,
+                          "alg_bytes": kernel_bytes(cfg, name)}
+    sim.profile(False)
+    copy_gbps = sim.stream_copy(1 << 30, 10)
+    log(f"stream copy (1 GiB x10): {copy_gbps:.0f} GB/s")
+    sim.close()
+    dom = max(kern, key=lambda k: kern[k]["total_ms"])
+    d = kern[dom]
+    achieved = d["alg_bytes"] / (d["avg_ms"] * 1e-3)
+    traffic = None
+    pmc = os.path.join(ROOT, "profiles", "ot3d256_fp64_pmc.json")
+    if n == 256 and os.path.exists(pmc):
+        # HBM bytes per dispatch measured with rocprofv3 --pmc FETCH_SIZE /
+        # WRITE_SIZE on this workload, scaled by a copy kernel of known size
+        with open(pmc) as fh:
+            rec = json.load(fh)["kernels"].get(dom)
+        if rec:
+            traffic = rec["hbm_bytes"]
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
+                "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic,
+                "alg_bytes_per_launch": d["alg_bytes"], "avg_launch_ms": d["avg_ms"]}
+    bzu = b_zu(cfg)
+    step_roof = {"bound": "hbm", "B_zu": bzu, "achieved": bzu * value / 1e9,
+                 "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": bzu * value / HBM_PEAK,
+                 "stream_copy_GBps": copy_gbps}
+    for k, v in kern.items():
+        log(f"  {k:8s} avg {v['avg_ms']:.3f} ms  x{v['launches']}  "
+            f"{v['alg_bytes'] / (v['avg_ms'] * 1e-3) / 1e9:.0f} GB/s algorithmic")
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        log(f"cpu baseline: oracle on {threads} threads, {args.cpu_steps} steps of {n}^3")
+        rate, wall = cpu_baseline(pb, args.cpu_steps, threads)
+        cpu = {"value": rate, "unit": "zone-updates/s", "cores": threads, "kind": "port",
+               "sample": f"{args.cpu_steps} full steps of the same {n}^3 workload "
+                         f"({wall:.1f} s wall), C FP64 restatement with OpenMP"}
+        log(f"cpu baseline {rate:.4e} zone-updates/s")
+
+    out = {
+        "metric": "zone-updates/s (plain FP64 path; HBM-roofline fraction)",
+        "value": value,
+        "unit": "zone-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded 3D Orszag-Tang initial data, counter-based PRNG)",
+        "config": {"workload": f"3D Orszag-Tang ideal MHD + CT, {n}^3, FP64, LHLLD + "
+                               f"LIM2ND, RK2, CFL 0.8, periodic",
+                   "zones": zones, "sim_time": sc["time"], "max_divB": divb},
+        "roofline": roofline,
+        "step_roofline": step_roof,
+        "kernels": {k: {"avg_ms": v["avg_ms"], "launches": v["launches"]} for k, v in kern.items()},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

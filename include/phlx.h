@@ -140,6 +140,10 @@ int phlx_divb_max(phlx_ctx *ctx, double *out);
  * 3 corner EMF, 4 face-B, 5 update, 6 misc. */
 int phlx_profile_enable(phlx_ctx *ctx, int32_t on);
 int phlx_profile_read(phlx_ctx *ctx, int32_t kernel, double *total_ms, int64_t *launches);
+/* Bandwidth probe: copies nbytes between two fresh device buffers `reps`
+ * times with a plain 16-byte-vector copy kernel and returns the achieved
+ * HBM rate (read + write bytes per second, in GB/s).  Diagnostic only. */
+int phlx_stream_copy(phlx_ctx *ctx, int64_t nbytes, int32_t reps, double *gbps);
 /* Opaque HIP stream of the context (hipStream_t), for callers that want
  * to order their own work after the library's. */
 void *phlx_stream(phlx_ctx *ctx);

@@ -27,7 +27,7 @@ struct phlx_ctx {
   double rk[3][3];
   bool initialized = false;
   std::vector<void *> allocs;
-  unsigned long long *scratch_bits = nullptr;
+  double *scratch_val = nullptr;
   // profiling
   bool prof = false;
   std::vector<ProfRec> recs;
@@ -191,7 +191,9 @@ extern "C" int phlx_create(const phlx_config *cfg, phlx_ctx **out) {
   const size_t ny = g.n2;
   e |= dalloc(ctx, &prof, 8 * (ny + 1));
   e |= dalloc(ctx, &g.sc, 1);
-  e |= dalloc(ctx, &ctx->scratch_bits, 1);
+  e |= dalloc(ctx, &ctx->scratch_val, 1);
+  g.nred = cell_grid_blocks(g);
+  e |= dalloc(ctx, &g.red, (size_t)g.nred);
   if (e) {
     phlx_destroy(ctx);
     return PHLX_E_NOMEM;
@@ -503,9 +505,8 @@ extern "C" int phlx_init(phlx_ctx *ctx) {
 extern "C" int phlx_compute_dt(phlx_ctx *ctx, double *dt_out, double *smax_out) {
   if (!ctx) return PHLX_E_ARG;
   if (!ctx->initialized) return PHLX_E_STATE;
-  launch_zero_bits(&ctx->g.sc->smax_bits, ctx->stream);
   ctx->kt.cfl(ctx->g, ctx->stream);
-  launch_step_end(ctx->g.sc, 0, ctx->stream);
+  launch_step_end(ctx->g, 0, ctx->stream);
   int rc = check_launch();
   if (rc) return rc;
   if (dt_out || smax_out) {
@@ -534,7 +535,7 @@ extern "C" int phlx_run(phlx_ctx *ctx, int32_t nsteps) {
     launch_step_begin(ctx->g.sc, ctx->stream);
     int rc = do_hydro_step(ctx, true);
     if (rc) return rc;
-    launch_step_end(ctx->g.sc, 1, ctx->stream);
+    launch_step_end(ctx->g, 1, ctx->stream);
     rc = check_launch();
     if (rc) return rc;
   }
@@ -581,16 +582,47 @@ extern "C" int phlx_divb_max(phlx_ctx *ctx, double *out) {
     *out = 0.0;
     return PHLX_OK;
   }
-  launch_zero_bits(ctx->scratch_bits, ctx->stream);
-  ctx->kt.divb(ctx->g, ctx->scratch_bits, ctx->stream);
+  ctx->kt.divb(ctx->g, ctx->stream);
+  launch_reduce_to(ctx->g, ctx->scratch_val, ctx->stream);
   int rc = check_launch();
   if (rc) return rc;
-  unsigned long long bits = 0;
-  HIPCHK(hipMemcpyAsync(&bits, ctx->scratch_bits, sizeof(bits), hipMemcpyDeviceToHost,
-                        ctx->stream));
+  double v = 0.0;
+  HIPCHK(hipMemcpyAsync(&v, ctx->scratch_val, sizeof(v), hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(hipStreamSynchronize(ctx->stream));
-  double v;
-  std::memcpy(&v, &bits, sizeof(v));
   *out = v;
   return PHLX_OK;
+}
+
+extern "C" int phlx_stream_copy(phlx_ctx *ctx, int64_t nbytes, int32_t reps, double *gbps) {
+  if (!ctx || nbytes < 16 || reps < 1 || !gbps) return PHLX_E_ARG;
+  HIPCHK(hipSetDevice(ctx->device));
+  const long long n16 = nbytes / 16;
+  void *a = nullptr, *b = nullptr;
+  if (hipMalloc(&a, n16 * 16) != hipSuccess) return PHLX_E_NOMEM;
+  if (hipMalloc(&b, n16 * 16) != hipSuccess) {
+    (void)hipFree(a);
+    return PHLX_E_NOMEM;
+  }
+  int rc = PHLX_OK;
+  hipEvent_t e0, e1;
+  if (hipMemsetAsync(a, 0, n16 * 16, ctx->stream) != hipSuccess ||
+      hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+    rc = PHLX_E_HIP;
+  } else {
+    launch_stream_copy(a, b, n16, ctx->stream);  // warm
+    (void)hipEventRecord(e0, ctx->stream);
+    for (int r = 0; r < reps; r++) launch_stream_copy(a, b, n16, ctx->stream);
+    (void)hipEventRecord(e1, ctx->stream);
+    float ms = 0.f;
+    if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+      rc = PHLX_E_HIP;
+    else
+      *gbps = 2.0 * (double)n16 * 16.0 * reps / (ms * 1e-3) / 1e9;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+  }
+  (void)hipStreamSynchronize(ctx->stream);
+  (void)hipFree(a);
+  (void)hipFree(b);
+  return rc;
 }

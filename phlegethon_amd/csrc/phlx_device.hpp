@@ -26,7 +26,7 @@ enum { FLUX_HLLC = 0, FLUX_LHLLC = 1, FLUX_HLLD = 2, FLUX_LHLLD = 3 };
 // Device-resident step control: dt/time/smax never round-trip to the host.
 struct Scalars {
   double time, dt, smax, tmax, cfl;
-  unsigned long long smax_bits;  // running max of positive doubles as bits
+  unsigned long long spare;
   long long step;
   int active;  // 0 once time >= tmax: the step's kernels return at once
   int pad;
@@ -48,6 +48,8 @@ struct Grid {
   double *cor[3];          // corner (edge) EMFs along x1, x2, x3
   const double *eq_rho_cc, *eq_p_cc, *eq_rho_f, *eq_p_f, *gravy, *phi_cc, *phi_f,
       *edot_cc;
+  double *red;       // per-block partial maxima of the cell-grid reductions
+  int nred;          // number of partials (blocks of a cell-grid launch)
   Scalars *sc;
 };
 

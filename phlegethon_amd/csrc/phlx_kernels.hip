@@ -298,7 +298,11 @@ __device__ __forceinline__ double wave_max(double v) {
   return v;
 }
 
-__device__ __forceinline__ void block_max_to(double s, unsigned long long *dst) {
+// Block maximum written to red[block]; the partials are reduced by one
+// small kernel afterwards (a single-address atomic per block serialises
+// tens of thousands of blocks at one L2 channel).  max is exact, so the
+// result does not depend on the order.
+__device__ __forceinline__ void block_max_to(double s, double *red) {
   __shared__ double part[16];
   s = wave_max(s);
   const int lane = threadIdx.x & 63, wid = (threadIdx.y * blockDim.x + threadIdx.x) >> 6;
@@ -308,9 +312,21 @@ __device__ __forceinline__ void block_max_to(double s, unsigned long long *dst) 
   if (threadIdx.x == 0 && threadIdx.y == 0) {
     double m = part[0];
     for (int w = 1; w < nw; w++) m = fmax(m, part[w]);
-    // positive doubles order like their bit patterns
-    atomicMax(dst, (unsigned long long)__double_as_longlong(m));
+    red[((long long)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = m;
   }
+}
+
+// Reduce n partials with one 1024-thread block.
+__device__ __forceinline__ double reduce_partials(const double *red, int n) {
+  __shared__ double part[16];
+  double m = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) m = fmax(m, red[i]);
+  m = wave_max(m);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = m;
+  __syncthreads();
+  m = part[0];
+  for (int w = 1; w < (int)(blockDim.x >> 6); w++) m = fmax(m, part[w]);
+  return m;
 }
 
 // Signal speed of one cell, summed over directions (source/source.F90
@@ -441,7 +457,7 @@ __global__ void __launch_bounds__(256) k_update(Grid g, int first, int last, dou
     g.eint[c] = eint;
     if (last) smax_local = cell_speed<SD, MHD>(g, p, b);
   }
-  if (last) block_max_to(smax_local, &g.sc->smax_bits);
+  if (last) block_max_to(smax_local, g.red);
 }
 
 // Standalone compute_hyperbolic_dt pass over the current prim / b_cc.
@@ -461,7 +477,7 @@ __global__ void __launch_bounds__(256) k_cfl(Grid g) {
       for (int v = 0; v < SD; v++) b[v] = g.bcc[v * N + c];
     s = cell_speed<SD, MHD>(g, p, b);
   }
-  block_max_to(s, &g.sc->smax_bits);
+  block_max_to(s, g.red);
 }
 
 // Step-0 initialisation: b_cc from faces and the ideal-gas eint
@@ -483,7 +499,7 @@ __global__ void __launch_bounds__(256) k_init(Grid g) {
 
 // max |div B| over cells, reduced like the CFL max.
 template <int SD>
-__global__ void __launch_bounds__(256) k_divb(Grid g, unsigned long long *out) {
+__global__ void __launch_bounds__(256) k_divb(Grid g) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y * blockDim.y + threadIdx.y;
   const int k = blockIdx.z;
@@ -494,7 +510,7 @@ __global__ void __launch_bounds__(256) k_divb(Grid g, unsigned long long *out) {
     if (SD == 3) d = d + (g.bf[2][fix(g, 2, i, j, k + 1)] - g.bf[2][fix(g, 2, i, j, k)]) * g.inv_dx[2];
     s = fabs(d);
   }
-  block_max_to(s, out);
+  block_max_to(s, g.red);
 }
 
 // Step bracket kernels (one thread): time_loop's dt clip before the step
@@ -504,17 +520,37 @@ __global__ void k_step_begin(Scalars *sc) {
   sc->active = (sc->time < sc->tmax) ? 1 : 0;
   if (!sc->active) return;
   if (sc->time + sc->dt > sc->tmax) sc->dt = sc->tmax - sc->time;
-  sc->smax_bits = 0ull;
 }
-__global__ void k_step_end(Scalars *sc, int advance) {
+// dt = cfl / max of the partials; with advance: time += dt first, step++.
+__global__ void __launch_bounds__(1024) k_step_end(Scalars *sc, const double *red, int n,
+                                                   int advance) {
   if (advance && !sc->active) return;
-  if (advance) sc->time = sc->time + sc->dt;
-  sc->smax = __longlong_as_double((long long)sc->smax_bits);
-  sc->dt = sc->cfl / sc->smax;
-  if (advance) sc->step = sc->step + 1;
-  sc->active = 1;
+  const double m = reduce_partials(red, n);
+  if (threadIdx.x == 0) {
+    if (advance) sc->time = sc->time + sc->dt;
+    sc->smax = m;
+    sc->dt = sc->cfl / m;
+    if (advance) sc->step = sc->step + 1;
+    sc->active = 1;
+  }
 }
-__global__ void k_zero_bits(unsigned long long *p) { *p = 0ull; }
+__global__ void __launch_bounds__(1024) k_reduce_to(const double *red, int n, double *out) {
+  const double m = reduce_partials(red, n);
+  if (threadIdx.x == 0) *out = m;
+}
+
+// Plain vector copy used as the bandwidth reference (16 B per lane access).
+__global__ void __launch_bounds__(256) k_stream_copy(const double2 *__restrict__ src,
+                                                     double2 *__restrict__ dst, long long n) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    dst[i] = src[i];
+}
+
+void launch_stream_copy(const void *src, void *dst, long long n16, hipStream_t st) {
+  const int blocks = 256 * 8 * 4;
+  k_stream_copy<<<blocks, 256, 0, st>>>((const double2 *)src, (double2 *)dst, n16);
+}
 
 // -------------------------------------------------------------- launchers
 static dim3 grid_for(int e1, int e2, int e3, dim3 blk) {
@@ -558,9 +594,9 @@ struct Kern {
     const dim3 blk(64, 4, 1);
     k_init<SD, MHD><<<grid_for(g.n1, g.n2, g.n3, blk), blk, 0, st>>>(g);
   }
-  static void divb(const Grid &g, unsigned long long *out, hipStream_t st) {
+  static void divb(const Grid &g, hipStream_t st) {
     const dim3 blk(64, 4, 1);
-    k_divb<SD><<<grid_for(g.n1, g.n2, g.n3, blk), blk, 0, st>>>(g, out);
+    k_divb<SD><<<grid_for(g.n1, g.n2, g.n3, blk), blk, 0, st>>>(g);
   }
 };
 
@@ -590,9 +626,16 @@ bool select_kernels(int sd, int nas, bool mhd, KernelTable &t) {
 }
 
 void launch_step_begin(Scalars *sc, hipStream_t st) { k_step_begin<<<1, 1, 0, st>>>(sc); }
-void launch_step_end(Scalars *sc, int advance, hipStream_t st) {
-  k_step_end<<<1, 1, 0, st>>>(sc, advance);
+void launch_step_end(const Grid &g, int advance, hipStream_t st) {
+  k_step_end<<<1, 1024, 0, st>>>(g.sc, g.red, g.nred, advance);
 }
-void launch_zero_bits(unsigned long long *p, hipStream_t st) { k_zero_bits<<<1, 1, 0, st>>>(p); }
+void launch_reduce_to(const Grid &g, double *out, hipStream_t st) {
+  k_reduce_to<<<1, 1024, 0, st>>>(g.red, g.nred, out);
+}
+int cell_grid_blocks(const Grid &g) {
+  const dim3 blk(64, 4, 1);
+  const dim3 gr = grid_for(g.n1, g.n2, g.n3, blk);
+  return (int)(gr.x * gr.y * gr.z);
+}
 
 }  // namespace phlx

@@ -68,6 +68,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.phlx_divb_max.argtypes = [P, DP]
     lib.phlx_profile_enable.argtypes = [P, I]
     lib.phlx_profile_read.argtypes = [P, I, DP, ctypes.POINTER(ctypes.c_int64)]
+    lib.phlx_stream_copy.argtypes = [P, ctypes.c_int64, I, DP]
     lib.phlx_stream.argtypes = [P]
     lib.phlx_stream.restype = P
     _lib = lib
@@ -205,6 +206,13 @@ class Simulation:
 
     def rk_update(self, irk, a1, a2, a3, last):
         _chk(self.lib.phlx_rk_update(self._ctx, irk, a1, a2, a3, int(last)), "phlx_rk_update")
+
+    def stream_copy(self, nbytes: int = 1 << 30, reps: int = 10) -> float:
+        """Achieved HBM GB/s of a plain device copy (read + write)."""
+        v = ctypes.c_double()
+        _chk(self.lib.phlx_stream_copy(self._ctx, int(nbytes), int(reps), ctypes.byref(v)),
+             "phlx_stream_copy")
+        return v.value
 
     # -- profiling -------------------------------------------------------
     def profile(self, on: bool):
