@@ -86,6 +86,35 @@ void orc_riemann_mhd(const orc_config *cfg, int n, const double *qL,
 void orc_reconstruct(int scheme, int limited_scalar, int n, const double *q,
                      double *qm, double *qp);
 
+/* ---- Helmholtz EoS (helm_oracle.c) ---- */
+typedef struct orc_helm orc_helm;
+/* f9_file / c4_file: the table in the reference file's column order,
+ * [j (T)][i (rho)][9 | 4]. */
+orc_helm *orc_helm_create(int nr, int nt, double ldlo, double ldhi, double ltlo,
+                          double lthi, const double *f9_file, const double *c4_file);
+void orc_helm_destroy(orc_helm *h);
+double orc_helm_node(const orc_helm *h, int which, int idx);
+void orc_helm_full(const orc_helm *h, double rho, double T, double abar, double zbar,
+                   double *P, double *E, double *sound, double *cv);
+void orc_helm_given1(const orc_helm *h, double rho, double T, double abar, double zbar,
+                     double *P, double *dPdT, int ret);
+void orc_helm_given2(const orc_helm *h, double rho, double T, double abar, double zbar,
+                     double *E, double *dEdT, int ret);
+int orc_helm_rhoe_given(const orc_helm *h, double rho, double E0, double abar, double zbar,
+                        double *T, double *P, double *sound, int ret);
+int orc_helm_rhoP_given(const orc_helm *h, double rho, double P0, double abar, double zbar,
+                        double *T, double *E, double *sound, int ret);
+void orc_helm_full_batch(const orc_helm *h, int n, const double *rho, const double *T,
+                         const double *abar, const double *zbar, double *P, double *E,
+                         double *cs, double *cv);
+void orc_helm_rhoe_batch(const orc_helm *h, int n, const double *rho, const double *e,
+                         const double *abar, const double *zbar, double *T, double *P,
+                         double *cs, int ret, int *iters);
+void orc_helm_rhoP_batch(const orc_helm *h, int n, const double *rho, const double *p,
+                         const double *abar, const double *zbar, double *T, double *E,
+                         double *cs, int ret, int *iters);
+void orc_helm_interp(const orc_helm *h, double rhos, double T, double *out6);
+
 #ifdef __cplusplus
 }
 #endif

@@ -69,6 +69,16 @@ def load() -> ctypes.CDLL:
     lib.orc_riemann_hydro.argtypes = [P, ctypes.c_int, P, P, P]
     lib.orc_riemann_mhd.argtypes = [P, ctypes.c_int, P, P, P, P, P, P]
     lib.orc_reconstruct.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, P, P, P]
+    D, I = ctypes.c_double, ctypes.c_int
+    lib.orc_helm_create.restype = P
+    lib.orc_helm_create.argtypes = [I, I, D, D, D, D, P, P]
+    lib.orc_helm_destroy.argtypes = [P]
+    lib.orc_helm_node.restype = D
+    lib.orc_helm_node.argtypes = [P, I, I]
+    lib.orc_helm_full_batch.argtypes = [P, I, P, P, P, P, P, P, P, P]
+    lib.orc_helm_rhoe_batch.argtypes = [P, I, P, P, P, P, P, P, P, I, P]
+    lib.orc_helm_rhoP_batch.argtypes = [P, I, P, P, P, P, P, P, P, I, P]
+    lib.orc_helm_interp.argtypes = [P, D, D, P]
     _lib = lib
     return lib
 
@@ -205,3 +215,64 @@ def run_problem(problem, nsteps: int, nthreads: int = 1) -> Dict[str, np.ndarray
               step=int(r.scalar("step")))
     r.close()
     return st
+
+
+class OracleHelm:
+    """CPU restatement of the Helmholtz EoS on a loaded table (HelmTable)."""
+
+    def __init__(self, table):
+        self.lib = load()
+        self.table = table
+        f9 = np.ascontiguousarray(table.f9, dtype=np.float64)
+        c4 = np.ascontiguousarray(table.c4, dtype=np.float64)
+        self.h = self.lib.orc_helm_create(table.nrho, table.nT, table.ldlo, table.ldhi,
+                                          table.ltlo, table.lthi, _p(f9), _p(c4))
+
+    def close(self):
+        if self.h:
+            self.lib.orc_helm_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def _arr(*xs):
+        n = max(np.size(x) for x in xs)
+        return [np.ascontiguousarray(np.broadcast_to(np.asarray(x, dtype=np.float64), (n,)))
+                for x in xs], n
+
+    def full(self, rho, T, abar, zbar):
+        (rho, T, abar, zbar), n = self._arr(rho, T, abar, zbar)
+        P, E, cs, cv = (np.empty(n) for _ in range(4))
+        self.lib.orc_helm_full_batch(self.h, n, _p(rho), _p(T), _p(abar), _p(zbar),
+                                     _p(P), _p(E), _p(cs), _p(cv))
+        return P, E, cs, cv
+
+    def rhoe_given(self, rho, e, abar, zbar, T0, ret=True):
+        (rho, e, abar, zbar, T), n = self._arr(rho, e, abar, zbar, T0)
+        T = T.copy()
+        P, cs = np.empty(n), np.empty(n)
+        it = np.zeros(n, dtype=np.int32)
+        self.lib.orc_helm_rhoe_batch(self.h, n, _p(rho), _p(e), _p(abar), _p(zbar), _p(T),
+                                     _p(P), _p(cs), int(ret),
+                                     it.ctypes.data_as(ctypes.c_void_p))
+        return T, P, cs, it
+
+    def rhoP_given(self, rho, p, abar, zbar, T0, ret=True):
+        (rho, p, abar, zbar, T), n = self._arr(rho, p, abar, zbar, T0)
+        T = T.copy()
+        E, cs = np.empty(n), np.empty(n)
+        it = np.zeros(n, dtype=np.int32)
+        self.lib.orc_helm_rhoP_batch(self.h, n, _p(rho), _p(p), _p(abar), _p(zbar), _p(T),
+                                     _p(E), _p(cs), int(ret),
+                                     it.ctypes.data_as(ctypes.c_void_p))
+        return T, E, cs, it
+
+    def interp(self, rhos, T):
+        out = np.empty(6)
+        self.lib.orc_helm_interp(self.h, float(rhos), float(T), _p(out))
+        return out
