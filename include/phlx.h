@@ -100,6 +100,16 @@ int phlx_set_profiles(phlx_ctx *ctx, const double *eq_rho_cc, const double *eq_p
                       const double *grav_cc, const double *phi_cc,
                       const double *phi_f, const double *edot_cc);
 
+/* USE_FASTEOS + USE_WB: equilibrium gamma_e = p_eq/e_eq + 1 at x2 cell
+ * centres (nx2) and x2 faces (nx2+1); the reference's eq_gammae_cc / _x1 /
+ * _x2 / _x3 (source/source.F90:1465-1478) reduced to plane-parallel
+ * profiles (x1 and x3 faces sit at cell-centre heights). */
+int phlx_set_eos_profiles(phlx_ctx *ctx, const double *eq_gammae_cc,
+                          const double *eq_gammae_f);
+/* gamma_e, gamma_c of the interior cells (the reference's gammaf(1:2,...)),
+ * host array [2][nx3][nx2][nx1]; Helmholtz contexts only. */
+int phlx_download_gammaf(phlx_ctx *ctx, double *gammaf);
+
 /* Step-0 part of time_loop (source/source.F90:6012-6062, 6271-6409):
  * b_cc from the face field, eint and T from (rho, p), then
  * compute_hyperbolic_dt.  Sets time = 0, step = 0. */
@@ -132,6 +142,38 @@ int phlx_sweep(phlx_ctx *ctx, int32_t dir);
 int phlx_ct(phlx_ctx *ctx, int32_t irk, double a1, double a2, double a3);
 int phlx_rk_update(phlx_ctx *ctx, int32_t irk, double a1, double a2, double a3,
                    int32_t last_stage);
+
+/* ---- Helmholtz EoS ----
+ * load_helm_table replaces load_helm_table (source/source.F90:23077-23156):
+ * f9 / c4 are the table's f block and dP/d(rho Ye) block in the reference
+ * file's column order (f, fd, ft, fdd, ftt, fdt, fddt, fdtt, fddtt and
+ * c, cd, ct, cdt), host memory laid out like the reference's
+ * helm_table_var(9, nrho, nT) in Fortran order (column fastest, then rho,
+ * then T).  Nodes are rho_i = 10**(ldlo + i*(ldhi-ldlo)/(nrho-1)) and
+ * T_j = 10**(ltlo + j*(lthi-ltlo)/(nT-1)) (:23094-23104).  The library keeps
+ * its own device copy; the host arrays may be freed on return. */
+int phlx_load_helm_table(phlx_ctx *ctx, const double *f9, const double *c4, int32_t nrho,
+                         int32_t nT, double ldlo, double ldhi, double ltlo, double lthi);
+/* Batched helm_rhoT_given_full (source/source.F90:23686-24232): n points,
+ * all pointers DEVICE memory of the context's GPU; P, E (specific internal
+ * energy, erg/g), sound speed (relativistic form) and c_v; any output may
+ * be NULL.  Asynchronous on the context stream. */
+int phlx_helm_rhoT_full_batch(phlx_ctx *ctx, int64_t n, const double *rho, const double *T,
+                              const double *abar, const double *zbar, double *P, double *E,
+                              double *sound, double *cv);
+/* Batched helm_rhoe_given (source/source.F90:24982-25019): Newton for
+ * T(rho, e) from the warm start in T (in/out), <= 100 iterations, tol 1e-11,
+ * T outside [10**ltlo, 10**lthi] resets to the start value.  Returns P and,
+ * when return_sound != 0, the sound speed (else sound = 0).  iters (may be
+ * NULL) receives the iterations taken per point.  Device pointers. */
+int phlx_helm_rhoe_given_batch(phlx_ctx *ctx, int64_t n, const double *rho, const double *e,
+                               const double *abar, const double *zbar, double *T, double *P,
+                               double *sound, int32_t return_sound, int32_t *iters);
+/* Batched helm_rhoP_given (source/source.F90:24943-24980): as above for
+ * T(rho, P); returns E. */
+int phlx_helm_rhoP_given_batch(phlx_ctx *ctx, int64_t n, const double *rho, const double *p,
+                               const double *abar, const double *zbar, double *T, double *E,
+                               double *sound, int32_t return_sound, int32_t *iters);
 
 /* Diagnostics: max over cells of |sum_d (b_d(+) - b_d(-))/dx_d|. */
 int phlx_divb_max(phlx_ctx *ctx, double *out);

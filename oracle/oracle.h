@@ -62,6 +62,16 @@ int orc_set_profiles(orc_grid *g, const double *eq_rho_cc, const double *eq_p_cc
                      const double *grav_cc, const double *phi_cc,
                      const double *phi_f, const double *edot_cc);
 
+/* FASTEOS well-balancing: equilibrium gamma_e at cell centres (nx2) and x2
+ * faces (nx2+1). */
+int orc_set_eos_profiles(orc_grid *g, const double *eq_gammae_cc, const double *eq_gammae_f);
+/* Helmholtz EoS (eos = ORC_EOS_HELMHOLTZ, nas = 2 with Ye, 1/abar): the table
+ * object is borrowed and must outlive the grid. */
+struct orc_helm;
+int orc_set_helm(orc_grid *g, const struct orc_helm *h);
+/* gamma_e, gamma_c of the interior cells, out[2][cell] (x fastest). */
+int orc_get_gammaf(const orc_grid *g, double *out);
+
 /* time_loop step-0 initialisation: b_cc from faces, eint/T from (rho,p),
  * compute_hyperbolic_dt (:6012-6062, :6271-6409). */
 int orc_init(orc_grid *g);

@@ -59,6 +59,9 @@ def load() -> ctypes.CDLL:
     lib.orc_set_state.argtypes = [P] * 7
     lib.orc_get_state.argtypes = [P] * 8
     lib.orc_set_profiles.argtypes = [P] * 9
+    lib.orc_set_eos_profiles.argtypes = [P] * 3
+    lib.orc_set_helm.argtypes = [P, P]
+    lib.orc_get_gammaf.argtypes = [P, P]
     lib.orc_init.argtypes = [P]
     lib.orc_run.argtypes = [P, ctypes.c_int]
     lib.orc_hydro_step.argtypes = [P]
@@ -118,7 +121,19 @@ class OracleRun:
         if problem.b_faces is not None:
             for d in range(self.cfg.sdims):
                 bs[d] = np.ascontiguousarray(problem.b_faces[d], dtype=np.float64)
-        self.lib.orc_set_state(self.g, _p(prim), None, None, _p(bs[0]), _p(bs[1]), _p(bs[2]))
+        temp = None if problem.temp is None else np.ascontiguousarray(problem.temp,
+                                                                      dtype=np.float64)
+        self.lib.orc_set_state(self.g, _p(prim), None, _p(temp), _p(bs[0]), _p(bs[1]),
+                               _p(bs[2]))
+        self.helm = None
+        if problem.helm is not None:
+            self.helm = OracleHelm(problem.helm)
+            self.lib.orc_set_helm(self.g, self.helm.h)
+        ge = [None if problem.profiles.get(n) is None else
+              np.ascontiguousarray(problem.profiles[n], dtype=np.float64)
+              for n in ("eq_gammae_cc", "eq_gammae_f")]
+        self._keep_ge = ge
+        self.lib.orc_set_eos_profiles(self.g, _p(ge[0]), _p(ge[1]))
         names = ["eq_rho_cc", "eq_p_cc", "eq_rho_f", "eq_p_f", "grav_cc", "phi_cc",
                  "phi_f", "edot_cc"]
         arrs = [None if problem.profiles.get(n) is None else
@@ -130,6 +145,9 @@ class OracleRun:
         if self.g:
             self.lib.orc_destroy(self.g)
             self.g = None
+        if getattr(self, "helm", None) is not None:
+            self.helm.close()
+            self.helm = None
 
     def __del__(self):
         try:
@@ -170,6 +188,10 @@ class OracleRun:
         for d in range(3):
             if bs[d] is not None:
                 out[f"b_x{d+1}"] = bs[d]
+        if self.helm is not None:
+            gf = np.empty((2,) + cfg.shape)
+            self.lib.orc_get_gammaf(self.g, _p(gf))
+            out["gammaf"] = gf
         return out
 
 

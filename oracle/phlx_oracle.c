@@ -46,6 +46,8 @@ struct orc_grid {
   int fe[3][3];                     /* extents of the ghosted face arrays */
   double *cor[3];                   /* corner EMFs along x1, x2, x3 */
   double *eq_rho_cc, *eq_p_cc, *eq_rho_f, *eq_p_f, *grav, *phi_cc, *phi_f, *edot;
+  double *eq_ge_cc, *eq_ge_f;       /* FASTEOS WB: equilibrium gamma_e */
+  const orc_helm *helm;             /* Helmholtz table (not owned) */
   double time, dt, smax;
   long step;
 };
@@ -138,6 +140,8 @@ orc_grid *orc_create(const orc_config *cfg) {
   g->phi_cc = (double *)calloc(ny, sizeof(double));
   g->phi_f = (double *)calloc(ny + 1, sizeof(double));
   g->edot = (double *)calloc(ny, sizeof(double));
+  g->eq_ge_cc = (double *)calloc(ny, sizeof(double));
+  g->eq_ge_f = (double *)calloc(ny + 1, sizeof(double));
 #ifdef _OPENMP
   if (cfg->nthreads > 0) omp_set_num_threads(cfg->nthreads);
 #endif
@@ -153,6 +157,7 @@ void orc_destroy(orc_grid *g) {
   }
   free(g->eq_rho_cc); free(g->eq_p_cc); free(g->eq_rho_f); free(g->eq_p_f);
   free(g->grav); free(g->phi_cc); free(g->phi_f); free(g->edot);
+  free(g->eq_ge_cc); free(g->eq_ge_f);
   free(g);
 }
 
@@ -216,6 +221,31 @@ int orc_set_profiles(orc_grid *g, const double *eq_rho_cc, const double *eq_p_cc
   if (phi_cc) memcpy(g->phi_cc, phi_cc, ny * sizeof(double));
   if (phi_f) memcpy(g->phi_f, phi_f, (ny + 1) * sizeof(double));
   if (edot_cc) memcpy(g->edot, edot_cc, ny * sizeof(double));
+  return 0;
+}
+
+int orc_set_eos_profiles(orc_grid *g, const double *eq_gammae_cc, const double *eq_gammae_f) {
+  int ny = g->n[1];
+  if (eq_gammae_cc) memcpy(g->eq_ge_cc, eq_gammae_cc, ny * sizeof(double));
+  if (eq_gammae_f) memcpy(g->eq_ge_f, eq_gammae_f, (ny + 1) * sizeof(double));
+  return 0;
+}
+
+int orc_set_helm(orc_grid *g, const orc_helm *h) {
+  g->helm = h;
+  return 0;
+}
+
+/* gamma_e, gamma_c of the interior cells, [2][cell] */
+int orc_get_gammaf(const orc_grid *g, double *out) {
+  long ni = (long)g->n[0] * g->n[1] * g->n[2];
+  for (int k = 0; k < g->n[2]; k++)
+    for (int j = 0; j < g->n[1]; j++)
+      for (int i = 0; i < g->n[0]; i++) {
+        long c = cidx(g, i, j, k), q = iidx(g, i, j, k);
+        out[q] = g->gamf[c * 2];
+        out[ni + q] = g->gamf[c * 2 + 1];
+      }
   return 0;
 }
 
@@ -933,6 +963,12 @@ static void sweep(orc_grid *g, int d) {
           qR[(long)f * nv] = qR[(long)f * nv] + rho_eq;
           qL[(long)f * nv + ip] = qL[(long)f * nv + ip] + p_eq;
           qR[(long)f * nv + ip] = qR[(long)f * nv + ip] + p_eq;
+          if (fasteos) {
+            /* x1 :8715-8727, x2 :10038-10048, x3 :11180-11190 */
+            double ge_eq = (d == 1) ? g->eq_ge_f[jy] : g->eq_ge_cc[jy];
+            gL[(long)f * 2] = gL[(long)f * 2] + ge_eq;
+            gR[(long)f * 2] = gR[(long)f * 2] + ge_eq;
+          }
         }
       /* Riemann */
       for (int f = 0; f <= n; f++) {
@@ -1206,9 +1242,11 @@ int orc_hydro_step(orc_grid *g) {
       for (int k = 0; k < n3; k++)
         for (int j = 0; j < n2; j++)
           for (int i = 0; i < n1; i++) {
-            double *p = g->prim + cidx(g, i, j, k) * nv;
+            long c = cidx(g, i, j, k);
+            double *p = g->prim + c * nv;
             p[0] = p[0] - g->eq_rho_cc[j];
             p[ip] = p[ip] - g->eq_p_cc[j];
+            if (g->c.fasteos) g->gamf[c * 2] = g->gamf[c * 2] - g->eq_ge_cc[j];
           }
     }
     halo_periodic(g, g->prim, nv);
@@ -1254,9 +1292,22 @@ int orc_hydro_step(orc_grid *g) {
           p[1] = m1 * inv_rho;
           p[2] = m2 * inv_rho;
           if (sd == 3) p[3] = m3 * inv_rho;
-          double pr = g->gmm1 * eint;
-          p[ip] = pr;
-          g->temp[c] = pr / (rho * CONST_RGAS * g->inv_mu);
+          if (g->c.eos == ORC_EOS_HELMHOLTZ) {
+            /* ADVECT_YE_IABAR + HELMHOLTZ_EOS + USE_FASTEOS (:13573-13578, :13648-13660) */
+            double ye = S[g->ias1] * inv_rho;
+            double abar = rho / S[g->ias1 + 1];
+            double zbar = ye * abar;
+            double T = g->temp[c], pr, sound;
+            orc_helm_rhoe_given(g->helm, rho, eint * inv_rho, abar, zbar, &T, &pr, &sound, 1);
+            g->gamf[c * 2] = pr / eint + 1.0;
+            g->gamf[c * 2 + 1] = sound * sound * rho / pr;
+            p[ip] = pr;
+            g->temp[c] = T;
+          } else {
+            double pr = g->gmm1 * eint;
+            p[ip] = pr;
+            g->temp[c] = pr / (rho * CONST_RGAS * g->inv_mu);
+          }
           g->eint[q] = eint;
           for (int v = g->ias1; v < nv; v++) p[v] = S[v] * inv_rho;
         }
@@ -1276,7 +1327,16 @@ int orc_compute_dt(orc_grid *g) {
         const double *p = g->prim + c * nv;
         double rho = p[0], vx1 = p[1], vx2 = p[2], vx3 = (sd == 3) ? p[3] : 0.0;
         double pr = p[g->ip];
-        double cs = sqrt(g->gm * pr / rho);
+        double cs;
+        if (g->c.eos == ORC_EOS_HELMHOLTZ) {
+          /* :13801-13847: full EoS at the carried T */
+          double ye = p[g->ias1];
+          double abar = 1.0 / p[g->ias1 + 1];
+          double zbar = ye * abar, ei, cv;
+          orc_helm_full(g->helm, rho, g->temp[c], abar, zbar, &pr, &ei, &cs, &cv);
+        } else {
+          cs = sqrt(g->gm * pr / rho);
+        }
         if (g->mhd) {
           const double *b = g->bcc + c * 3;
           double bx3 = (sd == 3) ? b[2] : 0.0;
@@ -1307,9 +1367,23 @@ int orc_init(orc_grid *g) {
       for (int i = 0; i < g->n[0]; i++) {
         long c = cidx(g, i, j, k), q = iidx(g, i, j, k);
         double rho = g->prim[c * nv], p = g->prim[c * nv + g->ip];
-        /* ideal gas branch of the step-0 EoS init (source.F90:6373-6376) */
-        g->temp[c] = p / (CONST_RGAS * rho * g->inv_mu);
-        g->eint[q] = p * g->igmm1;
+        if (g->c.eos == ORC_EOS_HELMHOLTZ) {
+          /* step-0 Helmholtz FASTEOS init (source.F90:6279-6285, :6351-6357) */
+          double abar = 1.0 / g->prim[c * nv + g->ias1 + 1];
+          double ye = g->prim[c * nv + g->ias1];
+          double zbar = abar * ye;
+          double T = g->temp[c], eint, sound;
+          orc_helm_rhoP_given(g->helm, rho, p, abar, zbar, &T, &eint, &sound, 1);
+          eint = rho * eint;
+          g->gamf[c * 2] = p / eint + 1.0;
+          g->gamf[c * 2 + 1] = sound * sound * rho / p;
+          g->eint[q] = eint;
+          g->temp[c] = T;
+        } else {
+          /* ideal gas branch of the step-0 EoS init (source.F90:6373-6376) */
+          g->temp[c] = p / (CONST_RGAS * rho * g->inv_mu);
+          g->eint[q] = p * g->igmm1;
+        }
       }
   g->time = 0.0;
   g->step = 0;

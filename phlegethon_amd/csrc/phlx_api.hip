@@ -2,6 +2,7 @@
 // host exchange, stepping and profiling.
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <new>
@@ -28,6 +29,11 @@ struct phlx_ctx {
   bool initialized = false;
   std::vector<void *> allocs;
   double *scratch_val = nullptr;
+  // Helmholtz table (device records + node arrays), owned separately so a
+  // table can be replaced
+  void *helm_buf = nullptr;
+  HelmDev helm{};
+  bool helm_loaded = false;
   // profiling
   bool prof = false;
   std::vector<ProfRec> recs;
@@ -64,6 +70,10 @@ extern "C" int phlx_version(void) { return 100; }
 // Validation of the supported option subset, mirroring the reference's
 // Make.checks rules that apply (exactly one stepper/recon/flux, HLLD
 // requires MHD, periodic in pairs, FASTEOS requires a complex EoS).
+// plane-parallel profile slots: eq_rho_cc, eq_p_cc, eq_rho_f, eq_p_f, grav,
+// phi_cc, phi_f, edot, eq_gammae_cc, eq_gammae_f; each ny+1 doubles
+constexpr int kNumProfiles = 10;
+
 static int validate(const phlx_config *c) {
   if (c->sdims != 2 && c->sdims != 3) return PHLX_E_CONFIG;
   if (c->nx1 < 2 || c->nx2 < 2 || (c->sdims == 3 && c->nx3 < 2)) return PHLX_E_CONFIG;
@@ -78,8 +88,15 @@ static int validate(const phlx_config *c) {
     if ((lo == PHLX_BC_PERIODIC) != (hi == PHLX_BC_PERIODIC)) return PHLX_E_CONFIG;
     if (mhd && (lo == PHLX_BC_DIODE || hi == PHLX_BC_DIODE)) return PHLX_E_CONFIG;
   }
-  if (c->eos != PHLX_EOS_IDEAL) return PHLX_E_CONFIG;  // Helmholtz: separate entry points
-  if (c->fasteos) return PHLX_E_CONFIG;
+  // Helmholtz runs use USE_FASTEOS with ADVECT_YE_IABAR (Ye, 1/abar as the
+  // two advected scalars); FASTEOS needs a complex EoS (Make.checks)
+  if (c->eos == PHLX_EOS_HELMHOLTZ) {
+    if (!c->fasteos || c->nas != 2) return PHLX_E_CONFIG;
+  } else if (c->eos == PHLX_EOS_IDEAL) {
+    if (c->fasteos) return PHLX_E_CONFIG;
+  } else {
+    return PHLX_E_CONFIG;
+  }
   if (c->precision != PHLX_PREC_FP64) return PHLX_E_CONFIG;
   if (c->cfl <= 0.0 || c->gamma <= 1.0 || c->mu <= 0.0) return PHLX_E_CONFIG;
   for (int r = 0; r < 8; r++)
@@ -161,7 +178,8 @@ extern "C" int phlx_create(const phlx_config *cfg, phlx_ctx **out) {
     ctx->rk[1][0] = 0.75; ctx->rk[1][1] = 0.25; ctx->rk[1][2] = -0.25;
     ctx->rk[2][0] = 1.0 / 3.0; ctx->rk[2][1] = 2.0 / 3.0; ctx->rk[2][2] = -(2.0 / 3.0);
   }
-  if (!select_kernels(g.sd, g.nas, g.mhd, ctx->kt)) {
+  const bool helm = cfg->eos == PHLX_EOS_HELMHOLTZ;
+  if (!select_kernels(g.sd, g.nas, g.mhd, helm, ctx->kt)) {
     phlx_destroy(ctx);
     return PHLX_E_CONFIG;
   }
@@ -187,9 +205,13 @@ extern "C" int phlx_create(const phlx_config *cfg, phlx_ctx **out) {
       e |= dalloc(ctx, &g.cor[1], (n1 + 1) * n2 * (n3 + 1));
     }
   }
+  if (helm) {
+    e |= dalloc(ctx, &g.temp, N);
+    e |= dalloc(ctx, &g.gam, 2 * N);
+  }
   double *prof = nullptr;
   const size_t ny = g.n2;
-  e |= dalloc(ctx, &prof, 8 * (ny + 1));
+  e |= dalloc(ctx, &prof, kNumProfiles * (ny + 1));
   e |= dalloc(ctx, &g.sc, 1);
   e |= dalloc(ctx, &ctx->scratch_val, 1);
   g.nred = cell_grid_blocks(g);
@@ -206,6 +228,8 @@ extern "C" int phlx_create(const phlx_config *cfg, phlx_ctx **out) {
   g.phi_cc = prof + 5 * (ny + 1);
   g.phi_f = prof + 6 * (ny + 1);
   g.edot_cc = prof + 7 * (ny + 1);
+  g.eq_ge_cc = prof + 8 * (ny + 1);
+  g.eq_ge_f = prof + 9 * (ny + 1);
   Scalars s0;
   std::memset(&s0, 0, sizeof(s0));
   s0.tmax = cfg->tmax;
@@ -229,6 +253,7 @@ extern "C" int phlx_destroy(phlx_ctx *ctx) {
     (void)hipEventDestroy(r.b);
   }
   for (void *p : ctx->allocs) (void)hipFree(p);
+  if (ctx->helm_buf) (void)hipFree(ctx->helm_buf);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return PHLX_OK;
@@ -257,7 +282,9 @@ extern "C" int phlx_upload_state(phlx_ctx *ctx, const double *prim, const double
   int rc = 0;
   if (prim) rc |= copy_in(ctx, g.P, prim, sizeof(double) * g.nv * g.N);
   if (eint) rc |= copy_in(ctx, g.eint, eint, sizeof(double) * g.N);
-  (void)temp;  // the ideal-gas EoS carries no temperature on the device
+  // the ideal-gas EoS carries no temperature on the device; Helmholtz runs
+  // take it as the warm start of the step-0 T(rho, p) solve
+  if (temp && g.temp) rc |= copy_in(ctx, g.temp, temp, sizeof(double) * g.N);
   const double *bs[3] = {b1, b2, b3};
   for (int d = 0; d < g.sd; d++)
     if (bs[d]) {
@@ -277,7 +304,9 @@ extern "C" int phlx_download_state(phlx_ctx *ctx, double *prim, double *eint, do
   int rc = 0;
   if (prim) rc |= copy_out(ctx, prim, g.P, sizeof(double) * g.nv * g.N);
   if (eint) rc |= copy_out(ctx, eint, g.eint, sizeof(double) * g.N);
-  if (temp) {
+  if (temp && g.temp) {
+    rc |= copy_out(ctx, temp, g.temp, sizeof(double) * g.N);
+  } else if (temp) {
     // T = p / (rho R / mu) for the ideal gas (source/source.F90:13670-13674)
     std::vector<double> p(2 * g.N);
     rc |= copy_out(ctx, p.data(), g.P, sizeof(double) * g.N);
@@ -343,6 +372,31 @@ extern "C" int phlx_download_prim_aos(phlx_ctx *ctx, double *prim, int32_t ngc) 
         const long long c = ((long long)k * g.n2 + j) * g.n1 + i;
         for (int v = 0; v < g.nv; v++) prim[a + v] = soa[(size_t)v * g.N + c];
       }
+  return PHLX_OK;
+}
+
+extern "C" int phlx_set_eos_profiles(phlx_ctx *ctx, const double *eq_gammae_cc,
+                                     const double *eq_gammae_f) {
+  if (!ctx) return PHLX_E_ARG;
+  HIPCHK(hipSetDevice(ctx->device));
+  Grid &g = ctx->g;
+  const size_t ny = g.n2;
+  std::vector<double> h(2 * (ny + 1), 0.0);
+  if (eq_gammae_cc) std::memcpy(h.data(), eq_gammae_cc, ny * sizeof(double));
+  if (eq_gammae_f) std::memcpy(h.data() + (ny + 1), eq_gammae_f, (ny + 1) * sizeof(double));
+  HIPCHK(hipMemcpyAsync((void *)g.eq_ge_cc, h.data(), h.size() * sizeof(double),
+                        hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return PHLX_OK;
+}
+
+extern "C" int phlx_download_gammaf(phlx_ctx *ctx, double *gammaf) {
+  if (!ctx || !gammaf) return PHLX_E_ARG;
+  if (!ctx->g.gam) return PHLX_E_STATE;
+  HIPCHK(hipSetDevice(ctx->device));
+  HIPCHK(hipMemcpyAsync(gammaf, ctx->g.gam, sizeof(double) * 2 * ctx->g.N,
+                        hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
   return PHLX_OK;
 }
 
@@ -488,6 +542,7 @@ static int do_hydro_step(phlx_ctx *ctx, bool with_cfl) {
 
 extern "C" int phlx_init(phlx_ctx *ctx) {
   if (!ctx) return PHLX_E_ARG;
+  if (ctx->cfg.eos == PHLX_EOS_HELMHOLTZ && !ctx->helm_loaded) return PHLX_E_TABLE;
   HIPCHK(hipSetDevice(ctx->device));
   Scalars s0;
   std::memset(&s0, 0, sizeof(s0));
@@ -625,4 +680,111 @@ extern "C" int phlx_stream_copy(phlx_ctx *ctx, int64_t nbytes, int32_t reps, dou
   (void)hipFree(a);
   (void)hipFree(b);
   return rc;
+}
+
+// ------------------------------------------------------------- Helmholtz EoS
+
+extern "C" int phlx_load_helm_table(phlx_ctx *ctx, const double *f9, const double *c4,
+                                    int32_t nrho, int32_t nT, double ldlo, double ldhi,
+                                    double ltlo, double lthi) {
+  if (!ctx || !f9 || !c4) return PHLX_E_ARG;
+  if (nrho < 2 || nT < 2 || !(ldhi > ldlo) || !(lthi > ltlo)) return PHLX_E_TABLE;
+  HIPCHK(hipSetDevice(ctx->device));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  const size_t np = (size_t)nrho * nT;
+  for (size_t p = 0; p < 9 * np; p++)
+    if (!std::isfinite(f9[p])) return PHLX_E_TABLE;
+  for (size_t p = 0; p < 4 * np; p++)
+    if (!std::isfinite(c4[p])) return PHLX_E_TABLE;
+  // node values computed on the host exactly as the reference builds its
+  // axes (source/source.F90:23094-23104)
+  HelmDev h{};
+  h.nr = nrho;
+  h.nt = nT;
+  h.dt = (lthi - ltlo) / (double)(nT - 1);
+  h.dr = (ldhi - ldlo) / (double)(nrho - 1);
+  std::vector<double> nodes((size_t)nrho + nT);
+  for (int i = 0; i < nrho; i++) nodes[i] = std::pow(10.0, ldlo + (double)i * h.dr);
+  for (int j = 0; j < nT; j++) nodes[nrho + j] = std::pow(10.0, ltlo + (double)j * h.dt);
+  h.tl = std::pow(10.0, ltlo);
+  h.th = std::pow(10.0, lthi);
+  h.lrho1 = std::log10(nodes[0]);
+  h.lT1 = std::log10(nodes[nrho]);
+  // one allocation: records | nodes | staging of the file-order blocks
+  const size_t rec_b = sizeof(double) * kHelmRec * np;
+  const size_t node_b = sizeof(double) * nodes.size();
+  const size_t stage_b = sizeof(double) * 13 * np;
+  void *buf = nullptr;
+  if (hipMalloc(&buf, rec_b + node_b + stage_b + 512) != hipSuccess) return PHLX_E_NOMEM;
+  char *base = (char *)buf;
+  double *tab = (double *)base;
+  double *nd = (double *)(base + rec_b);
+  double *st9 = (double *)(base + ((rec_b + node_b + 255) / 256) * 256);
+  double *st4 = st9 + 9 * np;
+  int rc = PHLX_OK;
+  if (hipMemcpyAsync(nd, nodes.data(), node_b, hipMemcpyHostToDevice, ctx->stream) != hipSuccess ||
+      hipMemcpyAsync(st9, f9, sizeof(double) * 9 * np, hipMemcpyHostToDevice, ctx->stream) !=
+          hipSuccess ||
+      hipMemcpyAsync(st4, c4, sizeof(double) * 4 * np, hipMemcpyHostToDevice, ctx->stream) !=
+          hipSuccess) {
+    rc = PHLX_E_HIP;
+  } else {
+    launch_helm_pack((long long)np, st9, st4, tab, ctx->stream);
+    rc = check_launch();
+    if (!rc && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = PHLX_E_HIP;
+  }
+  if (rc) {
+    (void)hipFree(buf);
+    return rc;
+  }
+  if (ctx->helm_buf) (void)hipFree(ctx->helm_buf);
+  ctx->helm_buf = buf;
+  h.tab = tab;
+  h.rho = nd;
+  h.T = nd + nrho;
+  ctx->helm = h;
+  ctx->g.helm = h;
+  ctx->helm_loaded = true;
+  return PHLX_OK;
+}
+
+extern "C" int phlx_helm_rhoT_full_batch(phlx_ctx *ctx, int64_t n, const double *rho,
+                                         const double *T, const double *abar,
+                                         const double *zbar, double *P, double *E,
+                                         double *sound, double *cv) {
+  if (!ctx || n < 0) return PHLX_E_ARG;
+  if (!ctx->helm_loaded) return PHLX_E_TABLE;
+  if (n == 0) return PHLX_OK;
+  if (!rho || !T || !abar || !zbar) return PHLX_E_ARG;
+  HIPCHK(hipSetDevice(ctx->device));
+  launch_helm_full(ctx->helm, n, rho, T, abar, zbar, P, E, sound, cv, ctx->stream);
+  return check_launch();
+}
+
+extern "C" int phlx_helm_rhoe_given_batch(phlx_ctx *ctx, int64_t n, const double *rho,
+                                          const double *e, const double *abar,
+                                          const double *zbar, double *T, double *P,
+                                          double *sound, int32_t return_sound, int32_t *iters) {
+  if (!ctx || n < 0) return PHLX_E_ARG;
+  if (!ctx->helm_loaded) return PHLX_E_TABLE;
+  if (n == 0) return PHLX_OK;
+  if (!rho || !e || !abar || !zbar || !T) return PHLX_E_ARG;
+  HIPCHK(hipSetDevice(ctx->device));
+  launch_helm_rhoe(ctx->helm, n, rho, e, abar, zbar, T, P, sound, return_sound, iters,
+                   ctx->stream);
+  return check_launch();
+}
+
+extern "C" int phlx_helm_rhoP_given_batch(phlx_ctx *ctx, int64_t n, const double *rho,
+                                          const double *p, const double *abar,
+                                          const double *zbar, double *T, double *E,
+                                          double *sound, int32_t return_sound, int32_t *iters) {
+  if (!ctx || n < 0) return PHLX_E_ARG;
+  if (!ctx->helm_loaded) return PHLX_E_TABLE;
+  if (n == 0) return PHLX_OK;
+  if (!rho || !p || !abar || !zbar || !T) return PHLX_E_ARG;
+  HIPCHK(hipSetDevice(ctx->device));
+  launch_helm_rhoP(ctx->helm, n, rho, p, abar, zbar, T, E, sound, return_sound, iters,
+                   ctx->stream);
+  return check_launch();
 }

@@ -15,6 +15,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "phlx_helm.hpp"
+
 namespace phlx {
 
 constexpr double kRgas = 8.31446261815324e7;  // source/source.F90:605
@@ -48,6 +50,11 @@ struct Grid {
   double *cor[3];          // corner (edge) EMFs along x1, x2, x3
   const double *eq_rho_cc, *eq_p_cc, *eq_rho_f, *eq_p_f, *gravy, *phi_cc, *phi_f,
       *edot_cc;
+  // Helmholtz FASTEOS: carried temperature (warm start of the Newton solve)
+  // and [gamma_e | gamma_c] per cell; WB equilibrium gamma_e profiles
+  double *temp, *gam;
+  const double *eq_ge_cc, *eq_ge_f;
+  HelmDev helm;
   double *red;       // per-block partial maxima of the cell-grid reductions
   int nred;          // number of partials (blocks of a cell-grid launch)
   Scalars *sc;
@@ -100,9 +107,9 @@ __device__ __forceinline__ bool flips(const Grid &g, int a, int side, long long 
 
 // Load primitive state (WB deviation when g.wb) and cell-centred B of a
 // possibly-ghost cell.
-template <int SD, int NV, bool MHD>
+template <int SD, int NV, bool MHD, bool GAM = false>
 __device__ __forceinline__ void load_cell(const Grid &g, int i, int j, int k, double *q,
-                                          double *b) {
+                                          double *b, double *ga = nullptr) {
   int si, sj, sk = 0;
   int mi = map_axis(g, 0, i, g.n1, si);
   int mj = map_axis(g, 1, j, g.n2, sj);
@@ -114,6 +121,13 @@ __device__ __forceinline__ void load_cell(const Grid &g, int i, int j, int k, do
   if (g.wb) {
     q[0] = q[0] - g.eq_rho_cc[mj];
     q[SD + 1] = q[SD + 1] - g.eq_p_cc[mj];
+  }
+  if (GAM) {
+    // gamma_e (WB deviation, source/source.F90:7160-7162) and gamma_c; the
+    // ghost copies carry no sign flip
+    ga[0] = g.gam[c];
+    ga[1] = g.gam[N + c];
+    if (g.wb) ga[0] = ga[0] - g.eq_ge_cc[mj];
   }
   if (MHD) {
 #pragma unroll
@@ -182,7 +196,7 @@ __device__ __forceinline__ void rec_pph_lim(double qm, double qc, double qp, dou
 // ------------------------------------------------------------ Riemann: hydro
 // HLLC / LHLLC in the face frame (slot 1 = normal velocity).
 // source/source.F90:15611-16260.  g_e/g_c: FASTEOS gamma_e, gamma_c.
-template <int SD, int NAS>
+template <int SD, int NAS, bool FE = false>
 __device__ __forceinline__ void riemann_hydro(const Grid &g, const double *qL,
                                               const double *qR, const double *gL,
                                               const double *gR, double *F) {
@@ -192,7 +206,7 @@ __device__ __forceinline__ void riemann_hydro(const Grid &g, const double *qL,
   const double v2L = uL * uL + vL * vL + wL * wL;
   const double pL = qL[IE];
   double eL, cL, eR, cR;
-  if (g.fasteos) {
+  if (FE) {
     eL = pL / (gL[0] - 1.0);
     cL = sqrt(gL[1] * pL * irL);
   } else {
@@ -204,7 +218,7 @@ __device__ __forceinline__ void riemann_hydro(const Grid &g, const double *qL,
   const double uR = qR[1], vR = qR[2], wR = (SD == 3) ? qR[3] : 0.0;
   const double v2R = uR * uR + vR * vR + wR * wR;
   const double pR = qR[IE];
-  if (g.fasteos) {
+  if (FE) {
     eR = pR / (gR[0] - 1.0);
     cR = sqrt(gR[1] * pR * irR);
   } else {
@@ -296,7 +310,7 @@ __device__ __forceinline__ void riemann_hydro(const Grid &g, const double *qL,
 // -------------------------------------------------------------- Riemann: MHD
 // HLLD / LHLLD in the face frame; b = (b_n, b_t1, b_t2).  Returns the
 // induction fluxes Fb (Fb[0] = 0).  source/source.F90:13969-15024.
-template <int SD, int NAS>
+template <int SD, int NAS, bool FE = false>
 __device__ __forceinline__ void riemann_mhd(const Grid &g, const double *qL,
                                             const double *qR, const double *bL,
                                             const double *bR, const double *gL,
@@ -307,7 +321,7 @@ __device__ __forceinline__ void riemann_mhd(const Grid &g, const double *qL,
   const double v2L = uL * uL + vL * vL + wL * wL;
   const double pL = qL[IE];
   double e, c2L, c2R;
-  if (g.fasteos) {
+  if (FE) {
     e = pL / (gL[0] - 1.0);
     c2L = gL[1] * pL * irL;
   } else {
@@ -320,7 +334,7 @@ __device__ __forceinline__ void riemann_mhd(const Grid &g, const double *qL,
   const double uR = qR[1], vR = qR[2], wR = (SD == 3) ? qR[3] : 0.0;
   const double v2R = uR * uR + vR * vR + wR * wR;
   const double pR = qR[IE];
-  if (g.fasteos) {
+  if (FE) {
     e = pR / (gR[0] - 1.0);
     c2R = gR[1] * pR * irR;
   } else {

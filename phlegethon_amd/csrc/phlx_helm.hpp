@@ -1,0 +1,334 @@
+// phlx_helm.hpp -- device Helmholtz EoS: biquintic Hermite interpolation of
+// the electron-positron free energy f(rho*Ye, T), bicubic dP/d(rho*Ye), ions
+// and radiation, and the Newton inversions T(rho, e) and T(rho, P).
+//
+// Reference: helm_rhoT_given_full (source/source.F90:23686-24232),
+// helm_rhoT_given1 (:24234-24569), helm_rhoT_given2 (:24571-24941),
+// helm_rhoP_given (:24943-24980), helm_rhoe_given (:24982-25019).
+// Coulomb corrections are off in the reference build and are not carried.
+//
+// Table layout in HBM: one 128-byte record per node, [jT][irho][16]:
+//   0 f, 1 df/dT, 2 d2f/dT2, 3 df/drho, 4 d2f/drho2, 5 d2f/drhodT,
+//   6 d3f/drho2dT, 7 d3f/drhodT2, 8 d4f/drho2dT2,
+//   9 c, 10 dc/dT, 11 dc/drho, 12 d2c/drhodT   (c = dP/d(rho*Ye)), 13-15 pad.
+// The four corner records of a cell are two adjacent pairs, so an evaluation
+// touches four 128-byte lines; the whole 541x271 table (18.8 MB) stays in the
+// last-level cache once the working set of a sweep has touched it.
+// Arithmetic is written term by term in the reference's order (no
+// contraction) so that results are reproducible bit for bit on any host.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace phlx {
+
+constexpr double kHelmRgas = 8.31446261815324e7;  // source/source.F90:605-616
+constexpr double kHelmCrad = 7.565767381646406e-15;
+constexpr double kHelmClight = 2.99792458e10;
+constexpr double kHelmClight2 = 8.987551787368177e20;
+constexpr int kHelmRec = 16;
+constexpr int kHelmMaxIter = 100;   // source/source.F90:24955, :24994
+constexpr double kHelmTol = 1e-11;  // source/source.F90:754
+
+struct HelmDev {
+  const double *tab;  // [nt][nr][16]
+  const double *rho;  // node values, nr
+  const double *T;    // node values, nt
+  int nr, nt;
+  double lrho1, lT1;  // log10 of the first nodes
+  double dr, dt;      // log10 spacings
+  double tl, th;      // Tl_eos, Th_eos (source/source.F90:750-751)
+};
+
+struct HelmCell {
+  int ih, jh;
+  double x, y, drho, dT;
+};
+
+struct HelmBasis {
+  double p0, p1, p2, p0m, p1m, p2m;
+};
+
+// cell index int((log10(rhoYe) - log10(rho_1)) / dlog) (source/source.F90
+// :23773-23784), kept inside the table.
+__device__ __forceinline__ HelmCell helm_locate(const HelmDev &h, double rhos, double T) {
+  HelmCell c;
+  c.ih = (int)((log10(rhos) - h.lrho1) / h.dr);
+  c.jh = (int)((log10(T) - h.lT1) / h.dt);
+  c.ih = c.ih < 0 ? 0 : (c.ih > h.nr - 2 ? h.nr - 2 : c.ih);
+  c.jh = c.jh < 0 ? 0 : (c.jh > h.nt - 2 ? h.nt - 2 : c.jh);
+  double tmp = h.rho[c.ih];
+  c.drho = h.rho[c.ih + 1] - tmp;
+  c.x = (rhos - tmp) / c.drho;
+  tmp = h.T[c.jh];
+  c.dT = h.T[c.jh + 1] - tmp;
+  c.y = (T - tmp) / c.dT;
+  return c;
+}
+
+// Quintic Hermite basis of one coordinate and its first / second derivative
+// (source/source.F90:23832-23862, :23900-23906, :24030-24036).
+__device__ __forceinline__ void helm_basis(double x, double d, HelmBasis &v) {
+  const double d2 = d * d;
+  const double x2 = x * x, x3 = x2 * x, x4 = x3 * x, x5 = x4 * x;
+  const double o = 1.0 - x;
+  const double o2 = o * o, o3 = o2 * o, o4 = o3 * o, o5 = o4 * o;
+  v.p0 = -6.0 * x5 + 15.0 * x4 - 10.0 * x3 + 1.0;
+  v.p1 = (-3.0 * x5 + 8.0 * x4 - 6.0 * x3 + x) * d;
+  v.p2 = (0.5 * (-x5 + 3.0 * x4 - 3.0 * x3 + x2)) * d2;
+  v.p0m = -6.0 * o5 + 15.0 * o4 - 10.0 * o3 + 1.0;
+  v.p1m = (3.0 * o5 - 8.0 * o4 + 6.0 * o3 - o) * d;
+  v.p2m = (0.5 * (-o5 + 3.0 * o4 - 3.0 * o3 + o2)) * d2;
+}
+
+__device__ __forceinline__ void helm_dbasis(double x, double d, HelmBasis &v) {
+  const double id = 1.0 / d;
+  const double x2 = x * x, x3 = x2 * x, x4 = x3 * x;
+  const double o = 1.0 - x;
+  const double o2 = o * o, o3 = o2 * o, o4 = o3 * o;
+  v.p0 = (-30.0 * x4 + 60.0 * x3 - 30.0 * x2) * id;
+  v.p1 = (-15.0 * x4 + 32.0 * x3 - 18.0 * x2 + 1.0);
+  v.p2 = (0.5 * (-5.0 * x4 + 12.0 * x3 - 9.0 * x2 + 2.0 * x)) * d;
+  v.p0m = -(-30.0 * o4 + 60.0 * o3 - 30.0 * o2) * id;
+  v.p1m = (-15.0 * o4 + 32.0 * o3 - 18.0 * o2 + 1.0);
+  v.p2m = -(0.5 * (-5.0 * o4 + 12.0 * o3 - 9.0 * o2 + 2.0 * o)) * d;
+}
+
+__device__ __forceinline__ void helm_ddbasis(double x, double d, HelmBasis &v) {
+  const double id = 1.0 / d, id2 = id * id;
+  const double x2 = x * x, x3 = x2 * x;
+  const double o = 1.0 - x;
+  const double o2 = o * o, o3 = o2 * o;
+  v.p0 = (-120.0 * x3 + 180.0 * x2 - 60.0 * x) * id2;
+  v.p1 = (-60.0 * x3 + 96.0 * x2 - 36.0 * x) * id;
+  v.p2 = (0.5 * (-20.0 * x3 + 36.0 * x2 - 18.0 * x + 2.0));
+  v.p0m = (-120.0 * o3 + 180.0 * o2 - 60.0 * o) * id2;
+  v.p1m = -(-60.0 * o3 + 96.0 * o2 - 36.0 * o) * id;
+  v.p2m = (0.5 * (-20.0 * o3 + 36.0 * o2 - 18.0 * o + 2.0));
+}
+
+// The 36-term biquintic sum in the reference's term order: entry v pairs the
+// rho basis of order (0,0,0,1,2,1,2,1,2)[v] with the T basis of order
+// (0,1,2,0,0,1,1,2,2)[v]; per entry the corners (i,j) (i+1,j) (i,j+1)
+// (i+1,j+1) are added in that order.
+__device__ __forceinline__ double helm_sum36(const HelmDev &h, const HelmCell &c,
+                                             const HelmBasis &r, const HelmBasis &t) {
+  const double *n00 = h.tab + (size_t)kHelmRec * ((size_t)c.jh * h.nr + c.ih);
+  const double *n10 = n00 + kHelmRec;
+  const double *n01 = n00 + (size_t)kHelmRec * h.nr;
+  const double *n11 = n01 + kHelmRec;
+  const double rb[3] = {r.p0, r.p1, r.p2}, rbm[3] = {r.p0m, r.p1m, r.p2m};
+  const double tb[3] = {t.p0, t.p1, t.p2}, tbm[3] = {t.p0m, t.p1m, t.p2m};
+  constexpr int RX[9] = {0, 0, 0, 1, 2, 1, 2, 1, 2};
+  constexpr int TY[9] = {0, 1, 2, 0, 0, 1, 1, 2, 2};
+  double s = 0.0;
+#pragma unroll
+  for (int v = 0; v < 9; v++) {
+    const double a = rb[RX[v]], am = rbm[RX[v]], b = tb[TY[v]], bm = tbm[TY[v]];
+    const double t0 = n00[v] * a * b, t1 = n10[v] * am * b;
+    const double t2 = n01[v] * a * bm, t3 = n11[v] * am * bm;
+    s = (v == 0) ? t0 : s + t0;
+    s = s + t1;
+    s = s + t2;
+    s = s + t3;
+  }
+  return s;
+}
+
+// Bicubic Hermite of the dP/d(rho*Ye) block (source/source.F90:24078-24125).
+__device__ __forceinline__ double helm_sum16(const HelmDev &h, const HelmCell &c) {
+  const double x = c.x, y = c.y, o = 1.0 - x, oy = 1.0 - y;
+  const double x2 = x * x, x3 = x2 * x, o2 = o * o, o3 = o2 * o;
+  const double y2 = y * y, y3 = y2 * y, oy2 = oy * oy, oy3 = oy2 * oy;
+  const double p0r = 2.0 * x3 - 3.0 * x2 + 1.0;
+  const double p1r = (x3 - 2.0 * x2 + x) * c.drho;
+  const double p0mr = 2.0 * o3 - 3.0 * o2 + 1.0;
+  const double p1mr = -(o3 - 2.0 * o2 + o) * c.drho;
+  const double p0t = 2.0 * y3 - 3.0 * y2 + 1.0;
+  const double p1t = (y3 - 2.0 * y2 + y) * c.dT;
+  const double p0mt = 2.0 * oy3 - 3.0 * oy2 + 1.0;
+  const double p1mt = -(oy3 - 2.0 * oy2 + oy) * c.dT;
+  const double *n00 = h.tab + (size_t)kHelmRec * ((size_t)c.jh * h.nr + c.ih) + 9;
+  const double *n10 = n00 + kHelmRec;
+  const double *n01 = n00 + (size_t)kHelmRec * h.nr;
+  const double *n11 = n01 + kHelmRec;
+  const double rb[4] = {p0r, p0r, p1r, p1r}, rbm[4] = {p0mr, p0mr, p1mr, p1mr};
+  const double tb[4] = {p0t, p1t, p0t, p1t}, tbm[4] = {p0mt, p1mt, p0mt, p1mt};
+  double s = 0.0;
+#pragma unroll
+  for (int v = 0; v < 4; v++) {
+    const double t0 = n00[v] * rb[v] * tb[v], t1 = n10[v] * rbm[v] * tb[v];
+    const double t2 = n01[v] * rb[v] * tbm[v], t3 = n11[v] * rbm[v] * tbm[v];
+    s = (v == 0) ? t0 : s + t0;
+    s = s + t1;
+    s = s + t2;
+    s = s + t3;
+  }
+  return s;
+}
+
+// helm_rhoT_given_full: P, E (specific), sound speed (relativistic form), c_v.
+__device__ __forceinline__ void helm_full(const HelmDev &h, double rho, double T, double abar,
+                                          double zbar, double &P, double &E, double &sound,
+                                          double &cv) {
+  const double Ye = zbar / abar, rhos = rho * Ye;
+  const HelmCell c = helm_locate(h, rhos, T);
+  HelmBasis r, dr, t, dt, ddt;
+  helm_basis(c.x, c.drho, r);
+  helm_dbasis(c.x, c.drho, dr);
+  helm_basis(c.y, c.dT, t);
+  helm_dbasis(c.y, c.dT, dt);
+  helm_ddbasis(c.y, c.dT, ddt);
+  const double f = helm_sum36(h, c, r, t);
+  const double df_drho = helm_sum36(h, c, dr, t);
+  const double df_dT = helm_sum36(h, c, r, dt);
+  const double df_drhodT = helm_sum36(h, c, dr, dt);
+  const double df_dT2 = helm_sum36(h, c, r, ddt);
+  double dP_drho = helm_sum16(h, c);
+  dP_drho = Ye * dP_drho;
+  double tmp = rhos * rhos;
+  double p = tmp * df_drho;
+  double dP_dT = tmp * df_drhodT;
+  const double s = -Ye * df_dT;
+  const double ds_dT = -Ye * df_dT2;
+  double e = Ye * f + T * s;
+  double dE_dT = T * ds_dT;
+  tmp = 1.0 / abar;
+  const double T2 = T * T, T3 = T2 * T, T4 = T3 * T;
+  const double pion = kHelmRgas * rho * T * tmp;
+  const double dpiondt = kHelmRgas * rho * tmp;
+  const double dpiondd = kHelmRgas * T * tmp;
+  p = p + pion + kHelmCrad * T4 * (1.0 / 3.0);
+  dP_dT = dP_dT + dpiondt + (4.0 / 3.0) * kHelmCrad * T3;
+  dP_drho = dP_drho + dpiondd;
+  e = e + 1.5 * kHelmRgas * T * tmp + kHelmCrad * T4 / rho;
+  dE_dT = dE_dT + 1.5 * kHelmRgas * tmp + 4.0 * kHelmCrad * T3 / rho;
+  const double zz = p / rho;
+  const double chiT = T / p * dP_dT;
+  const double chirho = dP_drho / zz;
+  const double x = zz * chiT / (T * dE_dT);
+  const double gam1 = chiT * x + chirho;
+  const double z = 1.0 + (e + kHelmClight2) / zz;
+  P = p;
+  E = e;
+  cv = dE_dT;
+  sound = kHelmClight * sqrt(gam1 / z);
+}
+
+// helm_rhoT_given1: P and (when ret) dP/dT.
+__device__ __forceinline__ void helm_given1(const HelmDev &h, double rho, double T, double abar,
+                                            double zbar, double &P, double &dPdT, bool ret) {
+  const double Ye = zbar / abar, rhos = rho * Ye;
+  const HelmCell c = helm_locate(h, rhos, T);
+  HelmBasis dr, t, dt;
+  helm_dbasis(c.x, c.drho, dr);
+  helm_basis(c.y, c.dT, t);
+  const double df_drho = helm_sum36(h, c, dr, t);
+  double df_drhodT = 0.0;
+  if (ret) {
+    helm_dbasis(c.y, c.dT, dt);
+    df_drhodT = helm_sum36(h, c, dr, dt);
+  }
+  double tmp = rhos * rhos;
+  double p = tmp * df_drho;
+  double dP_dT = tmp * df_drhodT;
+  const double T2 = T * T, T3 = T2 * T, T4 = T3 * T;
+  tmp = 1.0 / abar;
+  const double pion = kHelmRgas * rho * T * tmp;
+  const double dpiondt = kHelmRgas * rho * tmp;
+  p = p + pion + kHelmCrad * T4 * (1.0 / 3.0);
+  dP_dT = dP_dT + dpiondt + (4.0 / 3.0) * kHelmCrad * T3;
+  P = p;
+  dPdT = dP_dT;
+}
+
+// helm_rhoT_given2: E and (when ret) dE/dT.
+__device__ __forceinline__ void helm_given2(const HelmDev &h, double rho, double T, double abar,
+                                            double zbar, double &E, double &dEdT, bool ret) {
+  const double Ye = zbar / abar, rhos = rho * Ye;
+  const HelmCell c = helm_locate(h, rhos, T);
+  HelmBasis r, t, dt, ddt;
+  helm_basis(c.x, c.drho, r);
+  helm_basis(c.y, c.dT, t);
+  helm_dbasis(c.y, c.dT, dt);
+  const double f = helm_sum36(h, c, r, t);
+  const double df_dT = helm_sum36(h, c, r, dt);
+  double df_dT2 = 0.0;
+  if (ret) {
+    helm_ddbasis(c.y, c.dT, ddt);
+    df_dT2 = helm_sum36(h, c, r, ddt);
+  }
+  const double s = -Ye * df_dT;
+  const double ds_dT = -Ye * df_dT2;
+  double e = Ye * f + T * s;
+  double dE_dT = T * ds_dT;
+  const double T2 = T * T, T3 = T2 * T, T4 = T3 * T;
+  const double tmp = 1.0 / abar;
+  e = e + 1.5 * kHelmRgas * T * tmp + kHelmCrad * T4 / rho;
+  dE_dT = dE_dT + 1.5 * kHelmRgas * tmp + 4.0 * kHelmCrad * T3 / rho;
+  E = e;
+  dEdT = dE_dT;
+}
+
+// helm_rhoe_given: Newton for T(rho, e) from the warm start T (in/out).  The
+// residual of the incoming T is tested after the update, so the returned T is
+// one Newton step past the converged one; a T outside [Tl, Th] resets to the
+// start value and ends the loop.  Returns the iterations taken (<= 100).
+__device__ __forceinline__ int helm_rhoe_given(const HelmDev &h, double rho, double E0,
+                                               double abar, double zbar, double &T, double &P,
+                                               double &sound, bool ret) {
+  const double Told = T;
+  int iter;
+  sound = 0.0;
+  for (iter = 1; iter <= kHelmMaxIter; iter++) {
+    double e, dEdT;
+    helm_given2(h, rho, T, abar, zbar, e, dEdT, true);
+    const double res = e - E0;
+    T = T - res / dEdT;
+    double err = fabs(res / E0);
+    if ((T < h.tl) || (T > h.th)) {
+      T = Told;
+      err = 1e-15;
+    }
+    if (err < kHelmTol) break;
+  }
+  if (ret) {
+    double e, cv;
+    helm_full(h, rho, T, abar, zbar, P, e, sound, cv);
+  } else {
+    double dPdT;
+    helm_given1(h, rho, T, abar, zbar, P, dPdT, false);
+  }
+  return iter > kHelmMaxIter ? kHelmMaxIter : iter;
+}
+
+// helm_rhoP_given: Newton for T(rho, P); same conventions as helm_rhoe_given.
+__device__ __forceinline__ int helm_rhoP_given(const HelmDev &h, double rho, double P0,
+                                               double abar, double zbar, double &T, double &E,
+                                               double &sound, bool ret) {
+  const double Told = T;
+  int iter;
+  sound = 0.0;
+  for (iter = 1; iter <= kHelmMaxIter; iter++) {
+    double p, dPdT;
+    helm_given1(h, rho, T, abar, zbar, p, dPdT, true);
+    const double res = p - P0;
+    T = T - res / dPdT;
+    double err = fabs(res / P0);
+    if ((T < h.tl) || (T > h.th)) {
+      T = Told;
+      err = 1e-15;
+    }
+    if (err < kHelmTol) break;
+  }
+  if (ret) {
+    double p, cv;
+    helm_full(h, rho, T, abar, zbar, p, E, sound, cv);
+  } else {
+    double dEdT;
+    helm_given2(h, rho, T, abar, zbar, E, dEdT, false);
+  }
+  return iter > kHelmMaxIter ? kHelmMaxIter : iter;
+}
+
+}  // namespace phlx

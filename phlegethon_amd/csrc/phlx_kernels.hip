@@ -19,7 +19,7 @@ namespace phlx {
 // f), rotates into the face frame, solves, rotates back and applies the WB
 // flux correction and the ETOT potential flux.  MHD faces also store the
 // two tangential EMF components and the contact-upwind sign used by CT.
-template <int DIR, int SD, int NAS, bool MHD>
+template <int DIR, int SD, int NAS, bool MHD, bool HELM>
 __global__ void __launch_bounds__(256) k_flux(Grid g) {
   constexpr int NV = SD + 2 + NAS, IE = SD + 1;
   if (!g.sc->active) return;
@@ -29,12 +29,12 @@ __global__ void __launch_bounds__(256) k_flux(Grid g) {
   const int e1 = g.n1 + (DIR == 0), e2 = g.n2 + (DIR == 1);
   if (i >= e1 || j >= e2) return;
 
-  double q[4][NV], b[4][3];
+  double q[4][NV], b[4][3], ga[4][2];
 #pragma unroll
   for (int s = 0; s < 4; s++) {
     const int o = s - 2;
-    load_cell<SD, NV, MHD>(g, i + (DIR == 0 ? o : 0), j + (DIR == 1 ? o : 0),
-                           k + (DIR == 2 ? o : 0), q[s], b[s]);
+    load_cell<SD, NV, MHD, HELM>(g, i + (DIR == 0 ? o : 0), j + (DIR == 1 ? o : 0),
+                                 k + (DIR == 2 ? o : 0), q[s], b[s], ga[s]);
   }
   double qL[NV], qR[NV], bL[3] = {0.0, 0.0, 0.0}, bR[3] = {0.0, 0.0, 0.0};
   double l, r;
@@ -93,12 +93,32 @@ __global__ void __launch_bounds__(256) k_flux(Grid g) {
     qL[IE] = qL[IE] + peq;
     qR[IE] = qR[IE] + peq;
   }
+  // FASTEOS: gamma_e / gamma_c face values, unlimited PPH or the LIM2ND
+  // slope (source/source.F90:8497-8713), WB add-back of the face gamma_e
+  // (:8715-8727)
+  double gL[2] = {0.0, 0.0}, gR[2] = {0.0, 0.0};
+  if (HELM) {
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+      if (g.recon == REC_LIM2ND) {
+        rec_lim2nd(ga[0][c], ga[1][c], ga[2][c], l, gL[c]);
+        rec_lim2nd(ga[1][c], ga[2][c], ga[3][c], gR[c], r);
+      } else {
+        rec_pph(ga[0][c], ga[1][c], ga[2][c], l, gL[c]);
+        rec_pph(ga[1][c], ga[2][c], ga[3][c], gR[c], r);
+      }
+    }
+    if (g.wb) {
+      const double geq = (DIR == 1) ? g.eq_ge_f[jy] : g.eq_ge_cc[jy];
+      gL[0] = gL[0] + geq;
+      gR[0] = gR[0] + geq;
+    }
+  }
   double F[NV], Fb[3] = {0.0, 0.0, 0.0};
-  const double gdum[2] = {0.0, 0.0};
   if (MHD)
-    riemann_mhd<SD, NAS>(g, qL, qR, bL, bR, gdum, gdum, F, Fb);
+    riemann_mhd<SD, NAS, HELM>(g, qL, qR, bL, bR, gL, gR, F, Fb);
   else
-    riemann_hydro<SD, NAS>(g, qL, qR, gdum, gdum, F);
+    riemann_hydro<SD, NAS, HELM>(g, qL, qR, gL, gR, F);
   // back to the grid frame
   if (DIR == 1) {
     double t = F[1]; F[1] = -F[2]; F[2] = t;
@@ -331,9 +351,20 @@ __device__ __forceinline__ double reduce_partials(const double *red, int n) {
 
 // Signal speed of one cell, summed over directions (source/source.F90
 // :13891-13944): (|v_d| + c_f) / dx_d with c_f^2 = c^2 + B^2/rho.
-template <int SD, bool MHD>
-__device__ __forceinline__ double cell_speed(const Grid &g, const double *p, const double *b) {
-  double c = sqrt(g.gm * p[SD + 1] / p[0]);
+template <int SD, bool MHD, bool HELM>
+__device__ __forceinline__ double cell_speed(const Grid &g, const double *p, const double *b,
+                                             double T) {
+  double c;
+  if (HELM) {
+    // full Helmholtz evaluation at the carried T (source/source.F90:13801-13847)
+    const double ye = p[SD + 2];
+    const double abar = 1.0 / p[SD + 3];
+    const double zbar = ye * abar;
+    double pr, ei, cv;
+    helm_full(g.helm, p[0], T, abar, zbar, pr, ei, c, cv);
+  } else {
+    c = sqrt(g.gm * p[SD + 1] / p[0]);
+  }
   if (MHD) {
     const double bz = (SD == 3) ? b[2] : 0.0;
     const double ca2 = b[0] * b[0] + b[1] * b[1] + bz * bz;
@@ -353,7 +384,7 @@ __device__ __forceinline__ double cell_speed(const Grid &g, const double *p, con
 // gravity / heating sources, RK combine, b_cc from the updated faces,
 // cons->prim.  At the last stage the cell's CFL signal speed joins the
 // device-wide max.
-template <int SD, int NAS, bool MHD>
+template <int SD, int NAS, bool MHD, bool HELM>
 __global__ void __launch_bounds__(256) k_update(Grid g, int first, int last, double a1,
                                                 double a2, double c3) {
   constexpr int NV = SD + 2 + NAS, IE = SD + 1, IA = SD + 2;
@@ -436,7 +467,7 @@ __global__ void __launch_bounds__(256) k_update(Grid g, int first, int last, dou
 #pragma unroll
       for (int v = 0; v < SD; v++) g.bcc[v * N + c] = b[v];
     }
-    // cons -> prim (source/source.F90:13533-13686), ideal-gas EoS
+    // cons -> prim (source/source.F90:13533-13686)
     const double rho = S[0], m1 = S[1], m2 = S[2], m3 = (SD == 3) ? S[3] : 0.0;
     const double ir = 1.0 / rho;
     double eint = S[IE] - 0.5 * (m1 * m1 + m2 * m2 + m3 * m3) * ir;
@@ -446,7 +477,24 @@ __global__ void __launch_bounds__(256) k_update(Grid g, int first, int last, dou
     p[1] = m1 * ir;
     p[2] = m2 * ir;
     if (SD == 3) p[3] = m3 * ir;
-    p[IE] = g.gmm1 * eint;
+    double T = 0.0;
+    if (HELM) {
+      // Ye / 1/abar advected; T(rho, e) Newton from the carried T, then
+      // gamma_e = p/eint + 1 and gamma_c = c^2 rho / p (:13573-13578,
+      // :13648-13660)
+      const double ye = S[IA] * ir;
+      const double abar = rho / S[IA + 1];
+      const double zbar = ye * abar;
+      double pr, sound;
+      T = g.temp[c];
+      helm_rhoe_given(g.helm, rho, eint * ir, abar, zbar, T, pr, sound, true);
+      g.gam[c] = pr / eint + 1.0;
+      g.gam[N + c] = sound * sound * rho / pr;
+      g.temp[c] = T;
+      p[IE] = pr;
+    } else {
+      p[IE] = g.gmm1 * eint;
+    }
 #pragma unroll
     for (int v = IA; v < NV; v++) p[v] = S[v] * ir;
 #pragma unroll
@@ -455,41 +503,58 @@ __global__ void __launch_bounds__(256) k_update(Grid g, int first, int last, dou
       g.U[v * N + c] = S[v];
     }
     g.eint[c] = eint;
-    if (last) smax_local = cell_speed<SD, MHD>(g, p, b);
+    if (last) smax_local = cell_speed<SD, MHD, HELM>(g, p, b, T);
   }
   if (last) block_max_to(smax_local, g.red);
 }
 
 // Standalone compute_hyperbolic_dt pass over the current prim / b_cc.
-template <int SD, bool MHD>
+template <int SD, bool MHD, bool HELM>
 __global__ void __launch_bounds__(256) k_cfl(Grid g) {
+  constexpr int NL = HELM ? SD + 4 : SD + 2;  // + Ye, 1/abar
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y * blockDim.y + threadIdx.y;
   const int k = blockIdx.z;
   double s = 0.0;
   if (i < g.n1 && j < g.n2) {
     const long long N = g.N, c = cix(g, i, j, k);
-    double p[SD + 2], b[3] = {0.0, 0.0, 0.0};
+    double p[NL], b[3] = {0.0, 0.0, 0.0};
 #pragma unroll
-    for (int v = 0; v < SD + 2; v++) p[v] = g.P[v * N + c];
+    for (int v = 0; v < NL; v++) p[v] = g.P[v * N + c];
     if (MHD)
 #pragma unroll
       for (int v = 0; v < SD; v++) b[v] = g.bcc[v * N + c];
-    s = cell_speed<SD, MHD>(g, p, b);
+    s = cell_speed<SD, MHD, HELM>(g, p, b, HELM ? g.temp[c] : 0.0);
   }
   block_max_to(s, g.red);
 }
 
 // Step-0 initialisation: b_cc from faces and the ideal-gas eint
 // (source/source.F90:6012-6062, :6373-6376).
-template <int SD, bool MHD>
+template <int SD, bool MHD, bool HELM>
 __global__ void __launch_bounds__(256) k_init(Grid g) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y * blockDim.y + threadIdx.y;
   const int k = blockIdx.z;
   if (i >= g.n1 || j >= g.n2) return;
   const long long N = g.N, c = cix(g, i, j, k);
-  g.eint[c] = g.P[(SD + 1) * N + c] * g.igmm1;
+  if (HELM) {
+    // T(rho, p) Newton from the uploaded T, eint, gamma_e, gamma_c
+    // (source/source.F90:6279-6285, :6351-6357)
+    const double rho = g.P[c], p = g.P[(SD + 1) * N + c];
+    const double abar = 1.0 / g.P[(SD + 3) * N + c];
+    const double ye = g.P[(SD + 2) * N + c];
+    const double zbar = abar * ye;
+    double T = g.temp[c], eint, sound;
+    helm_rhoP_given(g.helm, rho, p, abar, zbar, T, eint, sound, true);
+    eint = rho * eint;
+    g.gam[c] = p / eint + 1.0;
+    g.gam[N + c] = sound * sound * rho / p;
+    g.eint[c] = eint;
+    g.temp[c] = T;
+  } else {
+    g.eint[c] = g.P[(SD + 1) * N + c] * g.igmm1;
+  }
   if (MHD) {
     g.bcc[c] = 0.5 * (g.bf[0][fix(g, 0, i + 1, j, k)] + g.bf[0][fix(g, 0, i, j, k)]);
     g.bcc[N + c] = 0.5 * (g.bf[1][fix(g, 1, i, j + 1, k)] + g.bf[1][fix(g, 1, i, j, k)]);
@@ -557,20 +622,22 @@ static dim3 grid_for(int e1, int e2, int e3, dim3 blk) {
   return dim3((e1 + blk.x - 1) / blk.x, (e2 + blk.y - 1) / blk.y, e3);
 }
 
-template <int SD, int NAS, bool MHD>
+template <int SD, int NAS, bool MHD, bool HELM>
 static void launch_flux(const Grid &g, int dir, hipStream_t st) {
   const dim3 blk(64, 4, 1);
   if (dir == 0)
-    k_flux<0, SD, NAS, MHD><<<grid_for(g.n1 + 1, g.n2, g.n3, blk), blk, 0, st>>>(g);
+    k_flux<0, SD, NAS, MHD, HELM><<<grid_for(g.n1 + 1, g.n2, g.n3, blk), blk, 0, st>>>(g);
   else if (dir == 1)
-    k_flux<1, SD, NAS, MHD><<<grid_for(g.n1, g.n2 + 1, g.n3, blk), blk, 0, st>>>(g);
+    k_flux<1, SD, NAS, MHD, HELM><<<grid_for(g.n1, g.n2 + 1, g.n3, blk), blk, 0, st>>>(g);
   else if (SD == 3)
-    k_flux<2, SD, NAS, MHD><<<grid_for(g.n1, g.n2, g.n3 + 1, blk), blk, 0, st>>>(g);
+    k_flux<2, SD, NAS, MHD, HELM><<<grid_for(g.n1, g.n2, g.n3 + 1, blk), blk, 0, st>>>(g);
 }
 
-template <int SD, int NAS, bool MHD>
+template <int SD, int NAS, bool MHD, bool HELM>
 struct Kern {
-  static void flux(const Grid &g, int dir, hipStream_t st) { launch_flux<SD, NAS, MHD>(g, dir, st); }
+  static void flux(const Grid &g, int dir, hipStream_t st) {
+    launch_flux<SD, NAS, MHD, HELM>(g, dir, st);
+  }
   static void corner(const Grid &g, hipStream_t st) {
     const dim3 blk(64, 4, 1);
     k_corner<SD><<<grid_for(g.n1 + 1, g.n2 + 1, SD == 3 ? g.n3 + 1 : 1, blk), blk, 0, st>>>(g);
@@ -583,16 +650,16 @@ struct Kern {
   static void update(const Grid &g, int first, int last, double a1, double a2, double c3,
                      hipStream_t st) {
     const dim3 blk(64, 4, 1);
-    k_update<SD, NAS, MHD><<<grid_for(g.n1, g.n2, g.n3, blk), blk, 0, st>>>(g, first, last, a1,
-                                                                           a2, c3);
+    k_update<SD, NAS, MHD, HELM><<<grid_for(g.n1, g.n2, g.n3, blk), blk, 0, st>>>(
+        g, first, last, a1, a2, c3);
   }
   static void cfl(const Grid &g, hipStream_t st) {
     const dim3 blk(64, 4, 1);
-    k_cfl<SD, MHD><<<grid_for(g.n1, g.n2, g.n3, blk), blk, 0, st>>>(g);
+    k_cfl<SD, MHD, HELM><<<grid_for(g.n1, g.n2, g.n3, blk), blk, 0, st>>>(g);
   }
   static void init(const Grid &g, hipStream_t st) {
     const dim3 blk(64, 4, 1);
-    k_init<SD, MHD><<<grid_for(g.n1, g.n2, g.n3, blk), blk, 0, st>>>(g);
+    k_init<SD, MHD, HELM><<<grid_for(g.n1, g.n2, g.n3, blk), blk, 0, st>>>(g);
   }
   static void divb(const Grid &g, hipStream_t st) {
     const dim3 blk(64, 4, 1);
@@ -600,27 +667,37 @@ struct Kern {
   }
 };
 
-template <int SD, int NAS, bool MHD>
+template <int SD, int NAS, bool MHD, bool HELM>
 static KernelTable make_table() {
+  using K = Kern<SD, NAS, MHD, HELM>;
   KernelTable t;
-  t.flux = &Kern<SD, NAS, MHD>::flux;
-  t.corner = &Kern<SD, NAS, MHD>::corner;
-  t.faceb = &Kern<SD, NAS, MHD>::faceb;
-  t.update = &Kern<SD, NAS, MHD>::update;
-  t.cfl = &Kern<SD, NAS, MHD>::cfl;
-  t.init = &Kern<SD, NAS, MHD>::init;
-  t.divb = &Kern<SD, NAS, MHD>::divb;
+  t.flux = &K::flux;
+  t.corner = &K::corner;
+  t.faceb = &K::faceb;
+  t.update = &K::update;
+  t.cfl = &K::cfl;
+  t.init = &K::init;
+  t.divb = &K::divb;
   return t;
 }
 
-bool select_kernels(int sd, int nas, bool mhd, KernelTable &t) {
-  if (sd == 2 && nas == 0 && !mhd) t = make_table<2, 0, false>();
-  else if (sd == 2 && nas == 0 && mhd) t = make_table<2, 0, true>();
-  else if (sd == 3 && nas == 0 && !mhd) t = make_table<3, 0, false>();
-  else if (sd == 3 && nas == 0 && mhd) t = make_table<3, 0, true>();
-  else if (sd == 2 && nas == 2 && !mhd) t = make_table<2, 2, false>();
-  else if (sd == 3 && nas == 2 && !mhd) t = make_table<3, 2, false>();
-  else if (sd == 3 && nas == 2 && mhd) t = make_table<3, 2, true>();
+bool select_kernels(int sd, int nas, bool mhd, bool helm, KernelTable &t) {
+  if (helm) {
+    // Helmholtz runs carry Ye and 1/abar as the two advected scalars
+    if (nas != 2) return false;
+    if (sd == 2 && !mhd) t = make_table<2, 2, false, true>();
+    else if (sd == 3 && !mhd) t = make_table<3, 2, false, true>();
+    else if (sd == 3 && mhd) t = make_table<3, 2, true, true>();
+    else return false;
+    return true;
+  }
+  if (sd == 2 && nas == 0 && !mhd) t = make_table<2, 0, false, false>();
+  else if (sd == 2 && nas == 0 && mhd) t = make_table<2, 0, true, false>();
+  else if (sd == 3 && nas == 0 && !mhd) t = make_table<3, 0, false, false>();
+  else if (sd == 3 && nas == 0 && mhd) t = make_table<3, 0, true, false>();
+  else if (sd == 2 && nas == 2 && !mhd) t = make_table<2, 2, false, false>();
+  else if (sd == 3 && nas == 2 && !mhd) t = make_table<3, 2, false, false>();
+  else if (sd == 3 && nas == 2 && mhd) t = make_table<3, 2, true, false>();
   else return false;
   return true;
 }

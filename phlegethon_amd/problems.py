@@ -19,8 +19,8 @@ from typing import Dict, List, Optional
 
 import numpy as np
 
-from .config import (BC_PERIODIC, BC_REFLECTIVE, BC_OUTFLOW, HLLC_FLUX, LHLLC_FLUX,
-                     HLLD_FLUX, LHLLD_FLUX, LIM2ND_REC, PPH_REC, PhlxConfig)
+from .config import (BC_PERIODIC, BC_REFLECTIVE, BC_OUTFLOW, EOS_HELMHOLTZ, HLLC_FLUX,
+                     LHLLC_FLUX, HLLD_FLUX, LHLLD_FLUX, LIM2ND_REC, PPH_REC, PhlxConfig)
 
 SEED = 20261202
 _M64 = np.uint64(0xFFFFFFFFFFFFFFFF)
@@ -51,12 +51,15 @@ class Problem:
     prim: np.ndarray
     b_faces: Optional[List[np.ndarray]] = None
     profiles: Dict[str, np.ndarray] = field(default_factory=dict)
+    temp: Optional[np.ndarray] = None      # initial T (Helmholtz warm start)
+    helm: Optional[object] = None          # helmholtz.HelmTable
 
     def copy(self) -> "Problem":
         return Problem(self.name, self.cfg, self.prim.copy(),
                        None if self.b_faces is None else
                        [None if b is None else b.copy() for b in self.b_faces],
-                       {k: v.copy() for k, v in self.profiles.items()})
+                       {k: v.copy() for k, v in self.profiles.items()},
+                       None if self.temp is None else self.temp.copy(), self.helm)
 
 
 def centres(cfg: PhlxConfig, d: int) -> np.ndarray:
@@ -270,10 +273,21 @@ def random_smooth(cfg: PhlxConfig, seed: int = SEED, amp: float = 0.2) -> Proble
 
 
 def _random_div_free(cfg: PhlxConfig, seed: int, amp: float = 0.3):
-    """Face field b = curl A with A on cell edges (discrete div b == 0)."""
+    """Face field b = curl A with A on cell edges (discrete div b == 0).
+    Coordinates are normalised by the domain lengths, so the modes are
+    periodic on any box (an exact no-op on the unit box)."""
     nz, ny, nx = cfg.shape
-    dx1, dx2, dx3 = cfg.dx
+    lo = (cfg.x1l, cfg.x2l, cfg.x3l)
+    ln = (cfg.x1u - cfg.x1l, cfg.x2u - cfg.x2l, cfg.x3u - cfg.x3l)
+    dx1, dx2, dx3 = (cfg.dx[d] / ln[d] for d in range(3))
     tp = 2.0 * math.pi
+
+    def nodes(cfg, d):  # noqa: F811  (normalised node / centre coordinates)
+        return (_nodes(cfg, d) - lo[d]) / ln[d]
+
+    def centres(cfg, d):  # noqa: F811
+        return (_centres(cfg, d) - lo[d]) / ln[d]
+
     xn, yn = nodes(cfg, 0), nodes(cfg, 1)
     xc, yc = centres(cfg, 0), centres(cfg, 1)
     if cfg.sdims == 2:
@@ -296,3 +310,82 @@ def _random_div_free(cfg: PhlxConfig, seed: int, amp: float = 0.3):
     b2 = (Ax[1:, :, :] - Ax[:-1, :, :]) / dx3 - (Az[:, :, 1:] - Az[:, :, :-1]) / dx1
     b3 = (Ay[:, :, 1:] - Ay[:, :, :-1]) / dx1 - (Ax[:, 1:, :] - Ax[:, :-1, :]) / dx2
     return [np.ascontiguousarray(b1), np.ascontiguousarray(b2), np.ascontiguousarray(b3)]
+
+
+_nodes, _centres = nodes, centres
+
+# --------------------------------------------------------------------- C3
+SLAB_L = 1.0e8  # cm
+
+
+def slab_profiles(y: np.ndarray, L: float = SLAB_L):
+    """log10 rho linear 6 -> 2, log10 T linear 9 -> 7 over [0, L]; abar a
+    tanh step 4 -> 16 centred at L/2 (width L/10); zbar = abar / 2."""
+    s = y / L
+    rho = 10.0 ** (6.0 - 4.0 * s)
+    T = 10.0 ** (9.0 - 2.0 * s)
+    abar = 10.0 + 6.0 * np.tanh((s - 0.5) / 0.1)
+    return rho, T, abar
+
+
+def helm_slab(eos, table, n: int = 256, ny: Optional[int] = None, sdims: int = 3,
+              recon: int = LIM2ND_REC, flux: int = LHLLC_FLUX, seed: int = SEED,
+              noise: float = 1e-4, mhd_beta: Optional[float] = None,
+              rk_stages: int = 2) -> Problem:
+    """Stratified Helmholtz slab: plane-parallel equilibrium rho(y), T(y),
+    abar(y) with P and e from the EoS, gravity from the discrete balance
+    (p_f[j+1] - p_f[j]) / dy = rho_cc[j] g_j and the potential integrated
+    from it; periodic x1/x3, reflecting x2; USE_FASTEOS + USE_WB +
+    EVOLVE_ETOT + ADVECT_YE_IABAR.  Zones get a seeded density perturbation
+    rho (1 + noise U(-1,1)) at fixed P (the step-0 T(rho, P) solve finds T).
+    Pattern of tests/hotbubble-helmholtz/app.F90:33-191.
+
+    ``eos`` evaluates the table (an object with ``full(rho, T, abar, zbar)``
+    returning P, E, cs, cv): the GPU ``runtime.HelmEoS`` or, in CPU-only
+    tests, the CPU restatement.  With ``mhd_beta`` set, LHLLD MHD with a weak
+    random divergence-free field of plasma beta ~ mhd_beta."""
+    ny = ny or n
+    mhd = mhd_beta is not None
+    cfg = PhlxConfig(sdims=sdims, nx1=n, nx2=ny, nx3=n if sdims == 3 else 1, nas=2,
+                     x1l=0.0, x1u=SLAB_L * n / ny, x2l=0.0, x2u=SLAB_L,
+                     x3l=0.0, x3u=SLAB_L * n / ny,
+                     recon=recon, flux=(LHLLD_FLUX if mhd else flux), rk_stages=rk_stages,
+                     cfl=0.8, mach_cutoff=1e-4,
+                     bc=(BC_PERIODIC, BC_PERIODIC, BC_REFLECTIVE, BC_REFLECTIVE,
+                         BC_PERIODIC, BC_PERIODIC),
+                     use_wb=True, use_gravity=True, evolve_etot=True,
+                     eos=EOS_HELMHOLTZ, fasteos=True).validate()
+    yc = centres(cfg, 1)
+    yf = nodes(cfg, 1)
+    dy = cfg.dx[1]
+    rho_cc, T_cc, abar_cc = slab_profiles(yc)
+    rho_f, T_f, abar_f = slab_profiles(yf)
+    P_cc, E_cc, _, _ = (np.asarray(a) for a in eos.full(rho_cc, T_cc, abar_cc, abar_cc / 2))
+    P_f, E_f, _, _ = (np.asarray(a) for a in eos.full(rho_f, T_f, abar_f, abar_f / 2))
+    grav = (P_f[1:] - P_f[:-1]) / dy / rho_cc
+    phi_f = np.zeros(ny + 1)
+    phi_f[1:] = -np.cumsum(grav * dy)
+    phi_cc = 0.5 * (phi_f[1:] + phi_f[:-1])
+    # gamma_e = p / (rho e) + 1 (tests/hotbubble-helmholtz/app.F90:74-76)
+    ge_cc = P_cc / (rho_cc * E_cc) + 1.0
+    ge_f = P_f / (rho_f * E_f) + 1.0
+    shape = cfg.shape
+    N = int(np.prod(shape))
+    u = (2.0 * uniform01(seed, 7, N) - 1.0).reshape(shape)
+    prim = np.zeros((cfg.nvars,) + shape)
+    prim[0] = rho_cc[None, :, None] * (1.0 + noise * u)
+    ip = cfg.sdims + 1
+    prim[ip] = np.broadcast_to(P_cc[None, :, None], shape)
+    prim[ip + 1] = 0.5
+    prim[ip + 2] = np.broadcast_to((1.0 / abar_cc)[None, :, None], shape)
+    temp = np.broadcast_to(T_cc[None, :, None], shape).copy()
+    prof = dict(eq_rho_cc=rho_cc, eq_p_cc=P_cc, eq_rho_f=rho_f, eq_p_f=P_f, grav_cc=grav,
+                phi_cc=phi_cc, phi_f=phi_f, eq_gammae_cc=ge_cc, eq_gammae_f=ge_f)
+    b_faces = None
+    if mhd:
+        b_faces = _random_div_free(cfg, seed, amp=1.0)
+        # scale to plasma beta ~ mhd_beta at the mid-plane pressure
+        bmag = max(np.sqrt(np.mean(b ** 2)) for b in b_faces if b is not None)
+        scale = np.sqrt(2.0 * P_cc[ny // 2] / mhd_beta) / bmag
+        b_faces = [None if b is None else b * scale for b in b_faces]
+    return Problem("helm_slab", cfg, prim, b_faces, prof, temp, table)
