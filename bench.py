@@ -8,6 +8,10 @@ time step (both RK stages, CT, CFL update) over the whole grid; the metric
 follows the reference's "updated cells/s" = steps * zones / loop wall time
 (source/source.F90:6834-6835) with the state resident in HBM.
 
+Other workload (``--workload helm_slab``): the 3D stratified Helmholtz slab,
+256^3, FP64, USE_FASTEOS + USE_WB + EVOLVE_ETOT + ADVECT_YE_IABAR, LHLLC +
+LIM2ND, RK2 (one T(rho, e) Newton solve per zone per stage).
+
 Prints ONE JSON line on stdout (progress goes to stderr).  Includes
   roofline      -- dominant kernel, algorithmic bytes / average launch time
                    (HIP events on the library's stream) vs 8 TB/s HBM peak;
@@ -42,6 +46,9 @@ def kernel_bytes(cfg, name: str) -> float:
     mhd = cfg.mhd
     nf = [(nx + 1) * ny * nz, nx * (ny + 1) * nz, nx * ny * (nz + 1)]
     cell_in = 8 * (nv + (sd if mhd else 0))           # prim (+ b_cc) per cell
+    helm = cfg.eos == 1                                 # + gamma_e, gamma_c per cell
+    if helm:
+        cell_in += 16
     if name.startswith("flux_x"):
         d = int(name[-1]) - 1
         per_face_out = 8 * nv + (8 * 2 + 1 if mhd else 0)  # fluxes (+2 EMFs + sign)
@@ -53,15 +60,19 @@ def kernel_bytes(cfg, name: str) -> float:
     if name == "update":   # fluxes, prim, U, U0 (or eint, b_cc), faces in; prim, U, eint, b_cc out
         rd = sum(nf[:sd]) * 8 * nv + N * 8 * (3 * nv) + (sum(nf[:sd]) * 8 if mhd else 0)
         wr = N * 8 * (2 * nv + 1 + (sd if mhd else 0))
+        if helm:  # T in; T, gamma_e, gamma_c out
+            rd += N * 8
+            wr += N * 24
         return rd + wr
     return 0.0
 
 
 def b_zu(cfg) -> int:
     """Algorithmic bytes per zone-update of the whole step, FP64, RK2:
-    8 * (5 n_q + 4 n_e) with n_q evolved fields (conserved + face B)."""
+    8 * (5 n_q + 4 n_e) with n_q evolved fields (conserved + face B) and n_e
+    carried EoS fields (T, gamma_e, gamma_c for Helmholtz FASTEOS)."""
     n_q = cfg.nvars + (cfg.sdims if cfg.mhd else 0)
-    n_e = 0
+    n_e = 3 if cfg.eos == 1 else 0
     stages = cfg.rk_stages
     if stages == 2:
         return 8 * (5 * n_q + 4 * n_e)
@@ -91,6 +102,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", choices=("ot3d", "helm_slab"), default="ot3d")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -103,12 +115,27 @@ def main():
     from phlegethon_amd.runtime import Simulation
 
     n = args.n
-    log(f"building 3D Orszag-Tang {n}^3 initial data")
-    pb = problems.orszag_tang_3d(n=n)
+    if args.workload == "ot3d":
+        log(f"building 3D Orszag-Tang {n}^3 initial data")
+        pb = problems.orszag_tang_3d(n=n)
+        desc = f"3D Orszag-Tang ideal MHD + CT, {n}^3, FP64, LHLLD + LIM2ND, RK2, CFL 0.8, periodic"
+        data = "synthetic (seeded 3D Orszag-Tang initial data, counter-based PRNG)"
+    else:
+        from phlegethon_amd import helmholtz
+        from phlegethon_amd.runtime import HelmEoS
+        table = helmholtz.default_table()
+        log(f"building the Helmholtz slab {n}^3 (table {table.nrho}x{table.nT})")
+        eos = HelmEoS(table)
+        pb = problems.helm_slab(eos, table, n=n)
+        eos.close()
+        desc = (f"3D stratified Helmholtz slab, {n}^3, FP64, USE_FASTEOS + USE_WB + "
+                f"EVOLVE_ETOT + ADVECT_YE_IABAR, LHLLC + LIM2ND, RK2, CFL 0.8")
+        data = ("synthetic (plane-parallel rho/T/abar profiles, seeded 1e-4 density noise; "
+                f"generated {table.nrho}x{table.nT} Helmholtz table)")
     cfg = pb.cfg
     zones = n ** 3
     sim = Simulation(cfg)
-    sim.upload(pb.prim, pb.b_faces)
+    sim.setup(pb)
     sim.init()
     log(f"warmup {args.warmup} steps")
     sim.run(args.warmup)
@@ -145,7 +172,7 @@ def main():
     achieved = d["alg_bytes"] / (d["avg_ms"] * 1e-3)
     traffic = None
     pmc = os.path.join(ROOT, "profiles", "ot3d256_fp64_pmc.json")
-    if n == 256 and os.path.exists(pmc):
+    if n == 256 and args.workload == "ot3d" and os.path.exists(pmc):
         # HBM bytes per dispatch measured with rocprofv3 --pmc FETCH_SIZE /
         # WRITE_SIZE on this workload, scaled by a copy kernel of known size
         with open(pmc) as fh:
@@ -185,10 +212,9 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (seeded 3D Orszag-Tang initial data, counter-based PRNG)",
-        "config": {"workload": f"3D Orszag-Tang ideal MHD + CT, {n}^3, FP64, LHLLD + "
-                               f"LIM2ND, RK2, CFL 0.8, periodic",
-                   "zones": zones, "sim_time": sc["time"], "max_divB": divb},
+        "data": data,
+        "config": {"workload": desc, "zones": zones, "sim_time": sc["time"],
+                   "max_divB": divb},
         "roofline": roofline,
         "step_roofline": step_roof,
         "kernels": {k: {"avg_ms": v["avg_ms"], "launches": v["launches"]} for k, v in kern.items()},
