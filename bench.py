@@ -102,7 +102,10 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=("ot3d", "helm_slab"), default="ot3d")
+    ap.add_argument("--workload", choices=("ot3d", "helm_slab", "lowmach"), default="ot3d")
+    ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64")
+    ap.add_argument("--heating", type=float, default=1e-3,
+                    help="lowmach: bottom heating amplitude")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -120,7 +123,7 @@ def main():
         pb = problems.orszag_tang_3d(n=n)
         desc = f"3D Orszag-Tang ideal MHD + CT, {n}^3, FP64, LHLLD + LIM2ND, RK2, CFL 0.8, periodic"
         data = "synthetic (seeded 3D Orszag-Tang initial data, counter-based PRNG)"
-    else:
+    elif args.workload == "helm_slab":
         from phlegethon_amd import helmholtz
         from phlegethon_amd.runtime import HelmEoS
         table = helmholtz.default_table()
@@ -132,6 +135,16 @@ def main():
                 f"EVOLVE_ETOT + ADVECT_YE_IABAR, LHLLC + LIM2ND, RK2, CFL 0.8")
         data = ("synthetic (plane-parallel rho/T/abar profiles, seeded 1e-4 density noise; "
                 f"generated {table.nrho}x{table.nT} Helmholtz table)")
+    else:
+        log(f"building the low-Mach heated box {n}^3")
+        pb = problems.lowmach_box(n=n, heating=args.heating)
+        desc = (f"3D low-Mach heated box, {n}^3, ideal gas + constant gravity, USE_WB + "
+                f"EVOLVE_ETOT + USE_EDOT, LHLLC + LIM2ND, RK2, CFL 0.8, heating {args.heating:g}")
+        data = "synthetic (polytrope + bottom heating layer, seeded 1e-6 density noise)"
+    if args.precision == "mixed":
+        from phlegethon_amd.config import PREC_MIXED
+        pb.cfg = pb.cfg.with_(precision=PREC_MIXED)
+        desc += " | mixed precision (FP32 recon/Riemann, FP64 deviations/update/CT/EoS)"
     cfg = pb.cfg
     zones = n ** 3
     sim = Simulation(cfg)
@@ -211,7 +224,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if args.precision == "fp64" else "f32+f64",
         "data": data,
         "config": {"workload": desc, "zones": zones, "sim_time": sc["time"],
                    "max_divB": divb},

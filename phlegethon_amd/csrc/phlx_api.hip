@@ -4,6 +4,8 @@
 
 #include <cmath>
 #include <cstdio>
+#include <algorithm>
+#include <cstdint>
 #include <cstring>
 #include <new>
 #include <vector>
@@ -34,6 +36,14 @@ struct phlx_ctx {
   void *helm_buf = nullptr;
   HelmDev helm{};
   bool helm_loaded = false;
+  // T(rho, e) surrogate: parameters, Adam moments, BF16 copies, gradient
+  // partials (one allocation, made on first use)
+  void *mlp_buf = nullptr;
+  float *mlp_theta = nullptr, *mlp_m = nullptr, *mlp_v = nullptr, *mlp_w1q = nullptr,
+        *mlp_w3q = nullptr, *mlp_gpart = nullptr, *mlp_lpart = nullptr, *mlp_gsum = nullptr;
+  unsigned short *mlp_w2t = nullptr;
+  bool mlp_ready = false;
+  bool eos_stats = false;
   // profiling
   bool prof = false;
   std::vector<ProfRec> recs;
@@ -97,7 +107,7 @@ static int validate(const phlx_config *c) {
   } else {
     return PHLX_E_CONFIG;
   }
-  if (c->precision != PHLX_PREC_FP64) return PHLX_E_CONFIG;
+  if (c->precision != PHLX_PREC_FP64 && c->precision != PHLX_PREC_MIXED) return PHLX_E_CONFIG;
   if (c->cfl <= 0.0 || c->gamma <= 1.0 || c->mu <= 0.0) return PHLX_E_CONFIG;
   for (int r = 0; r < 8; r++)
     if (c->reserved[r] != 0) return PHLX_E_CONFIG;
@@ -179,7 +189,7 @@ extern "C" int phlx_create(const phlx_config *cfg, phlx_ctx **out) {
     ctx->rk[2][0] = 1.0 / 3.0; ctx->rk[2][1] = 2.0 / 3.0; ctx->rk[2][2] = -(2.0 / 3.0);
   }
   const bool helm = cfg->eos == PHLX_EOS_HELMHOLTZ;
-  if (!select_kernels(g.sd, g.nas, g.mhd, helm, ctx->kt)) {
+  if (!select_kernels(g.sd, g.nas, g.mhd, helm, cfg->precision == PHLX_PREC_MIXED, ctx->kt)) {
     phlx_destroy(ctx);
     return PHLX_E_CONFIG;
   }
@@ -189,7 +199,10 @@ extern "C" int phlx_create(const phlx_config *cfg, phlx_ctx **out) {
   e |= dalloc(ctx, &g.U, nv * N);
   e |= dalloc(ctx, &g.U0, nv * N);
   e |= dalloc(ctx, &g.eint, N);
-  for (int d = 0; d < g.sd; d++) e |= dalloc(ctx, &g.F[d], nv * (size_t)g.nf[d]);
+  for (int d = 0; d < g.sd; d++) {
+    e |= dalloc(ctx, &g.F[d], nv * (size_t)g.nf[d]);
+    g.Ff[d] = (float *)g.F[d];  // the mixed path stores FP32 fluxes in the same buffer
+  }
   if (g.mhd) {
     e |= dalloc(ctx, &g.bcc, 3 * N);
     for (int d = 0; d < g.sd; d++) {
@@ -209,6 +222,9 @@ extern "C" int phlx_create(const phlx_config *cfg, phlx_ctx **out) {
     e |= dalloc(ctx, &g.temp, N);
     e |= dalloc(ctx, &g.gam, 2 * N);
   }
+  g.eos_guess = GUESS_WARM;
+  g.mlp.resid_tol = 0.25f;
+  g.mlp.box = 1.02f;
   double *prof = nullptr;
   const size_t ny = g.n2;
   e |= dalloc(ctx, &prof, kNumProfiles * (ny + 1));
@@ -216,6 +232,7 @@ extern "C" int phlx_create(const phlx_config *cfg, phlx_ctx **out) {
   e |= dalloc(ctx, &ctx->scratch_val, 1);
   g.nred = cell_grid_blocks(g);
   e |= dalloc(ctx, &g.red, (size_t)g.nred);
+  if (helm) e |= dalloc(ctx, &g.eos_part, 3 * (size_t)g.nred);
   if (e) {
     phlx_destroy(ctx);
     return PHLX_E_NOMEM;
@@ -254,6 +271,7 @@ extern "C" int phlx_destroy(phlx_ctx *ctx) {
   }
   for (void *p : ctx->allocs) (void)hipFree(p);
   if (ctx->helm_buf) (void)hipFree(ctx->helm_buf);
+  if (ctx->mlp_buf) (void)hipFree(ctx->mlp_buf);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return PHLX_OK;
@@ -518,7 +536,11 @@ extern "C" int phlx_rk_update(phlx_ctx *ctx, int32_t irk, double a1, double a2, 
   if (!ctx) return PHLX_E_ARG;
   ProfRec r;
   int on = kbegin(ctx, KID_UPDATE, r);
-  ctx->kt.update(ctx->g, irk == 0, last_stage != 0, a1, a2, a3, ctx->stream);
+  const bool guess = ctx->kt.update_guess && (ctx->g.eos_guess != GUESS_WARM || ctx->eos_stats);
+  if (guess)
+    ctx->kt.update_guess(ctx->g, irk == 0, last_stage != 0, a1, a2, a3, ctx->stream);
+  else
+    ctx->kt.update(ctx->g, irk == 0, last_stage != 0, a1, a2, a3, ctx->stream);
   kend(ctx, on, r);
   return check_launch();
 }
@@ -787,4 +809,293 @@ extern "C" int phlx_helm_rhoP_given_batch(phlx_ctx *ctx, int64_t n, const double
   launch_helm_rhoP(ctx->helm, n, rho, p, abar, zbar, T, E, sound, return_sound, iters,
                    ctx->stream);
   return check_launch();
+}
+
+// ------------------------------------------------------------- T(rho, e) surrogate
+
+constexpr int kMlpMaxGradBlocks = 1024;  // gradient-check batches up to 131072 samples
+
+static int mlp_alloc(phlx_ctx *ctx) {
+  if (ctx->mlp_buf) return PHLX_OK;
+  const size_t P = kMlpNParam;
+  const size_t floats = 3 * P + 256 + 64 + (size_t)kMlpMaxGradBlocks * P + kMlpMaxGradBlocks + P;
+  const size_t bytes = floats * sizeof(float) + 16 * 64 * 4 * sizeof(unsigned short) + 4096;
+  void *buf = nullptr;
+  if (hipMalloc(&buf, bytes) != hipSuccess) return PHLX_E_NOMEM;
+  HIPCHK(hipMemsetAsync(buf, 0, bytes, ctx->stream));
+  float *f = (float *)buf;
+  ctx->mlp_theta = f; f += P;
+  ctx->mlp_m = f; f += P;
+  ctx->mlp_v = f; f += P;
+  ctx->mlp_w1q = f; f += 256;
+  ctx->mlp_w3q = f; f += 64;
+  ctx->mlp_gpart = f; f += (size_t)kMlpMaxGradBlocks * P;
+  ctx->mlp_lpart = f; f += kMlpMaxGradBlocks;
+  ctx->mlp_gsum = f; f += P;
+  uintptr_t a = ((uintptr_t)f + 255) & ~(uintptr_t)255;
+  ctx->mlp_w2t = (unsigned short *)a;
+  ctx->mlp_buf = buf;
+  MlpDev &m = ctx->g.mlp;
+  m.theta = ctx->mlp_theta;
+  m.w1q = ctx->mlp_w1q;
+  m.w3q = ctx->mlp_w3q;
+  m.w2t = (const mlp_s4 *)ctx->mlp_w2t;
+  return PHLX_OK;
+}
+
+static void mlp_set_norm(MlpDev &m, const float *norm) {
+  for (int c = 0; c < 4; c++) {
+    m.in_off[c] = norm[c];
+    m.in_scale[c] = norm[4 + c];
+  }
+  m.out_off = norm[8];
+  m.out_scale = norm[9];
+}
+
+extern "C" int phlx_mlp_set_params(phlx_ctx *ctx, const float *theta, const float *norm) {
+  if (!ctx || !theta || !norm) return PHLX_E_ARG;
+  HIPCHK(hipSetDevice(ctx->device));
+  int rc = mlp_alloc(ctx);
+  if (rc) return rc;
+  for (int i = 0; i < kMlpNParam; i++)
+    if (!std::isfinite(theta[i])) return PHLX_E_ARG;
+  if (!(norm[9] > 0.0f)) return PHLX_E_ARG;
+  HIPCHK(hipMemcpyAsync(ctx->mlp_theta, theta, sizeof(float) * kMlpNParam, hipMemcpyHostToDevice,
+                        ctx->stream));
+  launch_mlp_quantize(ctx->mlp_theta, ctx->mlp_w1q, ctx->mlp_w3q, ctx->mlp_w2t, ctx->stream);
+  rc = check_launch();
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  mlp_set_norm(ctx->g.mlp, norm);
+  ctx->mlp_ready = true;
+  return PHLX_OK;
+}
+
+extern "C" int phlx_mlp_get_params(phlx_ctx *ctx, float *theta, float *norm) {
+  if (!ctx) return PHLX_E_ARG;
+  if (!ctx->mlp_ready) return PHLX_E_STATE;
+  if (theta) {
+    HIPCHK(hipMemcpyAsync(theta, ctx->mlp_theta, sizeof(float) * kMlpNParam,
+                          hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+  }
+  if (norm) {
+    const MlpDev &m = ctx->g.mlp;
+    for (int c = 0; c < 4; c++) {
+      norm[c] = m.in_off[c];
+      norm[4 + c] = m.in_scale[c];
+    }
+    norm[8] = m.out_off;
+    norm[9] = m.out_scale;
+  }
+  return PHLX_OK;
+}
+
+extern "C" int phlx_mlp_forward(phlx_ctx *ctx, int64_t n, const float *raw, float *y,
+                                int32_t *inbox) {
+  if (!ctx || n < 0) return PHLX_E_ARG;
+  if (!ctx->mlp_ready) return PHLX_E_STATE;
+  if (n == 0) return PHLX_OK;
+  if (!raw || !y) return PHLX_E_ARG;
+  HIPCHK(hipSetDevice(ctx->device));
+  launch_mlp_forward(ctx->g.mlp, n, raw, y, inbox, ctx->stream);
+  return check_launch();
+}
+
+extern "C" int phlx_mlp_grad(phlx_ctx *ctx, int64_t n, const float *xn, const float *tn,
+                             float *grad, double *loss) {
+  if (!ctx || n <= 0 || !xn || !tn || !grad) return PHLX_E_ARG;
+  if (!ctx->mlp_ready) return PHLX_E_STATE;
+  if (n > (int64_t)kMlpMaxGradBlocks * 128) return PHLX_E_ARG;
+  HIPCHK(hipSetDevice(ctx->device));
+  const int nblk = launch_mlp_grad(ctx->mlp_theta, ctx->mlp_w1q, ctx->mlp_w3q, xn, tn, 0, n,
+                                   ctx->mlp_gpart, ctx->mlp_lpart, ctx->stream);
+  launch_mlp_adam(ctx->mlp_theta, ctx->mlp_m, ctx->mlp_v, ctx->mlp_gpart, nblk, 1, 0.0f,
+                  ctx->mlp_w1q, ctx->mlp_w3q, ctx->mlp_w2t, grad, ctx->stream);
+  int rc = check_launch();
+  if (rc) return rc;
+  if (loss) {
+    std::vector<float> lp(nblk);
+    HIPCHK(hipMemcpyAsync(lp.data(), ctx->mlp_lpart, sizeof(float) * nblk, hipMemcpyDeviceToHost,
+                          ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    double s = 0.0;
+    for (float v : lp) s += v;
+    *loss = s / (double)n;
+  }
+  return PHLX_OK;
+}
+
+extern "C" int phlx_mlp_train(phlx_ctx *ctx, int64_t nsamples, int32_t epochs, int32_t batch,
+                              double lr, uint64_t seed, double *train_loss, double *val_rmse) {
+  if (!ctx || nsamples < 4096 || epochs < 1 || batch < 128 || batch % 128 ||
+      batch > kMlpMaxGradBlocks * 128 || !(lr > 0.0))
+    return PHLX_E_ARG;
+  if (!ctx->helm_loaded) return PHLX_E_TABLE;
+  HIPCHK(hipSetDevice(ctx->device));
+  int rc = mlp_alloc(ctx);
+  if (rc) return rc;
+  const HelmDev &h = ctx->helm;
+  const long long n = nsamples, nval = n / 16, ntr = n - nval;
+  if (ntr < batch) return PHLX_E_ARG;
+  // sample box: the table's rho*Ye and T ranges (0.05 dex inside the edges),
+  // abar in [4, 16], Ye in [0.45, 0.5]
+  const double ldlo = h.lrho1, ldhi = h.lrho1 + h.dr * (h.nr - 1);
+  const double ltlo = h.lT1, lthi = h.lT1 + h.dt * (h.nt - 1);
+  const double box[8] = {ldlo + 0.05, ldhi - 0.05, ltlo + 0.05, lthi - 0.05, 4.0, 16.0, 0.45, 0.5};
+  float *data = nullptr;
+  if (hipMalloc(&data, sizeof(float) * 10 * (size_t)n) != hipSuccess) return PHLX_E_NOMEM;
+  float *raw = data, *label = data + 4 * n, *xn = data + 5 * n, *tn = data + 9 * n;
+  launch_mlp_data(h, n, seed, box, raw, label, ctx->stream);
+  rc = check_launch();
+  std::vector<float> hraw;
+  if (!rc) {
+    hraw.resize(4 * (size_t)n);
+    if (hipMemcpyAsync(hraw.data(), raw, sizeof(float) * 4 * n, hipMemcpyDeviceToHost,
+                       ctx->stream) != hipSuccess ||
+        hipStreamSynchronize(ctx->stream) != hipSuccess)
+      rc = PHLX_E_HIP;
+  }
+  if (rc) {
+    (void)hipFree(data);
+    return rc;
+  }
+  float norm[10];
+  for (int c = 0; c < 4; c++) {
+    float lo = hraw[c], hi = hraw[c];
+    for (long long i = 1; i < n; i++) {
+      lo = std::min(lo, hraw[4 * i + c]);
+      hi = std::max(hi, hraw[4 * i + c]);
+    }
+    norm[c] = 0.5f * (lo + hi);
+    norm[4 + c] = hi > lo ? 2.0f / (hi - lo) : 1.0f;
+  }
+  norm[8] = (float)(0.5 * (box[2] + box[3]));
+  norm[9] = (float)(0.5 * (box[3] - box[2]));
+  mlp_set_norm(ctx->g.mlp, norm);
+  launch_mlp_normalize(ctx->g.mlp, n, raw, label, xn, tn, ctx->stream);
+  // Xavier-uniform initial weights (host RNG), zero biases and moments
+  std::vector<float> th(kMlpNParam, 0.0f);
+  unsigned long long st = seed ^ 0x5DEECE66Dull;
+  auto next = [&st]() {
+    st = st * 6364136223846793005ull + 1442695040888963407ull;
+    return (double)(st >> 11) * (1.0 / 9007199254740992.0);
+  };
+  const double a1 = std::sqrt(6.0 / (4 + 64)), a2 = std::sqrt(6.0 / (64 + 64)),
+               a3 = std::sqrt(6.0 / (64 + 1));
+  for (int i = 0; i < 256; i++) th[kMlpOffW1 + i] = (float)((2.0 * next() - 1.0) * a1);
+  for (int i = 0; i < 4096; i++) th[kMlpOffW2 + i] = (float)((2.0 * next() - 1.0) * a2);
+  for (int i = 0; i < 64; i++) th[kMlpOffW3 + i] = (float)((2.0 * next() - 1.0) * a3);
+  if (hipMemcpyAsync(ctx->mlp_theta, th.data(), sizeof(float) * kMlpNParam, hipMemcpyHostToDevice,
+                     ctx->stream) != hipSuccess ||
+      hipMemsetAsync(ctx->mlp_m, 0, sizeof(float) * 2 * kMlpNParam, ctx->stream) != hipSuccess) {
+    (void)hipFree(data);
+    return PHLX_E_HIP;
+  }
+  launch_mlp_quantize(ctx->mlp_theta, ctx->mlp_w1q, ctx->mlp_w3q, ctx->mlp_w2t, ctx->stream);
+  const long long spe = ntr / batch;  // steps per epoch
+  const long long steps = spe * epochs;
+  int nblk = 1;
+  for (long long t = 1; t <= steps; t++) {
+    const long long n0 = ((t - 1) % spe) * batch;
+    const double cosf_ = 0.5 * (1.0 + std::cos(3.141592653589793 * (double)(t - 1) / (double)steps));
+    const float lr_t = (float)(lr * (0.05 + 0.95 * cosf_));
+    nblk = launch_mlp_grad(ctx->mlp_theta, ctx->mlp_w1q, ctx->mlp_w3q, xn, tn, n0, batch,
+                           ctx->mlp_gpart, ctx->mlp_lpart, ctx->stream);
+    launch_mlp_adam(ctx->mlp_theta, ctx->mlp_m, ctx->mlp_v, ctx->mlp_gpart, nblk, (int)t, lr_t,
+                    ctx->mlp_w1q, ctx->mlp_w3q, ctx->mlp_w2t, nullptr, ctx->stream);
+  }
+  rc = check_launch();
+  ctx->mlp_ready = true;
+  if (!rc && train_loss) {
+    std::vector<float> lp(nblk);
+    if (hipMemcpyAsync(lp.data(), ctx->mlp_lpart, sizeof(float) * nblk, hipMemcpyDeviceToHost,
+                       ctx->stream) != hipSuccess ||
+        hipStreamSynchronize(ctx->stream) != hipSuccess) {
+      rc = PHLX_E_HIP;
+    } else {
+      double s = 0.0;
+      for (float v : lp) s += v;
+      *train_loss = s / (double)batch;
+    }
+  }
+  if (!rc && val_rmse) {
+    // held-out tail: RMS error of log10 T
+    float *y = xn;  // reuse: the normalised inputs of the training part are done
+    launch_mlp_forward(ctx->g.mlp, nval, raw + 4 * ntr, y, nullptr, ctx->stream);
+    std::vector<float> hy(nval), ht(nval);
+    if (check_launch() ||
+        hipMemcpyAsync(hy.data(), y, sizeof(float) * nval, hipMemcpyDeviceToHost, ctx->stream) !=
+            hipSuccess ||
+        hipMemcpyAsync(ht.data(), tn + ntr, sizeof(float) * nval, hipMemcpyDeviceToHost,
+                       ctx->stream) != hipSuccess ||
+        hipStreamSynchronize(ctx->stream) != hipSuccess) {
+      rc = PHLX_E_HIP;
+    } else {
+      double s = 0.0;
+      for (long long i = 0; i < nval; i++) {
+        const double d = ((double)hy[i] - (double)ht[i]) * norm[9];
+        s += d * d;
+      }
+      *val_rmse = std::sqrt(s / (double)nval);
+    }
+  }
+  (void)hipStreamSynchronize(ctx->stream);
+  (void)hipFree(data);
+  return rc;
+}
+
+extern "C" int phlx_helm_rhoe_given_guess_batch(phlx_ctx *ctx, int64_t n, const double *rho,
+                                                const double *e, const double *abar,
+                                                const double *zbar, double *T, double *P,
+                                                double *sound, int32_t return_sound,
+                                                int32_t guess_mode, int32_t *iters,
+                                                int32_t *fallback) {
+  if (!ctx || n < 0) return PHLX_E_ARG;
+  if (guess_mode < GUESS_WARM || guess_mode > GUESS_BRACKET) return PHLX_E_ARG;
+  if (!ctx->helm_loaded) return PHLX_E_TABLE;
+  if (guess_mode == GUESS_MLP && !ctx->mlp_ready) return PHLX_E_STATE;
+  if (n == 0) return PHLX_OK;
+  if (!rho || !e || !abar || !zbar || !T) return PHLX_E_ARG;
+  HIPCHK(hipSetDevice(ctx->device));
+  launch_helm_rhoe_guess(ctx->helm, ctx->g.mlp, guess_mode, n, rho, e, abar, zbar, T, P, sound,
+                         return_sound, iters, fallback, ctx->stream);
+  return check_launch();
+}
+
+extern "C" int phlx_set_eos_guess(phlx_ctx *ctx, int32_t mode, double resid_tol,
+                                  int32_t count_iterations) {
+  if (!ctx) return PHLX_E_ARG;
+  if (mode < GUESS_WARM || mode > GUESS_BRACKET || !(resid_tol > 0.0)) return PHLX_E_ARG;
+  if (ctx->cfg.eos != PHLX_EOS_HELMHOLTZ) return PHLX_E_CONFIG;
+  if (mode == GUESS_MLP && !ctx->mlp_ready) return PHLX_E_STATE;
+  // the guess variant of the update needs every block full of interior cells
+  if ((mode != GUESS_WARM || count_iterations) && (ctx->g.n1 % 64 || ctx->g.n2 % 4))
+    return PHLX_E_CONFIG;
+  ctx->g.eos_guess = mode;
+  ctx->g.mlp.resid_tol = (float)resid_tol;
+  ctx->eos_stats = count_iterations != 0;
+  return PHLX_OK;
+}
+
+extern "C" int phlx_eos_stats(phlx_ctx *ctx, int64_t *newton_iters, int64_t *solves,
+                              int64_t *fallbacks, int32_t reset) {
+  if (!ctx) return PHLX_E_ARG;
+  if (!ctx->g.eos_part) return PHLX_E_CONFIG;
+  HIPCHK(hipSetDevice(ctx->device));
+  std::vector<unsigned int> part(3 * (size_t)ctx->g.nred);
+  HIPCHK(hipMemcpyAsync(part.data(), ctx->g.eos_part, sizeof(unsigned int) * part.size(),
+                        hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  long long s[3] = {0, 0, 0};
+  for (size_t b = 0; b < part.size(); b += 3)
+    for (int q = 0; q < 3; q++) s[q] += part[b + q];
+  if (newton_iters) *newton_iters = s[0];
+  if (solves) *solves = s[1];
+  if (fallbacks) *fallbacks = s[2];
+  if (reset) {
+    HIPCHK(hipMemsetAsync(ctx->g.eos_part, 0, sizeof(unsigned int) * part.size(), ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+  }
+  return PHLX_OK;
 }

@@ -16,6 +16,7 @@
 #include <stdint.h>
 
 #include "phlx_helm.hpp"
+#include "phlx_mlp.hpp"
 
 namespace phlx {
 
@@ -45,6 +46,7 @@ struct Grid {
   double *P, *U, *U0, *eint, *bcc;
   double *bf[3], *b0f[3];
   double *F[3];
+  float *Ff[3];            // FP32 view of F[d] (mixed precision: fluxes stored as FP32)
   double *emf[3];          // two EMF components per face (see k_flux)
   signed char *sgn[3];     // contact-upwind sign per face
   double *cor[3];          // corner (edge) EMFs along x1, x2, x3
@@ -55,6 +57,12 @@ struct Grid {
   double *temp, *gam;
   const double *eq_ge_cc, *eq_ge_f;
   HelmDev helm;
+  // T(rho, e) start: GUESS_WARM (the reference), GUESS_MLP, GUESS_BRACKET;
+  // per-block Newton iteration / zone / fallback counters of the guess
+  // variant of the update
+  MlpDev mlp;
+  int eos_guess;
+  unsigned int *eos_part;
   double *red;       // per-block partial maxima of the cell-grid reductions
   int nred;          // number of partials (blocks of a cell-grid launch)
   Scalars *sc;
@@ -504,5 +512,351 @@ __device__ __forceinline__ void riemann_mhd(const Grid &g, const double *qL,
 #pragma unroll
   for (int iv = 0; iv < 3; iv++) Fb[iv] = fb[iv] + s * (Ubs[iv] - Ub[iv]) + ss * (Ubss[iv] - Ubs[iv]);
 }
+
+// ====================================================================
+// Mixed precision (PHLX_PREC_MIXED): reconstruction, wave speeds and the
+// Riemann fluxes in FP32 on the FP64-formed well-balanced deviations.
+//
+// The pressure slot of a face state holds the deviation p' = p - p_eq
+// (formed in FP64 before the cast); the full pressure p = p_eq + p' is
+// passed beside it for the sound speeds and energies.  The normal momentum
+// flux is assembled from deviations, so it carries p* - p_eq directly
+// (what the reference obtains by subtracting p_eq from the FP64 flux,
+// source/source.F90:8953-8959): in FP32 the subtraction of two O(p_eq)
+// numbers would leave only ~6e-8 p_eq of precision, far above the
+// low-Mach pressure fluctuation M^2 p.  The star pressures are affine in
+// the left/right total pressures with weights summing to one, so the same
+// formula evaluated on deviations gives p* - p_eq exactly.
+// ====================================================================
+namespace mx {
+
+__device__ __forceinline__ void rec_lim2nd(float qm, float qc, float qp, float &l, float &r) {
+  float sf = qp - qc, sb = qc - qm;
+  float prod = sf * sb, slope = 0.0f;
+  if (prod > 0.0f) slope = prod / (sf + sb);
+  float h = 0.5f * slope;
+  l = qc - h;
+  r = qc + h;
+}
+__device__ __forceinline__ void rec_pph(float qm, float qc, float qp, float &l, float &r) {
+  float c = (5.0f / 6.0f) * qc;
+  l = (1.0f / 3.0f) * qm + c - (1.0f / 6.0f) * qp;
+  r = -(1.0f / 6.0f) * qm + c + (1.0f / 3.0f) * qp;
+}
+__device__ __forceinline__ void rec_pph_lim(float qm, float qc, float qp, float &l, float &r) {
+  float c = (5.0f / 6.0f) * qc;
+  float am = (1.0f / 3.0f) * qm + c - (1.0f / 6.0f) * qp;
+  float ap = -(1.0f / 6.0f) * qm + c + (1.0f / 3.0f) * qp;
+  am = fminf(fmaxf(am, fminf(qm, qc)), fmaxf(qm, qc));
+  ap = fminf(fmaxf(ap, fminf(qc, qp)), fmaxf(qc, qp));
+  am = am - qc;
+  ap = ap - qc;
+  if (ap * am >= 0.0f) {
+    ap = 0.0f;
+    am = 0.0f;
+  } else {
+    if (fabsf(ap) >= 2.0f * fabsf(am)) ap = -2.0f * am;
+    if (fabsf(am) >= 2.0f * fabsf(ap)) am = -2.0f * ap;
+  }
+  l = am + qc;
+  r = ap + qc;
+}
+
+// HLLC / LHLLC, deviation form.  q[IE] = p', pf = full p.
+template <int SD, int NAS, bool FE>
+__device__ __forceinline__ void riemann_hydro(const Grid &g, const float *qL, const float *qR,
+                                              float pfL, float pfR, const float *gL,
+                                              const float *gR, float *F) {
+  constexpr int IE = SD + 1, IA = SD + 2, NV = SD + 2 + NAS;
+  const float gm = (float)g.gm, igmm1 = (float)g.igmm1;
+  const float rhoL = qL[0], irL = 1.0f / rhoL;
+  const float uL = qL[1], vL = qL[2], wL = (SD == 3) ? qL[3] : 0.0f;
+  const float v2L = uL * uL + vL * vL + wL * wL;
+  const float rhoR = qR[0], irR = 1.0f / rhoR;
+  const float uR = qR[1], vR = qR[2], wR = (SD == 3) ? qR[3] : 0.0f;
+  const float v2R = uR * uR + vR * vR + wR * wR;
+  float eL, cL, eR, cR;
+  if (FE) {
+    eL = pfL / (gL[0] - 1.0f);
+    cL = sqrtf(gL[1] * pfL * irL);
+    eR = pfR / (gR[0] - 1.0f);
+    cR = sqrtf(gR[1] * pfR * irR);
+  } else {
+    eL = igmm1 * pfL;
+    cL = sqrtf(gm * pfL * irL);
+    eR = igmm1 * pfR;
+    cR = sqrtf(gm * pfR * irR);
+  }
+  const float EL = eL + 0.5f * rhoL * v2L, ER = eR + 0.5f * rhoR * v2R;
+  const float dpL = qL[IE], dpR = qR[IE];
+  const float cmax = fmaxf(cL, cR);
+  const float sL = fminf(uL, uR) - cmax, sR = fmaxf(uL, uR) + cmax;
+  const float dL = sL - uL, dR = sR - uR;
+  const float us = (dpR - dpL + rhoL * uL * dL - rhoR * uR * dR) / (rhoL * dL - rhoR * dR);
+  const float rsL = rhoL * (dL / (sL - us));
+  const float EsL = rsL * (EL * irL + (us - uL) * (us + pfL * irL / dL));
+  const float rsR = rhoR * (dR / (sR - us));
+  const float EsR = rsR * (ER * irR + (us - uR) * (us + pfR * irR / dR));
+  if (g.flux == FLUX_LHLLC) {
+    float chi = fminf(1.0f, fmaxf(sqrtf(v2L) / cL, sqrtf(v2R) / cR));
+    chi = fmaxf((float)g.mach_cutoff, chi);
+    const float phi = chi * (2.0f - chi);
+    const float rbar = 0.5f * (rhoL + rhoR), cbar = 0.5f * (cL + cR);
+    const float diss = 0.5f * phi * rbar * cbar * (uR - uL);
+    const float psd = 0.5f * (dpL + dpR) - diss;   // p* - p_eq
+    const float psf = 0.5f * (pfL + pfR) - diss;   // p*
+    const float s = copysignf(1.0f, us);
+    const float hp = 0.5f * (1.0f + s), hm = 0.5f * (1.0f - s);
+    const float rs = hp * rsL + hm * rsR;
+    if (g.all_mach && sL > 0.0f) {
+      F[0] = rhoL * uL;
+      F[1] = rhoL * uL * uL + dpL;
+      F[2] = rhoL * uL * vL;
+      if (SD == 3) F[3] = rhoL * uL * wL;
+      F[IE] = (EL + pfL) * uL;
+#pragma unroll
+      for (int v = IA; v < NV; v++) F[v] = rhoL * qL[v] * uL;
+    } else if (g.all_mach && sR < 0.0f) {
+      F[0] = rhoR * uR;
+      F[1] = rhoR * uR * uR + dpR;
+      F[2] = rhoR * uR * vR;
+      if (SD == 3) F[3] = rhoR * uR * wR;
+      F[IE] = (ER + pfR) * uR;
+#pragma unroll
+      for (int v = IA; v < NV; v++) F[v] = rhoR * qR[v] * uR;
+    } else {
+      F[0] = rs * us;
+      F[1] = rs * us * us + psd;
+      F[2] = rs * us * (hp * vL + hm * vR);
+      if (SD == 3) F[3] = rs * us * (hp * wL + hm * wR);
+      F[IE] = (hp * EsL + hm * EsR + psf) * us;
+#pragma unroll
+      for (int v = IA; v < NV; v++) F[v] = rs * us * (hp * qL[v] + hm * qR[v]);
+    }
+    return;
+  }
+  const bool left = us > 0.0f;
+  const float rho = left ? rhoL : rhoR, u = left ? uL : uR, vv = left ? vL : vR,
+              ww = left ? wL : wR, E = left ? EL : ER, pf = left ? pfL : pfR,
+              pd = left ? dpL : dpR;
+  const float s = left ? sL : sR, rs = left ? rsL : rsR, Es = left ? EsL : EsR;
+  const float *q = left ? qL : qR;
+  float f[NV], U[NV];
+  U[0] = rho; U[1] = rho * u; U[2] = rho * vv;
+  if (SD == 3) U[3] = rho * ww;
+  U[IE] = E;
+  f[0] = rho * u; f[1] = rho * u * u + pd; f[2] = rho * u * vv;
+  if (SD == 3) f[3] = rho * u * ww;
+  f[IE] = (E + pf) * u;
+#pragma unroll
+  for (int v = IA; v < NV; v++) {
+    U[v] = rho * q[v];
+    f[v] = rho * u * q[v];
+  }
+  const bool outer = left ? (sL >= 0.0f) : (sR < 0.0f);
+  if (outer) {
+#pragma unroll
+    for (int v = 0; v < NV; v++) F[v] = f[v];
+  } else {
+    float Us[NV];
+    Us[0] = rs; Us[1] = rs * us; Us[2] = rs * vv;
+    if (SD == 3) Us[3] = rs * ww;
+    Us[IE] = Es;
+#pragma unroll
+    for (int v = IA; v < NV; v++) Us[v] = rs * q[v];
+#pragma unroll
+    for (int v = 0; v < NV; v++) F[v] = f[v] + s * (Us[v] - U[v]);
+  }
+}
+
+// HLLD / LHLLD, deviation form (total pressure deviation p' + B^2/2 in
+// the momentum flux, full total pressure in the energy flux).
+template <int SD, int NAS, bool FE>
+__device__ __forceinline__ void riemann_mhd(const Grid &g, const float *qL, const float *qR,
+                                            float pfL, float pfR, const float *bL,
+                                            const float *bR, const float *gL, const float *gR,
+                                            float *F, float *Fb) {
+  constexpr int IE = SD + 1, IA = SD + 2, NV = SD + 2 + NAS;
+  const float gm = (float)g.gm, igmm1 = (float)g.igmm1;
+  const float rhoL = qL[0], irL = 1.0f / rhoL;
+  const float uL = qL[1], vL = qL[2], wL = (SD == 3) ? qL[3] : 0.0f;
+  const float v2L = uL * uL + vL * vL + wL * wL;
+  const float rhoR = qR[0], irR = 1.0f / rhoR;
+  const float uR = qR[1], vR = qR[2], wR = (SD == 3) ? qR[3] : 0.0f;
+  const float v2R = uR * uR + vR * vR + wR * wR;
+  float eL, eR, c2L, c2R;
+  if (FE) {
+    eL = pfL / (gL[0] - 1.0f);
+    c2L = gL[1] * pfL * irL;
+    eR = pfR / (gR[0] - 1.0f);
+    c2R = gR[1] * pfR * irR;
+  } else {
+    eL = pfL * igmm1;
+    c2L = gm * pfL * irL;
+    eR = pfR * igmm1;
+    c2R = gm * pfR * irR;
+  }
+  const float byL = bL[1], bzL = (SD == 3) ? bL[2] : 0.0f;
+  const float byR = bR[1], bzR = (SD == 3) ? bR[2] : 0.0f;
+  const float bx = 0.5f * (bL[0] + bR[0]), bx2 = bx * bx;
+  const float pbL = 0.5f * (bx2 + byL * byL + bzL * bzL);
+  const float pbR = 0.5f * (bx2 + byR * byR + bzR * bzR);
+  const float EL = eL + 0.5f * rhoL * v2L + pbL;
+  const float ER = eR + 0.5f * rhoR * v2R + pbR;
+  const float ptL = pfL + pbL, ptR = pfR + pbR;          // full
+  const float pdL = qL[IE] + pbL, pdR = qR[IE] + pbR;    // deviation
+  const float a2L = 2.0f * pbL * irL, a2R = 2.0f * pbR * irR;
+  const float an2L = bx2 * irL, an2R = bx2 * irR;
+  float t = c2L + a2L;
+  const float cL = sqrtf(0.5f * (c2L + a2L + sqrtf(fmaxf(t * t - 4.0f * c2L * an2L, 0.0f))));
+  t = c2R + a2R;
+  const float cR = sqrtf(0.5f * (c2R + a2R + sqrtf(fmaxf(t * t - 4.0f * c2R * an2R, 0.0f))));
+  const float cmax = fmaxf(cL, cR);
+  const float sL = fminf(uL, uR) - cmax, sR = fmaxf(uL, uR) + cmax;
+  const float dL = sL - uL, dR = sR - uR;
+  const float it1 = 1.0f / (dR * rhoR - dL * rhoL);
+  const float us = (dR * rhoR * uR - dL * rhoL * uL - pdR + pdL) * it1;
+  const float dmL = sL - us, dmR = sR - us;
+  float phi = 1.0f;
+  if (g.flux == FLUX_LHLLD) {
+    t = v2L + a2L;
+    const float cu2L = 0.5f * (v2L + a2L + sqrtf(fmaxf(t * t - 4.0f * v2L * an2L, 0.0f)));
+    t = v2R + a2R;
+    const float cu2R = 0.5f * (v2R + a2R + sqrtf(fmaxf(t * t - 4.0f * v2R * an2R, 0.0f)));
+    const float mL = sqrtf(cu2L) / cL, mR = sqrtf(cu2R) / cR;
+    if (fmaxf(mL, mR) < 0.6f) {
+      float chi = fminf(1.0f, fmaxf(mL, mR));
+      chi = fmaxf((float)g.mach_cutoff, chi);
+      phi = chi * (2.0f - chi);
+    }
+  }
+  const float diss = phi * rhoL * rhoR * dR * dL * (uL - uR);
+  const float psf = (dR * rhoR * ptL - dL * rhoL * ptR - diss) * it1;
+  const float psd = (dR * rhoR * pdL - dL * rhoL * pdR - diss) * it1;
+  const float rsL = rhoL * dL / dmL, rsR = rhoR * dR / dmR;
+  const float ia1 = 1.0f / (rhoL * dL * dmL - bx2);
+  const float vsL = vL - bx * byL * (us - uL) * ia1;
+  const float bysL = byL * (rhoL * dL * dL - bx2) * ia1;
+  const float wsL = wL - bx * bzL * (us - uL) * ia1;
+  const float bzsL = bzL * (rhoL * dL * dL - bx2) * ia1;
+  const float ia2 = 1.0f / (rhoR * dR * dmR - bx2);
+  const float vsR = vR - bx * byR * (us - uR) * ia2;
+  const float bysR = byR * (rhoR * dR * dR - bx2) * ia2;
+  const float wsR = wR - bx * bzR * (us - uR) * ia2;
+  const float bzsR = bzR * (rhoR * dR * dR - bx2) * ia2;
+  const float vbL = uL * bx + vL * byL + wL * bzL;
+  const float vbR = uR * bx + vR * byR + wR * bzR;
+  const float vbsL = us * bx + vsL * bysL + wsL * bzsL;
+  const float vbsR = us * bx + vsR * bysR + wsR * bzsR;
+  const float EsL = (dL * EL - ptL * uL + psf * us + bx * (vbL - vbsL)) / dmL;
+  const float EsR = (dR * ER - ptR * uR + psf * us + bx * (vbR - vbsR)) / dmR;
+  const float sqL = sqrtf(rsL), sqR = sqrtf(rsR);
+  const float abx = fabsf(bx);
+  const float ssL = us - abx / sqL, ssR = us + abx / sqR;
+  const float isq = 1.0f / (sqL + sqR);
+  const float sg = copysignf(1.0f, bx);
+  const float vss = (sqL * vsL + sqR * vsR + (bysR - bysL) * sg) * isq;
+  const float byss = (sqL * bysR + sqR * bysL + sqL * sqR * (vsR - vsL) * sg) * isq;
+  const float wss = (sqL * wsL + sqR * wsR + (bzsR - bzsL) * sg) * isq;
+  const float bzss = (sqL * bzsR + sqR * bzsL + sqL * sqR * (wsR - wsL) * sg) * isq;
+  const float vbss = us * bx + vss * byss + wss * bzss;
+  const float EssL = EsL - sqL * (vbsL - vbss) * sg;
+  const float EssR = EsR + sqR * (vbsR - vbss) * sg;
+
+  if (g.flux == FLUX_LHLLD) {
+    float rho, u, v, w, E, by, bz, ptd, ptf;
+    if (g.all_mach && sL > 0.0f) {
+      rho = rhoL; u = uL; v = vL; w = wL; E = EL; by = byL; bz = bzL; ptd = pdL; ptf = ptL;
+    } else if (ssL > 0.0f) {
+      rho = rsL; u = us; v = vsL; w = wsL; E = EsL; by = bysL; bz = bzsL; ptd = psd; ptf = psf;
+    } else if (us > 0.0f) {
+      rho = rsL; u = us; v = vss; w = wss; E = EssL; by = byss; bz = bzss; ptd = psd; ptf = psf;
+    } else if (ssR > 0.0f) {
+      rho = rsR; u = us; v = vss; w = wss; E = EssR; by = byss; bz = bzss; ptd = psd; ptf = psf;
+    } else if (!g.all_mach || sR > 0.0f) {
+      rho = rsR; u = us; v = vsR; w = wsR; E = EsR; by = bysR; bz = bzsR; ptd = psd; ptf = psf;
+    } else {
+      rho = rhoR; u = uR; v = vR; w = wR; E = ER; by = byR; bz = bzR; ptd = pdR; ptf = ptR;
+    }
+    F[0] = rho * u;
+    F[1] = rho * u * u + ptd - bx2;
+    F[2] = rho * u * v - bx * by;
+    if (SD == 3) F[3] = rho * u * w - bx * bz;
+    F[IE] = (E + ptf) * u - bx * (bx * u + v * by + w * bz);
+    if (NAS > 0) {
+      const float s2 = copysignf(1.0f, u), hp = 0.5f * (1.0f + s2), hm = 0.5f * (1.0f - s2);
+#pragma unroll
+      for (int iv = IA; iv < NV; iv++) F[iv] = rho * u * (hp * qL[iv] + hm * qR[iv]);
+    }
+    Fb[0] = 0.0f;
+    Fb[1] = u * by - bx * v;
+    Fb[2] = u * bz - bx * w;
+    return;
+  }
+  const bool left = us > 0.0f;
+  const float rho = left ? rhoL : rhoR, u = left ? uL : uR, v = left ? vL : vR,
+              w = left ? wL : wR, E = left ? EL : ER, ptf = left ? ptL : ptR,
+              ptd = left ? pdL : pdR, by = left ? byL : byR, bz = left ? bzL : bzR,
+              vb = left ? vbL : vbR;
+  const float *q = left ? qL : qR;
+  float f[NV], U[NV], fb[3], Ub[3];
+  U[0] = rho; U[1] = rho * u; U[2] = rho * v;
+  if (SD == 3) U[3] = rho * w;
+  U[IE] = E;
+  f[0] = rho * u;
+  f[1] = rho * u * u + ptd - bx2;
+  f[2] = rho * u * v - bx * by;
+  if (SD == 3) f[3] = rho * u * w - bx * bz;
+  f[IE] = (E + ptf) * u - bx * vb;
+#pragma unroll
+  for (int iv = IA; iv < NV; iv++) {
+    U[iv] = rho * q[iv];
+    f[iv] = rho * u * q[iv];
+  }
+  Ub[0] = bx; Ub[1] = by; Ub[2] = bz;
+  fb[0] = 0.0f;
+  fb[1] = u * by - bx * v;
+  fb[2] = u * bz - bx * w;
+  const float s = left ? sL : sR;
+  const bool outer = left ? (sL > 0.0f) : (sR < 0.0f);
+  if (outer) {
+#pragma unroll
+    for (int iv = 0; iv < NV; iv++) F[iv] = f[iv];
+#pragma unroll
+    for (int iv = 0; iv < 3; iv++) Fb[iv] = fb[iv];
+    return;
+  }
+  const float rs = left ? rsL : rsR, vs = left ? vsL : vsR, ws = left ? wsL : wsR,
+              Es = left ? EsL : EsR, bys = left ? bysL : bysR, bzs = left ? bzsL : bzsR;
+  float Us[NV], Ubs[3];
+  Us[0] = rs; Us[1] = rs * us; Us[2] = rs * vs;
+  if (SD == 3) Us[3] = rs * ws;
+  Us[IE] = Es;
+#pragma unroll
+  for (int iv = IA; iv < NV; iv++) Us[iv] = rs * q[iv];
+  Ubs[0] = bx; Ubs[1] = bys; Ubs[2] = bzs;
+  const bool single = left ? (ssL > 0.0f) : (ssR < 0.0f);
+  if (single) {
+#pragma unroll
+    for (int iv = 0; iv < NV; iv++) F[iv] = f[iv] + s * (Us[iv] - U[iv]);
+#pragma unroll
+    for (int iv = 0; iv < 3; iv++) Fb[iv] = fb[iv] + s * (Ubs[iv] - Ub[iv]);
+    return;
+  }
+  const float ss = left ? ssL : ssR, Ess = left ? EssL : EssR;
+  float Uss[NV], Ubss[3];
+  Uss[0] = rs; Uss[1] = rs * us; Uss[2] = rs * vss;
+  if (SD == 3) Uss[3] = rs * wss;
+  Uss[IE] = Ess;
+#pragma unroll
+  for (int iv = IA; iv < NV; iv++) Uss[iv] = rs * q[iv];
+  Ubss[0] = bx; Ubss[1] = byss; Ubss[2] = bzss;
+#pragma unroll
+  for (int iv = 0; iv < NV; iv++) F[iv] = f[iv] + s * (Us[iv] - U[iv]) + ss * (Uss[iv] - Us[iv]);
+#pragma unroll
+  for (int iv = 0; iv < 3; iv++) Fb[iv] = fb[iv] + s * (Ubs[iv] - Ub[iv]) + ss * (Ubss[iv] - Ubs[iv]);
+}
+
+}  // namespace mx
 
 }  // namespace phlx

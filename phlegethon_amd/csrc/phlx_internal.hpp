@@ -3,6 +3,7 @@
 
 #include "phlx_device.hpp"
 #include "phlx_helm.hpp"
+#include "phlx_mlp.hpp"
 
 namespace phlx {
 
@@ -12,12 +13,13 @@ struct KernelTable {
   void (*corner)(const Grid &, hipStream_t);
   void (*faceb)(const Grid &, int, double, double, double, hipStream_t);
   void (*update)(const Grid &, int, int, double, double, double, hipStream_t);
+  void (*update_guess)(const Grid &, int, int, double, double, double, hipStream_t);
   void (*cfl)(const Grid &, hipStream_t);
   void (*init)(const Grid &, hipStream_t);
   void (*divb)(const Grid &, hipStream_t);
 };
 
-bool select_kernels(int sd, int nas, bool mhd, bool helm, KernelTable &t);
+bool select_kernels(int sd, int nas, bool mhd, bool helm, bool mixed, KernelTable &t);
 void launch_step_begin(Scalars *sc, hipStream_t st);
 void launch_step_end(const Grid &g, int advance, hipStream_t st);
 void launch_reduce_to(const Grid &g, double *out, hipStream_t st);
@@ -36,6 +38,26 @@ void launch_helm_rhoe(const HelmDev &h, long long n, const double *rho, const do
 void launch_helm_rhoP(const HelmDev &h, long long n, const double *rho, const double *p,
                       const double *abar, const double *zbar, double *T, double *E, double *cs,
                       int ret, int *iters, hipStream_t st);
+
+// T(rho, e) surrogate (phlx_mlp.hip)
+void launch_mlp_data(const HelmDev &h, long long n, unsigned long long seed, const double box[8],
+                     float *raw, float *label, hipStream_t st);
+void launch_mlp_normalize(const MlpDev &m, long long n, const float *raw, const float *label,
+                          float *xn, float *tn, hipStream_t st);
+int launch_mlp_grad(const float *theta, const float *w1q, const float *w3q, const float *xn,
+                    const float *tn, long long n0, long long nb, float *gpart, float *lpart,
+                    hipStream_t st);
+void launch_mlp_adam(float *theta, float *m, float *v, const float *gpart, int nblk, int t,
+                     float lr, float *w1q, float *w3q, unsigned short *w2t, float *gout,
+                     hipStream_t st);
+void launch_mlp_quantize(const float *theta, float *w1q, float *w3q, unsigned short *w2t,
+                         hipStream_t st);
+void launch_mlp_forward(const MlpDev &m, long long n, const float *raw, float *y, int *inbox,
+                        hipStream_t st);
+void launch_helm_rhoe_guess(const HelmDev &h, const MlpDev &m, int mode, long long n,
+                            const double *rho, const double *e, const double *abar,
+                            const double *zbar, double *T, double *P, double *cs, int ret,
+                            int *iters, int *fb, hipStream_t st);
 
 enum { KID_FLUX1 = 0, KID_FLUX2, KID_FLUX3, KID_CORNER, KID_FACEB, KID_UPDATE, KID_MISC, KID_COUNT };
 

@@ -156,6 +156,148 @@ __global__ void __launch_bounds__(256) k_flux(Grid g) {
   for (int v = 0; v < NV; v++) g.F[DIR][v * nf + f] = F[v];
 }
 
+// ------------------------------------------------------- flux, mixed precision
+// As k_flux with the mixed-precision policy: cells are loaded and the WB
+// deviations formed in FP64, then reconstruction, wave speeds and the
+// Riemann solve run in FP32 (deviation form, phlx_device.hpp); the face
+// equilibrium is added back in FP64 before the cast; fluxes are stored as
+// FP32 and accumulated in FP64 by k_update.
+template <int DIR, int SD, int NAS, bool MHD, bool HELM>
+__global__ void __launch_bounds__(256) k_flux_mx(Grid g) {
+  constexpr int NV = SD + 2 + NAS, IE = SD + 1;
+  if (!g.sc->active) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = blockIdx.z;
+  const int e1 = g.n1 + (DIR == 0), e2 = g.n2 + (DIR == 1);
+  if (i >= e1 || j >= e2) return;
+
+  float q[4][NV], b[4][3], ga[4][2];
+#pragma unroll
+  for (int s = 0; s < 4; s++) {
+    const int o = s - 2;
+    double qd[NV], bd[3] = {0.0, 0.0, 0.0}, gd[2] = {0.0, 0.0};
+    load_cell<SD, NV, MHD, HELM>(g, i + (DIR == 0 ? o : 0), j + (DIR == 1 ? o : 0),
+                                 k + (DIR == 2 ? o : 0), qd, bd, gd);
+#pragma unroll
+    for (int v = 0; v < NV; v++) q[s][v] = (float)qd[v];
+#pragma unroll
+    for (int v = 0; v < 3; v++) b[s][v] = (float)bd[v];
+    ga[s][0] = (float)gd[0];
+    ga[s][1] = (float)gd[1];
+  }
+  float qL[NV], qR[NV], bL[3] = {0.0f, 0.0f, 0.0f}, bR[3] = {0.0f, 0.0f, 0.0f};
+  float l, r;
+#pragma unroll
+  for (int v = 0; v < NV; v++) {
+    if (g.recon == REC_LIM2ND) {
+      mx::rec_lim2nd(q[0][v], q[1][v], q[2][v], l, qL[v]);
+      mx::rec_lim2nd(q[1][v], q[2][v], q[3][v], qR[v], r);
+    } else if (v <= IE) {
+      mx::rec_pph(q[0][v], q[1][v], q[2][v], l, qL[v]);
+      mx::rec_pph(q[1][v], q[2][v], q[3][v], qR[v], r);
+    } else {
+      mx::rec_pph_lim(q[0][v], q[1][v], q[2][v], l, qL[v]);
+      mx::rec_pph_lim(q[1][v], q[2][v], q[3][v], qR[v], r);
+    }
+  }
+  const long long f = fix(g, DIR, i, j, k);
+  if (MHD) {
+#pragma unroll
+    for (int c = 0; c < SD; c++) {
+      if (c == DIR) continue;
+      if (g.recon == REC_LIM2ND) {
+        mx::rec_lim2nd(b[0][c], b[1][c], b[2][c], l, bL[c]);
+        mx::rec_lim2nd(b[1][c], b[2][c], b[3][c], bR[c], r);
+      } else {
+        mx::rec_pph(b[0][c], b[1][c], b[2][c], l, bL[c]);
+        mx::rec_pph(b[1][c], b[2][c], b[3][c], bR[c], r);
+      }
+    }
+    const float bn = (float)g.bf[DIR][f];
+    bL[DIR] = bn;
+    bR[DIR] = bn;
+  }
+  if (DIR == 1) {
+    float t = qL[1]; qL[1] = qL[2]; qL[2] = -t;
+    t = qR[1]; qR[1] = qR[2]; qR[2] = -t;
+    if (MHD) {
+      t = bL[0]; bL[0] = bL[1]; bL[1] = -t;
+      t = bR[0]; bR[0] = bR[1]; bR[1] = -t;
+    }
+  } else if (DIR == 2) {
+    float t = qL[1]; qL[1] = qL[3]; qL[3] = qL[2]; qL[2] = t;
+    t = qR[1]; qR[1] = qR[3]; qR[3] = qR[2]; qR[2] = t;
+    if (MHD) {
+      t = bL[0]; bL[0] = bL[2]; bL[2] = bL[1]; bL[1] = t;
+      t = bR[0]; bR[0] = bR[2]; bR[2] = bR[1]; bR[1] = t;
+    }
+  }
+  const int jy = j;
+  // full face pressures for the wave speeds / energies; the pressure slot
+  // keeps the deviation (FP64 add-back of the face equilibrium, then cast)
+  float pfL = qL[IE], pfR = qR[IE];
+  if (g.wb) {
+    const double req = (DIR == 1) ? g.eq_rho_f[jy] : g.eq_rho_cc[jy];
+    const double peq = (DIR == 1) ? g.eq_p_f[jy] : g.eq_p_cc[jy];
+    qL[0] = (float)(req + (double)qL[0]);
+    qR[0] = (float)(req + (double)qR[0]);
+    pfL = (float)(peq + (double)qL[IE]);
+    pfR = (float)(peq + (double)qR[IE]);
+  }
+  float gL[2] = {0.0f, 0.0f}, gR[2] = {0.0f, 0.0f};
+  if (HELM) {
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+      if (g.recon == REC_LIM2ND) {
+        mx::rec_lim2nd(ga[0][c], ga[1][c], ga[2][c], l, gL[c]);
+        mx::rec_lim2nd(ga[1][c], ga[2][c], ga[3][c], gR[c], r);
+      } else {
+        mx::rec_pph(ga[0][c], ga[1][c], ga[2][c], l, gL[c]);
+        mx::rec_pph(ga[1][c], ga[2][c], ga[3][c], gR[c], r);
+      }
+    }
+    if (g.wb) {
+      const double geq = (DIR == 1) ? g.eq_ge_f[jy] : g.eq_ge_cc[jy];
+      gL[0] = (float)(geq + (double)gL[0]);
+      gR[0] = (float)(geq + (double)gR[0]);
+    }
+  }
+  float F[NV], Fb[3] = {0.0f, 0.0f, 0.0f};
+  if (MHD)
+    mx::riemann_mhd<SD, NAS, HELM>(g, qL, qR, pfL, pfR, bL, bR, gL, gR, F, Fb);
+  else
+    mx::riemann_hydro<SD, NAS, HELM>(g, qL, qR, pfL, pfR, gL, gR, F);
+  if (DIR == 1) {
+    float t = F[1]; F[1] = -F[2]; F[2] = t;
+    if (MHD) { t = Fb[0]; Fb[0] = -Fb[1]; Fb[1] = t; }
+  } else if (DIR == 2) {
+    float t = F[1]; F[1] = F[2]; F[2] = F[3]; F[3] = t;
+    if (MHD) { t = Fb[0]; Fb[0] = Fb[1]; Fb[1] = Fb[2]; Fb[2] = t; }
+  }
+  const long long nf = g.nf[DIR];
+  if (MHD) {
+    double e0, e1v;
+    if (DIR == 0) { e0 = Fb[2]; e1v = -Fb[1]; }
+    else if (DIR == 1) { e0 = -Fb[2]; e1v = Fb[0]; }
+    else { e0 = Fb[1]; e1v = -Fb[0]; }
+    g.emf[DIR][f] = e0;
+    g.emf[DIR][nf + f] = e1v;
+    const double den = 0.5 * ((double)qL[0] + (double)qR[0]) * g.sc->smax;
+    const double mach = (double)F[0] / den;
+    signed char sgn = 0;
+    if (mach > g.mach_ct) sgn = 1;
+    if (mach < -g.mach_ct) sgn = -1;
+    g.sgn[DIR][f] = sgn;
+  }
+  if (g.etot) {
+    const double phi = (DIR == 1) ? g.phi_f[jy] : g.phi_cc[jy];
+    F[IE] = (float)((double)F[IE] + (double)F[0] * phi);
+  }
+#pragma unroll
+  for (int v = 0; v < NV; v++) g.Ff[DIR][v * nf + f] = F[v];
+}
+
 // ------------------------------------------------------------------ CT
 // Face-centred EMF component `comp` (0..2 = E1..E3) of the face array of
 // direction fd at a possibly-ghost transverse position; ghosts follow the
@@ -384,7 +526,17 @@ __device__ __forceinline__ double cell_speed(const Grid &g, const double *p, con
 // gravity / heating sources, RK combine, b_cc from the updated faces,
 // cons->prim.  At the last stage the cell's CFL signal speed joins the
 // device-wide max.
-template <int SD, int NAS, bool MHD, bool HELM>
+template <bool MIX>
+__device__ __forceinline__ double flux_at(const Grid &g, int d, long long idx) {
+  return MIX ? (double)g.Ff[d][idx] : g.F[d][idx];
+}
+
+// GUESS (Helmholtz only): the T(rho, e) start comes from g.eos_guess (warm,
+// surrogate or bracketed) and the Newton iterations / fallbacks are counted
+// per block.  The host admits this variant only when every thread of every
+// block is an interior cell (nx1 % 64 == 0, nx2 % 4 == 0), so the surrogate's
+// wave-wide MFMA and block barriers are reached uniformly.
+template <int SD, int NAS, bool MHD, bool HELM, bool MIX, bool GUESS = false>
 __global__ void __launch_bounds__(256) k_update(Grid g, int first, int last, double a1,
                                                 double a2, double c3) {
   constexpr int NV = SD + 2 + NAS, IE = SD + 1, IA = SD + 2;
@@ -433,19 +585,22 @@ __global__ void __launch_bounds__(256) k_update(Grid g, int first, int last, dou
       const long long nf = g.nf[0];
       const long long fm = fix(g, 0, i, j, k), fp = fm + 1;
 #pragma unroll
-      for (int v = 0; v < NV; v++) R[v] = (g.F[0][v * nf + fp] - g.F[0][v * nf + fm]) * g.inv_dx[0];
+      for (int v = 0; v < NV; v++)
+        R[v] = (flux_at<MIX>(g, 0, v * nf + fp) - flux_at<MIX>(g, 0, v * nf + fm)) * g.inv_dx[0];
     }
     {
       const long long nf = g.nf[1];
       const long long fm = fix(g, 1, i, j, k), fp = fm + g.n1;
 #pragma unroll
-      for (int v = 0; v < NV; v++) R[v] = R[v] + (g.F[1][v * nf + fp] - g.F[1][v * nf + fm]) * g.inv_dx[1];
+      for (int v = 0; v < NV; v++)
+        R[v] = R[v] + (flux_at<MIX>(g, 1, v * nf + fp) - flux_at<MIX>(g, 1, v * nf + fm)) * g.inv_dx[1];
     }
     if (SD == 3) {
       const long long nf = g.nf[2];
       const long long fm = fix(g, 2, i, j, k), fp = fm + (long long)g.n1 * g.n2;
 #pragma unroll
-      for (int v = 0; v < NV; v++) R[v] = R[v] + (g.F[2][v * nf + fp] - g.F[2][v * nf + fm]) * g.inv_dx[2];
+      for (int v = 0; v < NV; v++)
+        R[v] = R[v] + (flux_at<MIX>(g, 2, v * nf + fp) - flux_at<MIX>(g, 2, v * nf + fm)) * g.inv_dx[2];
     }
     if (g.grav) {
       // source/source.F90:12606-12644 on the WB deviation rho'
@@ -487,7 +642,36 @@ __global__ void __launch_bounds__(256) k_update(Grid g, int first, int last, dou
       const double zbar = ye * abar;
       double pr, sound;
       T = g.temp[c];
-      helm_rhoe_given(g.helm, rho, eint * ir, abar, zbar, T, pr, sound, true);
+      if constexpr (GUESS) {
+        __shared__ unsigned int lds[4 * kMlpLdsWaveDw];
+        __shared__ unsigned int cnt[3];
+        const int mode = g.eos_guess;
+        double tg = -1.0;
+        if (mode == GUESS_MLP) {
+          float x[4];
+          bool ok;
+          mlp_inputs(g.mlp, rho, eint * ir, abar, zbar, x, ok);
+          const float y = mlp_forward_wave(g.mlp, x, lds + threadIdx.y * kMlpLdsWaveDw);
+          if (ok) tg = exp10((double)(y * g.mlp.out_scale + g.mlp.out_off));
+        }
+        int fb;
+        const int it = helm_rhoe_guess(g.helm, mode, tg, g.mlp.resid_tol, rho, eint * ir, abar,
+                                       zbar, T, pr, sound, true, fb);
+        const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+        if (tid < 3) cnt[tid] = 0u;
+        __syncthreads();
+        atomicAdd(&cnt[0], (unsigned int)it);
+        atomicAdd(&cnt[1], 1u);
+        atomicAdd(&cnt[2], (unsigned int)fb);
+        __syncthreads();
+        if (tid < 3) {
+          const long long blk =
+              blockIdx.x + (long long)gridDim.x * (blockIdx.y + (long long)gridDim.y * blockIdx.z);
+          g.eos_part[3 * blk + tid] = g.eos_part[3 * blk + tid] + cnt[tid];
+        }
+      } else {
+        helm_rhoe_given(g.helm, rho, eint * ir, abar, zbar, T, pr, sound, true);
+      }
       g.gam[c] = pr / eint + 1.0;
       g.gam[N + c] = sound * sound * rho / pr;
       g.temp[c] = T;
@@ -622,9 +806,18 @@ static dim3 grid_for(int e1, int e2, int e3, dim3 blk) {
   return dim3((e1 + blk.x - 1) / blk.x, (e2 + blk.y - 1) / blk.y, e3);
 }
 
-template <int SD, int NAS, bool MHD, bool HELM>
+template <int SD, int NAS, bool MHD, bool HELM, bool MIX>
 static void launch_flux(const Grid &g, int dir, hipStream_t st) {
   const dim3 blk(64, 4, 1);
+  if (MIX) {
+    if (dir == 0)
+      k_flux_mx<0, SD, NAS, MHD, HELM><<<grid_for(g.n1 + 1, g.n2, g.n3, blk), blk, 0, st>>>(g);
+    else if (dir == 1)
+      k_flux_mx<1, SD, NAS, MHD, HELM><<<grid_for(g.n1, g.n2 + 1, g.n3, blk), blk, 0, st>>>(g);
+    else if (SD == 3)
+      k_flux_mx<2, SD, NAS, MHD, HELM><<<grid_for(g.n1, g.n2, g.n3 + 1, blk), blk, 0, st>>>(g);
+    return;
+  }
   if (dir == 0)
     k_flux<0, SD, NAS, MHD, HELM><<<grid_for(g.n1 + 1, g.n2, g.n3, blk), blk, 0, st>>>(g);
   else if (dir == 1)
@@ -633,10 +826,10 @@ static void launch_flux(const Grid &g, int dir, hipStream_t st) {
     k_flux<2, SD, NAS, MHD, HELM><<<grid_for(g.n1, g.n2, g.n3 + 1, blk), blk, 0, st>>>(g);
 }
 
-template <int SD, int NAS, bool MHD, bool HELM>
+template <int SD, int NAS, bool MHD, bool HELM, bool MIX>
 struct Kern {
   static void flux(const Grid &g, int dir, hipStream_t st) {
-    launch_flux<SD, NAS, MHD, HELM>(g, dir, st);
+    launch_flux<SD, NAS, MHD, HELM, MIX>(g, dir, st);
   }
   static void corner(const Grid &g, hipStream_t st) {
     const dim3 blk(64, 4, 1);
@@ -650,7 +843,13 @@ struct Kern {
   static void update(const Grid &g, int first, int last, double a1, double a2, double c3,
                      hipStream_t st) {
     const dim3 blk(64, 4, 1);
-    k_update<SD, NAS, MHD, HELM><<<grid_for(g.n1, g.n2, g.n3, blk), blk, 0, st>>>(
+    k_update<SD, NAS, MHD, HELM, MIX><<<grid_for(g.n1, g.n2, g.n3, blk), blk, 0, st>>>(
+        g, first, last, a1, a2, c3);
+  }
+  static void update_guess(const Grid &g, int first, int last, double a1, double a2, double c3,
+                           hipStream_t st) {
+    const dim3 blk(64, 4, 1);
+    k_update<SD, NAS, MHD, HELM, MIX, true><<<grid_for(g.n1, g.n2, g.n3, blk), blk, 0, st>>>(
         g, first, last, a1, a2, c3);
   }
   static void cfl(const Grid &g, hipStream_t st) {
@@ -667,39 +866,46 @@ struct Kern {
   }
 };
 
-template <int SD, int NAS, bool MHD, bool HELM>
+template <int SD, int NAS, bool MHD, bool HELM, bool MIX>
 static KernelTable make_table() {
-  using K = Kern<SD, NAS, MHD, HELM>;
+  using K = Kern<SD, NAS, MHD, HELM, MIX>;
   KernelTable t;
   t.flux = &K::flux;
   t.corner = &K::corner;
   t.faceb = &K::faceb;
   t.update = &K::update;
+  t.update_guess = HELM ? &K::update_guess : nullptr;
   t.cfl = &K::cfl;
   t.init = &K::init;
   t.divb = &K::divb;
   return t;
 }
 
-bool select_kernels(int sd, int nas, bool mhd, bool helm, KernelTable &t) {
+template <bool MIX>
+static bool select_prec(int sd, int nas, bool mhd, bool helm, KernelTable &t) {
   if (helm) {
     // Helmholtz runs carry Ye and 1/abar as the two advected scalars
     if (nas != 2) return false;
-    if (sd == 2 && !mhd) t = make_table<2, 2, false, true>();
-    else if (sd == 3 && !mhd) t = make_table<3, 2, false, true>();
-    else if (sd == 3 && mhd) t = make_table<3, 2, true, true>();
+    if (sd == 2 && !mhd) t = make_table<2, 2, false, true, MIX>();
+    else if (sd == 3 && !mhd) t = make_table<3, 2, false, true, MIX>();
+    else if (sd == 3 && mhd) t = make_table<3, 2, true, true, MIX>();
     else return false;
     return true;
   }
-  if (sd == 2 && nas == 0 && !mhd) t = make_table<2, 0, false, false>();
-  else if (sd == 2 && nas == 0 && mhd) t = make_table<2, 0, true, false>();
-  else if (sd == 3 && nas == 0 && !mhd) t = make_table<3, 0, false, false>();
-  else if (sd == 3 && nas == 0 && mhd) t = make_table<3, 0, true, false>();
-  else if (sd == 2 && nas == 2 && !mhd) t = make_table<2, 2, false, false>();
-  else if (sd == 3 && nas == 2 && !mhd) t = make_table<3, 2, false, false>();
-  else if (sd == 3 && nas == 2 && mhd) t = make_table<3, 2, true, false>();
+  if (sd == 2 && nas == 0 && !mhd) t = make_table<2, 0, false, false, MIX>();
+  else if (sd == 2 && nas == 0 && mhd) t = make_table<2, 0, true, false, MIX>();
+  else if (sd == 3 && nas == 0 && !mhd) t = make_table<3, 0, false, false, MIX>();
+  else if (sd == 3 && nas == 0 && mhd) t = make_table<3, 0, true, false, MIX>();
+  else if (sd == 2 && nas == 2 && !mhd) t = make_table<2, 2, false, false, MIX>();
+  else if (sd == 3 && nas == 2 && !mhd) t = make_table<3, 2, false, false, MIX>();
+  else if (sd == 3 && nas == 2 && mhd) t = make_table<3, 2, true, false, MIX>();
   else return false;
   return true;
+}
+
+bool select_kernels(int sd, int nas, bool mhd, bool helm, bool mixed, KernelTable &t) {
+  return mixed ? select_prec<true>(sd, nas, mhd, helm, t)
+               : select_prec<false>(sd, nas, mhd, helm, t);
 }
 
 void launch_step_begin(Scalars *sc, hipStream_t st) { k_step_begin<<<1, 1, 0, st>>>(sc); }

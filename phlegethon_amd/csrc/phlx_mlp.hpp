@@ -1,0 +1,197 @@
+// phlx_mlp.hpp -- the T(rho, e) surrogate: a 4 -> 64 -> 64 -> 1 tanh MLP
+// that supplies the initial guess of the Helmholtz Newton inversion
+// (helm_rhoe_given, source/source.F90:24982-25019; the reference starts from
+// the carried temperature and has no surrogate).
+//
+// Inputs (log10 rho, log10 e, abar, zbar) are normalised affinely to about
+// [-1, 1]; the output is normalised log10 T.  Inference runs one wavefront
+// per 64 zones: layer 1 (4 -> 64) and layer 3 (64 -> 1) on the VALU in FP32
+// with BF16-rounded weights, layer 2 (64 x 64) on the matrix cores with
+// v_mfma_f32_16x16x16bf16_1k (BF16 operands, FP32 accumulation): 16 tiles x
+// 4 k-steps = 64 MFMA per wavefront.  The exact-table Newton then runs from
+// the guess to the reference's tolerance, so the surrogate changes the
+// iteration count, never the converged state beyond that tolerance; a guess
+// whose inputs leave the training box, or whose first Newton residual
+// |e(T_g) - e| / e exceeds a threshold, is replaced by the bracketed guess
+// (bisection in log T over the table range).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "phlx_helm.hpp"
+
+namespace phlx {
+
+constexpr int kMlpIn = 4, kMlpH = 64;
+// flat parameter vector: w1 [4][64], b1 [64], w2 [64][64], b2 [64], w3 [64], b3
+constexpr int kMlpOffW1 = 0, kMlpOffB1 = 256, kMlpOffW2 = 320, kMlpOffB2 = 4416,
+              kMlpOffW3 = 4480, kMlpOffB3 = 4544, kMlpNParam = 4545;
+constexpr int kBracketSteps = 16;  // bisection steps of the fallback guess
+
+enum { GUESS_WARM = 0, GUESS_MLP = 1, GUESS_BRACKET = 2 };
+
+typedef short mlp_s4 __attribute__((ext_vector_type(4)));
+typedef float mlp_f4 __attribute__((ext_vector_type(4)));
+
+struct MlpDev {
+  const float *theta;   // FP32 master parameters (biases are read from here)
+  const float *w1q;     // BF16-rounded w1 as FP32, [4][64]
+  const float *w3q;     // BF16-rounded w3 as FP32, [64]
+  const mlp_s4 *w2t;    // w2 as BF16 MFMA B tiles: [kb][nb][lane] x 4 (k = 16kb + 4(lane/16) + j)
+  float in_off[4], in_scale[4];  // x = (raw - off) * scale
+  float out_off, out_scale;      // log10 T = y * out_scale + out_off
+  float resid_tol;               // fallback when the first residual exceeds this
+  float box;                     // accepted |x| (training box + margin)
+};
+
+// LDS per wavefront for the inference: h1 as BF16 rows of 33 dwords (64
+// values + 1 pad dword against bank conflicts) and the layer-3 partial sums
+// (64 rows x 17 floats).
+constexpr int kMlpLdsH1Dw = 64 * 33;
+constexpr int kMlpLdsPart = 64 * 17;
+constexpr int kMlpLdsWaveDw = kMlpLdsH1Dw + kMlpLdsPart;
+
+__device__ __forceinline__ unsigned short bf16_bits(float f) {
+  unsigned int u = __float_as_uint(f);
+  u = u + 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even (finite inputs)
+  return (unsigned short)(u >> 16);
+}
+__device__ __forceinline__ float bf16_round(float f) {
+  return __uint_as_float(((unsigned int)bf16_bits(f)) << 16);
+}
+// tanh(z) = 1 - 2 / (exp(2z) + 1): exp2 + reciprocal, ~1e-7 relative, exact
+// limits; far below the BF16 resolution of the activations it feeds.
+__device__ __forceinline__ float fast_tanh(float z) {
+  z = fminf(fmaxf(z, -15.0f), 15.0f);
+  const float e = __expf(2.0f * z);
+  return 1.0f - 2.0f / (e + 1.0f);
+}
+
+__device__ __forceinline__ void mlp_inputs(const MlpDev &m, double rho, double e, double abar,
+                                           double zbar, float x[4], bool &inbox) {
+  const float raw[4] = {(float)log10(rho), (float)log10(fabs(e) > 0.0 ? fabs(e) : 1e-300),
+                        (float)abar, (float)zbar};
+  inbox = e > 0.0;
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    x[i] = (raw[i] - m.in_off[i]) * m.in_scale[i];
+    inbox = inbox && fabsf(x[i]) <= m.box;
+  }
+  if (!inbox) {
+#pragma unroll
+    for (int i = 0; i < 4; i++) x[i] = 0.0f;  // keep the wave's arithmetic finite
+  }
+}
+
+// Normalised network output for this lane's zone.  Every lane of the
+// wavefront must call it (MFMA and LDS exchange are wave-wide); lds points
+// at this wavefront's kMlpLdsWaveDw dwords.  Ends with all lanes having
+// passed a block barrier, so all waves of the block must call it together.
+__device__ __forceinline__ float mlp_forward_wave(const MlpDev &m, const float x[4],
+                                                  unsigned int *lds) {
+  const int lane = threadIdx.x & 63;
+  const float *b1 = m.theta + kMlpOffB1;
+  const float *b2 = m.theta + kMlpOffB2;
+  unsigned int *h1 = lds;
+  float *part = (float *)(lds + kMlpLdsH1Dw);
+  // layer 1 (VALU): this lane's 64 hidden units, stored as BF16 pairs
+#pragma unroll 4
+  for (int n = 0; n < kMlpH; n += 2) {
+    float z0 = b1[n], z1 = b1[n + 1];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      z0 = z0 + x[i] * m.w1q[i * kMlpH + n];
+      z1 = z1 + x[i] * m.w1q[i * kMlpH + n + 1];
+    }
+    const unsigned int lo = bf16_bits(fast_tanh(z0)), hi = bf16_bits(fast_tanh(z1));
+    h1[lane * 33 + (n >> 1)] = lo | (hi << 16);
+  }
+  __syncthreads();
+  // layer 2 (MFMA 16x16x16 BF16): rows = zones 16rb..16rb+15 of this wave;
+  // A lane l holds row l%16, k = 16kb + 4(l/16) .. +3; D lane l holds
+  // rows 4(l/16)+r, column l%16
+  const int row = lane & 15, kq = lane >> 4;
+  const float *w3 = m.w3q;
+#pragma unroll 1
+  for (int rb = 0; rb < 4; rb++) {
+    mlp_f4 acc[4];
+#pragma unroll
+    for (int nb = 0; nb < 4; nb++) acc[nb] = (mlp_f4){0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int kb = 0; kb < 4; kb++) {
+      const unsigned int *src = h1 + (16 * rb + row) * 33 + 8 * kb + 2 * kq;
+      const unsigned int d0 = src[0], d1 = src[1];
+      mlp_s4 a;
+      a.x = (short)(d0 & 0xFFFF);
+      a.y = (short)(d0 >> 16);
+      a.z = (short)(d1 & 0xFFFF);
+      a.w = (short)(d1 >> 16);
+#pragma unroll
+      for (int nb = 0; nb < 4; nb++)
+        acc[nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, m.w2t[(kb * 4 + nb) * 64 + lane],
+                                                            acc[nb], 0, 0, 0);
+    }
+    // epilogue: tanh(z2 + b2) * w3 summed over this lane's four columns
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      float s = 0.0f;
+#pragma unroll
+      for (int nb = 0; nb < 4; nb++) {
+        const int col = 16 * nb + row;
+        s = s + fast_tanh(acc[nb][r] + b2[col]) * w3[col];
+      }
+      part[(16 * rb + 4 * kq + r) * 17 + row] = s;
+    }
+  }
+  __syncthreads();
+  float y = m.theta[kMlpOffB3];
+#pragma unroll
+  for (int c = 0; c < 16; c++) y = y + part[lane * 17 + c];
+  return y;
+}
+
+// Fallback: bisection of e(T) - e0 in log10 T over [Tl, Th].
+__device__ __forceinline__ double helm_bracket_guess(const HelmDev &h, double rho, double E0,
+                                                     double abar, double zbar) {
+  double lo = log10(h.tl), hi = log10(h.th);
+#pragma unroll 1
+  for (int it = 0; it < kBracketSteps; it++) {
+    const double mid = 0.5 * (lo + hi);
+    double e, dedt;
+    helm_given2(h, rho, exp10(mid), abar, zbar, e, dedt, false);
+    if (e > E0) hi = mid; else lo = mid;
+  }
+  return exp10(0.5 * (lo + hi));
+}
+
+// T(rho, e) with a chosen start: the reference's Newton loop
+// (helm_rhoe_given) from T (GUESS_WARM: T as given; GUESS_MLP: T = tg, the
+// surrogate's guess, which must be > 0 or is rejected; GUESS_BRACKET: the
+// bisection guess).  A surrogate guess whose first residual exceeds
+// resid_tol restarts from the bracketed guess (fallback = 1).  Returns the
+// Newton iterations of the final solve.
+__device__ __forceinline__ int helm_rhoe_guess(const HelmDev &h, int mode, double tg,
+                                               float resid_tol, double rho, double E0,
+                                               double abar, double zbar, double &T, double &P,
+                                               double &sound, bool ret, int &fallback) {
+  fallback = 0;
+  if (mode == GUESS_BRACKET) {
+    T = helm_bracket_guess(h, rho, E0, abar, zbar);
+  } else if (mode == GUESS_MLP) {
+    bool ok = tg > 0.0 && tg >= h.tl && tg <= h.th;
+    if (ok) {
+      double e, dedt;
+      helm_given2(h, rho, tg, abar, zbar, e, dedt, false);
+      ok = fabs((e - E0) / E0) <= (double)resid_tol;
+    }
+    if (ok) {
+      T = tg;
+    } else {
+      fallback = 1;
+      T = helm_bracket_guess(h, rho, E0, abar, zbar);
+    }
+  }
+  return helm_rhoe_given(h, rho, E0, abar, zbar, T, P, sound, ret);
+}
+
+}  // namespace phlx

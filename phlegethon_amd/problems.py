@@ -389,3 +389,39 @@ def helm_slab(eos, table, n: int = 256, ny: Optional[int] = None, sdims: int = 3
         scale = np.sqrt(2.0 * P_cc[ny // 2] / mhd_beta) / bmag
         b_faces = [None if b is None else b * scale for b in b_faces]
     return Problem("helm_slab", cfg, prim, b_faces, prof, temp, table)
+
+
+# --------------------------------------------------------------------- C4
+def lowmach_box(n: int = 384, heating: float = 1e-6, sdims: int = 3, recon: int = LIM2ND_REC,
+                flux: int = LHLLC_FLUX, precision: int = 0, seed: int = SEED,
+                noise: float = 1e-6, mach_cutoff: float = 1e-5) -> Problem:
+    """Low-Mach convection box: ideal gas gamma = 5/3, constant gravity
+    g = -1 in x2, the C1 polytrope rho = (1 - 0.4 y)^1.5, p = rho^gamma on
+    the unit cube (periodic x1/x3, reflecting x2), heated at the bottom by
+    edot = heating * sin(8 pi y) for y < 1/8 (pattern of
+    tests/andrassy2022/app.F90:132-167), seeded density noise.  USE_WB +
+    USE_GRAVITY + EVOLVE_ETOT + USE_EDOT, LHLLC, RK2, CFL 0.8.  The flow
+    speed grows with the heating (Mach ~ heating^(1/3) once convective)."""
+    cfg = PhlxConfig(sdims=sdims, nx1=n, nx2=n, nx3=n if sdims == 3 else 1,
+                     recon=recon, flux=flux, rk_stages=2, cfl=0.8, mach_cutoff=mach_cutoff,
+                     bc=(BC_PERIODIC, BC_PERIODIC, BC_REFLECTIVE, BC_REFLECTIVE,
+                         BC_PERIODIC, BC_PERIODIC),
+                     use_wb=True, use_gravity=True, evolve_etot=True, use_edot=True,
+                     precision=precision).validate()
+    gm = cfg.gamma
+    yc = centres(cfg, 1)
+    yf = nodes(cfg, 1)
+    rho_cc = (1.0 - 0.4 * yc) ** 1.5
+    p_cc = rho_cc ** gm
+    rho_f = (1.0 - 0.4 * yf) ** 1.5
+    p_f = rho_f ** gm
+    edot = np.where(yc < 0.125, heating * np.sin(8.0 * math.pi * yc), 0.0)
+    shape = cfg.shape
+    N = int(np.prod(shape))
+    u = (2.0 * uniform01(seed, 11, N) - 1.0).reshape(shape)
+    prim = np.zeros((cfg.nvars,) + shape)
+    prim[0] = rho_cc[None, :, None] * (1.0 + noise * u)
+    prim[cfg.sdims + 1] = np.broadcast_to(p_cc[None, :, None], shape)
+    prof = dict(eq_rho_cc=rho_cc, eq_p_cc=p_cc, eq_rho_f=rho_f, eq_p_f=p_f,
+                grav_cc=np.full(n, -1.0), phi_cc=yc.copy(), phi_f=yf.copy(), edot_cc=edot)
+    return Problem("lowmach_box", cfg, prim, None, prof)

@@ -77,6 +77,16 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.phlx_helm_rhoT_full_batch.argtypes = [P, ctypes.c_int64] + [P] * 8
     lib.phlx_helm_rhoe_given_batch.argtypes = [P, ctypes.c_int64] + [P] * 7 + [I, P]
     lib.phlx_helm_rhoP_given_batch.argtypes = [P, ctypes.c_int64] + [P] * 7 + [I, P]
+    lib.phlx_helm_rhoe_given_guess_batch.argtypes = ([P, ctypes.c_int64] + [P] * 7 +
+                                                     [I, I, P, P])
+    lib.phlx_mlp_set_params.argtypes = [P, P, P]
+    lib.phlx_mlp_get_params.argtypes = [P, P, P]
+    lib.phlx_mlp_forward.argtypes = [P, ctypes.c_int64, P, P, P]
+    lib.phlx_mlp_grad.argtypes = [P, ctypes.c_int64, P, P, P, DP]
+    lib.phlx_mlp_train.argtypes = [P, ctypes.c_int64, I, I, D, ctypes.c_uint64, DP, DP]
+    lib.phlx_set_eos_guess.argtypes = [P, I, D, I]
+    I64P = ctypes.POINTER(ctypes.c_int64)
+    lib.phlx_eos_stats.argtypes = [P, I64P, I64P, I64P, I]
     _lib = lib
     return lib
 
@@ -92,6 +102,9 @@ def _chk(rc: int, what: str):
     if rc != 0:
         raise PhlxError(rc, what)
 
+
+MLP_NPARAM = 4545
+GUESS_MODES = {"warm": 0, "mlp": 1, "bracket": 2}
 
 KERNELS = {"flux_x1": 0, "flux_x2": 1, "flux_x3": 2, "corner": 3, "faceb": 4,
            "update": 5, "misc": 6}
@@ -255,6 +268,46 @@ class Simulation:
     def ctx(self):
         return self._ctx
 
+    # -- T(rho, e) surrogate ---------------------------------------------
+    def mlp_train(self, nsamples: int = 1 << 22, epochs: int = 4, batch: int = 4096,
+                  lr: float = 3e-3, seed: int = 20261202):
+        """Generate (rho, e, abar, zbar) -> T pairs with the loaded table and
+        train the surrogate on the GPU; returns (final batch loss, held-out
+        RMS error of log10 T)."""
+        loss, rmse = ctypes.c_double(), ctypes.c_double()
+        _chk(self.lib.phlx_mlp_train(self._ctx, int(nsamples), int(epochs), int(batch),
+                                     float(lr), int(seed), ctypes.byref(loss),
+                                     ctypes.byref(rmse)), "phlx_mlp_train")
+        return loss.value, rmse.value
+
+    def mlp_params(self):
+        theta = np.empty(MLP_NPARAM, dtype=np.float32)
+        norm = np.empty(10, dtype=np.float32)
+        _chk(self.lib.phlx_mlp_get_params(self._ctx, theta.ctypes.data_as(ctypes.c_void_p),
+                                          norm.ctypes.data_as(ctypes.c_void_p)),
+             "phlx_mlp_get_params")
+        return theta, norm
+
+    def mlp_set_params(self, theta, norm):
+        theta = np.ascontiguousarray(theta, dtype=np.float32)
+        norm = np.ascontiguousarray(norm, dtype=np.float32)
+        assert theta.size == MLP_NPARAM and norm.size == 10
+        _chk(self.lib.phlx_mlp_set_params(self._ctx, theta.ctypes.data_as(ctypes.c_void_p),
+                                          norm.ctypes.data_as(ctypes.c_void_p)),
+             "phlx_mlp_set_params")
+
+    def set_eos_guess(self, mode: str = "warm", resid_tol: float = 0.25, count: bool = False):
+        """T(rho, e) start of the per-zone Newton: "warm" (the carried T, as
+        the reference), "mlp" (surrogate, bracketed fallback) or "bracket"."""
+        _chk(self.lib.phlx_set_eos_guess(self._ctx, GUESS_MODES[mode], float(resid_tol),
+                                         int(count)), "phlx_set_eos_guess")
+
+    def eos_stats(self, reset: bool = False):
+        it, n, fb = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        _chk(self.lib.phlx_eos_stats(self._ctx, ctypes.byref(it), ctypes.byref(n),
+                                     ctypes.byref(fb), int(reset)), "phlx_eos_stats")
+        return {"newton_iters": it.value, "solves": n.value, "fallbacks": fb.value}
+
     # -- profiling -------------------------------------------------------
     def profile(self, on: bool):
         _chk(self.lib.phlx_profile_enable(self._ctx, int(on)), "phlx_profile_enable")
@@ -338,3 +391,42 @@ class HelmEoS:
             self._p(it)), "phlx_helm_rhoP_given_batch")
         self._sync_out()
         return tuple(x.cpu().numpy() for x in (T, E, cs, it))
+
+    # -- surrogate checks (device staging through torch) -----------------
+    def rhoe_given_guess(self, rho, e, abar, zbar, T0, mode: str = "mlp", ret=True):
+        (rho, e, abar, zbar, T), n = self._dev(rho, e, abar, zbar, T0)
+        P, cs = self._empty(n), self._empty(n)
+        it = self._empty(n, self.torch.int32)
+        fb = self._empty(n, self.torch.int32)
+        self._sync_in()
+        _chk(self.lib.phlx_helm_rhoe_given_guess_batch(
+            self.sim.ctx, n, *map(self._p, (rho, e, abar, zbar, T, P, cs)), int(ret),
+            GUESS_MODES[mode], self._p(it), self._p(fb)), "phlx_helm_rhoe_given_guess_batch")
+        self._sync_out()
+        return tuple(x.cpu().numpy() for x in (T, P, cs, it, fb))
+
+    def mlp_forward(self, raw):
+        raw = np.ascontiguousarray(raw, dtype=np.float32)
+        n = raw.shape[0]
+        r = self.torch.from_numpy(raw.copy()).to(self.device)
+        y = self._empty(n, self.torch.float32)
+        inbox = self._empty(n, self.torch.int32)
+        self._sync_in()
+        _chk(self.lib.phlx_mlp_forward(self.sim.ctx, n, self._p(r), self._p(y), self._p(inbox)),
+             "phlx_mlp_forward")
+        self._sync_out()
+        return y.cpu().numpy(), inbox.cpu().numpy()
+
+    def mlp_grad(self, xn, tn):
+        xn = np.ascontiguousarray(xn, dtype=np.float32)
+        tn = np.ascontiguousarray(tn, dtype=np.float32)
+        n = xn.shape[0]
+        x = self.torch.from_numpy(xn.copy()).to(self.device)
+        t = self.torch.from_numpy(tn.copy()).to(self.device)
+        g = self._empty(MLP_NPARAM, self.torch.float32)
+        loss = ctypes.c_double()
+        self._sync_in()
+        _chk(self.lib.phlx_mlp_grad(self.sim.ctx, n, self._p(x), self._p(t), self._p(g),
+                                    ctypes.byref(loss)), "phlx_mlp_grad")
+        self._sync_out()
+        return g.cpu().numpy(), loss.value
