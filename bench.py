@@ -92,6 +92,118 @@ def cpu_baseline(pb, steps: int, threads: int):
     return zones * steps / dt, dt
 
 
+def build_problem(args):
+    from phlegethon_amd import problems
+    n = args.n
+    if args.workload == "ot3d":
+        log(f"building 3D Orszag-Tang {n}^3 initial data")
+        pb = problems.orszag_tang_3d(n=n)
+        desc = f"3D Orszag-Tang ideal MHD + CT, {n}^3, LHLLD + LIM2ND, RK2, CFL 0.8, periodic"
+        data = "synthetic (seeded 3D Orszag-Tang initial data, counter-based PRNG)"
+    elif args.workload == "helm_slab":
+        from phlegethon_amd import helmholtz
+        from phlegethon_amd.runtime import HelmEoS
+        table = helmholtz.default_table()
+        mhd = args.mhd_beta is not None
+        log(f"building the Helmholtz slab {n}^3 (table {table.nrho}x{table.nT}, "
+            f"MHD beta {args.mhd_beta})")
+        eos = HelmEoS(table)
+        pb = problems.helm_slab(eos, table, n=n, mhd_beta=args.mhd_beta)
+        eos.close()
+        desc = (f"3D stratified Helmholtz slab, {n}^3, USE_FASTEOS + USE_WB + EVOLVE_ETOT + "
+                f"ADVECT_YE_IABAR, {'LHLLD + CT, beta ' + format(args.mhd_beta, 'g') if mhd else 'LHLLC'}"
+                f" + LIM2ND, RK2, CFL 0.8")
+        data = ("synthetic (plane-parallel rho/T/abar profiles, seeded 1e-4 density noise"
+                f"{', seeded random-phase divergence-free field' if mhd else ''}; generated "
+                f"{table.nrho}x{table.nT} Helmholtz table)")
+    else:
+        log(f"building the low-Mach heated box {n}^3")
+        pb = problems.lowmach_box(n=n, heating=args.heating)
+        desc = (f"3D low-Mach heated box, {n}^3, ideal gas + constant gravity, USE_WB + "
+                f"EVOLVE_ETOT + USE_EDOT, LHLLC + LIM2ND, RK2, CFL 0.8, heating {args.heating:g}")
+        data = "synthetic (polytrope + bottom heating layer, seeded 1e-6 density noise)"
+    return pb, desc, data
+
+
+def conserved(st, cfg):
+    import numpy as np
+    p = st["prim"]
+    rho = p[0]
+    out = {"rho": rho}
+    ke = np.zeros_like(rho)
+    for d in range(cfg.sdims):
+        out[f"m{d + 1}"] = rho * p[1 + d]
+        ke += 0.5 * rho * p[1 + d] ** 2
+    E = st["eint"] + ke
+    if cfg.mhd:
+        E += 0.5 * (st["b_cc"] ** 2).sum(axis=0)
+    out["E"] = E
+    for v in range(cfg.sdims + 2, cfg.nvars):
+        out[f"rho_s{v - cfg.sdims - 2}"] = rho * p[v]
+    return out
+
+
+def run_variant(pb, cfg, args, guess="warm", keep_state=False, profile=True):
+    """Upload, init, warm up, time args.steps steps; optionally profile."""
+    from phlegethon_amd.runtime import Simulation
+    sim = Simulation(cfg)
+    sim.setup(pb)
+    res = {}
+    if guess == "mlp":
+        t0 = time.perf_counter()
+        loss, rmse = sim.mlp_train(nsamples=args.mlp_samples, epochs=args.mlp_epochs,
+                                   batch=4096, lr=3e-3)
+        res["mlp"] = {"train_s": time.perf_counter() - t0, "samples": args.mlp_samples,
+                      "epochs": args.mlp_epochs, "final_batch_mse": loss,
+                      "heldout_rmse_log10T": rmse}
+        log(f"surrogate trained in {res['mlp']['train_s']:.2f} s, held-out RMS log10 T {rmse:.4f}")
+        sim.set_eos_guess("mlp", resid_tol=0.25, count=False)
+    sim.init()
+    log(f"warmup {args.warmup} steps")
+    sim.run(args.warmup)
+    sim.synchronize()
+    log(f"timing {args.steps} steps")
+    t0 = time.perf_counter()
+    sim.run(args.steps)
+    sim.synchronize()
+    elapsed = time.perf_counter() - t0
+    zones = cfg.nx1 * cfg.nx2 * (cfg.nx3 if cfg.sdims == 3 else 1)
+    res.update(value=zones * args.steps / elapsed, ms_per_step=1e3 * elapsed / args.steps,
+               scalars=sim.scalars(), divb=sim.divb_max())
+    log(f"{res['value']:.4e} zone-updates/s, {res['ms_per_step']:.3f} ms/step, "
+        f"t={res['scalars']['time']:.6g}, max|divB|={res['divb']:.2e}")
+    if keep_state:
+        res["state"] = sim.download()
+    if cfg.eos == 1:
+        # Newton iterations per solve over two counted steps
+        sim.set_eos_guess(guess, resid_tol=0.25, count=True)
+        sim.eos_stats(reset=True)
+        sim.run(2)
+        st = sim.eos_stats()
+        res["eos"] = {"newton_iters_per_solve": st["newton_iters"] / max(st["solves"], 1),
+                      "fallback_rate": st["fallbacks"] / max(st["solves"], 1),
+                      "guess": guess}
+        log(f"EoS: {res['eos']['newton_iters_per_solve']:.3f} Newton iterations per solve, "
+            f"fallback rate {res['eos']['fallback_rate']:.4f}")
+        sim.set_eos_guess(guess, resid_tol=0.25, count=False)
+    if profile:
+        sim.profile(True)
+        sim.run(args.profile_steps)
+        sim.synchronize()
+        kern = {}
+        for name in ("flux_x1", "flux_x2", "flux_x3", "corner", "faceb", "update"):
+            ms, cnt = sim.profile_read(name)
+            if cnt:
+                kern[name] = {"total_ms": ms, "launches": cnt, "avg_ms": ms / cnt,
+                              "alg_bytes": kernel_bytes(cfg, name)}
+        sim.profile(False)
+        res["kern"] = kern
+        res["copy_gbps"] = sim.stream_copy(1 << 30, 10)
+        log(f"stream copy (1 GiB x10): {res['copy_gbps']:.0f} GB/s")
+    sim.close()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -104,8 +216,17 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", choices=("ot3d", "helm_slab", "lowmach"), default="ot3d")
     ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64")
+    ap.add_argument("--eos-guess", choices=("warm", "mlp"), default="warm",
+                    help="Helmholtz: Newton start (carried T, or the trained surrogate)")
+    ap.add_argument("--mhd-beta", type=float, default=None,
+                    help="helm_slab: add a random divergence-free field at this plasma beta")
     ap.add_argument("--heating", type=float, default=1e-3,
                     help="lowmach: bottom heating amplitude")
+    ap.add_argument("--compare", action="store_true",
+                    help="also run the plain FP64 (warm-start) path on the same initial data "
+                         "and report the speed-up and the rel. L1 error per conserved field")
+    ap.add_argument("--mlp-samples", type=int, default=1 << 22)
+    ap.add_argument("--mlp-epochs", type=int, default=4)
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -114,94 +235,59 @@ def main():
         sys.exit(2)
 
     import numpy as np
-    from phlegethon_amd import problems
-    from phlegethon_amd.runtime import Simulation
+    from phlegethon_amd.config import PREC_MIXED
 
+    pb, desc, data = build_problem(args)
     n = args.n
-    if args.workload == "ot3d":
-        log(f"building 3D Orszag-Tang {n}^3 initial data")
-        pb = problems.orszag_tang_3d(n=n)
-        desc = f"3D Orszag-Tang ideal MHD + CT, {n}^3, FP64, LHLLD + LIM2ND, RK2, CFL 0.8, periodic"
-        data = "synthetic (seeded 3D Orszag-Tang initial data, counter-based PRNG)"
-    elif args.workload == "helm_slab":
-        from phlegethon_amd import helmholtz
-        from phlegethon_amd.runtime import HelmEoS
-        table = helmholtz.default_table()
-        log(f"building the Helmholtz slab {n}^3 (table {table.nrho}x{table.nT})")
-        eos = HelmEoS(table)
-        pb = problems.helm_slab(eos, table, n=n)
-        eos.close()
-        desc = (f"3D stratified Helmholtz slab, {n}^3, FP64, USE_FASTEOS + USE_WB + "
-                f"EVOLVE_ETOT + ADVECT_YE_IABAR, LHLLC + LIM2ND, RK2, CFL 0.8")
-        data = ("synthetic (plane-parallel rho/T/abar profiles, seeded 1e-4 density noise; "
-                f"generated {table.nrho}x{table.nT} Helmholtz table)")
-    else:
-        log(f"building the low-Mach heated box {n}^3")
-        pb = problems.lowmach_box(n=n, heating=args.heating)
-        desc = (f"3D low-Mach heated box, {n}^3, ideal gas + constant gravity, USE_WB + "
-                f"EVOLVE_ETOT + USE_EDOT, LHLLC + LIM2ND, RK2, CFL 0.8, heating {args.heating:g}")
-        data = "synthetic (polytrope + bottom heating layer, seeded 1e-6 density noise)"
-    if args.precision == "mixed":
-        from phlegethon_amd.config import PREC_MIXED
-        pb.cfg = pb.cfg.with_(precision=PREC_MIXED)
-        desc += " | mixed precision (FP32 recon/Riemann, FP64 deviations/update/CT/EoS)"
+    zones = pb.cfg.nx1 * pb.cfg.nx2 * (pb.cfg.nx3 if pb.cfg.sdims == 3 else 1)
     cfg = pb.cfg
-    zones = n ** 3
-    sim = Simulation(cfg)
-    sim.setup(pb)
-    sim.init()
-    log(f"warmup {args.warmup} steps")
-    sim.run(args.warmup)
-    sim.synchronize()
-    log(f"timing {args.steps} steps")
-    t0 = time.perf_counter()
-    sim.run(args.steps)
-    sim.synchronize()
-    t1 = time.perf_counter()
-    elapsed = t1 - t0
-    value = zones * args.steps / elapsed
-    ms_per_step = 1e3 * elapsed / args.steps
-    sc = sim.scalars()
-    divb = sim.divb_max()
-    log(f"{value:.4e} zone-updates/s, {ms_per_step:.3f} ms/step, t={sc['time']:.6f}, "
-        f"max|divB|={divb:.2e}")
-
-    # per-kernel device time (HIP events on the library stream)
-    sim.profile(True)
-    sim.run(args.profile_steps)
-    sim.synchronize()
-    kern = {}
-    for name in ("flux_x1", "flux_x2", "flux_x3", "corner", "faceb", "update"):
-        ms, cnt = sim.profile_read(name)
-        if cnt:
-            kern[name] = {"total_ms": ms, "launches": cnt, "avg_ms": ms / cnt,
-                          "alg_bytes": kernel_bytes(cfg, name)}
-    sim.profile(False)
-    copy_gbps = sim.stream_copy(1 << 30, 10)
-    log(f"stream copy (1 GiB x10): {copy_gbps:.0f} GB/s")
-    sim.close()
+    if args.precision == "mixed":
+        cfg = cfg.with_(precision=PREC_MIXED)
+        desc += " | mixed precision (FP32 recon/Riemann, FP64 deviations/update/CT/EoS)"
+    else:
+        desc += " | FP64"
+    if args.eos_guess == "mlp":
+        desc += " | surrogate Newton guess"
+    res = run_variant(pb, cfg, args, guess=args.eos_guess, keep_state=args.compare)
+    value, ms_per_step = res["value"], res["ms_per_step"]
+    kern = res["kern"]
     dom = max(kern, key=lambda k: kern[k]["total_ms"])
     d = kern[dom]
     achieved = d["alg_bytes"] / (d["avg_ms"] * 1e-3)
     traffic = None
     pmc = os.path.join(ROOT, "profiles", "ot3d256_fp64_pmc.json")
-    if n == 256 and args.workload == "ot3d" and os.path.exists(pmc):
+    if (n == 256 and args.workload == "ot3d" and args.precision == "fp64"
+            and os.path.exists(pmc)):
         # HBM bytes per dispatch measured with rocprofv3 --pmc FETCH_SIZE /
         # WRITE_SIZE on this workload, scaled by a copy kernel of known size
         with open(pmc) as fh:
             rec = json.load(fh)["kernels"].get(dom)
         if rec:
             traffic = rec["hbm_bytes"]
-    roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
-                "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic,
-                "alg_bytes_per_launch": d["alg_bytes"], "avg_launch_ms": d["avg_ms"]}
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved / 1e9,
+                "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": achieved / HBM_PEAK,
+                "traffic": traffic, "alg_bytes_per_launch": d["alg_bytes"],
+                "avg_launch_ms": d["avg_ms"]}
     bzu = b_zu(cfg)
     step_roof = {"bound": "hbm", "B_zu": bzu, "achieved": bzu * value / 1e9,
                  "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": bzu * value / HBM_PEAK,
-                 "stream_copy_GBps": copy_gbps}
+                 "stream_copy_GBps": res["copy_gbps"]}
     for k, v in kern.items():
         log(f"  {k:8s} avg {v['avg_ms']:.3f} ms  x{v['launches']}  "
             f"{v['alg_bytes'] / (v['avg_ms'] * 1e-3) / 1e9:.0f} GB/s algorithmic")
+
+    compare = None
+    if args.compare:
+        log("plain FP64 warm-start run on the same initial data")
+        base = run_variant(pb, pb.cfg, args, guess="warm", keep_state=True, profile=False)
+        cv, cb = conserved(res["state"], pb.cfg), conserved(base["state"], pb.cfg)
+        rel = {k: float(np.abs(cv[k] - cb[k]).sum() / max(np.abs(cb[k]).sum(), 1e-300))
+               for k in cb}
+        compare = {"plain_value": base["value"], "plain_ms_per_step": base["ms_per_step"],
+                   "speedup": value / base["value"], "rel_l1_vs_plain_fp64": rel,
+                   "plain_eos": base.get("eos"), "steps_compared": args.warmup + args.steps}
+        log(f"speed-up {compare['speedup']:.3f}x; rel L1 " +
+            " ".join(f"{k} {v:.2e}" for k, v in rel.items()))
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -226,13 +312,20 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if args.precision == "fp64" else "f32+f64",
         "data": data,
-        "config": {"workload": desc, "zones": zones, "sim_time": sc["time"],
-                   "max_divB": divb},
+        "config": {"workload": desc, "zones": zones, "sim_time": res["scalars"]["time"],
+                   "max_divB": res["divb"]},
         "roofline": roofline,
         "step_roofline": step_roof,
-        "kernels": {k: {"avg_ms": v["avg_ms"], "launches": v["launches"]} for k, v in kern.items()},
+        "kernels": {k: {"avg_ms": v["avg_ms"], "launches": v["launches"]}
+                    for k, v in kern.items()},
         "cpu_baseline": cpu,
     }
+    if "eos" in res:
+        out["eos"] = res["eos"]
+    if "mlp" in res:
+        out["surrogate"] = res["mlp"]
+    if compare:
+        out["compare"] = compare
     print(json.dumps(out), flush=True)
 
 

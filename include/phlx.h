@@ -38,7 +38,7 @@ extern "C" {
 #define PHLX_E_TABLE (-7)      /* EoS table missing or malformed */
 
 /* option values (names follow the reference's Makefile OPTS) */
-enum { PHLX_LIM2ND_REC = 0, PHLX_PPH_REC = 1 };
+enum { PHLX_LIM2ND_REC = 0, PHLX_PPH_REC = 1, PHLX_LIM5TH_REC = 2 };
 enum { PHLX_HLLC_FLUX = 0, PHLX_LHLLC_FLUX = 1, PHLX_HLLD_FLUX = 2, PHLX_LHLLD_FLUX = 3 };
 enum { PHLX_BC_PERIODIC = 0, PHLX_BC_REFLECTIVE = 1, PHLX_BC_OUTFLOW = 2, PHLX_BC_DIODE = 3 };
 enum { PHLX_EOS_IDEAL = 0, PHLX_EOS_HELMHOLTZ = 1 };
@@ -52,7 +52,7 @@ typedef struct phlx_config {
   int32_t nas;              /* nas_make: advected scalars */
   double x1l, x1u, x2l, x2u, x3l, x3u;  /* domain (initialize_simulation args) */
   double gamma, mu;         /* gamma_ad, mu */
-  int32_t recon;            /* LIM2ND_REC / PPH_REC */
+  int32_t recon;            /* LIM2ND_REC / PPH_REC / LIM5TH_REC (ghosts are virtual: no ngc) */
   int32_t flux;             /* HLLC / LHLLC / HLLD / LHLLD (HLLD* imply USE_MHD) */
   int32_t all_mach;         /* FLUX_IS_ALL_MACH */
   int32_t rk_stages;        /* RK2_STEPPER (2) / RK3_STEPPER (3) */

@@ -9,7 +9,7 @@ module phlx_iface
   use, intrinsic :: iso_c_binding
   implicit none
 
-  integer(c_int), parameter :: PHLX_LIM2ND_REC = 0, PHLX_PPH_REC = 1
+  integer(c_int), parameter :: PHLX_LIM2ND_REC = 0, PHLX_PPH_REC = 1, PHLX_LIM5TH_REC = 2
   integer(c_int), parameter :: PHLX_HLLC_FLUX = 0, PHLX_LHLLC_FLUX = 1, &
                                PHLX_HLLD_FLUX = 2, PHLX_LHLLD_FLUX = 3
   integer(c_int), parameter :: PHLX_BC_PERIODIC = 0, PHLX_BC_REFLECTIVE = 1, &

@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-enum { ORC_LIM2ND = 0, ORC_PPH = 1 };
+enum { ORC_LIM2ND = 0, ORC_PPH = 1, ORC_LIM5TH = 2 };
 enum { ORC_HLLC = 0, ORC_LHLLC = 1, ORC_HLLD = 2, ORC_LHLLD = 3 };
 enum { ORC_BC_PERIODIC = 0, ORC_BC_REFLECTIVE = 1, ORC_BC_OUTFLOW = 2, ORC_BC_DIODE = 3 };
 enum { ORC_EOS_IDEAL = 0, ORC_EOS_HELMHOLTZ = 1 };
@@ -91,8 +91,9 @@ void orc_riemann_hydro(const orc_config *cfg, int n, const double *qL,
 void orc_riemann_mhd(const orc_config *cfg, int n, const double *qL,
                      const double *qR, const double *bL, const double *bR,
                      double *flux, double *fluxb);
-/* Reconstruct one pencil q[0..n-1] (n>=3): faces 1..n-2 of cells 1..n-2;
- * out qm[c] = left-edge value of cell c, qp[c] = right-edge value. */
+/* Reconstruct one pencil q[0..n-1] (n>=3): faces 1..n-2 of cells 1..n-2
+ * (LIM5TH: cells 2..n-3); out qm[c] = left-edge value of cell c, qp[c] =
+ * right-edge value. */
 void orc_reconstruct(int scheme, int limited_scalar, int n, const double *q,
                      double *qm, double *qp);
 

@@ -85,7 +85,7 @@ orc_grid *orc_create(const orc_config *cfg) {
   g->c = *cfg;
   g->sd = cfg->sdims;
   g->nv = cfg->sdims + 2 + cfg->nas;            /* source.F90:472-477 */
-  g->ng = 2;
+  g->ng = (cfg->recon == ORC_LIM5TH) ? 3 : 2; /* LIM5TH reads i +- 2 of cells lx1-1..ux1+1 */
   g->mhd = (cfg->flux == ORC_HLLD || cfg->flux == ORC_LHLLD);
   g->n[0] = cfg->nx1;
   g->n[1] = cfg->nx2;
@@ -462,8 +462,41 @@ static inline void rec_pph_lim(double qm, double qc, double qp, double *l, doubl
   *r = ap + qc;
 }
 
+/* LIM5TH_REC: 5th-order face values clipped to the neighbour range + the
+ * extremum limiter (source.F90:8261-8315, b :8401-8455, gamma :8607-8711) */
+static inline void rec_lim5(double qmm, double qm, double qc, double qp, double qpp, double *l,
+                            double *r) {
+  const double o420 = 1.0 / 420.0;
+  double tmp = 329.0 * qc;
+  double am = (-21.0 * qmm + 189.0 * qm + tmp - 91.0 * qp + 14.0 * qpp) * o420;
+  double ap = (14.0 * qmm - 91.0 * qm + tmp + 189.0 * qp - 21.0 * qpp) * o420;
+  double mn = fmin(qm, qc), mx = fmax(qm, qc);
+  if (am < mn) am = mn;
+  if (am > mx) am = mx;
+  mn = fmin(qc, qp);
+  mx = fmax(qc, qp);
+  if (ap < mn) ap = mn;
+  if (ap > mx) ap = mx;
+  am = am - qc;
+  ap = ap - qc;
+  if (ap * am >= 0.0) {
+    ap = 0.0;
+    am = 0.0;
+  } else {
+    if (fabs(ap) >= 2.0 * fabs(am)) ap = -2.0 * am;
+    if (fabs(am) >= 2.0 * fabs(ap)) am = -2.0 * ap;
+  }
+  *l = am + qc;
+  *r = ap + qc;
+}
+
 void orc_reconstruct(int scheme, int limited_scalar, int n, const double *q,
                      double *qm, double *qp) {
+  if (scheme == ORC_LIM5TH) {
+    for (int c = 2; c < n - 2; c++)
+      rec_lim5(q[c - 2], q[c - 1], q[c], q[c + 1], q[c + 2], &qm[c], &qp[c]);
+    return;
+  }
   for (int c = 1; c < n - 1; c++) {
     if (scheme == ORC_LIM2ND)
       rec_lim2nd(q[c - 1], q[c], q[c + 1], &qm[c], &qp[c]);
@@ -880,10 +913,15 @@ static void sweep(orc_grid *g, int d) {
       /* reconstruction of cells -1..n (faces 0..n) */
       for (int c = -1; c <= n; c++) {
         long cm = c0 + (long)(c - 1) * stride, cc = c0 + (long)c * stride, cp = c0 + (long)(c + 1) * stride;
+        /* LIM5TH: two more cells (ng = 3); the indices stay inside the ghosts */
+        const int r5 = g->c.recon == ORC_LIM5TH;
+        long cmm = r5 ? c0 + (long)(c - 2) * stride : cm, cpp = r5 ? c0 + (long)(c + 2) * stride : cp;
         double l, r;
         for (int v = 0; v < nv; v++) {
           double qm = pr[cm * nv + v], qc = pr[cc * nv + v], qp = pr[cp * nv + v];
-          if (g->c.recon == ORC_LIM2ND)
+          if (r5)
+            rec_lim5(pr[cmm * nv + v], qm, qc, qp, pr[cpp * nv + v], &l, &r);
+          else if (g->c.recon == ORC_LIM2ND)
             rec_lim2nd(qm, qc, qp, &l, &r);
           else if (v <= ip)
             rec_pph(qm, qc, qp, &l, &r);
@@ -896,7 +934,9 @@ static void sweep(orc_grid *g, int d) {
           for (int comp = 0; comp < sd; comp++) {
             if (comp == d) continue;
             double qm = bc[cm * 3 + comp], qc = bc[cc * 3 + comp], qp = bc[cp * 3 + comp];
-            if (g->c.recon == ORC_LIM2ND)
+            if (r5)
+              rec_lim5(bc[cmm * 3 + comp], qm, qc, qp, bc[cpp * 3 + comp], &l, &r);
+            else if (g->c.recon == ORC_LIM2ND)
               rec_lim2nd(qm, qc, qp, &l, &r);
             else
               rec_pph(qm, qc, qp, &l, &r);
@@ -907,7 +947,9 @@ static void sweep(orc_grid *g, int d) {
         if (fasteos) {
           for (int comp = 0; comp < 2; comp++) {
             double qm = gf[cm * 2 + comp], qc = gf[cc * 2 + comp], qp = gf[cp * 2 + comp];
-            if (g->c.recon == ORC_LIM2ND)
+            if (r5)
+              rec_lim5(gf[cmm * 2 + comp], qm, qc, qp, gf[cpp * 2 + comp], &l, &r);
+            else if (g->c.recon == ORC_LIM2ND)
               rec_lim2nd(qm, qc, qp, &l, &r);
             else
               rec_pph(qm, qc, qp, &l, &r);

@@ -12,13 +12,13 @@ import ctypes
 from dataclasses import dataclass, field, replace
 from typing import Sequence
 
-LIM2ND_REC, PPH_REC = 0, 1
+LIM2ND_REC, PPH_REC, LIM5TH_REC = 0, 1, 2
 HLLC_FLUX, LHLLC_FLUX, HLLD_FLUX, LHLLD_FLUX = 0, 1, 2, 3
 BC_PERIODIC, BC_REFLECTIVE, BC_OUTFLOW, BC_DIODE = 0, 1, 2, 3
 EOS_IDEAL, EOS_HELMHOLTZ = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1
 
-_RECON = {"LIM2ND_REC": LIM2ND_REC, "PPH_REC": PPH_REC}
+_RECON = {"LIM2ND_REC": LIM2ND_REC, "PPH_REC": PPH_REC, "LIM5TH_REC": LIM5TH_REC}
 _FLUX = {"HLLC_FLUX": HLLC_FLUX, "LHLLC_FLUX": LHLLC_FLUX,
          "HLLD_FLUX": HLLD_FLUX, "LHLLD_FLUX": LHLLD_FLUX}
 _BC = {"PERIODIC": BC_PERIODIC, "REFLECTIVE": BC_REFLECTIVE,
@@ -127,8 +127,8 @@ class PhlxConfig:
             raise ConfigError("sdims_make must be 2 or 3")
         if min(self.nx1, self.nx2) < 2 or (self.sdims == 3 and self.nx3 < 2):
             raise ConfigError("each nx*_make must be >= 2")
-        if self.recon not in (LIM2ND_REC, PPH_REC):
-            raise ConfigError("exactly one of LIM2ND_REC / PPH_REC is supported")
+        if self.recon not in (LIM2ND_REC, PPH_REC, LIM5TH_REC):
+            raise ConfigError("exactly one of LIM2ND_REC / PPH_REC / LIM5TH_REC")
         if self.flux not in (HLLC_FLUX, LHLLC_FLUX, HLLD_FLUX, LHLLD_FLUX):
             raise ConfigError("flux must be one of HLLC/LHLLC/HLLD/LHLLD")
         if self.rk_stages not in (2, 3):

@@ -87,7 +87,8 @@ constexpr int kNumProfiles = 10;
 static int validate(const phlx_config *c) {
   if (c->sdims != 2 && c->sdims != 3) return PHLX_E_CONFIG;
   if (c->nx1 < 2 || c->nx2 < 2 || (c->sdims == 3 && c->nx3 < 2)) return PHLX_E_CONFIG;
-  if (c->recon != PHLX_LIM2ND_REC && c->recon != PHLX_PPH_REC) return PHLX_E_CONFIG;
+  if (c->recon != PHLX_LIM2ND_REC && c->recon != PHLX_PPH_REC && c->recon != PHLX_LIM5TH_REC)
+    return PHLX_E_CONFIG;
   if (c->flux < PHLX_HLLC_FLUX || c->flux > PHLX_LHLLD_FLUX) return PHLX_E_CONFIG;
   if (c->rk_stages != 2 && c->rk_stages != 3) return PHLX_E_CONFIG;
   if (c->nas != 0 && c->nas != 2) return PHLX_E_CONFIG;
@@ -189,7 +190,8 @@ extern "C" int phlx_create(const phlx_config *cfg, phlx_ctx **out) {
     ctx->rk[2][0] = 1.0 / 3.0; ctx->rk[2][1] = 2.0 / 3.0; ctx->rk[2][2] = -(2.0 / 3.0);
   }
   const bool helm = cfg->eos == PHLX_EOS_HELMHOLTZ;
-  if (!select_kernels(g.sd, g.nas, g.mhd, helm, cfg->precision == PHLX_PREC_MIXED, ctx->kt)) {
+  if (!select_kernels(g.sd, g.nas, g.mhd, helm, cfg->precision == PHLX_PREC_MIXED,
+                      cfg->recon == PHLX_LIM5TH_REC, ctx->kt)) {
     phlx_destroy(ctx);
     return PHLX_E_CONFIG;
   }
@@ -205,6 +207,7 @@ extern "C" int phlx_create(const phlx_config *cfg, phlx_ctx **out) {
   }
   if (g.mhd) {
     e |= dalloc(ctx, &g.bcc, 3 * N);
+    e |= dalloc(ctx, &g.ecell, 3 * N);
     for (int d = 0; d < g.sd; d++) {
       e |= dalloc(ctx, &g.bf[d], (size_t)g.nf[d]);
       e |= dalloc(ctx, &g.b0f[d], (size_t)g.nf[d]);
@@ -232,7 +235,10 @@ extern "C" int phlx_create(const phlx_config *cfg, phlx_ctx **out) {
   e |= dalloc(ctx, &ctx->scratch_val, 1);
   g.nred = cell_grid_blocks(g);
   e |= dalloc(ctx, &g.red, (size_t)g.nred);
-  if (helm) e |= dalloc(ctx, &g.eos_part, 3 * (size_t)g.nred);
+  if (helm) {
+    e |= dalloc(ctx, &g.eos_part, 3 * (size_t)g.nred);
+    e |= dalloc(ctx, &g.mlp_y, N);
+  }
   if (e) {
     phlx_destroy(ctx);
     return PHLX_E_NOMEM;
@@ -1072,6 +1078,10 @@ extern "C" int phlx_set_eos_guess(phlx_ctx *ctx, int32_t mode, double resid_tol,
   // the guess variant of the update needs every block full of interior cells
   if ((mode != GUESS_WARM || count_iterations) && (ctx->g.n1 % 64 || ctx->g.n2 % 4))
     return PHLX_E_CONFIG;
+  if (mode == GUESS_MLP && ctx->g.eos_guess != GUESS_MLP) {
+    // no previous network output yet: all-ones bytes are a NaN pattern
+    HIPCHK(hipMemsetAsync(ctx->g.mlp_y, 0xFF, sizeof(float) * ctx->g.N, ctx->stream));
+  }
   ctx->g.eos_guess = mode;
   ctx->g.mlp.resid_tol = (float)resid_tol;
   ctx->eos_stats = count_iterations != 0;

@@ -23,7 +23,7 @@ namespace phlx {
 constexpr double kRgas = 8.31446261815324e7;  // source/source.F90:605
 
 enum { BC_PERIODIC = 0, BC_REFLECTIVE = 1, BC_OUTFLOW = 2, BC_DIODE = 3 };
-enum { REC_LIM2ND = 0, REC_PPH = 1 };
+enum { REC_LIM2ND = 0, REC_PPH = 1, REC_LIM5TH = 2 };
 enum { FLUX_HLLC = 0, FLUX_LHLLC = 1, FLUX_HLLD = 2, FLUX_LHLLD = 3 };
 
 // Device-resident step control: dt/time/smax never round-trip to the host.
@@ -50,6 +50,7 @@ struct Grid {
   double *emf[3];          // two EMF components per face (see k_flux)
   signed char *sgn[3];     // contact-upwind sign per face
   double *cor[3];          // corner (edge) EMFs along x1, x2, x3
+  double *ecell;           // cell EMF -(v x B), [comp][cell], written by update / init
   const double *eq_rho_cc, *eq_p_cc, *eq_rho_f, *eq_p_f, *gravy, *phi_cc, *phi_f,
       *edot_cc;
   // Helmholtz FASTEOS: carried temperature (warm start of the Newton solve)
@@ -63,6 +64,7 @@ struct Grid {
   MlpDev mlp;
   int eos_guess;
   unsigned int *eos_part;
+  float *mlp_y;            // network output at each zone's last solve (NaN: none yet)
   double *red;       // per-block partial maxima of the cell-grid reductions
   int nred;          // number of partials (blocks of a cell-grid launch)
   Scalars *sc;
@@ -199,6 +201,55 @@ __device__ __forceinline__ void rec_pph_lim(double qm, double qc, double qp, dou
   }
   l = am + qc;
   r = ap + qc;
+}
+
+__device__ __forceinline__ void rec_lim5(double qmm, double qm, double qc, double qp,
+                                         double qpp, double &l, double &r) {
+  // 5th-order face values, clipped to the neighbour range, extremum
+  // limiter (source/source.F90:8261-8315)
+  const double o420 = 1.0 / 420.0;
+  const double c = 329.0 * qc;
+  double am = (-21.0 * qmm + 189.0 * qm + c - 91.0 * qp + 14.0 * qpp) * o420;
+  double ap = (14.0 * qmm - 91.0 * qm + c + 189.0 * qp - 21.0 * qpp) * o420;
+  double lo = fmin(qm, qc), hi = fmax(qm, qc);
+  am = am < lo ? lo : am;
+  am = am > hi ? hi : am;
+  lo = fmin(qc, qp);
+  hi = fmax(qc, qp);
+  ap = ap < lo ? lo : ap;
+  ap = ap > hi ? hi : ap;
+  am = am - qc;
+  ap = ap - qc;
+  if (ap * am >= 0.0) {
+    ap = 0.0;
+    am = 0.0;
+  } else {
+    if (fabs(ap) >= 2.0 * fabs(am)) ap = -2.0 * am;
+    if (fabs(am) >= 2.0 * fabs(ap)) am = -2.0 * ap;
+  }
+  l = am + qc;
+  r = ap + qc;
+}
+
+// Face states of one face from the cell window w[0..NS-1] (cells f-O ..):
+// left = right edge of cell f-1, right = left edge of cell f.
+template <bool R5>
+__device__ __forceinline__ void face_states(int recon, bool limited_scalar, const double *w,
+                                            int stride, double &qL, double &qR) {
+  double l, r;
+  if (R5) {
+    rec_lim5(w[0], w[stride], w[2 * stride], w[3 * stride], w[4 * stride], l, qL);
+    rec_lim5(w[stride], w[2 * stride], w[3 * stride], w[4 * stride], w[5 * stride], qR, r);
+  } else if (recon == REC_LIM2ND) {
+    rec_lim2nd(w[0], w[stride], w[2 * stride], l, qL);
+    rec_lim2nd(w[stride], w[2 * stride], w[3 * stride], qR, r);
+  } else if (!limited_scalar) {
+    rec_pph(w[0], w[stride], w[2 * stride], l, qL);
+    rec_pph(w[stride], w[2 * stride], w[3 * stride], qR, r);
+  } else {
+    rec_pph_lim(w[0], w[stride], w[2 * stride], l, qL);
+    rec_pph_lim(w[stride], w[2 * stride], w[3 * stride], qR, r);
+  }
 }
 
 // ------------------------------------------------------------ Riemann: hydro
@@ -543,6 +594,25 @@ __device__ __forceinline__ void rec_pph(float qm, float qc, float qp, float &l, 
   l = (1.0f / 3.0f) * qm + c - (1.0f / 6.0f) * qp;
   r = -(1.0f / 6.0f) * qm + c + (1.0f / 3.0f) * qp;
 }
+__device__ __forceinline__ void rec_lim5(float qmm, float qm, float qc, float qp, float qpp,
+                                         float &l, float &r) {
+  const float c = 329.0f * qc;
+  float am = (-21.0f * qmm + 189.0f * qm + c - 91.0f * qp + 14.0f * qpp) * (1.0f / 420.0f);
+  float ap = (14.0f * qmm - 91.0f * qm + c + 189.0f * qp - 21.0f * qpp) * (1.0f / 420.0f);
+  am = fminf(fmaxf(am, fminf(qm, qc)), fmaxf(qm, qc));
+  ap = fminf(fmaxf(ap, fminf(qc, qp)), fmaxf(qc, qp));
+  am = am - qc;
+  ap = ap - qc;
+  if (ap * am >= 0.0f) {
+    ap = 0.0f;
+    am = 0.0f;
+  } else {
+    if (fabsf(ap) >= 2.0f * fabsf(am)) ap = -2.0f * am;
+    if (fabsf(am) >= 2.0f * fabsf(ap)) am = -2.0f * ap;
+  }
+  l = am + qc;
+  r = ap + qc;
+}
 __device__ __forceinline__ void rec_pph_lim(float qm, float qc, float qp, float &l, float &r) {
   float c = (5.0f / 6.0f) * qc;
   float am = (1.0f / 3.0f) * qm + c - (1.0f / 6.0f) * qp;
@@ -560,6 +630,25 @@ __device__ __forceinline__ void rec_pph_lim(float qm, float qc, float qp, float 
   }
   l = am + qc;
   r = ap + qc;
+}
+
+template <bool R5>
+__device__ __forceinline__ void face_states(int recon, bool limited_scalar, const float *w,
+                                            int stride, float &qL, float &qR) {
+  float l, r;
+  if (R5) {
+    rec_lim5(w[0], w[stride], w[2 * stride], w[3 * stride], w[4 * stride], l, qL);
+    rec_lim5(w[stride], w[2 * stride], w[3 * stride], w[4 * stride], w[5 * stride], qR, r);
+  } else if (recon == REC_LIM2ND) {
+    rec_lim2nd(w[0], w[stride], w[2 * stride], l, qL);
+    rec_lim2nd(w[stride], w[2 * stride], w[3 * stride], qR, r);
+  } else if (!limited_scalar) {
+    rec_pph(w[0], w[stride], w[2 * stride], l, qL);
+    rec_pph(w[stride], w[2 * stride], w[3 * stride], qR, r);
+  } else {
+    rec_pph_lim(w[0], w[stride], w[2 * stride], l, qL);
+    rec_pph_lim(w[stride], w[2 * stride], w[3 * stride], qR, r);
+  }
 }
 
 // HLLC / LHLLC, deviation form.  q[IE] = p', pf = full p.

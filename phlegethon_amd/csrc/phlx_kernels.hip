@@ -19,9 +19,10 @@ namespace phlx {
 // f), rotates into the face frame, solves, rotates back and applies the WB
 // flux correction and the ETOT potential flux.  MHD faces also store the
 // two tangential EMF components and the contact-upwind sign used by CT.
-template <int DIR, int SD, int NAS, bool MHD, bool HELM>
+template <int DIR, int SD, int NAS, bool MHD, bool HELM, bool R5>
 __global__ void __launch_bounds__(256) k_flux(Grid g) {
   constexpr int NV = SD + 2 + NAS, IE = SD + 1;
+  constexpr int NS = R5 ? 6 : 4, O = R5 ? 3 : 2;  // window: cells f-O .. f+NS-O-1
   if (!g.sc->active) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y * blockDim.y + threadIdx.y;
@@ -29,40 +30,23 @@ __global__ void __launch_bounds__(256) k_flux(Grid g) {
   const int e1 = g.n1 + (DIR == 0), e2 = g.n2 + (DIR == 1);
   if (i >= e1 || j >= e2) return;
 
-  double q[4][NV], b[4][3], ga[4][2];
+  double q[NS][NV], b[NS][3], ga[NS][2];
 #pragma unroll
-  for (int s = 0; s < 4; s++) {
-    const int o = s - 2;
+  for (int s = 0; s < NS; s++) {
+    const int o = s - O;
     load_cell<SD, NV, MHD, HELM>(g, i + (DIR == 0 ? o : 0), j + (DIR == 1 ? o : 0),
                                  k + (DIR == 2 ? o : 0), q[s], b[s], ga[s]);
   }
   double qL[NV], qR[NV], bL[3] = {0.0, 0.0, 0.0}, bR[3] = {0.0, 0.0, 0.0};
-  double l, r;
+  // scalars take the limited parabola under PPH (source/source.F90:8194-8243)
 #pragma unroll
-  for (int v = 0; v < NV; v++) {
-    if (g.recon == REC_LIM2ND) {
-      rec_lim2nd(q[0][v], q[1][v], q[2][v], l, qL[v]);
-      rec_lim2nd(q[1][v], q[2][v], q[3][v], qR[v], r);
-    } else if (v <= IE) {
-      rec_pph(q[0][v], q[1][v], q[2][v], l, qL[v]);
-      rec_pph(q[1][v], q[2][v], q[3][v], qR[v], r);
-    } else {
-      rec_pph_lim(q[0][v], q[1][v], q[2][v], l, qL[v]);
-      rec_pph_lim(q[1][v], q[2][v], q[3][v], qR[v], r);
-    }
-  }
+  for (int v = 0; v < NV; v++) face_states<R5>(g.recon, v > IE, &q[0][v], NV, qL[v], qR[v]);
   const long long f = fix(g, DIR, i, j, k);
   if (MHD) {
 #pragma unroll
     for (int c = 0; c < SD; c++) {
       if (c == DIR) continue;
-      if (g.recon == REC_LIM2ND) {
-        rec_lim2nd(b[0][c], b[1][c], b[2][c], l, bL[c]);
-        rec_lim2nd(b[1][c], b[2][c], b[3][c], bR[c], r);
-      } else {
-        rec_pph(b[0][c], b[1][c], b[2][c], l, bL[c]);
-        rec_pph(b[1][c], b[2][c], b[3][c], bR[c], r);
-      }
+      face_states<R5>(g.recon, false, &b[0][c], 3, bL[c], bR[c]);
     }
     const double bn = g.bf[DIR][f];
     bL[DIR] = bn;
@@ -99,15 +83,7 @@ __global__ void __launch_bounds__(256) k_flux(Grid g) {
   double gL[2] = {0.0, 0.0}, gR[2] = {0.0, 0.0};
   if (HELM) {
 #pragma unroll
-    for (int c = 0; c < 2; c++) {
-      if (g.recon == REC_LIM2ND) {
-        rec_lim2nd(ga[0][c], ga[1][c], ga[2][c], l, gL[c]);
-        rec_lim2nd(ga[1][c], ga[2][c], ga[3][c], gR[c], r);
-      } else {
-        rec_pph(ga[0][c], ga[1][c], ga[2][c], l, gL[c]);
-        rec_pph(ga[1][c], ga[2][c], ga[3][c], gR[c], r);
-      }
-    }
+    for (int c = 0; c < 2; c++) face_states<R5>(g.recon, false, &ga[0][c], 2, gL[c], gR[c]);
     if (g.wb) {
       const double geq = (DIR == 1) ? g.eq_ge_f[jy] : g.eq_ge_cc[jy];
       gL[0] = gL[0] + geq;
@@ -162,9 +138,10 @@ __global__ void __launch_bounds__(256) k_flux(Grid g) {
 // Riemann solve run in FP32 (deviation form, phlx_device.hpp); the face
 // equilibrium is added back in FP64 before the cast; fluxes are stored as
 // FP32 and accumulated in FP64 by k_update.
-template <int DIR, int SD, int NAS, bool MHD, bool HELM>
+template <int DIR, int SD, int NAS, bool MHD, bool HELM, bool R5>
 __global__ void __launch_bounds__(256) k_flux_mx(Grid g) {
   constexpr int NV = SD + 2 + NAS, IE = SD + 1;
+  constexpr int NS = R5 ? 6 : 4, O = R5 ? 3 : 2;
   if (!g.sc->active) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y * blockDim.y + threadIdx.y;
@@ -172,10 +149,10 @@ __global__ void __launch_bounds__(256) k_flux_mx(Grid g) {
   const int e1 = g.n1 + (DIR == 0), e2 = g.n2 + (DIR == 1);
   if (i >= e1 || j >= e2) return;
 
-  float q[4][NV], b[4][3], ga[4][2];
+  float q[NS][NV], b[NS][3], ga[NS][2];
 #pragma unroll
-  for (int s = 0; s < 4; s++) {
-    const int o = s - 2;
+  for (int s = 0; s < NS; s++) {
+    const int o = s - O;
     double qd[NV], bd[3] = {0.0, 0.0, 0.0}, gd[2] = {0.0, 0.0};
     load_cell<SD, NV, MHD, HELM>(g, i + (DIR == 0 ? o : 0), j + (DIR == 1 ? o : 0),
                                  k + (DIR == 2 ? o : 0), qd, bd, gd);
@@ -187,32 +164,14 @@ __global__ void __launch_bounds__(256) k_flux_mx(Grid g) {
     ga[s][1] = (float)gd[1];
   }
   float qL[NV], qR[NV], bL[3] = {0.0f, 0.0f, 0.0f}, bR[3] = {0.0f, 0.0f, 0.0f};
-  float l, r;
 #pragma unroll
-  for (int v = 0; v < NV; v++) {
-    if (g.recon == REC_LIM2ND) {
-      mx::rec_lim2nd(q[0][v], q[1][v], q[2][v], l, qL[v]);
-      mx::rec_lim2nd(q[1][v], q[2][v], q[3][v], qR[v], r);
-    } else if (v <= IE) {
-      mx::rec_pph(q[0][v], q[1][v], q[2][v], l, qL[v]);
-      mx::rec_pph(q[1][v], q[2][v], q[3][v], qR[v], r);
-    } else {
-      mx::rec_pph_lim(q[0][v], q[1][v], q[2][v], l, qL[v]);
-      mx::rec_pph_lim(q[1][v], q[2][v], q[3][v], qR[v], r);
-    }
-  }
+  for (int v = 0; v < NV; v++) mx::face_states<R5>(g.recon, v > IE, &q[0][v], NV, qL[v], qR[v]);
   const long long f = fix(g, DIR, i, j, k);
   if (MHD) {
 #pragma unroll
     for (int c = 0; c < SD; c++) {
       if (c == DIR) continue;
-      if (g.recon == REC_LIM2ND) {
-        mx::rec_lim2nd(b[0][c], b[1][c], b[2][c], l, bL[c]);
-        mx::rec_lim2nd(b[1][c], b[2][c], b[3][c], bR[c], r);
-      } else {
-        mx::rec_pph(b[0][c], b[1][c], b[2][c], l, bL[c]);
-        mx::rec_pph(b[1][c], b[2][c], b[3][c], bR[c], r);
-      }
+      mx::face_states<R5>(g.recon, false, &b[0][c], 3, bL[c], bR[c]);
     }
     const float bn = (float)g.bf[DIR][f];
     bL[DIR] = bn;
@@ -248,15 +207,7 @@ __global__ void __launch_bounds__(256) k_flux_mx(Grid g) {
   float gL[2] = {0.0f, 0.0f}, gR[2] = {0.0f, 0.0f};
   if (HELM) {
 #pragma unroll
-    for (int c = 0; c < 2; c++) {
-      if (g.recon == REC_LIM2ND) {
-        mx::rec_lim2nd(ga[0][c], ga[1][c], ga[2][c], l, gL[c]);
-        mx::rec_lim2nd(ga[1][c], ga[2][c], ga[3][c], gR[c], r);
-      } else {
-        mx::rec_pph(ga[0][c], ga[1][c], ga[2][c], l, gL[c]);
-        mx::rec_pph(ga[1][c], ga[2][c], ga[3][c], gR[c], r);
-      }
-    }
+    for (int c = 0; c < 2; c++) mx::face_states<R5>(g.recon, false, &ga[0][c], 2, gL[c], gR[c]);
     if (g.wb) {
       const double geq = (DIR == 1) ? g.eq_ge_f[jy] : g.eq_ge_cc[jy];
       gL[0] = (float)(geq + (double)gL[0]);
@@ -331,16 +282,34 @@ __device__ __forceinline__ void face_emf(const Grid &g, int fd, int comp, int i,
   s = (double)g.sgn[fd][f];
 }
 
-// -(v x B)_comp of a (possibly ghost) cell from v and b_cc
+// -(v x B)_comp of a cell from v and b_cc (the reference forms it from the
+// ghost-filled prim and b_cc inside the corner loops, :11966-12185).
 template <int SD>
-__device__ __forceinline__ double cell_emf(const Grid &g, int comp, int i, int j, int k) {
-  constexpr int NV = SD + 2;  // rho, velocities, p (only the velocities are used)
-  double q[NV], b[3] = {0.0, 0.0, 0.0};
-  load_cell<SD, NV, true>(g, i, j, k, q, b);
-  const double v1 = q[1], v2 = q[2], v3 = (SD == 3) ? q[3] : 0.0;
+__device__ __forceinline__ double emf_of(int comp, const double *v, const double *b) {
+  const double v1 = v[0], v2 = v[1], v3 = (SD == 3) ? v[2] : 0.0;
   if (comp == 2) return -v1 * b[1] + v2 * b[0];
   if (comp == 0) return v3 * b[1] - v2 * b[2];
   return v1 * b[2] - v3 * b[0];
+}
+
+// Stored cell EMF of a (possibly ghost) cell.  The update kernel stores
+// -(v x B) of every interior cell; a mirrored ghost flips v_a and b_a of the
+// mirrored axis a, which negates every component but E_a (each product
+// holds exactly one axis-a factor), and IEEE negation commutes with
+// rounding, so this equals the value formed from the flipped ghost state
+// bit for bit.
+template <int SD>
+__device__ __forceinline__ double cell_emf(const Grid &g, int comp, int i, int j, int k) {
+  int si, sj, sk = 0;
+  const int mi = map_axis(g, 0, i, g.n1, si);
+  const int mj = map_axis(g, 1, j, g.n2, sj);
+  const int mk = (SD == 3) ? map_axis(g, 2, k, g.n3, sk) : k;
+  double e = g.ecell[(long long)comp * g.N + cix(g, mi, mj, mk)];
+  if (comp != 0 && si && flips(g, 0, si, cix(g, si < 0 ? 0 : g.n1 - 1, mj, mk))) e = -e;
+  if (comp != 1 && sj && flips(g, 1, sj, cix(g, mi, sj < 0 ? 0 : g.n2 - 1, mk))) e = -e;
+  if (SD == 3 && comp != 2 && sk && flips(g, 2, sk, cix(g, mi, mj, sk < 0 ? 0 : g.n3 - 1)))
+    e = -e;
+  return e;
 }
 
 // Gardiner-Stone CT-contact corner EMF (source/source.F90:11966-12027)
@@ -646,13 +615,17 @@ __global__ void __launch_bounds__(256) k_update(Grid g, int first, int last, dou
         __shared__ unsigned int lds[4 * kMlpLdsWaveDw];
         __shared__ unsigned int cnt[3];
         const int mode = g.eos_guess;
-        double tg = -1.0;
+        double tg = T;
         if (mode == GUESS_MLP) {
+          // corrected warm start: T_prev * 10^(s (y_now - y_prev)); the
+          // previous output is NaN before the first solve of a zone
           float x[4];
           bool ok;
           mlp_inputs(g.mlp, rho, eint * ir, abar, zbar, x, ok);
           const float y = mlp_forward_wave(g.mlp, x, lds + threadIdx.y * kMlpLdsWaveDw);
-          if (ok) tg = exp10((double)(y * g.mlp.out_scale + g.mlp.out_off));
+          const float yp = g.mlp_y[c];
+          if (ok && yp == yp) tg = T * exp10((double)(g.mlp.out_scale * (y - yp)));
+          g.mlp_y[c] = ok ? y : __int_as_float(0x7fffffff);
         }
         int fb;
         const int it = helm_rhoe_guess(g.helm, mode, tg, g.mlp.resid_tol, rho, eint * ir, abar,
@@ -687,6 +660,14 @@ __global__ void __launch_bounds__(256) k_update(Grid g, int first, int last, dou
       g.U[v * N + c] = S[v];
     }
     g.eint[c] = eint;
+    if (MHD) {
+      const double vv[3] = {p[1], p[2], SD == 3 ? p[3] : 0.0};
+      g.ecell[2 * N + c] = emf_of<SD>(2, vv, b);
+      if (SD == 3) {
+        g.ecell[c] = emf_of<SD>(0, vv, b);
+        g.ecell[N + c] = emf_of<SD>(1, vv, b);
+      }
+    }
     if (last) smax_local = cell_speed<SD, MHD, HELM>(g, p, b, T);
   }
   if (last) block_max_to(smax_local, g.red);
@@ -740,9 +721,19 @@ __global__ void __launch_bounds__(256) k_init(Grid g) {
     g.eint[c] = g.P[(SD + 1) * N + c] * g.igmm1;
   }
   if (MHD) {
-    g.bcc[c] = 0.5 * (g.bf[0][fix(g, 0, i + 1, j, k)] + g.bf[0][fix(g, 0, i, j, k)]);
-    g.bcc[N + c] = 0.5 * (g.bf[1][fix(g, 1, i, j + 1, k)] + g.bf[1][fix(g, 1, i, j, k)]);
-    if (SD == 3) g.bcc[2 * N + c] = 0.5 * (g.bf[2][fix(g, 2, i, j, k + 1)] + g.bf[2][fix(g, 2, i, j, k)]);
+    double b[3] = {0.0, 0.0, 0.0};
+    b[0] = 0.5 * (g.bf[0][fix(g, 0, i + 1, j, k)] + g.bf[0][fix(g, 0, i, j, k)]);
+    b[1] = 0.5 * (g.bf[1][fix(g, 1, i, j + 1, k)] + g.bf[1][fix(g, 1, i, j, k)]);
+    if (SD == 3) b[2] = 0.5 * (g.bf[2][fix(g, 2, i, j, k + 1)] + g.bf[2][fix(g, 2, i, j, k)]);
+    g.bcc[c] = b[0];
+    g.bcc[N + c] = b[1];
+    if (SD == 3) g.bcc[2 * N + c] = b[2];
+    const double vv[3] = {g.P[N + c], g.P[2 * N + c], SD == 3 ? g.P[3 * N + c] : 0.0};
+    g.ecell[2 * N + c] = emf_of<SD>(2, vv, b);
+    if (SD == 3) {
+      g.ecell[c] = emf_of<SD>(0, vv, b);
+      g.ecell[N + c] = emf_of<SD>(1, vv, b);
+    }
   }
 }
 
@@ -806,30 +797,30 @@ static dim3 grid_for(int e1, int e2, int e3, dim3 blk) {
   return dim3((e1 + blk.x - 1) / blk.x, (e2 + blk.y - 1) / blk.y, e3);
 }
 
-template <int SD, int NAS, bool MHD, bool HELM, bool MIX>
+template <int SD, int NAS, bool MHD, bool HELM, bool MIX, bool R5>
 static void launch_flux(const Grid &g, int dir, hipStream_t st) {
   const dim3 blk(64, 4, 1);
   if (MIX) {
     if (dir == 0)
-      k_flux_mx<0, SD, NAS, MHD, HELM><<<grid_for(g.n1 + 1, g.n2, g.n3, blk), blk, 0, st>>>(g);
+      k_flux_mx<0, SD, NAS, MHD, HELM, R5><<<grid_for(g.n1 + 1, g.n2, g.n3, blk), blk, 0, st>>>(g);
     else if (dir == 1)
-      k_flux_mx<1, SD, NAS, MHD, HELM><<<grid_for(g.n1, g.n2 + 1, g.n3, blk), blk, 0, st>>>(g);
+      k_flux_mx<1, SD, NAS, MHD, HELM, R5><<<grid_for(g.n1, g.n2 + 1, g.n3, blk), blk, 0, st>>>(g);
     else if (SD == 3)
-      k_flux_mx<2, SD, NAS, MHD, HELM><<<grid_for(g.n1, g.n2, g.n3 + 1, blk), blk, 0, st>>>(g);
+      k_flux_mx<2, SD, NAS, MHD, HELM, R5><<<grid_for(g.n1, g.n2, g.n3 + 1, blk), blk, 0, st>>>(g);
     return;
   }
   if (dir == 0)
-    k_flux<0, SD, NAS, MHD, HELM><<<grid_for(g.n1 + 1, g.n2, g.n3, blk), blk, 0, st>>>(g);
+    k_flux<0, SD, NAS, MHD, HELM, R5><<<grid_for(g.n1 + 1, g.n2, g.n3, blk), blk, 0, st>>>(g);
   else if (dir == 1)
-    k_flux<1, SD, NAS, MHD, HELM><<<grid_for(g.n1, g.n2 + 1, g.n3, blk), blk, 0, st>>>(g);
+    k_flux<1, SD, NAS, MHD, HELM, R5><<<grid_for(g.n1, g.n2 + 1, g.n3, blk), blk, 0, st>>>(g);
   else if (SD == 3)
-    k_flux<2, SD, NAS, MHD, HELM><<<grid_for(g.n1, g.n2, g.n3 + 1, blk), blk, 0, st>>>(g);
+    k_flux<2, SD, NAS, MHD, HELM, R5><<<grid_for(g.n1, g.n2, g.n3 + 1, blk), blk, 0, st>>>(g);
 }
 
-template <int SD, int NAS, bool MHD, bool HELM, bool MIX>
+template <int SD, int NAS, bool MHD, bool HELM, bool MIX, bool R5 = false>
 struct Kern {
   static void flux(const Grid &g, int dir, hipStream_t st) {
-    launch_flux<SD, NAS, MHD, HELM, MIX>(g, dir, st);
+    launch_flux<SD, NAS, MHD, HELM, MIX, R5>(g, dir, st);
   }
   static void corner(const Grid &g, hipStream_t st) {
     const dim3 blk(64, 4, 1);
@@ -866,9 +857,9 @@ struct Kern {
   }
 };
 
-template <int SD, int NAS, bool MHD, bool HELM, bool MIX>
+template <int SD, int NAS, bool MHD, bool HELM, bool MIX, bool R5>
 static KernelTable make_table() {
-  using K = Kern<SD, NAS, MHD, HELM, MIX>;
+  using K = Kern<SD, NAS, MHD, HELM, MIX, R5>;
   KernelTable t;
   t.flux = &K::flux;
   t.corner = &K::corner;
@@ -881,31 +872,34 @@ static KernelTable make_table() {
   return t;
 }
 
-template <bool MIX>
+template <bool MIX, bool R5>
 static bool select_prec(int sd, int nas, bool mhd, bool helm, KernelTable &t) {
   if (helm) {
     // Helmholtz runs carry Ye and 1/abar as the two advected scalars
     if (nas != 2) return false;
-    if (sd == 2 && !mhd) t = make_table<2, 2, false, true, MIX>();
-    else if (sd == 3 && !mhd) t = make_table<3, 2, false, true, MIX>();
-    else if (sd == 3 && mhd) t = make_table<3, 2, true, true, MIX>();
+    if (sd == 2 && !mhd) t = make_table<2, 2, false, true, MIX, R5>();
+    else if (sd == 3 && !mhd) t = make_table<3, 2, false, true, MIX, R5>();
+    else if (sd == 3 && mhd) t = make_table<3, 2, true, true, MIX, R5>();
     else return false;
     return true;
   }
-  if (sd == 2 && nas == 0 && !mhd) t = make_table<2, 0, false, false, MIX>();
-  else if (sd == 2 && nas == 0 && mhd) t = make_table<2, 0, true, false, MIX>();
-  else if (sd == 3 && nas == 0 && !mhd) t = make_table<3, 0, false, false, MIX>();
-  else if (sd == 3 && nas == 0 && mhd) t = make_table<3, 0, true, false, MIX>();
-  else if (sd == 2 && nas == 2 && !mhd) t = make_table<2, 2, false, false, MIX>();
-  else if (sd == 3 && nas == 2 && !mhd) t = make_table<3, 2, false, false, MIX>();
-  else if (sd == 3 && nas == 2 && mhd) t = make_table<3, 2, true, false, MIX>();
+  if (sd == 2 && nas == 0 && !mhd) t = make_table<2, 0, false, false, MIX, R5>();
+  else if (sd == 2 && nas == 0 && mhd) t = make_table<2, 0, true, false, MIX, R5>();
+  else if (sd == 3 && nas == 0 && !mhd) t = make_table<3, 0, false, false, MIX, R5>();
+  else if (sd == 3 && nas == 0 && mhd) t = make_table<3, 0, true, false, MIX, R5>();
+  else if (sd == 2 && nas == 2 && !mhd) t = make_table<2, 2, false, false, MIX, R5>();
+  else if (sd == 3 && nas == 2 && !mhd) t = make_table<3, 2, false, false, MIX, R5>();
+  else if (sd == 3 && nas == 2 && mhd) t = make_table<3, 2, true, false, MIX, R5>();
   else return false;
   return true;
 }
 
-bool select_kernels(int sd, int nas, bool mhd, bool helm, bool mixed, KernelTable &t) {
-  return mixed ? select_prec<true>(sd, nas, mhd, helm, t)
-               : select_prec<false>(sd, nas, mhd, helm, t);
+bool select_kernels(int sd, int nas, bool mhd, bool helm, bool mixed, bool r5, KernelTable &t) {
+  if (mixed)
+    return r5 ? select_prec<true, true>(sd, nas, mhd, helm, t)
+              : select_prec<true, false>(sd, nas, mhd, helm, t);
+  return r5 ? select_prec<false, true>(sd, nas, mhd, helm, t)
+            : select_prec<false, false>(sd, nas, mhd, helm, t);
 }
 
 void launch_step_begin(Scalars *sc, hipStream_t st) { k_step_begin<<<1, 1, 0, st>>>(sc); }

@@ -4,7 +4,12 @@
 // the carried temperature and has no surrogate).
 //
 // Inputs (log10 rho, log10 e, abar, zbar) are normalised affinely to about
-// [-1, 1]; the output is normalised log10 T.  Inference runs one wavefront
+// [-1, 1]; the output is normalised log10 T.  Two uses: a cold guess
+// (log10 T = network output) for callers without a carried temperature, and
+// inside the hydro update a corrected warm start,
+//   log10 T_g = log10 T_prev + s (y(state now) - y(state at the last solve)),
+// where the network's smooth approximation error cancels in the difference
+// while the change of T with the new internal energy is kept.  Inference runs one wavefront
 // per 64 zones: layer 1 (4 -> 64) and layer 3 (64 -> 1) on the VALU in FP32
 // with BF16-rounded weights, layer 2 (64 x 64) on the matrix cores with
 // v_mfma_f32_16x16x16bf16_1k (BF16 operands, FP32 accumulation): 16 tiles x
@@ -164,32 +169,63 @@ __device__ __forceinline__ double helm_bracket_guess(const HelmDev &h, double rh
   return exp10(0.5 * (lo + hi));
 }
 
-// T(rho, e) with a chosen start: the reference's Newton loop
-// (helm_rhoe_given) from T (GUESS_WARM: T as given; GUESS_MLP: T = tg, the
-// surrogate's guess, which must be > 0 or is rejected; GUESS_BRACKET: the
-// bisection guess).  A surrogate guess whose first residual exceeds
-// resid_tol restarts from the bracketed guess (fallback = 1).  Returns the
-// Newton iterations of the final solve.
+// The reference's Newton loop for T(rho, e) (helm_rhoe_given) entered with
+// the first evaluation (e1, dedt1 = E and dE/dT at the incoming T) already
+// done: identical arithmetic to helm_rhoe_given from the same T, without
+// repeating that evaluation.
+__device__ __forceinline__ int helm_rhoe_given_from(const HelmDev &h, double rho, double E0,
+                                                    double abar, double zbar, double &T,
+                                                    double &P, double &sound, bool ret,
+                                                    double e1, double dedt1) {
+  const double Told = T;
+  int iter;
+  sound = 0.0;
+  for (iter = 1; iter <= kHelmMaxIter; iter++) {
+    double e = e1, dEdT = dedt1;
+    if (iter > 1) helm_given2(h, rho, T, abar, zbar, e, dEdT, true);
+    const double res = e - E0;
+    T = T - res / dEdT;
+    double err = fabs(res / E0);
+    if ((T < h.tl) || (T > h.th)) {
+      T = Told;
+      err = 1e-15;
+    }
+    if (err < kHelmTol) break;
+  }
+  if (ret) {
+    double e, cv;
+    helm_full(h, rho, T, abar, zbar, P, e, sound, cv);
+  } else {
+    double dPdT;
+    helm_given1(h, rho, T, abar, zbar, P, dPdT, false);
+  }
+  return iter > kHelmMaxIter ? kHelmMaxIter : iter;
+}
+
+// T(rho, e) with a chosen start (GUESS_WARM: T as given, the reference;
+// GUESS_BRACKET: the bisection guess; GUESS_MLP: tg, the surrogate's guess).
+// A surrogate guess is judged by the first Newton evaluation itself: if it
+// is outside the table or |e(tg) - e| / e > resid_tol the solve restarts
+// from the bracketed guess (fallback = 1).  Returns the Newton iterations of
+// the final solve.
 __device__ __forceinline__ int helm_rhoe_guess(const HelmDev &h, int mode, double tg,
                                                float resid_tol, double rho, double E0,
                                                double abar, double zbar, double &T, double &P,
                                                double &sound, bool ret, int &fallback) {
   fallback = 0;
-  if (mode == GUESS_BRACKET) {
-    T = helm_bracket_guess(h, rho, E0, abar, zbar);
-  } else if (mode == GUESS_MLP) {
-    bool ok = tg > 0.0 && tg >= h.tl && tg <= h.th;
-    if (ok) {
+  if (mode == GUESS_MLP) {
+    if (tg >= h.tl && tg <= h.th) {
       double e, dedt;
-      helm_given2(h, rho, tg, abar, zbar, e, dedt, false);
-      ok = fabs((e - E0) / E0) <= (double)resid_tol;
+      helm_given2(h, rho, tg, abar, zbar, e, dedt, true);
+      if (fabs((e - E0) / E0) <= (double)resid_tol) {
+        T = tg;
+        return helm_rhoe_given_from(h, rho, E0, abar, zbar, T, P, sound, ret, e, dedt);
+      }
     }
-    if (ok) {
-      T = tg;
-    } else {
-      fallback = 1;
-      T = helm_bracket_guess(h, rho, E0, abar, zbar);
-    }
+    fallback = 1;
+    T = helm_bracket_guess(h, rho, E0, abar, zbar);
+  } else if (mode == GUESS_BRACKET) {
+    T = helm_bracket_guess(h, rho, E0, abar, zbar);
   }
   return helm_rhoe_given(h, rho, E0, abar, zbar, T, P, sound, ret);
 }

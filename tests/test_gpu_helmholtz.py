@@ -162,7 +162,7 @@ def _rel_l1(a, b):
     return float(np.abs(a - b).sum() / (den if den > 0 else 1.0))
 
 
-from phlegethon_amd.config import HLLC_FLUX, LHLLC_FLUX, LIM2ND_REC, PPH_REC  # noqa: E402
+from phlegethon_amd.config import HLLC_FLUX, LHLLC_FLUX, LIM2ND_REC, LIM5TH_REC, PPH_REC  # noqa: E402
 
 
 @pytest.mark.parametrize("case", [
@@ -170,7 +170,8 @@ from phlegethon_amd.config import HLLC_FLUX, LHLLC_FLUX, LIM2ND_REC, PPH_REC  # 
     dict(n=12, sdims=3, recon=PPH_REC, flux=HLLC_FLUX),
     dict(n=24, ny=32, sdims=2, recon=LIM2ND_REC, flux=LHLLC_FLUX),
     dict(n=10, sdims=3, recon=LIM2ND_REC, mhd_beta=1e3),
-], ids=["3d-lim2nd-lhllc", "3d-pph-hllc", "2d-lim2nd-lhllc", "3d-mhd-lhlld"])
+    dict(n=10, sdims=3, recon=LIM5TH_REC, flux=LHLLC_FLUX),
+], ids=["3d-lim2nd-lhllc", "3d-pph-hllc", "2d-lim2nd-lhllc", "3d-mhd-lhlld", "3d-lim5th"])
 def test_helm_slab_parity(table, oracle_mod, case):
     nsteps = 6
     pb = _slab(table, oracle_mod, **case)

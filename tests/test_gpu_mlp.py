@@ -120,8 +120,10 @@ def test_guess_gain(eos, trained, oracle_mod, table):
     res = {}
     for mode in ("warm", "bracket", "mlp"):
         Tm, Pm, csm, it, fb = eos.rhoe_given_guess(rho, E, abar, zbar, cold, mode=mode)
+        # the surrogate's first Newton evaluation doubles as its residual
+        # check; a rejected guess costs that evaluation plus the bisection
         evals = it + (BRACKET_STEPS if mode == "bracket" else 0) + \
-            (1 + BRACKET_STEPS * fb if mode == "mlp" else 0)
+            ((1 + BRACKET_STEPS) * fb if mode == "mlp" else 0)
         res[mode] = (Tm, it, evals, fb)
         print(f"  {mode:8s} Newton it/zone {it.mean():.2f}  EoS evals/zone {evals.mean():.2f}  "
               f"fallback {fb.mean():.4f}", flush=True)

@@ -14,7 +14,7 @@ import pytest
 from phlegethon_amd import problems as P
 from phlegethon_amd.config import (BC_DIODE, BC_OUTFLOW, BC_PERIODIC, BC_REFLECTIVE,
                                    HLLC_FLUX, HLLD_FLUX, LHLLC_FLUX, LHLLD_FLUX,
-                                   LIM2ND_REC, PPH_REC, PhlxConfig)
+                                   LIM2ND_REC, LIM5TH_REC, PPH_REC, PhlxConfig)
 
 pytestmark = pytest.mark.gpu
 
@@ -120,6 +120,31 @@ def test_random_3d_mhd_reflect_outflow(flux):
                      bc=(BC_PERIODIC, BC_PERIODIC, BC_REFLECTIVE, BC_REFLECTIVE,
                          BC_OUTFLOW, BC_OUTFLOW))
     check_parity(P.random_smooth(cfg), 8)
+
+
+@pytest.mark.parametrize("case", ["hot_bubble", "ot2d", "mhd3d_mixed_bcs", "hydro3d_diode"])
+def test_lim5th(case):
+    """LIM5TH_REC: 6-cell stencil, ghosts three deep (mirrors of three
+    interior cells at reflecting / outflow / diode walls)."""
+    if case == "hot_bubble":
+        pb = P.hot_bubble(n=48, recon=LIM5TH_REC)
+        n = 12
+    elif case == "ot2d":
+        pb = P.orszag_tang_2d(n=40, flux=LHLLD_FLUX, recon=LIM5TH_REC)
+        n = 12
+    elif case == "mhd3d_mixed_bcs":
+        cfg = PhlxConfig(sdims=3, nx1=16, nx2=12, nx3=12, flux=HLLD_FLUX, recon=LIM5TH_REC,
+                         bc=(BC_PERIODIC, BC_PERIODIC, BC_REFLECTIVE, BC_REFLECTIVE,
+                             BC_OUTFLOW, BC_OUTFLOW))
+        pb = P.random_smooth(cfg)
+        n = 6
+    else:
+        cfg = PhlxConfig(sdims=3, nx1=16, nx2=12, nx3=10, nas=2, flux=LHLLC_FLUX,
+                         recon=LIM5TH_REC, bc=(BC_DIODE, BC_DIODE, BC_REFLECTIVE,
+                                               BC_OUTFLOW, BC_PERIODIC, BC_PERIODIC))
+        pb = P.random_smooth(cfg, amp=0.5)
+        n = 6
+    check_parity(pb, n)
 
 
 def test_scalars_nas2():

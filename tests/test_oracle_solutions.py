@@ -18,7 +18,7 @@ import pytest
 
 from phlegethon_amd import problems as P
 from phlegethon_amd.config import (HLLC_FLUX, LHLLC_FLUX, HLLD_FLUX, LHLLD_FLUX,
-                                   LIM2ND_REC, PPH_REC, PhlxConfig)
+                                   LIM2ND_REC, LIM5TH_REC, PPH_REC, PhlxConfig)
 
 import exact_riemann
 
@@ -216,6 +216,52 @@ def test_lim2nd_bounds_and_constant(oracle_mod):
     lin = 0.25 * np.arange(10)
     qm, qp = oracle_mod.reconstruct(LIM2ND_REC, lin)
     np.testing.assert_allclose(qp[1:-1] - qm[1:-1], 0.125, rtol=1e-14)
+
+
+def test_lim5th_exact_monotone_quartic_and_bounds(oracle_mod):
+    # cell averages of a monotone quartic: the 5th-order face values are
+    # exact (degree <= 4) and the clip / extremum limiter leaves them alone
+    p = lambda x: 0.2 + x + 0.1 * x ** 2 + 0.01 * x ** 3 + 0.001 * x ** 4
+    P = lambda x: 0.2 * x + x ** 2 / 2 + 0.1 * x ** 3 / 3 + 0.01 * x ** 4 / 4 + 0.001 * x ** 5 / 5
+    h = 0.1
+    x0 = np.arange(20) * h
+    avg = (P(x0 + h) - P(x0)) / h
+    qm, qp = oracle_mod.reconstruct(LIM5TH_REC, avg)
+    np.testing.assert_allclose(qm[2:-2], p(x0[2:-2]), rtol=1e-12)
+    np.testing.assert_allclose(qp[2:-2], p(x0[2:-2] + h), rtol=1e-12)
+    # a quintic term is not reproduced: the pin is not trivially satisfied
+    avg5 = avg + ((x0 + h) ** 6 - x0 ** 6) / 6 / h
+    qm5, _ = oracle_mod.reconstruct(LIM5TH_REC, avg5)
+    assert np.max(np.abs(qm5[2:-2] - (p(x0[2:-2]) + x0[2:-2] ** 5))) > 1e-7
+    # random data: faces inside the neighbour range, constants preserved
+    r = np.random.default_rng(8)
+    q = r.uniform(-1, 1, 300)
+    qm, qp = oracle_mod.reconstruct(LIM5TH_REC, q)
+    c = slice(2, -2)
+    lo_m, hi_m = np.minimum(q[1:-3], q[2:-2]), np.maximum(q[1:-3], q[2:-2])
+    lo_p, hi_p = np.minimum(q[2:-2], q[3:-1]), np.maximum(q[2:-2], q[3:-1])
+    assert np.all(qm[c] >= lo_m - 1e-15) and np.all(qm[c] <= hi_m + 1e-15)
+    assert np.all(qp[c] >= lo_p - 1e-15) and np.all(qp[c] <= hi_p + 1e-15)
+    # local extremum -> flat cell
+    ext = np.array([0.0, 1.0, 0.0, 1.0, 0.0, 1.0, 0.0])
+    qm, qp = oracle_mod.reconstruct(LIM5TH_REC, ext)
+    assert qm[3] == qp[3] == 1.0
+    k = np.full(12, -0.3)
+    qm, qp = oracle_mod.reconstruct(LIM5TH_REC, k)
+    np.testing.assert_allclose(qm[2:-2], -0.3, rtol=3e-16)
+
+
+def test_lim5th_sod_converges(oracle_mod):
+    errs = []
+    for n in (200, 400):
+        pb = P.sod_tube(n=n, flux=HLLC_FLUX, recon=LIM5TH_REC)
+        pb.cfg = pb.cfg.with_(tmax=0.2)
+        st = _run(oracle_mod, pb)
+        x = P.centres(pb.cfg, 0)
+        ex = exact_riemann.sample(x, 0.2, 0.5, (1.0, 0.0, 1.0), (0.125, 0.0, 0.1), 1.4)
+        errs.append(np.mean(np.abs(st["prim"][0, 0, 0] - ex[0])))
+    assert errs[0] < 1.2e-2, errs
+    assert errs[1] < 0.75 * errs[0], errs
 
 
 def test_cfl_closed_form(oracle_mod):
