@@ -175,6 +175,54 @@ int phlx_helm_rhoP_given_batch(phlx_ctx *ctx, int64_t n, const double *rho, cons
                                const double *abar, const double *zbar, double *T, double *E,
                                double *sound, int32_t return_sound, int32_t *iters);
 
+/* ---- T(rho, e) surrogate: guess of the Newton inversion ----
+ * No counterpart in the reference, whose helm_rhoe_given
+ * (source/source.F90:24982-25019) starts from the carried temperature.  A
+ * 4 -> 64 -> 64 -> 1 tanh network maps normalised (log10 rho, log10 e,
+ * abar, zbar) to normalised log10 T; the 64 x 64 layer runs on BF16 MFMA
+ * with FP32 accumulation.  Parameters are a flat FP32 vector of 4545 values
+ * (w1[4][64], b1[64], w2[64][64], b2[64], w3[64], b3) and a normalisation
+ * norm[10] = (in_off[4], in_scale[4], out_off, out_scale):
+ * x = (raw - in_off) * in_scale, log10 T = y * out_scale + out_off.
+ *
+ * phlx_mlp_train: generates nsamples (rho, e, abar, zbar) -> T pairs with
+ * the loaded table (log-uniform over the table box, abar in [4, 16], Ye in
+ * [0.45, 0.5], labels from the FP64 Newton solve) and trains on the GPU
+ * (Adam, cosine learning rate, loss weighted by (d ln E / d ln T)^2, i.e.
+ * measured in the energy residual).  Outputs (may be NULL): the last batch
+ * loss and the held-out RMS errors of log10 T and of the implied log10 e. */
+int phlx_mlp_train(phlx_ctx *ctx, int64_t nsamples, int32_t epochs, int32_t batch, double lr,
+                   uint64_t seed, double *train_loss, double *val_rms_logT,
+                   double *val_rms_loge);
+int phlx_mlp_set_params(phlx_ctx *ctx, const float *theta, const float *norm);
+int phlx_mlp_get_params(phlx_ctx *ctx, float *theta, float *norm);
+/* Inference on raw inputs raw[n][4] (device): normalised outputs y[n] and
+ * in-box flags (may be NULL). */
+int phlx_mlp_forward(phlx_ctx *ctx, int64_t n, const float *raw, float *y, int32_t *inbox);
+/* Gradient of (1/n) sum wt (y - tn)^2 over normalised inputs xn[n][4] (all
+ * device; wt may be NULL for unit weights) into grad[4545] (device); loss to
+ * the host.  For checking the training kernels. */
+int phlx_mlp_grad(phlx_ctx *ctx, int64_t n, const float *xn, const float *tn, const float *wt,
+                  float *grad, double *loss);
+/* helm_rhoe_given with a chosen start: guess_mode 0 = the given T (the
+ * reference), 1 = the surrogate (falls back to the bracketed guess when the
+ * inputs leave its training box or its first energy residual exceeds the
+ * threshold of phlx_set_eos_guess, default 0.25), 2 = bracketed guess
+ * (bisection in log T over the table).  fallback (may be NULL) flags points
+ * that fell back.  Device pointers. */
+int phlx_helm_rhoe_given_guess_batch(phlx_ctx *ctx, int64_t n, const double *rho,
+                                     const double *e, const double *abar, const double *zbar,
+                                     double *T, double *P, double *sound, int32_t return_sound,
+                                     int32_t guess_mode, int32_t *iters, int32_t *fallback);
+/* Start of the per-zone T(rho, e) Newton inside the Helmholtz update: 0 the
+ * carried T (the reference), 1 the carried T corrected by the surrogate
+ * (log10 T_prev + s (y_now - y_prev)), 2 bracketed.  count_iterations != 0
+ * accumulates Newton iterations / solves / fallbacks (phlx_eos_stats).
+ * Modes other than 0, and counting, need nx1 % 64 == 0 and nx2 % 4 == 0. */
+int phlx_set_eos_guess(phlx_ctx *ctx, int32_t mode, double resid_tol, int32_t count_iterations);
+int phlx_eos_stats(phlx_ctx *ctx, int64_t *newton_iters, int64_t *solves, int64_t *fallbacks,
+                   int32_t reset);
+
 /* Diagnostics: max over cells of |sum_d (b_d(+) - b_d(-))/dx_d|. */
 int phlx_divb_max(phlx_ctx *ctx, double *out);
 /* Per-kernel device time accumulated while profiling is on (HIP events on

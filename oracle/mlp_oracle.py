@@ -60,13 +60,16 @@ def forward(theta: np.ndarray, x: np.ndarray, quant: bool = True):
     return y, dict(x=x, w1=w1, w2=w2, w3=w3, h1=h1, h1q=h1q, h2=h2)
 
 
-def loss_and_grad(theta: np.ndarray, x: np.ndarray, t: np.ndarray, quant: bool = True):
-    """Mean squared error over the batch and its gradient w.r.t. theta."""
+def loss_and_grad(theta: np.ndarray, x: np.ndarray, t: np.ndarray, quant: bool = True,
+                  w: np.ndarray = None):
+    """Weighted mean squared error (1/B) sum w (y - t)^2 over the batch (w = 1
+    by default) and its gradient w.r.t. theta."""
     y, c = forward(theta, x, quant)
     B = y.shape[0]
     r = y - t
-    L = float(np.mean(r * r))
-    d3 = 2.0 * r / B                          # dL/dy
+    w = np.ones_like(r) if w is None else np.asarray(w, dtype=r.dtype)
+    L = float(np.mean(w * r * r))
+    d3 = 2.0 * w * r / B                      # dL/dy
     g = {}
     g["w3"] = c["h2"].T @ d3
     g["b3"] = np.array([d3.sum()])

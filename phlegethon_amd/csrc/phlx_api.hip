@@ -909,12 +909,12 @@ extern "C" int phlx_mlp_forward(phlx_ctx *ctx, int64_t n, const float *raw, floa
 }
 
 extern "C" int phlx_mlp_grad(phlx_ctx *ctx, int64_t n, const float *xn, const float *tn,
-                             float *grad, double *loss) {
+                             const float *wt, float *grad, double *loss) {
   if (!ctx || n <= 0 || !xn || !tn || !grad) return PHLX_E_ARG;
   if (!ctx->mlp_ready) return PHLX_E_STATE;
   if (n > (int64_t)kMlpMaxGradBlocks * 128) return PHLX_E_ARG;
   HIPCHK(hipSetDevice(ctx->device));
-  const int nblk = launch_mlp_grad(ctx->mlp_theta, ctx->mlp_w1q, ctx->mlp_w3q, xn, tn, 0, n,
+  const int nblk = launch_mlp_grad(ctx->mlp_theta, ctx->mlp_w1q, ctx->mlp_w3q, xn, tn, wt, 0, n,
                                    ctx->mlp_gpart, ctx->mlp_lpart, ctx->stream);
   launch_mlp_adam(ctx->mlp_theta, ctx->mlp_m, ctx->mlp_v, ctx->mlp_gpart, nblk, 1, 0.0f,
                   ctx->mlp_w1q, ctx->mlp_w3q, ctx->mlp_w2t, grad, ctx->stream);
@@ -933,7 +933,8 @@ extern "C" int phlx_mlp_grad(phlx_ctx *ctx, int64_t n, const float *xn, const fl
 }
 
 extern "C" int phlx_mlp_train(phlx_ctx *ctx, int64_t nsamples, int32_t epochs, int32_t batch,
-                              double lr, uint64_t seed, double *train_loss, double *val_rmse) {
+                              double lr, uint64_t seed, double *train_loss, double *val_rms_logT,
+                              double *val_rms_loge) {
   if (!ctx || nsamples < 4096 || epochs < 1 || batch < 128 || batch % 128 ||
       batch > kMlpMaxGradBlocks * 128 || !(lr > 0.0))
     return PHLX_E_ARG;
@@ -950,9 +951,10 @@ extern "C" int phlx_mlp_train(phlx_ctx *ctx, int64_t nsamples, int32_t epochs, i
   const double ltlo = h.lT1, lthi = h.lT1 + h.dt * (h.nt - 1);
   const double box[8] = {ldlo + 0.05, ldhi - 0.05, ltlo + 0.05, lthi - 0.05, 4.0, 16.0, 0.45, 0.5};
   float *data = nullptr;
-  if (hipMalloc(&data, sizeof(float) * 10 * (size_t)n) != hipSuccess) return PHLX_E_NOMEM;
-  float *raw = data, *label = data + 4 * n, *xn = data + 5 * n, *tn = data + 9 * n;
-  launch_mlp_data(h, n, seed, box, raw, label, ctx->stream);
+  if (hipMalloc(&data, sizeof(float) * 11 * (size_t)n) != hipSuccess) return PHLX_E_NOMEM;
+  float *raw = data, *label = data + 4 * n, *xn = data + 5 * n, *tn = data + 9 * n,
+        *wt = data + 10 * n;
+  launch_mlp_data(h, n, seed, box, raw, label, wt, ctx->stream);
   rc = check_launch();
   std::vector<float> hraw;
   if (!rc) {
@@ -1006,7 +1008,7 @@ extern "C" int phlx_mlp_train(phlx_ctx *ctx, int64_t nsamples, int32_t epochs, i
     const long long n0 = ((t - 1) % spe) * batch;
     const double cosf_ = 0.5 * (1.0 + std::cos(3.141592653589793 * (double)(t - 1) / (double)steps));
     const float lr_t = (float)(lr * (0.05 + 0.95 * cosf_));
-    nblk = launch_mlp_grad(ctx->mlp_theta, ctx->mlp_w1q, ctx->mlp_w3q, xn, tn, n0, batch,
+    nblk = launch_mlp_grad(ctx->mlp_theta, ctx->mlp_w1q, ctx->mlp_w3q, xn, tn, wt, n0, batch,
                            ctx->mlp_gpart, ctx->mlp_lpart, ctx->stream);
     launch_mlp_adam(ctx->mlp_theta, ctx->mlp_m, ctx->mlp_v, ctx->mlp_gpart, nblk, (int)t, lr_t,
                     ctx->mlp_w1q, ctx->mlp_w3q, ctx->mlp_w2t, nullptr, ctx->stream);
@@ -1025,25 +1027,30 @@ extern "C" int phlx_mlp_train(phlx_ctx *ctx, int64_t nsamples, int32_t epochs, i
       *train_loss = s / (double)batch;
     }
   }
-  if (!rc && val_rmse) {
-    // held-out tail: RMS error of log10 T
+  if (!rc && (val_rms_logT || val_rms_loge)) {
+    // held-out tail: RMS error of log10 T and of the implied log10 E
+    // (|d ln E / d ln T| times the log10 T error)
     float *y = xn;  // reuse: the normalised inputs of the training part are done
     launch_mlp_forward(ctx->g.mlp, nval, raw + 4 * ntr, y, nullptr, ctx->stream);
-    std::vector<float> hy(nval), ht(nval);
+    std::vector<float> hy(nval), ht(nval), hw(nval);
     if (check_launch() ||
         hipMemcpyAsync(hy.data(), y, sizeof(float) * nval, hipMemcpyDeviceToHost, ctx->stream) !=
             hipSuccess ||
         hipMemcpyAsync(ht.data(), tn + ntr, sizeof(float) * nval, hipMemcpyDeviceToHost,
                        ctx->stream) != hipSuccess ||
+        hipMemcpyAsync(hw.data(), wt + ntr, sizeof(float) * nval, hipMemcpyDeviceToHost,
+                       ctx->stream) != hipSuccess ||
         hipStreamSynchronize(ctx->stream) != hipSuccess) {
       rc = PHLX_E_HIP;
     } else {
-      double s = 0.0;
+      double s = 0.0, se = 0.0;
       for (long long i = 0; i < nval; i++) {
         const double d = ((double)hy[i] - (double)ht[i]) * norm[9];
         s += d * d;
+        se += (double)hw[i] * d * d;
       }
-      *val_rmse = std::sqrt(s / (double)nval);
+      if (val_rms_logT) *val_rms_logT = std::sqrt(s / (double)nval);
+      if (val_rms_loge) *val_rms_loge = std::sqrt(se / (double)nval);
     }
   }
   (void)hipStreamSynchronize(ctx->stream);

@@ -117,13 +117,15 @@ __device__ __forceinline__ bool flips(const Grid &g, int a, int side, long long 
 
 // Load primitive state (WB deviation when g.wb) and cell-centred B of a
 // possibly-ghost cell.
-template <int SD, int NV, bool MHD, bool GAM = false>
+// FAST: the caller has established that (i, j, k) is an interior cell, so
+// the boundary map and the mirror flips are skipped (same values).
+template <int SD, int NV, bool MHD, bool GAM = false, bool FAST = false>
 __device__ __forceinline__ void load_cell(const Grid &g, int i, int j, int k, double *q,
                                           double *b, double *ga = nullptr) {
-  int si, sj, sk = 0;
-  int mi = map_axis(g, 0, i, g.n1, si);
-  int mj = map_axis(g, 1, j, g.n2, sj);
-  int mk = (SD == 3) ? map_axis(g, 2, k, g.n3, sk) : k;
+  int si = 0, sj = 0, sk = 0;
+  int mi = FAST ? i : map_axis(g, 0, i, g.n1, si);
+  int mj = FAST ? j : map_axis(g, 1, j, g.n2, sj);
+  int mk = (SD == 3 && !FAST) ? map_axis(g, 2, k, g.n3, sk) : k;
   long long c = cix(g, mi, mj, mk);
   const long long N = g.N;
 #pragma unroll
@@ -143,6 +145,7 @@ __device__ __forceinline__ void load_cell(const Grid &g, int i, int j, int k, do
 #pragma unroll
     for (int v = 0; v < SD; v++) b[v] = g.bcc[v * N + c];
   }
+  if (FAST) return;
   if (si && flips(g, 0, si, cix(g, si < 0 ? 0 : g.n1 - 1, mj, mk))) {
     q[1] = -q[1];
     if (MHD) b[0] = -b[0];
@@ -724,37 +727,39 @@ __device__ __forceinline__ void riemann_hydro(const Grid &g, const float *qL, co
     }
     return;
   }
+  // HLLC: F = f + s (U* - U).  At low Mach U* - U is a small difference of
+  // O(1) states multiplied by s ~ c, so it is formed from the small
+  // quantities directly: rho* - rho = rho (u* - u) / (s - u*), and
+  // E* - E = (rho* - rho) E / rho + rho* (u* - u) (u* + p / (rho (s - u))).
   const bool left = us > 0.0f;
   const float rho = left ? rhoL : rhoR, u = left ? uL : uR, vv = left ? vL : vR,
               ww = left ? wL : wR, E = left ? EL : ER, pf = left ? pfL : pfR,
               pd = left ? dpL : dpR;
-  const float s = left ? sL : sR, rs = left ? rsL : rsR, Es = left ? EsL : EsR;
+  const float s = left ? sL : sR, rs = left ? rsL : rsR, d = left ? dL : dR;
   const float *q = left ? qL : qR;
-  float f[NV], U[NV];
-  U[0] = rho; U[1] = rho * u; U[2] = rho * vv;
-  if (SD == 3) U[3] = rho * ww;
-  U[IE] = E;
+  float f[NV];
   f[0] = rho * u; f[1] = rho * u * u + pd; f[2] = rho * u * vv;
   if (SD == 3) f[3] = rho * u * ww;
   f[IE] = (E + pf) * u;
 #pragma unroll
-  for (int v = IA; v < NV; v++) {
-    U[v] = rho * q[v];
-    f[v] = rho * u * q[v];
-  }
+  for (int v = IA; v < NV; v++) f[v] = rho * u * q[v];
   const bool outer = left ? (sL >= 0.0f) : (sR < 0.0f);
   if (outer) {
 #pragma unroll
     for (int v = 0; v < NV; v++) F[v] = f[v];
   } else {
-    float Us[NV];
-    Us[0] = rs; Us[1] = rs * us; Us[2] = rs * vv;
-    if (SD == 3) Us[3] = rs * ww;
-    Us[IE] = Es;
+    const float du = us - u;
+    const float D = rho * du / (s - us);
+    float dU[NV];
+    dU[0] = D;
+    dU[1] = D * us + rho * du;
+    dU[2] = D * vv;
+    if (SD == 3) dU[3] = D * ww;
+    dU[IE] = D * (E / rho) + rs * du * (us + pf / (rho * d));
 #pragma unroll
-    for (int v = IA; v < NV; v++) Us[v] = rs * q[v];
+    for (int v = IA; v < NV; v++) dU[v] = D * q[v];
 #pragma unroll
-    for (int v = 0; v < NV; v++) F[v] = f[v] + s * (Us[v] - U[v]);
+    for (int v = 0; v < NV; v++) F[v] = f[v] + s * dU[v];
   }
 }
 
@@ -882,27 +887,25 @@ __device__ __forceinline__ void riemann_mhd(const Grid &g, const float *qL, cons
     Fb[2] = u * bz - bx * w;
     return;
   }
+  // HLLD: F = f + s (U* - U) [+ s** (U** - U*)], the differences formed
+  // from small quantities (as in the HLLC branch): with du = u* - u and
+  // dm = s - u*, rho* - rho = rho du / dm, v* - v = -bx by du ia,
+  // b*_y - b_y = b_y rho (s - u) du ia, and
+  // E* - E = (du (E + pt) + (p* - pt) u* + bx (vb - vb*)) / dm.
   const bool left = us > 0.0f;
   const float rho = left ? rhoL : rhoR, u = left ? uL : uR, v = left ? vL : vR,
               w = left ? wL : wR, E = left ? EL : ER, ptf = left ? ptL : ptR,
               ptd = left ? pdL : pdR, by = left ? byL : byR, bz = left ? bzL : bzR,
               vb = left ? vbL : vbR;
   const float *q = left ? qL : qR;
-  float f[NV], U[NV], fb[3], Ub[3];
-  U[0] = rho; U[1] = rho * u; U[2] = rho * v;
-  if (SD == 3) U[3] = rho * w;
-  U[IE] = E;
+  float f[NV], fb[3];
   f[0] = rho * u;
   f[1] = rho * u * u + ptd - bx2;
   f[2] = rho * u * v - bx * by;
   if (SD == 3) f[3] = rho * u * w - bx * bz;
   f[IE] = (E + ptf) * u - bx * vb;
 #pragma unroll
-  for (int iv = IA; iv < NV; iv++) {
-    U[iv] = rho * q[iv];
-    f[iv] = rho * u * q[iv];
-  }
-  Ub[0] = bx; Ub[1] = by; Ub[2] = bz;
+  for (int iv = IA; iv < NV; iv++) f[iv] = rho * u * q[iv];
   fb[0] = 0.0f;
   fb[1] = u * by - bx * v;
   fb[2] = u * bz - bx * w;
@@ -916,34 +919,47 @@ __device__ __forceinline__ void riemann_mhd(const Grid &g, const float *qL, cons
     return;
   }
   const float rs = left ? rsL : rsR, vs = left ? vsL : vsR, ws = left ? wsL : wsR,
-              Es = left ? EsL : EsR, bys = left ? bysL : bysR, bzs = left ? bzsL : bzsR;
-  float Us[NV], Ubs[3];
-  Us[0] = rs; Us[1] = rs * us; Us[2] = rs * vs;
-  if (SD == 3) Us[3] = rs * ws;
-  Us[IE] = Es;
+              bys = left ? bysL : bysR, bzs = left ? bzsL : bzsR,
+              vbs = left ? vbsL : vbsR, d = left ? dL : dR, dm = left ? dmL : dmR,
+              ia = left ? ia1 : ia2;
+  const float du = us - u;
+  const float D = rho * du / dm;
+  float dU[NV], dUb[3];
+  dU[0] = D;
+  dU[1] = D * us + rho * du;
+  dU[2] = D * vs + rho * (-bx * by * du * ia);
+  if (SD == 3) dU[3] = D * ws + rho * (-bx * bz * du * ia);
+  dU[IE] = (du * (E + ptf) + (psd - ptd) * us + bx * (vb - vbs)) / dm;
 #pragma unroll
-  for (int iv = IA; iv < NV; iv++) Us[iv] = rs * q[iv];
-  Ubs[0] = bx; Ubs[1] = bys; Ubs[2] = bzs;
+  for (int iv = IA; iv < NV; iv++) dU[iv] = D * q[iv];
+  dUb[0] = 0.0f;
+  dUb[1] = by * rho * d * du * ia;
+  dUb[2] = bz * rho * d * du * ia;
   const bool single = left ? (ssL > 0.0f) : (ssR < 0.0f);
   if (single) {
 #pragma unroll
-    for (int iv = 0; iv < NV; iv++) F[iv] = f[iv] + s * (Us[iv] - U[iv]);
+    for (int iv = 0; iv < NV; iv++) F[iv] = f[iv] + s * dU[iv];
 #pragma unroll
-    for (int iv = 0; iv < 3; iv++) Fb[iv] = fb[iv] + s * (Ubs[iv] - Ub[iv]);
+    for (int iv = 0; iv < 3; iv++) Fb[iv] = fb[iv] + s * dUb[iv];
     return;
   }
-  const float ss = left ? ssL : ssR, Ess = left ? EssL : EssR;
-  float Uss[NV], Ubss[3];
-  Uss[0] = rs; Uss[1] = rs * us; Uss[2] = rs * vss;
-  if (SD == 3) Uss[3] = rs * wss;
-  Uss[IE] = Ess;
+  // double-star state: same density and normal velocity as the star state
+  const float ss = left ? ssL : ssR, sq = left ? sqL : sqR;
+  float dU2[NV], dUb2[3];
+  dU2[0] = 0.0f;
+  dU2[1] = 0.0f;
+  dU2[2] = rs * (vss - vs);
+  if (SD == 3) dU2[3] = rs * (wss - ws);
+  dU2[IE] = left ? -sq * (vbs - vbss) * sg : sq * (vbs - vbss) * sg;
 #pragma unroll
-  for (int iv = IA; iv < NV; iv++) Uss[iv] = rs * q[iv];
-  Ubss[0] = bx; Ubss[1] = byss; Ubss[2] = bzss;
+  for (int iv = IA; iv < NV; iv++) dU2[iv] = 0.0f;
+  dUb2[0] = 0.0f;
+  dUb2[1] = byss - bys;
+  dUb2[2] = bzss - bzs;
 #pragma unroll
-  for (int iv = 0; iv < NV; iv++) F[iv] = f[iv] + s * (Us[iv] - U[iv]) + ss * (Uss[iv] - Us[iv]);
+  for (int iv = 0; iv < NV; iv++) F[iv] = f[iv] + s * dU[iv] + ss * dU2[iv];
 #pragma unroll
-  for (int iv = 0; iv < 3; iv++) Fb[iv] = fb[iv] + s * (Ubs[iv] - Ub[iv]) + ss * (Ubss[iv] - Ubs[iv]);
+  for (int iv = 0; iv < 3; iv++) Fb[iv] = fb[iv] + s * dUb[iv] + ss * dUb2[iv];
 }
 
 }  // namespace mx

@@ -41,12 +41,12 @@ void launch_helm_rhoP(const HelmDev &h, long long n, const double *rho, const do
 
 // T(rho, e) surrogate (phlx_mlp.hip)
 void launch_mlp_data(const HelmDev &h, long long n, unsigned long long seed, const double box[8],
-                     float *raw, float *label, hipStream_t st);
+                     float *raw, float *label, float *wt, hipStream_t st);
 void launch_mlp_normalize(const MlpDev &m, long long n, const float *raw, const float *label,
                           float *xn, float *tn, hipStream_t st);
 int launch_mlp_grad(const float *theta, const float *w1q, const float *w3q, const float *xn,
-                    const float *tn, long long n0, long long nb, float *gpart, float *lpart,
-                    hipStream_t st);
+                    const float *tn, const float *wt, long long n0, long long nb, float *gpart,
+                    float *lpart, hipStream_t st);
 void launch_mlp_adam(float *theta, float *m, float *v, const float *gpart, int nblk, int t,
                      float lr, float *w1q, float *w3q, unsigned short *w2t, float *gout,
                      hipStream_t st);

@@ -13,6 +13,15 @@
 
 namespace phlx {
 
+// Whether every cell of this face's stencil (cells x-O .. x+NS-O-1 along
+// DIR) is interior; a wave whose lanes all say yes takes the unmapped loads.
+template <int DIR, int NS, int O>
+__device__ __forceinline__ bool stencil_inside(const Grid &g, int i, int j, int k) {
+  const int x = (DIR == 0) ? i : (DIR == 1) ? j : k;
+  const int n = (DIR == 0) ? g.n1 : (DIR == 1) ? g.n2 : g.n3;
+  return x - O >= 0 && x + NS - O - 1 < n;
+}
+
 // ---------------------------------------------------------------- flux
 // One thread per face along DIR.  Loads the four cells f-2..f+1, builds the
 // left state (right edge of cell f-1) and right state (left edge of cell
@@ -31,11 +40,20 @@ __global__ void __launch_bounds__(256) k_flux(Grid g) {
   if (i >= e1 || j >= e2) return;
 
   double q[NS][NV], b[NS][3], ga[NS][2];
+  if (__all(stencil_inside<DIR, NS, O>(g, i, j, k))) {
 #pragma unroll
-  for (int s = 0; s < NS; s++) {
-    const int o = s - O;
-    load_cell<SD, NV, MHD, HELM>(g, i + (DIR == 0 ? o : 0), j + (DIR == 1 ? o : 0),
-                                 k + (DIR == 2 ? o : 0), q[s], b[s], ga[s]);
+    for (int s = 0; s < NS; s++) {
+      const int o = s - O;
+      load_cell<SD, NV, MHD, HELM, true>(g, i + (DIR == 0 ? o : 0), j + (DIR == 1 ? o : 0),
+                                         k + (DIR == 2 ? o : 0), q[s], b[s], ga[s]);
+    }
+  } else {
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+      const int o = s - O;
+      load_cell<SD, NV, MHD, HELM>(g, i + (DIR == 0 ? o : 0), j + (DIR == 1 ? o : 0),
+                                   k + (DIR == 2 ? o : 0), q[s], b[s], ga[s]);
+    }
   }
   double qL[NV], qR[NV], bL[3] = {0.0, 0.0, 0.0}, bR[3] = {0.0, 0.0, 0.0};
   // scalars take the limited parabola under PPH (source/source.F90:8194-8243)
@@ -150,12 +168,17 @@ __global__ void __launch_bounds__(256) k_flux_mx(Grid g) {
   if (i >= e1 || j >= e2) return;
 
   float q[NS][NV], b[NS][3], ga[NS][2];
+  const bool fast = __all(stencil_inside<DIR, NS, O>(g, i, j, k));
 #pragma unroll
   for (int s = 0; s < NS; s++) {
     const int o = s - O;
     double qd[NV], bd[3] = {0.0, 0.0, 0.0}, gd[2] = {0.0, 0.0};
-    load_cell<SD, NV, MHD, HELM>(g, i + (DIR == 0 ? o : 0), j + (DIR == 1 ? o : 0),
-                                 k + (DIR == 2 ? o : 0), qd, bd, gd);
+    if (fast)
+      load_cell<SD, NV, MHD, HELM, true>(g, i + (DIR == 0 ? o : 0), j + (DIR == 1 ? o : 0),
+                                         k + (DIR == 2 ? o : 0), qd, bd, gd);
+    else
+      load_cell<SD, NV, MHD, HELM>(g, i + (DIR == 0 ? o : 0), j + (DIR == 1 ? o : 0),
+                                   k + (DIR == 2 ? o : 0), qd, bd, gd);
 #pragma unroll
     for (int v = 0; v < NV; v++) q[s][v] = (float)qd[v];
 #pragma unroll

@@ -4,7 +4,11 @@
 // Training data are generated on the GPU by the Helmholtz EoS itself:
 // (log10 rho*Ye, log10 T, abar, Ye) drawn uniformly over the table box,
 // e = E(rho, T) from the table, and the label log10 T from the FP64 Newton
-// solve of T(rho, e) started at the bracketed guess.  One optimisation step
+// solve of T(rho, e) started at the bracketed guess.  The loss weights each
+// sample's squared log T error by (d ln E / d ln T)^2 at the label, i.e. it
+// measures the guess in the energy residual the Newton iteration works on:
+// in degenerate matter E hardly depends on T (d ln T / d ln E reaches 1e4),
+// so log T there is ill-determined and irrelevant to the iteration count.  One optimisation step
 // is two kernels: k_mlp_grad (128 samples per block: forward with the same
 // BF16 rounding points as the inference, backward in FP32, block-partial
 // gradients reduced through LDS) and k_mlp_adam (sums the partials, Adam
@@ -35,7 +39,7 @@ __device__ __forceinline__ double unit01(unsigned long long seed, long long i, i
 __global__ void __launch_bounds__(256)
 k_mlp_data(HelmDev h, long long n, unsigned long long seed, double lrs_lo, double lrs_hi,
            double lt_lo, double lt_hi, double a_lo, double a_hi, double ye_lo, double ye_hi,
-           float *__restrict__ raw, float *__restrict__ label) {
+           float *__restrict__ raw, float *__restrict__ label, float *__restrict__ wt) {
   const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
   const double lrs = lrs_lo + unit01(seed, i, 0) * (lrs_hi - lrs_lo);
@@ -47,6 +51,10 @@ k_mlp_data(HelmDev h, long long n, unsigned long long seed, double lrs_lo, doubl
   helm_given2(h, rho, exp10(lt), abar, zbar, e, dedt, false);
   double T = helm_bracket_guess(h, rho, e, abar, zbar), P, cs;
   helm_rhoe_given(h, rho, e, abar, zbar, T, P, cs, false);
+  double el, dedtl;
+  helm_given2(h, rho, T, abar, zbar, el, dedtl, true);
+  const double slope = dedtl * T / el;  // d ln E / d ln T at the label
+  wt[i] = (float)fmin(fmax(slope * slope, 1e-8), 1e4);
   raw[4 * i + 0] = (float)log10(rho);
   raw[4 * i + 1] = (float)log10(e);
   raw[4 * i + 2] = (float)abar;
@@ -69,8 +77,8 @@ k_mlp_normalize(MlpDev m, long long n, const float *__restrict__ raw,
 __global__ void __launch_bounds__(kGradBlock)
 k_mlp_grad(const float *__restrict__ theta, const float *__restrict__ w1q,
            const float *__restrict__ w3q, const float *__restrict__ xn,
-           const float *__restrict__ tn, long long n0, long long nb, float inv_b,
-           float *__restrict__ gpart, float *__restrict__ lpart) {
+           const float *__restrict__ tn, const float *__restrict__ wt, long long n0,
+           long long nb, float inv_b, float *__restrict__ gpart, float *__restrict__ lpart) {
   __shared__ float w2s[kMlpH * kMlpH];
   __shared__ float A[kGradBlock * 65];
   __shared__ float Bm[kGradBlock * 65];
@@ -81,10 +89,13 @@ k_mlp_grad(const float *__restrict__ theta, const float *__restrict__ w1q,
   const long long sl = (long long)blockIdx.x * kGradBlock + tid;
   const bool valid = sl < nb;
   const long long s = n0 + sl;
-  float x[4], t = 0.0f;
+  float x[4], t = 0.0f, w = 0.0f;
 #pragma unroll
   for (int c = 0; c < 4; c++) x[c] = valid ? xn[4 * s + c] : 0.0f;
-  if (valid) t = tn[s];
+  if (valid) {
+    t = tn[s];
+    w = wt ? wt[s] : 1.0f;
+  }
   float h1[kMlpH], h2[kMlpH];
 #pragma unroll
   for (int k = 0; k < kMlpH; k++) {
@@ -104,7 +115,7 @@ k_mlp_grad(const float *__restrict__ theta, const float *__restrict__ w1q,
     y = y + h2[n] * w3q[n];
   }
   const float r = valid ? y - t : 0.0f;
-  const float d3 = 2.0f * r * inv_b;
+  const float d3 = 2.0f * w * r * inv_b;
   // phase 1: dW2 = h1q^T d2, db2 = sum d2
 #pragma unroll
   for (int k = 0; k < kMlpH; k++) A[tid * 65 + k] = bf16_round(h1[k]);
@@ -164,7 +175,7 @@ k_mlp_grad(const float *__restrict__ theta, const float *__restrict__ w1q,
   }
   // loss and db3 through the now-free d3s / A scratch
   __syncthreads();
-  A[tid] = r * r;
+  A[tid] = w * r * r;
   __syncthreads();
   if (tid == 0) {
     float acc = 0.0f, l = 0.0f;
@@ -284,9 +295,9 @@ inline unsigned nblocks(long long n, int b) { return (unsigned)((n + b - 1) / b)
 }  // namespace
 
 void launch_mlp_data(const HelmDev &h, long long n, unsigned long long seed, const double box[8],
-                     float *raw, float *label, hipStream_t st) {
+                     float *raw, float *label, float *wt, hipStream_t st) {
   hipLaunchKernelGGL(k_mlp_data, dim3(nblocks(n, 256)), dim3(256), 0, st, h, n, seed, box[0],
-                     box[1], box[2], box[3], box[4], box[5], box[6], box[7], raw, label);
+                     box[1], box[2], box[3], box[4], box[5], box[6], box[7], raw, label, wt);
 }
 void launch_mlp_normalize(const MlpDev &m, long long n, const float *raw, const float *label,
                           float *xn, float *tn, hipStream_t st) {
@@ -294,11 +305,11 @@ void launch_mlp_normalize(const MlpDev &m, long long n, const float *raw, const 
                      xn, tn);
 }
 int launch_mlp_grad(const float *theta, const float *w1q, const float *w3q, const float *xn,
-                    const float *tn, long long n0, long long nb, float *gpart, float *lpart,
-                    hipStream_t st) {
+                    const float *tn, const float *wt, long long n0, long long nb, float *gpart,
+                    float *lpart, hipStream_t st) {
   const int nblk = (int)nblocks(nb, kGradBlock);
-  hipLaunchKernelGGL(k_mlp_grad, dim3(nblk), dim3(kGradBlock), 0, st, theta, w1q, w3q, xn, tn, n0,
-                     nb, 1.0f / (float)nb, gpart, lpart);
+  hipLaunchKernelGGL(k_mlp_grad, dim3(nblk), dim3(kGradBlock), 0, st, theta, w1q, w3q, xn, tn, wt,
+                     n0, nb, 1.0f / (float)nb, gpart, lpart);
   return nblk;
 }
 void launch_mlp_adam(float *theta, float *m, float *v, const float *gpart, int nblk, int t,

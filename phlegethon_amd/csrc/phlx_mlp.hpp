@@ -214,7 +214,11 @@ __device__ __forceinline__ int helm_rhoe_guess(const HelmDev &h, int mode, doubl
                                                double &sound, bool ret, int &fallback) {
   fallback = 0;
   if (mode == GUESS_MLP) {
-    if (tg >= h.tl && tg <= h.th) {
+    // a guess outside the table's T range is clamped into it (in degenerate
+    // matter log T is ill-determined and any T there has a small energy
+    // residual); a missing guess (tg <= 0 or NaN) falls back
+    if (tg > 0.0) {
+      tg = fmin(fmax(tg, h.tl * (1.0 + 1e-9)), h.th * (1.0 - 1e-9));
       double e, dedt;
       helm_given2(h, rho, tg, abar, zbar, e, dedt, true);
       if (fabs((e - E0) / E0) <= (double)resid_tol) {

@@ -82,8 +82,8 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.phlx_mlp_set_params.argtypes = [P, P, P]
     lib.phlx_mlp_get_params.argtypes = [P, P, P]
     lib.phlx_mlp_forward.argtypes = [P, ctypes.c_int64, P, P, P]
-    lib.phlx_mlp_grad.argtypes = [P, ctypes.c_int64, P, P, P, DP]
-    lib.phlx_mlp_train.argtypes = [P, ctypes.c_int64, I, I, D, ctypes.c_uint64, DP, DP]
+    lib.phlx_mlp_grad.argtypes = [P, ctypes.c_int64, P, P, P, P, DP]
+    lib.phlx_mlp_train.argtypes = [P, ctypes.c_int64, I, I, D, ctypes.c_uint64, DP, DP, DP]
     lib.phlx_set_eos_guess.argtypes = [P, I, D, I]
     I64P = ctypes.POINTER(ctypes.c_int64)
     lib.phlx_eos_stats.argtypes = [P, I64P, I64P, I64P, I]
@@ -273,12 +273,12 @@ class Simulation:
                   lr: float = 3e-3, seed: int = 20261202):
         """Generate (rho, e, abar, zbar) -> T pairs with the loaded table and
         train the surrogate on the GPU; returns (final batch loss, held-out
-        RMS error of log10 T)."""
-        loss, rmse = ctypes.c_double(), ctypes.c_double()
+        RMS error of log10 T, held-out RMS of the implied log10 E error)."""
+        loss, rmst, rmse = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
         _chk(self.lib.phlx_mlp_train(self._ctx, int(nsamples), int(epochs), int(batch),
                                      float(lr), int(seed), ctypes.byref(loss),
-                                     ctypes.byref(rmse)), "phlx_mlp_train")
-        return loss.value, rmse.value
+                                     ctypes.byref(rmst), ctypes.byref(rmse)), "phlx_mlp_train")
+        return loss.value, rmst.value, rmse.value
 
     def mlp_params(self):
         theta = np.empty(MLP_NPARAM, dtype=np.float32)
@@ -417,16 +417,19 @@ class HelmEoS:
         self._sync_out()
         return y.cpu().numpy(), inbox.cpu().numpy()
 
-    def mlp_grad(self, xn, tn):
+    def mlp_grad(self, xn, tn, wt=None):
         xn = np.ascontiguousarray(xn, dtype=np.float32)
         tn = np.ascontiguousarray(tn, dtype=np.float32)
         n = xn.shape[0]
         x = self.torch.from_numpy(xn.copy()).to(self.device)
         t = self.torch.from_numpy(tn.copy()).to(self.device)
+        w = None if wt is None else self.torch.from_numpy(
+            np.ascontiguousarray(wt, dtype=np.float32).copy()).to(self.device)
         g = self._empty(MLP_NPARAM, self.torch.float32)
         loss = ctypes.c_double()
         self._sync_in()
-        _chk(self.lib.phlx_mlp_grad(self.sim.ctx, n, self._p(x), self._p(t), self._p(g),
+        _chk(self.lib.phlx_mlp_grad(self.sim.ctx, n, self._p(x), self._p(t),
+                                    None if w is None else self._p(w), self._p(g),
                                     ctypes.byref(loss)), "phlx_mlp_grad")
         self._sync_out()
         return g.cpu().numpy(), loss.value

@@ -4,11 +4,13 @@
 - inference (layer 2 on BF16 MFMA, FP32 accumulation) equals the oracle's
   quantised forward: median |dy| <= 1e-5, max <= 5e-3 (the tail is a BF16
   rounding of h1 that flips between fast tanh and numpy's tanh);
-- the training kernels' gradient of the batch MSE equals the oracle's
-  analytic gradient (itself pinned by finite differences) to 1e-3 relative
-  per parameter block;
-- GPU training on EoS-generated pairs reaches a held-out RMS error in
-  log10 T below 0.05 dex;
+- the training kernels' gradient of the weighted batch MSE equals the
+  oracle's analytic gradient (itself pinned by finite differences) to 1e-3
+  relative per parameter block, with unit and with random weights;
+- GPU training on EoS-generated pairs reaches a held-out RMS error of the
+  implied log10 e below 0.02 dex (the error the Newton iteration sees; log10
+  T itself is ill-determined in degenerate matter, where d ln T / d ln e
+  reaches 1e4);
 - with the trained network the Newton inversion converges to the same T as
   the plain Newton (1e-9 relative) and needs fewer EoS evaluations than
   the bracketed guess and than a cold fixed start; fallbacks are rare;
@@ -80,30 +82,32 @@ def test_gradients_match_oracle(eos, mlp_oracle):
     n = 1000
     xn = rng.uniform(-1, 1, (n, 4)).astype(np.float32)
     tn = rng.uniform(-1, 1, n).astype(np.float32)
-    g, loss = eos.mlp_grad(xn, tn)
-    lo, go = M.loss_and_grad(theta, xn, tn, quant=True)
-    print(f"loss gpu {loss:.6e} oracle {lo:.6e}", flush=True)
-    assert abs(loss / lo - 1.0) < 1e-4
-    o = 0
-    for name, shp in M.SHAPES:
-        k = int(np.prod(shp))
-        a, b = g[o:o + k], go[o:o + k]
-        rel = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
-        print(f"  grad {name}: rel {rel:.2e}", flush=True)
-        assert rel < 1e-3, name
-        o += k
+    for wt in (None, rng.uniform(0.01, 16.0, n).astype(np.float32)):
+        g, loss = eos.mlp_grad(xn, tn, wt)
+        lo, go = M.loss_and_grad(theta, xn, tn, quant=True, w=wt)
+        print(f"loss gpu {loss:.6e} oracle {lo:.6e}", flush=True)
+        assert abs(loss / lo - 1.0) < 1e-4
+        o = 0
+        for name, shp in M.SHAPES:
+            k = int(np.prod(shp))
+            a, b = g[o:o + k], go[o:o + k]
+            rel = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
+            print(f"  grad {name}: rel {rel:.2e}", flush=True)
+            assert rel < 1e-3, name
+            o += k
 
 
 @pytest.fixture(scope="module")
 def trained(eos):
-    loss, rmse = eos.sim.mlp_train(nsamples=1 << 18, epochs=6, batch=1024, lr=3e-3)
-    print(f"trained: final batch loss {loss:.3e}, held-out RMS log10 T {rmse:.4f}", flush=True)
-    return loss, rmse
+    loss, rmst, rmse = eos.sim.mlp_train(nsamples=1 << 20, epochs=8, batch=2048, lr=5e-3)
+    print(f"trained: final batch loss {loss:.3e}, held-out RMS log10 T {rmst:.4f}, "
+          f"implied log10 e {rmse:.4f}", flush=True)
+    return loss, rmst, rmse
 
 
 def test_training_reaches_accuracy(trained):
-    loss, rmse = trained
-    assert rmse < 0.05
+    loss, rmst, rmse = trained
+    assert rmse < 0.02
 
 
 def test_guess_gain(eos, trained, oracle_mod, table):
@@ -143,7 +147,7 @@ def test_hydro_with_surrogate_guess(table, oracle_mod):
     nsteps = 4
     sim = Simulation(pb.cfg)
     sim.setup(pb)
-    loss, rmse = sim.mlp_train(nsamples=1 << 18, epochs=6, batch=1024, lr=3e-3)
+    loss, rmst, rmse = sim.mlp_train(nsamples=1 << 20, epochs=8, batch=2048, lr=5e-3)
     sim.set_eos_guess("mlp", resid_tol=0.25, count=True)
     sim.init()
     sim.run(nsteps)
