@@ -1092,6 +1092,7 @@ extern "C" int phlx_set_eos_guess(phlx_ctx *ctx, int32_t mode, double resid_tol,
   ctx->g.eos_guess = mode;
   ctx->g.mlp.resid_tol = (float)resid_tol;
   ctx->eos_stats = count_iterations != 0;
+  ctx->g.eos_count = ctx->eos_stats ? 1 : 0;
   return PHLX_OK;
 }
 

@@ -63,6 +63,7 @@ struct Grid {
   // variant of the update
   MlpDev mlp;
   int eos_guess;
+  int eos_count;           // accumulate the counters (the guess variant only)
   unsigned int *eos_part;
   float *mlp_y;            // network output at each zone's last solve (NaN: none yet)
   double *red;       // per-block partial maxima of the cell-grid reductions

@@ -653,17 +653,19 @@ __global__ void __launch_bounds__(256) k_update(Grid g, int first, int last, dou
         int fb;
         const int it = helm_rhoe_guess(g.helm, mode, tg, g.mlp.resid_tol, rho, eint * ir, abar,
                                        zbar, T, pr, sound, true, fb);
-        const int tid = threadIdx.y * blockDim.x + threadIdx.x;
-        if (tid < 3) cnt[tid] = 0u;
-        __syncthreads();
-        atomicAdd(&cnt[0], (unsigned int)it);
-        atomicAdd(&cnt[1], 1u);
-        atomicAdd(&cnt[2], (unsigned int)fb);
-        __syncthreads();
-        if (tid < 3) {
-          const long long blk =
-              blockIdx.x + (long long)gridDim.x * (blockIdx.y + (long long)gridDim.y * blockIdx.z);
-          g.eos_part[3 * blk + tid] = g.eos_part[3 * blk + tid] + cnt[tid];
+        if (g.eos_count) {  // uniform across the grid
+          const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+          if (tid < 3) cnt[tid] = 0u;
+          __syncthreads();
+          atomicAdd(&cnt[0], (unsigned int)it);
+          atomicAdd(&cnt[1], 1u);
+          atomicAdd(&cnt[2], (unsigned int)fb);
+          __syncthreads();
+          if (tid < 3) {
+            const long long blk = blockIdx.x + (long long)gridDim.x *
+                                                   (blockIdx.y + (long long)gridDim.y * blockIdx.z);
+            g.eos_part[3 * blk + tid] = g.eos_part[3 * blk + tid] + cnt[tid];
+          }
         }
       } else {
         helm_rhoe_given(g.helm, rho, eint * ir, abar, zbar, T, pr, sound, true);
