@@ -151,9 +151,12 @@ def run_variant(pb, cfg, args, guess="warm", keep_state=False, profile=True):
     res = {}
     if guess == "mlp":
         t0 = time.perf_counter()
+        from phlegethon_amd.problems import state_box
+        box = state_box(pb)
         loss, rmst, rmse = sim.mlp_train(nsamples=args.mlp_samples, epochs=args.mlp_epochs,
-                                         batch=4096, lr=5e-3)
+                                         batch=4096, lr=5e-3, box=box)
         res["mlp"] = {"train_s": time.perf_counter() - t0, "samples": args.mlp_samples,
+                      "box": [round(v, 4) for v in box],
                       "epochs": args.mlp_epochs, "final_batch_loss": loss,
                       "heldout_rms_log10T": rmst, "heldout_rms_log10e": rmse}
         log(f"surrogate trained in {res['mlp']['train_s']:.2f} s, held-out RMS log10 T "

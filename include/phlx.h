@@ -186,11 +186,18 @@ int phlx_helm_rhoP_given_batch(phlx_ctx *ctx, int64_t n, const double *rho, cons
  * x = (raw - in_off) * in_scale, log10 T = y * out_scale + out_off.
  *
  * phlx_mlp_train: generates nsamples (rho, e, abar, zbar) -> T pairs with
- * the loaded table (log-uniform over the table box, abar in [4, 16], Ye in
- * [0.45, 0.5], labels from the FP64 Newton solve) and trains on the GPU
+ * the loaded table (log-uniform over the training box, by default the table
+ * box with abar in [4, 16] and Ye in [0.45, 0.5]; labels from the FP64
+ * Newton solve) and trains on the GPU
  * (Adam, cosine learning rate, loss weighted by (d ln E / d ln T)^2, i.e.
  * measured in the energy residual).  Outputs (may be NULL): the last batch
  * loss and the held-out RMS errors of log10 T and of the implied log10 e. */
+/* Training box of later phlx_mlp_train calls, box[8] = {log10(rho*Ye) lo, hi,
+ * log10 T lo, hi, abar lo, hi, Ye lo, hi} (host array; clipped to the table;
+ * NULL restores the table box).  A network trained on the states a run
+ * occupies is far more accurate there than one spread over the table's 27
+ * decades of density. */
+int phlx_mlp_set_train_box(phlx_ctx *ctx, const double *box);
 int phlx_mlp_train(phlx_ctx *ctx, int64_t nsamples, int32_t epochs, int32_t batch, double lr,
                    uint64_t seed, double *train_loss, double *val_rms_logT,
                    double *val_rms_loge);
@@ -218,8 +225,19 @@ int phlx_helm_rhoe_given_guess_batch(phlx_ctx *ctx, int64_t n, const double *rho
  * carried T (the reference), 1 the carried T corrected by the surrogate
  * (log10 T_prev + s (y_now - y_prev)), 2 bracketed.  count_iterations != 0
  * accumulates Newton iterations / solves / fallbacks (phlx_eos_stats).
- * Modes other than 0, and counting, need nx1 % 64 == 0 and nx2 % 4 == 0. */
+ * Modes other than 0, and counting, need nx1 % 64 == 0 and nx2 % 4 == 0.
+ * resid_tol is also the surrogate threshold of
+ * phlx_helm_rhoe_given_guess_batch; a context of another EoS may set it once
+ * a Helmholtz table is loaded. */
 int phlx_set_eos_guess(phlx_ctx *ctx, int32_t mode, double resid_tol, int32_t count_iterations);
+/* Mixed precision with the Helmholtz EoS: the per-zone T(rho, e) solve makes
+ * one FP64 evaluation at its start and, when |e(T) - e| / e < lin_tol
+ * (default 1e-5; 0 disables the shortcut), applies the Newton step linearly
+ * to T and P instead of iterating (error second order in the residual);
+ * larger residuals run the reference Newton.  No counterpart in the
+ * reference (its hydro_step always iterates helm_rhoe_given,
+ * source/source.F90:24982-25019).  lin_tol in [0, 1e-2]. */
+int phlx_set_eos_lin_tol(phlx_ctx *ctx, double lin_tol);
 int phlx_eos_stats(phlx_ctx *ctx, int64_t *newton_iters, int64_t *solves, int64_t *fallbacks,
                    int32_t reset);
 

@@ -43,6 +43,10 @@ struct phlx_ctx {
         *mlp_w3q = nullptr, *mlp_gpart = nullptr, *mlp_lpart = nullptr, *mlp_gsum = nullptr;
   unsigned short *mlp_w2t = nullptr;
   bool mlp_ready = false;
+  // training box {log10 rhoYe lo, hi, log10 T lo, hi, abar lo, hi, Ye lo, hi};
+  // has_box false: the table's range
+  double mlp_box[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  bool mlp_has_box = false;
   bool eos_stats = false;
   // profiling
   bool prof = false;
@@ -226,6 +230,7 @@ extern "C" int phlx_create(const phlx_config *cfg, phlx_ctx **out) {
     e |= dalloc(ctx, &g.gam, 2 * N);
   }
   g.eos_guess = GUESS_WARM;
+  g.eos_lin_tol = 1e-5;
   g.mlp.resid_tol = 0.25f;
   g.mlp.box = 1.02f;
   double *prof = nullptr;
@@ -932,6 +937,21 @@ extern "C" int phlx_mlp_grad(phlx_ctx *ctx, int64_t n, const float *xn, const fl
   return PHLX_OK;
 }
 
+extern "C" int phlx_mlp_set_train_box(phlx_ctx *ctx, const double *box) {
+  if (!ctx) return PHLX_E_ARG;
+  if (!box) {
+    ctx->mlp_has_box = false;
+    return PHLX_OK;
+  }
+  for (int c = 0; c < 8; c += 2)
+    if (!(box[c + 1] > box[c]) || !std::isfinite(box[c]) || !std::isfinite(box[c + 1]))
+      return PHLX_E_ARG;
+  if (!(box[4] > 0.0) || !(box[6] > 0.0) || box[7] > 1.0) return PHLX_E_ARG;
+  for (int c = 0; c < 8; c++) ctx->mlp_box[c] = box[c];
+  ctx->mlp_has_box = true;
+  return PHLX_OK;
+}
+
 extern "C" int phlx_mlp_train(phlx_ctx *ctx, int64_t nsamples, int32_t epochs, int32_t batch,
                               double lr, uint64_t seed, double *train_loss, double *val_rms_logT,
                               double *val_rms_loge) {
@@ -946,10 +966,19 @@ extern "C" int phlx_mlp_train(phlx_ctx *ctx, int64_t nsamples, int32_t epochs, i
   const long long n = nsamples, nval = n / 16, ntr = n - nval;
   if (ntr < batch) return PHLX_E_ARG;
   // sample box: the table's rho*Ye and T ranges (0.05 dex inside the edges),
-  // abar in [4, 16], Ye in [0.45, 0.5]
-  const double ldlo = h.lrho1, ldhi = h.lrho1 + h.dr * (h.nr - 1);
-  const double ltlo = h.lT1, lthi = h.lT1 + h.dt * (h.nt - 1);
-  const double box[8] = {ldlo + 0.05, ldhi - 0.05, ltlo + 0.05, lthi - 0.05, 4.0, 16.0, 0.45, 0.5};
+  // abar in [4, 16], Ye in [0.45, 0.5]; a box set by phlx_mlp_set_train_box
+  // replaces it, clipped to the same table range
+  const double ldlo = h.lrho1 + 0.05, ldhi = h.lrho1 + h.dr * (h.nr - 1) - 0.05;
+  const double ltlo = h.lT1 + 0.05, lthi = h.lT1 + h.dt * (h.nt - 1) - 0.05;
+  double box[8] = {ldlo, ldhi, ltlo, lthi, 4.0, 16.0, 0.45, 0.5};
+  if (ctx->mlp_has_box) {
+    for (int c = 0; c < 8; c++) box[c] = ctx->mlp_box[c];
+    box[0] = std::max(box[0], ldlo);
+    box[1] = std::min(box[1], ldhi);
+    box[2] = std::max(box[2], ltlo);
+    box[3] = std::min(box[3], lthi);
+    if (!(box[1] > box[0]) || !(box[3] > box[2])) return PHLX_E_ARG;
+  }
   float *data = nullptr;
   if (hipMalloc(&data, sizeof(float) * 11 * (size_t)n) != hipSuccess) return PHLX_E_NOMEM;
   float *raw = data, *label = data + 4 * n, *xn = data + 5 * n, *tn = data + 9 * n,
@@ -1080,10 +1109,11 @@ extern "C" int phlx_set_eos_guess(phlx_ctx *ctx, int32_t mode, double resid_tol,
                                   int32_t count_iterations) {
   if (!ctx) return PHLX_E_ARG;
   if (mode < GUESS_WARM || mode > GUESS_BRACKET || !(resid_tol > 0.0)) return PHLX_E_ARG;
-  if (ctx->cfg.eos != PHLX_EOS_HELMHOLTZ) return PHLX_E_CONFIG;
+  if (ctx->cfg.eos != PHLX_EOS_HELMHOLTZ && !ctx->helm_loaded) return PHLX_E_CONFIG;
   if (mode == GUESS_MLP && !ctx->mlp_ready) return PHLX_E_STATE;
   // the guess variant of the update needs every block full of interior cells
-  if ((mode != GUESS_WARM || count_iterations) && (ctx->g.n1 % 64 || ctx->g.n2 % 4))
+  if (ctx->cfg.eos == PHLX_EOS_HELMHOLTZ && (mode != GUESS_WARM || count_iterations) &&
+      (ctx->g.n1 % 64 || ctx->g.n2 % 4))
     return PHLX_E_CONFIG;
   if (mode == GUESS_MLP && ctx->g.eos_guess != GUESS_MLP) {
     // no previous network output yet: all-ones bytes are a NaN pattern
@@ -1093,6 +1123,12 @@ extern "C" int phlx_set_eos_guess(phlx_ctx *ctx, int32_t mode, double resid_tol,
   ctx->g.mlp.resid_tol = (float)resid_tol;
   ctx->eos_stats = count_iterations != 0;
   ctx->g.eos_count = ctx->eos_stats ? 1 : 0;
+  return PHLX_OK;
+}
+
+extern "C" int phlx_set_eos_lin_tol(phlx_ctx *ctx, double lin_tol) {
+  if (!ctx || !(lin_tol >= 0.0) || lin_tol > 1e-2) return PHLX_E_ARG;
+  ctx->g.eos_lin_tol = lin_tol;
   return PHLX_OK;
 }
 

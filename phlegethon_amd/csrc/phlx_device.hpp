@@ -64,6 +64,7 @@ struct Grid {
   MlpDev mlp;
   int eos_guess;
   int eos_count;           // accumulate the counters (the guess variant only)
+  double eos_lin_tol;      // mixed precision: residual below which the EoS step is linear
   unsigned int *eos_part;
   float *mlp_y;            // network output at each zone's last solve (NaN: none yet)
   double *red;       // per-block partial maxima of the cell-grid reductions

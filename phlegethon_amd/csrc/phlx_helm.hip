@@ -21,6 +21,10 @@ k_helm_full(HelmDev h, long long n, const double *__restrict__ rho,
   if (i >= n) return;
   double p, e, s, c;
   helm_full(h, rho[i], T[i], abar[i], zbar[i], p, e, s, c);
+  helm_pin(p);
+  helm_pin(e);
+  helm_pin(s);
+  helm_pin(c);
   if (P) P[i] = p;
   if (E) E[i] = e;
   if (cs) cs[i] = s;
@@ -36,6 +40,8 @@ k_helm_rhoe(HelmDev h, long long n, const double *__restrict__ rho,
   if (i >= n) return;
   double t = T[i], p, s;
   const int it = helm_rhoe_given(h, rho[i], e[i], abar[i], zbar[i], t, p, s, ret != 0);
+  helm_pin(p);
+  helm_pin(s);
   T[i] = t;
   if (P) P[i] = p;
   if (cs) cs[i] = s;
@@ -51,6 +57,8 @@ k_helm_rhoP(HelmDev h, long long n, const double *__restrict__ rho,
   if (i >= n) return;
   double t = T[i], e, s;
   const int it = helm_rhoP_given(h, rho[i], p[i], abar[i], zbar[i], t, e, s, ret != 0);
+  helm_pin(e);
+  helm_pin(s);
   T[i] = t;
   if (E) E[i] = e;
   if (cs) cs[i] = s;

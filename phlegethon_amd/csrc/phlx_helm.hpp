@@ -45,8 +45,15 @@ struct HelmCell {
   double x, y, drho, dT;
 };
 
+// Marks v as computed at this point.  Without it the compiler sinks a whole
+// evaluation into the conditional stores that follow (one copy per branch),
+// which holds every table value of the evaluation live at once.
+__device__ __forceinline__ void helm_pin(double &v) { asm volatile("" : "+v"(v)); }
+
+// Basis values of one coordinate at the cell's low corner (a[0..2]: order
+// 0, 1, 2 of the Hermite family) and high corner (am[0..2]).
 struct HelmBasis {
-  double p0, p1, p2, p0m, p1m, p2m;
+  double a[3], am[3];
 };
 
 // cell index int((log10(rhoYe) - log10(rho_1)) / dlog) (source/source.F90
@@ -73,12 +80,12 @@ __device__ __forceinline__ void helm_basis(double x, double d, HelmBasis &v) {
   const double x2 = x * x, x3 = x2 * x, x4 = x3 * x, x5 = x4 * x;
   const double o = 1.0 - x;
   const double o2 = o * o, o3 = o2 * o, o4 = o3 * o, o5 = o4 * o;
-  v.p0 = -6.0 * x5 + 15.0 * x4 - 10.0 * x3 + 1.0;
-  v.p1 = (-3.0 * x5 + 8.0 * x4 - 6.0 * x3 + x) * d;
-  v.p2 = (0.5 * (-x5 + 3.0 * x4 - 3.0 * x3 + x2)) * d2;
-  v.p0m = -6.0 * o5 + 15.0 * o4 - 10.0 * o3 + 1.0;
-  v.p1m = (3.0 * o5 - 8.0 * o4 + 6.0 * o3 - o) * d;
-  v.p2m = (0.5 * (-o5 + 3.0 * o4 - 3.0 * o3 + o2)) * d2;
+  v.a[0] = -6.0 * x5 + 15.0 * x4 - 10.0 * x3 + 1.0;
+  v.a[1] = (-3.0 * x5 + 8.0 * x4 - 6.0 * x3 + x) * d;
+  v.a[2] = (0.5 * (-x5 + 3.0 * x4 - 3.0 * x3 + x2)) * d2;
+  v.am[0] = -6.0 * o5 + 15.0 * o4 - 10.0 * o3 + 1.0;
+  v.am[1] = (3.0 * o5 - 8.0 * o4 + 6.0 * o3 - o) * d;
+  v.am[2] = (0.5 * (-o5 + 3.0 * o4 - 3.0 * o3 + o2)) * d2;
 }
 
 __device__ __forceinline__ void helm_dbasis(double x, double d, HelmBasis &v) {
@@ -86,12 +93,12 @@ __device__ __forceinline__ void helm_dbasis(double x, double d, HelmBasis &v) {
   const double x2 = x * x, x3 = x2 * x, x4 = x3 * x;
   const double o = 1.0 - x;
   const double o2 = o * o, o3 = o2 * o, o4 = o3 * o;
-  v.p0 = (-30.0 * x4 + 60.0 * x3 - 30.0 * x2) * id;
-  v.p1 = (-15.0 * x4 + 32.0 * x3 - 18.0 * x2 + 1.0);
-  v.p2 = (0.5 * (-5.0 * x4 + 12.0 * x3 - 9.0 * x2 + 2.0 * x)) * d;
-  v.p0m = -(-30.0 * o4 + 60.0 * o3 - 30.0 * o2) * id;
-  v.p1m = (-15.0 * o4 + 32.0 * o3 - 18.0 * o2 + 1.0);
-  v.p2m = -(0.5 * (-5.0 * o4 + 12.0 * o3 - 9.0 * o2 + 2.0 * o)) * d;
+  v.a[0] = (-30.0 * x4 + 60.0 * x3 - 30.0 * x2) * id;
+  v.a[1] = (-15.0 * x4 + 32.0 * x3 - 18.0 * x2 + 1.0);
+  v.a[2] = (0.5 * (-5.0 * x4 + 12.0 * x3 - 9.0 * x2 + 2.0 * x)) * d;
+  v.am[0] = -(-30.0 * o4 + 60.0 * o3 - 30.0 * o2) * id;
+  v.am[1] = (-15.0 * o4 + 32.0 * o3 - 18.0 * o2 + 1.0);
+  v.am[2] = -(0.5 * (-5.0 * o4 + 12.0 * o3 - 9.0 * o2 + 2.0 * o)) * d;
 }
 
 __device__ __forceinline__ void helm_ddbasis(double x, double d, HelmBasis &v) {
@@ -99,40 +106,46 @@ __device__ __forceinline__ void helm_ddbasis(double x, double d, HelmBasis &v) {
   const double x2 = x * x, x3 = x2 * x;
   const double o = 1.0 - x;
   const double o2 = o * o, o3 = o2 * o;
-  v.p0 = (-120.0 * x3 + 180.0 * x2 - 60.0 * x) * id2;
-  v.p1 = (-60.0 * x3 + 96.0 * x2 - 36.0 * x) * id;
-  v.p2 = (0.5 * (-20.0 * x3 + 36.0 * x2 - 18.0 * x + 2.0));
-  v.p0m = (-120.0 * o3 + 180.0 * o2 - 60.0 * o) * id2;
-  v.p1m = -(-60.0 * o3 + 96.0 * o2 - 36.0 * o) * id;
-  v.p2m = (0.5 * (-20.0 * o3 + 36.0 * o2 - 18.0 * o + 2.0));
+  v.a[0] = (-120.0 * x3 + 180.0 * x2 - 60.0 * x) * id2;
+  v.a[1] = (-60.0 * x3 + 96.0 * x2 - 36.0 * x) * id;
+  v.a[2] = (0.5 * (-20.0 * x3 + 36.0 * x2 - 18.0 * x + 2.0));
+  v.am[0] = (-120.0 * o3 + 180.0 * o2 - 60.0 * o) * id2;
+  v.am[1] = -(-60.0 * o3 + 96.0 * o2 - 36.0 * o) * id;
+  v.am[2] = (0.5 * (-20.0 * o3 + 36.0 * o2 - 18.0 * o + 2.0));
 }
 
-// The 36-term biquintic sum in the reference's term order: entry v pairs the
-// rho basis of order (0,0,0,1,2,1,2,1,2)[v] with the T basis of order
-// (0,1,2,0,0,1,1,2,2)[v]; per entry the corners (i,j) (i+1,j) (i,j+1)
-// (i+1,j+1) are added in that order.
-__device__ __forceinline__ double helm_sum36(const HelmDev &h, const HelmCell &c,
-                                             const HelmBasis &r, const HelmBasis &t) {
+// NS biquintic sums over the cell in ONE pass over the table: sum k pairs
+// the rho basis R[k] with the T basis T[k].  Each sum keeps the reference's
+// term order: entry v pairs the rho basis of order (0,0,0,1,2,1,2,1,2)[v]
+// with the T basis of order (0,1,2,0,0,1,1,2,2)[v]; per entry the corners
+// (i,j) (i+1,j) (i,j+1) (i+1,j+1) are added in that order.  Reading each
+// table value once for all sums keeps ~4 values in flight per lane instead
+// of the whole 36-value block per sum.
+template <int NS>
+__device__ __forceinline__ void helm_sums36(const HelmDev &h, const HelmCell &c,
+                                            const HelmBasis *const (&R)[NS],
+                                            const HelmBasis *const (&T)[NS], double (&s)[NS]) {
   const double *n00 = h.tab + (size_t)kHelmRec * ((size_t)c.jh * h.nr + c.ih);
   const double *n10 = n00 + kHelmRec;
   const double *n01 = n00 + (size_t)kHelmRec * h.nr;
   const double *n11 = n01 + kHelmRec;
-  const double rb[3] = {r.p0, r.p1, r.p2}, rbm[3] = {r.p0m, r.p1m, r.p2m};
-  const double tb[3] = {t.p0, t.p1, t.p2}, tbm[3] = {t.p0m, t.p1m, t.p2m};
   constexpr int RX[9] = {0, 0, 0, 1, 2, 1, 2, 1, 2};
   constexpr int TY[9] = {0, 1, 2, 0, 0, 1, 1, 2, 2};
-  double s = 0.0;
 #pragma unroll
   for (int v = 0; v < 9; v++) {
-    const double a = rb[RX[v]], am = rbm[RX[v]], b = tb[TY[v]], bm = tbm[TY[v]];
-    const double t0 = n00[v] * a * b, t1 = n10[v] * am * b;
-    const double t2 = n01[v] * a * bm, t3 = n11[v] * am * bm;
-    s = (v == 0) ? t0 : s + t0;
-    s = s + t1;
-    s = s + t2;
-    s = s + t3;
+    const double q0 = n00[v], q1 = n10[v], q2 = n01[v], q3 = n11[v];
+#pragma unroll
+    for (int k = 0; k < NS; k++) {
+      const double a = R[k]->a[RX[v]], am = R[k]->am[RX[v]];
+      const double b = T[k]->a[TY[v]], bm = T[k]->am[TY[v]];
+      const double t0 = q0 * a * b, t1 = q1 * am * b;
+      const double t2 = q2 * a * bm, t3 = q3 * am * bm;
+      s[k] = (v == 0) ? t0 : s[k] + t0;
+      s[k] = s[k] + t1;
+      s[k] = s[k] + t2;
+      s[k] = s[k] + t3;
+    }
   }
-  return s;
 }
 
 // Bicubic Hermite of the dP/d(rho*Ye) block (source/source.F90:24078-24125).
@@ -167,10 +180,11 @@ __device__ __forceinline__ double helm_sum16(const HelmDev &h, const HelmCell &c
   return s;
 }
 
-// helm_rhoT_given_full: P, E (specific), sound speed (relativistic form), c_v.
+// helm_rhoT_given_full: P, E (specific), sound speed (relativistic form), c_v;
+// dPdT (when non-null) receives dP/dT.
 __device__ __forceinline__ void helm_full(const HelmDev &h, double rho, double T, double abar,
                                           double zbar, double &P, double &E, double &sound,
-                                          double &cv) {
+                                          double &cv, double *dPdT_out = nullptr) {
   const double Ye = zbar / abar, rhos = rho * Ye;
   const HelmCell c = helm_locate(h, rhos, T);
   HelmBasis r, dr, t, dt, ddt;
@@ -179,11 +193,13 @@ __device__ __forceinline__ void helm_full(const HelmDev &h, double rho, double T
   helm_basis(c.y, c.dT, t);
   helm_dbasis(c.y, c.dT, dt);
   helm_ddbasis(c.y, c.dT, ddt);
-  const double f = helm_sum36(h, c, r, t);
-  const double df_drho = helm_sum36(h, c, dr, t);
-  const double df_dT = helm_sum36(h, c, r, dt);
-  const double df_drhodT = helm_sum36(h, c, dr, dt);
-  const double df_dT2 = helm_sum36(h, c, r, ddt);
+  double sm[5];
+  {
+    const HelmBasis *const RR[5] = {&r, &dr, &r, &dr, &r};
+    const HelmBasis *const TT[5] = {&t, &t, &dt, &dt, &ddt};
+    helm_sums36<5>(h, c, RR, TT, sm);
+  }
+  const double f = sm[0], df_drho = sm[1], df_dT = sm[2], df_drhodT = sm[3], df_dT2 = sm[4];
   double dP_drho = helm_sum16(h, c);
   dP_drho = Ye * dP_drho;
   double tmp = rhos * rhos;
@@ -213,6 +229,7 @@ __device__ __forceinline__ void helm_full(const HelmDev &h, double rho, double T
   E = e;
   cv = dE_dT;
   sound = kHelmClight * sqrt(gam1 / z);
+  if (dPdT_out) *dPdT_out = dP_dT;
 }
 
 // helm_rhoT_given1: P and (when ret) dP/dT.
@@ -223,11 +240,21 @@ __device__ __forceinline__ void helm_given1(const HelmDev &h, double rho, double
   HelmBasis dr, t, dt;
   helm_dbasis(c.x, c.drho, dr);
   helm_basis(c.y, c.dT, t);
-  const double df_drho = helm_sum36(h, c, dr, t);
-  double df_drhodT = 0.0;
+  double df_drho, df_drhodT = 0.0;
   if (ret) {
     helm_dbasis(c.y, c.dT, dt);
-    df_drhodT = helm_sum36(h, c, dr, dt);
+    const HelmBasis *const RR[2] = {&dr, &dr};
+    const HelmBasis *const TT[2] = {&t, &dt};
+    double sm[2];
+    helm_sums36<2>(h, c, RR, TT, sm);
+    df_drho = sm[0];
+    df_drhodT = sm[1];
+  } else {
+    const HelmBasis *const RR[1] = {&dr};
+    const HelmBasis *const TT[1] = {&t};
+    double sm[1];
+    helm_sums36<1>(h, c, RR, TT, sm);
+    df_drho = sm[0];
   }
   double tmp = rhos * rhos;
   double p = tmp * df_drho;
@@ -251,12 +278,23 @@ __device__ __forceinline__ void helm_given2(const HelmDev &h, double rho, double
   helm_basis(c.x, c.drho, r);
   helm_basis(c.y, c.dT, t);
   helm_dbasis(c.y, c.dT, dt);
-  const double f = helm_sum36(h, c, r, t);
-  const double df_dT = helm_sum36(h, c, r, dt);
-  double df_dT2 = 0.0;
+  double f, df_dT, df_dT2 = 0.0;
   if (ret) {
     helm_ddbasis(c.y, c.dT, ddt);
-    df_dT2 = helm_sum36(h, c, r, ddt);
+    const HelmBasis *const RR[3] = {&r, &r, &r};
+    const HelmBasis *const TT[3] = {&t, &dt, &ddt};
+    double sm[3];
+    helm_sums36<3>(h, c, RR, TT, sm);
+    f = sm[0];
+    df_dT = sm[1];
+    df_dT2 = sm[2];
+  } else {
+    const HelmBasis *const RR[2] = {&r, &r};
+    const HelmBasis *const TT[2] = {&t, &dt};
+    double sm[2];
+    helm_sums36<2>(h, c, RR, TT, sm);
+    f = sm[0];
+    df_dT = sm[1];
   }
   const double s = -Ye * df_dT;
   const double ds_dT = -Ye * df_dT2;

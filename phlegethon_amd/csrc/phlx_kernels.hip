@@ -485,11 +485,14 @@ __device__ __forceinline__ double reduce_partials(const double *red, int n) {
 
 // Signal speed of one cell, summed over directions (source/source.F90
 // :13891-13944): (|v_d| + c_f) / dx_d with c_f^2 = c^2 + B^2/rho.
-template <int SD, bool MHD, bool HELM>
+template <int SD, bool MHD, bool HELM, bool KNOWN_CS = false>
 __device__ __forceinline__ double cell_speed(const Grid &g, const double *p, const double *b,
-                                             double T) {
+                                             double T, double cs_known = 0.0) {
   double c;
-  if (HELM) {
+  if (KNOWN_CS) {
+    // the sound speed of the zone's own EoS solve (mixed precision)
+    c = cs_known;
+  } else if (HELM) {
     // full Helmholtz evaluation at the carried T (source/source.F90:13801-13847)
     const double ye = p[SD + 2];
     const double abar = 1.0 / p[SD + 3];
@@ -528,8 +531,16 @@ __device__ __forceinline__ double flux_at(const Grid &g, int d, long long idx) {
 // per block.  The host admits this variant only when every thread of every
 // block is an interior cell (nx1 % 64 == 0, nx2 % 4 == 0), so the surrogate's
 // wave-wide MFMA and block barriers are reached uniformly.
+// MIX with HELM: the T(rho, e) solve is one FP64 evaluation plus a
+// linearised Newton step (helm_rhoe_lin) whenever the start's residual is
+// small, and that evaluation's sound speed also serves the CFL.
+#ifndef PHLX_HELM_UPDATE_WAVES
+#define PHLX_HELM_UPDATE_WAVES 1
+#endif
 template <int SD, int NAS, bool MHD, bool HELM, bool MIX, bool GUESS = false>
-__global__ void __launch_bounds__(256) k_update(Grid g, int first, int last, double a1,
+__global__ void __launch_bounds__(256)
+    __attribute__((amdgpu_waves_per_eu(HELM ? PHLX_HELM_UPDATE_WAVES : 1)))
+    k_update(Grid g, int first, int last, double a1,
                                                 double a2, double c3) {
   constexpr int NV = SD + 2 + NAS, IE = SD + 1, IA = SD + 2;
   if (!g.sc->active) return;
@@ -614,7 +625,9 @@ __global__ void __launch_bounds__(256) k_update(Grid g, int first, int last, dou
 #pragma unroll
       for (int v = 0; v < SD; v++) g.bcc[v * N + c] = b[v];
     }
-    // cons -> prim (source/source.F90:13533-13686)
+    // cons -> prim (source/source.F90:13533-13686).  Everything but the
+    // EoS outputs is stored before the Helmholtz solve, so that little state
+    // stays live across it.
     const double rho = S[0], m1 = S[1], m2 = S[2], m3 = (SD == 3) ? S[3] : 0.0;
     const double ir = 1.0 / rho;
     double eint = S[IE] - 0.5 * (m1 * m1 + m2 * m2 + m3 * m3) * ir;
@@ -624,12 +637,29 @@ __global__ void __launch_bounds__(256) k_update(Grid g, int first, int last, dou
     p[1] = m1 * ir;
     p[2] = m2 * ir;
     if (SD == 3) p[3] = m3 * ir;
-    double T = 0.0;
+#pragma unroll
+    for (int v = IA; v < NV; v++) p[v] = S[v] * ir;
+    if (!HELM) p[IE] = g.gmm1 * eint;
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+      if (!HELM || v != IE) g.P[v * N + c] = p[v];
+      g.U[v * N + c] = S[v];
+    }
+    g.eint[c] = eint;
+    if (MHD) {
+      const double vv[3] = {p[1], p[2], SD == 3 ? p[3] : 0.0};
+      g.ecell[2 * N + c] = emf_of<SD>(2, vv, b);
+      if (SD == 3) {
+        g.ecell[c] = emf_of<SD>(0, vv, b);
+        g.ecell[N + c] = emf_of<SD>(1, vv, b);
+      }
+    }
+    double T = 0.0, sound_out = 0.0;
     if (HELM) {
       // Ye / 1/abar advected; T(rho, e) Newton from the carried T, then
       // gamma_e = p/eint + 1 and gamma_c = c^2 rho / p (:13573-13578,
       // :13648-13660)
-      const double ye = S[IA] * ir;
+      const double ye = p[IA];
       const double abar = rho / S[IA + 1];
       const double zbar = ye * abar;
       double pr, sound;
@@ -650,9 +680,16 @@ __global__ void __launch_bounds__(256) k_update(Grid g, int first, int last, dou
           if (ok && yp == yp) tg = T * exp10((double)(g.mlp.out_scale * (y - yp)));
           g.mlp_y[c] = ok ? y : __int_as_float(0x7fffffff);
         }
-        int fb;
-        const int it = helm_rhoe_guess(g.helm, mode, tg, g.mlp.resid_tol, rho, eint * ir, abar,
-                                       zbar, T, pr, sound, true, fb);
+        int fb, it;
+        if constexpr (MIX) {
+          if (mode == GUESS_BRACKET) tg = helm_bracket_guess(g.helm, rho, eint * ir, abar, zbar);
+          if (tg >= g.helm.tl && tg <= g.helm.th) T = tg;
+          it = helm_rhoe_lin(g.helm, g.eos_lin_tol, rho, eint * ir, abar, zbar, T, pr, sound,
+                             fb);
+        } else {
+          it = helm_rhoe_guess(g.helm, mode, tg, g.mlp.resid_tol, rho, eint * ir, abar, zbar, T,
+                               pr, sound, true, fb);
+        }
         if (g.eos_count) {  // uniform across the grid
           const int tid = threadIdx.y * blockDim.x + threadIdx.x;
           if (tid < 3) cnt[tid] = 0u;
@@ -667,33 +704,22 @@ __global__ void __launch_bounds__(256) k_update(Grid g, int first, int last, dou
             g.eos_part[3 * blk + tid] = g.eos_part[3 * blk + tid] + cnt[tid];
           }
         }
+      } else if constexpr (MIX) {
+        int fb;
+        helm_rhoe_lin(g.helm, g.eos_lin_tol, rho, eint * ir, abar, zbar, T, pr, sound, fb);
       } else {
         helm_rhoe_given(g.helm, rho, eint * ir, abar, zbar, T, pr, sound, true);
       }
+      helm_pin(pr);
+      helm_pin(sound);
       g.gam[c] = pr / eint + 1.0;
       g.gam[N + c] = sound * sound * rho / pr;
       g.temp[c] = T;
+      g.P[IE * N + c] = pr;
       p[IE] = pr;
-    } else {
-      p[IE] = g.gmm1 * eint;
+      sound_out = sound;
     }
-#pragma unroll
-    for (int v = IA; v < NV; v++) p[v] = S[v] * ir;
-#pragma unroll
-    for (int v = 0; v < NV; v++) {
-      g.P[v * N + c] = p[v];
-      g.U[v * N + c] = S[v];
-    }
-    g.eint[c] = eint;
-    if (MHD) {
-      const double vv[3] = {p[1], p[2], SD == 3 ? p[3] : 0.0};
-      g.ecell[2 * N + c] = emf_of<SD>(2, vv, b);
-      if (SD == 3) {
-        g.ecell[c] = emf_of<SD>(0, vv, b);
-        g.ecell[N + c] = emf_of<SD>(1, vv, b);
-      }
-    }
-    if (last) smax_local = cell_speed<SD, MHD, HELM>(g, p, b, T);
+    if (last) smax_local = cell_speed<SD, MHD, HELM, HELM && MIX>(g, p, b, T, sound_out);
   }
   if (last) block_max_to(smax_local, g.red);
 }

@@ -283,6 +283,8 @@ k_helm_rhoe_guess(HelmDev h, MlpDev m, int mode, long long n, const double *__re
   int f;
   const int it = helm_rhoe_guess(h, mode, tg, m.resid_tol, rho[i], e[i], abar[i], zbar[i], t, p,
                                  s, ret != 0, f);
+  helm_pin(p);
+  helm_pin(s);
   T[i] = t;
   if (P) P[i] = p;
   if (cs) cs[i] = s;

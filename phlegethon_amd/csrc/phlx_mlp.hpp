@@ -64,12 +64,13 @@ __device__ __forceinline__ unsigned short bf16_bits(float f) {
 __device__ __forceinline__ float bf16_round(float f) {
   return __uint_as_float(((unsigned int)bf16_bits(f)) << 16);
 }
-// tanh(z) = 1 - 2 / (exp(2z) + 1): exp2 + reciprocal, ~1e-7 relative, exact
-// limits; far below the BF16 resolution of the activations it feeds.
+// tanh(z) = 1 - 2 / (exp(2z) + 1) with the hardware exp2 and reciprocal
+// (v_exp_f32, v_rcp_f32: ~1 ulp each), exact limits; the error (~2e-7) is
+// far below the BF16 resolution of the activations it feeds.
 __device__ __forceinline__ float fast_tanh(float z) {
   z = fminf(fmaxf(z, -15.0f), 15.0f);
-  const float e = __expf(2.0f * z);
-  return 1.0f - 2.0f / (e + 1.0f);
+  const float e = __builtin_amdgcn_exp2f(z * 2.8853900817779268f);  // 2 / ln 2
+  return 1.0f - 2.0f * __builtin_amdgcn_rcpf(e + 1.0f);
 }
 
 __device__ __forceinline__ void mlp_inputs(const MlpDev &m, double rho, double e, double abar,
@@ -200,6 +201,35 @@ __device__ __forceinline__ int helm_rhoe_given_from(const HelmDev &h, double rho
     helm_given1(h, rho, T, abar, zbar, P, dPdT, false);
   }
   return iter > kHelmMaxIter ? kHelmMaxIter : iter;
+}
+
+// Mixed-precision T(rho, e): ONE FP64 evaluation (helm_rhoT_given_full
+// plus dP/dT) at the start T.  When its energy residual |e(T) - e0| / e0 is
+// below lin_tol, T and P take that Newton step linearly, dT = -(e - e0) / c_v,
+// T += dT, P += dP/dT dT, which leaves an error of second order in the
+// residual (~1e-10 relative for a 1e-5 residual); the sound speed stays that
+// of the start (relative error ~dT/T, it feeds the FP32 wave speeds only).
+// Otherwise the reference Newton (helm_rhoe_given) runs from the stepped T.
+// Returns the EoS evaluations before the final one; fallback = 1 when the
+// reference loop ran.
+__device__ __forceinline__ int helm_rhoe_lin(const HelmDev &h, double lin_tol, double rho,
+                                             double E0, double abar, double zbar, double &T,
+                                             double &P, double &sound, int &fallback) {
+  double E, cv, dPdT;
+  helm_full(h, rho, T, abar, zbar, P, E, sound, cv, &dPdT);
+  const double res = E - E0;
+  const double dT = -res / cv;
+  const double T1 = T + dT;
+  const bool inside = (T1 >= h.tl) && (T1 <= h.th);
+  if (fabs(res / E0) < lin_tol && inside) {
+    T = T1;
+    P = P + dPdT * dT;
+    fallback = 0;
+    return 1;
+  }
+  fallback = 1;
+  if (inside) T = T1;
+  return 1 + helm_rhoe_given(h, rho, E0, abar, zbar, T, P, sound, true);
 }
 
 // T(rho, e) with a chosen start (GUESS_WARM: T as given, the reference;

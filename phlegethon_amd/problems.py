@@ -391,6 +391,22 @@ def helm_slab(eos, table, n: int = 256, ny: Optional[int] = None, sdims: int = 3
     return Problem("helm_slab", cfg, prim, b_faces, prof, temp, table)
 
 
+def state_box(pb: Problem, dex: float = 0.5) -> tuple:
+    """Training box of the T(rho, e) surrogate around a Helmholtz problem's
+    states: (log10 rhoYe lo, hi, log10 T lo, hi, abar lo, hi, Ye lo, hi),
+    widened by ``dex`` decades in density and temperature, 10 % in abar and
+    0.01 in Ye, for the states the flow may still reach."""
+    ip = pb.cfg.sdims + 1
+    rho, ye, iabar = pb.prim[0], pb.prim[ip + 1], pb.prim[ip + 2]
+    lrs = np.log10(rho * ye)
+    lt = np.log10(pb.temp)
+    abar = 1.0 / iabar
+    return (float(lrs.min()) - dex, float(lrs.max()) + dex,
+            float(lt.min()) - dex, float(lt.max()) + dex,
+            float(abar.min()) * 0.9, float(abar.max()) * 1.1,
+            max(float(ye.min()) - 0.01, 0.05), min(float(ye.max()) + 0.01, 1.0))
+
+
 # --------------------------------------------------------------------- C4
 def lowmach_box(n: int = 384, heating: float = 1e-6, sdims: int = 3, recon: int = LIM2ND_REC,
                 flux: int = LHLLC_FLUX, precision: int = 0, seed: int = SEED,

@@ -84,7 +84,9 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.phlx_mlp_forward.argtypes = [P, ctypes.c_int64, P, P, P]
     lib.phlx_mlp_grad.argtypes = [P, ctypes.c_int64, P, P, P, P, DP]
     lib.phlx_mlp_train.argtypes = [P, ctypes.c_int64, I, I, D, ctypes.c_uint64, DP, DP, DP]
+    lib.phlx_mlp_set_train_box.argtypes = [P, P]
     lib.phlx_set_eos_guess.argtypes = [P, I, D, I]
+    lib.phlx_set_eos_lin_tol.argtypes = [P, D]
     I64P = ctypes.POINTER(ctypes.c_int64)
     lib.phlx_eos_stats.argtypes = [P, I64P, I64P, I64P, I]
     _lib = lib
@@ -270,10 +272,19 @@ class Simulation:
 
     # -- T(rho, e) surrogate ---------------------------------------------
     def mlp_train(self, nsamples: int = 1 << 22, epochs: int = 4, batch: int = 4096,
-                  lr: float = 3e-3, seed: int = 20261202):
+                  lr: float = 3e-3, seed: int = 20261202, box=None):
         """Generate (rho, e, abar, zbar) -> T pairs with the loaded table and
         train the surrogate on the GPU; returns (final batch loss, held-out
-        RMS error of log10 T, held-out RMS of the implied log10 E error)."""
+        RMS error of log10 T, held-out RMS of the implied log10 E error).
+        box: (log10 rhoYe lo, hi, log10 T lo, hi, abar lo, hi, Ye lo, hi) to
+        sample from (e.g. problems.state_box); None = the table box."""
+        if box is None:
+            _chk(self.lib.phlx_mlp_set_train_box(self._ctx, None), "phlx_mlp_set_train_box")
+        else:
+            b = np.ascontiguousarray(box, dtype=np.float64)
+            assert b.size == 8
+            _chk(self.lib.phlx_mlp_set_train_box(self._ctx, b.ctypes.data_as(ctypes.c_void_p)),
+                 "phlx_mlp_set_train_box")
         loss, rmst, rmse = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
         _chk(self.lib.phlx_mlp_train(self._ctx, int(nsamples), int(epochs), int(batch),
                                      float(lr), int(seed), ctypes.byref(loss),
@@ -301,6 +312,12 @@ class Simulation:
         the reference), "mlp" (surrogate, bracketed fallback) or "bracket"."""
         _chk(self.lib.phlx_set_eos_guess(self._ctx, GUESS_MODES[mode], float(resid_tol),
                                          int(count)), "phlx_set_eos_guess")
+
+    def set_eos_lin_tol(self, lin_tol: float = 1e-5):
+        """Mixed precision + Helmholtz: energy residual below which the
+        per-zone T(rho, e) solve takes its Newton step linearly after one
+        FP64 evaluation (0: always the reference Newton)."""
+        _chk(self.lib.phlx_set_eos_lin_tol(self._ctx, float(lin_tol)), "phlx_set_eos_lin_tol")
 
     def eos_stats(self, reset: bool = False):
         it, n, fb = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()

@@ -4,7 +4,9 @@ oracle on the same seeded initial data.
 Policy: reconstruction, wave speeds and Riemann fluxes in FP32 on the WB
 deviations, which are formed in FP64; flux divergence, sources, the RK
 combine (conservative total-energy accumulation), the CT face-B update and
-the EoS Newton residual stay FP64.
+the EoS Newton residual stay FP64.  With the Helmholtz EoS the per-zone
+T(rho, e) solve makes one FP64 evaluation and takes its Newton step linearly
+when the residual is below 1e-5 (reference Newton otherwise).
 
 Bar: relative L1 per conserved variable <= max(1e-6, 0.1 * S), where S is
 the same field's response of the FP64 path to a 1e-7 relative perturbation
@@ -26,10 +28,12 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-6
 
 
-def gpu_run(pb, nsteps):
+def gpu_run(pb, nsteps, lin_tol=None):
     from phlegethon_amd.runtime import Simulation
     sim = Simulation(pb.cfg)
     sim.setup(pb)
+    if lin_tol is not None:
+        sim.set_eos_lin_tol(lin_tol)
     sim.init()
     sim.run(nsteps)
     st = sim.download()
@@ -61,12 +65,12 @@ def rel_l1(a, b):
     return float(np.abs(a - b).sum() / (den if den > 0 else 1.0))
 
 
-def check(pb, nsteps, oracle_mod, tol=TOL):
+def check(pb, nsteps, oracle_mod, tol=TOL, lin_tol=None):
     mixed = pb.copy()
     mixed.cfg = pb.cfg.with_(precision=PREC_MIXED)
     print(f"[mixed] {pb.name} shape={pb.cfg.shape} flux={pb.cfg.flux} recon={pb.cfg.recon} "
-          f"steps={nsteps}", flush=True)
-    got = gpu_run(mixed, nsteps)
+          f"steps={nsteps} lin_tol={lin_tol}", flush=True)
+    got = gpu_run(mixed, nsteps, lin_tol)
     ref = oracle_mod.run_problem(pb, nsteps)
     pert = pb.copy()
     n = pert.prim[0].size
@@ -113,5 +117,9 @@ def test_helm_slab(oracle_mod):
     H.generate(141, 81, 0.0, 7.0, 6.0, 6.0 + 80 * 10.0 / 270.0, path=p)
     table = H.load(p)
     orc = oracle_mod.OracleHelm(table)
-    check(P.helm_slab(orc, table, n=10, sdims=3), 6, oracle_mod)
-    check(P.helm_slab(orc, table, n=8, sdims=3, mhd_beta=1e3), 6, oracle_mod)
+    # the EoS solve: one FP64 evaluation + linear Newton step (default), and
+    # the reference Newton throughout (lin_tol 0)
+    for lin_tol in (None, 0.0):
+        check(P.helm_slab(orc, table, n=10, sdims=3), 6, oracle_mod, lin_tol=lin_tol)
+        check(P.helm_slab(orc, table, n=8, sdims=3, mhd_beta=1e3), 6, oracle_mod,
+              lin_tol=lin_tol)

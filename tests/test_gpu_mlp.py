@@ -123,6 +123,9 @@ def test_guess_gain(eos, trained, oracle_mod, table):
     _, E, _, _ = orc.full(rho, T, abar, zbar)
     cold = np.full(n, 1.0e8)
     res = {}
+    # a surrogate guess is kept while its first residual is below 100 %
+    # (Newton from there still converges; the bracket costs 16 evaluations)
+    eos.sim.set_eos_guess("warm", resid_tol=1.0)
     for mode in ("warm", "bracket", "mlp"):
         Tm, Pm, csm, it, fb = eos.rhoe_given_guess(rho, E, abar, zbar, cold, mode=mode)
         # the surrogate's first Newton evaluation doubles as its residual
@@ -132,19 +135,34 @@ def test_guess_gain(eos, trained, oracle_mod, table):
         res[mode] = (Tm, it, evals, fb)
         print(f"  {mode:8s} Newton it/zone {it.mean():.2f}  EoS evals/zone {evals.mean():.2f}  "
               f"fallback {fb.mean():.4f}", flush=True)
-    # every start converges to the same energy residual (< 1e-11 relative);
-    # T itself agrees to that residual times d ln T / d ln e (up to ~1e4 in
-    # degenerate matter)
+    # converged solves agree in energy (< 1e-10 relative) and in T to that
+    # residual times d ln T / d ln e (up to ~1e4 in degenerate matter).  A
+    # Newton step that leaves the table resets T to its start and ends the
+    # loop (the reference's rule): frequent from the cold fixed start, rare
+    # from the surrogate's guess.
     Tref = res["bracket"][0]
-    for mode in ("mlp", "warm"):
+    for mode, start, max_reset in (("mlp", None, 2e-3), ("warm", cold, 0.2)):
         Tm, it = res[mode][0], res[mode][1]
-        ok = it < 100  # a cold fixed start may hit the reset path
-        _, Em, _, _ = orc.full(rho[ok], Tm[ok], abar[ok], zbar[ok])
-        assert np.max(np.abs(Em / E[ok] - 1.0)) < 1e-10, mode
-        np.testing.assert_allclose(Tm[ok], Tref[ok], rtol=1e-6)
+        _, Em, _, _ = orc.full(rho, Tm, abar, zbar)
+        conv = np.abs(Em / E - 1.0) < 1e-10
+        print(f"  {mode}: {np.mean(~conv):.4f} of the solves ended on the reset", flush=True)
+        assert np.mean(~conv) <= max_reset, mode
+        if start is not None:
+            assert np.all(Tm[~conv] == start[~conv])
+        np.testing.assert_allclose(Tm[conv], Tref[conv], rtol=1e-6)
     assert res["mlp"][2].mean() < res["bracket"][2].mean()
     assert res["mlp"][2].mean() < res["warm"][2].mean()
     assert res["mlp"][3].mean() < 0.05
+
+
+def test_train_box_arguments(eos):
+    from phlegethon_amd.runtime import PhlxError
+    for bad in ((3.0, 2.0, 6.5, 8.5, 4.0, 16.0, 0.45, 0.5),     # empty rho range
+                (2.0, 3.0, 6.5, 8.5, 0.0, 16.0, 0.45, 0.5),     # abar <= 0
+                (2.0, 3.0, 6.5, 8.5, 4.0, 16.0, 0.45, 1.5),     # Ye > 1
+                (20.0, 21.0, 6.5, 8.5, 4.0, 16.0, 0.45, 0.5)):  # outside the table
+        with pytest.raises(PhlxError):
+            eos.sim.mlp_train(nsamples=1 << 13, epochs=1, batch=1024, box=bad)
 
 
 def test_hydro_with_surrogate_guess(table, oracle_mod):
@@ -154,14 +172,16 @@ def test_hydro_with_surrogate_guess(table, oracle_mod):
     nsteps = 4
     sim = Simulation(pb.cfg)
     sim.setup(pb)
-    loss, rmst, rmse = sim.mlp_train(nsamples=1 << 20, epochs=8, batch=2048, lr=5e-3)
+    box = P.state_box(pb)
+    loss, rmst, rmse = sim.mlp_train(nsamples=1 << 20, epochs=8, batch=2048, lr=5e-3, box=box)
     sim.set_eos_guess("mlp", resid_tol=0.25, count=True)
     sim.init()
     sim.run(nsteps)
     got = sim.download()
     st = sim.eos_stats()
     sim.close()
-    print(f"hydro MLP guess: rmse {rmse:.4f}; Newton it/solve "
+    assert rmse < 0.03
+    print(f"hydro MLP guess: box {np.round(box, 3)}, rmse {rmse:.4f}; Newton it/solve "
           f"{st['newton_iters'] / st['solves']:.3f}, fallback rate "
           f"{st['fallbacks'] / st['solves']:.4f}", flush=True)
     assert st["solves"] == 2 * nsteps * 64 * 16 * 64
