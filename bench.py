@@ -117,11 +117,14 @@ def build_problem(args):
                 f"{', seeded random-phase divergence-free field' if mhd else ''}; generated "
                 f"{table.nrho}x{table.nT} Helmholtz table)")
     else:
-        log(f"building the low-Mach heated box {n}^3")
-        pb = problems.lowmach_box(n=n, heating=args.heating)
+        log(f"building the low-Mach heated box {n}^3 (Mach {args.mach})")
+        pb = problems.lowmach_box(n=n, heating=args.heating, mach=args.mach)
+        heat = args.mach ** 3 if args.mach is not None else args.heating
         desc = (f"3D low-Mach heated box, {n}^3, ideal gas + constant gravity, USE_WB + "
-                f"EVOLVE_ETOT + USE_EDOT, LHLLC + LIM2ND, RK2, CFL 0.8, heating {args.heating:g}")
-        data = "synthetic (polytrope + bottom heating layer, seeded 1e-6 density noise)"
+                f"EVOLVE_ETOT + USE_EDOT, LHLLC + LIM2ND, RK2, CFL 0.8, heating {heat:g}"
+                + (f", initial rolls at RMS Mach {args.mach:g}" if args.mach is not None else ""))
+        data = ("synthetic (polytrope + bottom heating layer, seeded 1e-6 density noise"
+                + (", divergence-free rolls" if args.mach is not None else "") + ")")
     return pb, desc, data
 
 
@@ -143,6 +146,25 @@ def conserved(st, cfg):
     return out
 
 
+def deviation_errors(st, ref, pb):
+    """rel. L1 of rho' = rho - rho_eq, p' = p - p_eq and of each velocity
+    component, each normalised by the FP64 field's own L1 norm."""
+    import numpy as np
+    sd = pb.cfg.sdims
+    eqr = pb.profiles["eq_rho_cc"][None, :, None]
+    eqp = pb.profiles["eq_p_cc"][None, :, None]
+    ip = sd + 1
+    out = {}
+    a, b = st["prim"][0] - eqr, ref["prim"][0] - eqr
+    out["rho'"] = float(np.abs(a - b).sum() / max(np.abs(b).sum(), 1e-300))
+    a, b = st["prim"][ip] - eqp, ref["prim"][ip] - eqp
+    out["p'"] = float(np.abs(a - b).sum() / max(np.abs(b).sum(), 1e-300))
+    for d in range(sd):
+        a, b = st["prim"][1 + d], ref["prim"][1 + d]
+        out[f"v{d + 1}"] = float(np.abs(a - b).sum() / max(np.abs(b).sum(), 1e-300))
+    return out
+
+
 def run_variant(pb, cfg, args, guess="warm", keep_state=False, profile=True):
     """Upload, init, warm up, time args.steps steps; optionally profile."""
     from phlegethon_amd.runtime import Simulation
@@ -152,11 +174,11 @@ def run_variant(pb, cfg, args, guess="warm", keep_state=False, profile=True):
     if guess == "mlp":
         t0 = time.perf_counter()
         from phlegethon_amd.problems import state_box
-        box = state_box(pb)
+        box = state_box(pb) if args.mlp_box == "state" else None
         loss, rmst, rmse = sim.mlp_train(nsamples=args.mlp_samples, epochs=args.mlp_epochs,
                                          batch=4096, lr=5e-3, box=box)
         res["mlp"] = {"train_s": time.perf_counter() - t0, "samples": args.mlp_samples,
-                      "box": [round(v, 4) for v in box],
+                      "box": ([round(v, 4) for v in box] if box is not None else "table"),
                       "epochs": args.mlp_epochs, "final_batch_loss": loss,
                       "heldout_rms_log10T": rmst, "heldout_rms_log10e": rmse}
         log(f"surrogate trained in {res['mlp']['train_s']:.2f} s, held-out RMS log10 T "
@@ -229,6 +251,12 @@ def main():
     ap.add_argument("--compare", action="store_true",
                     help="also run the plain FP64 (warm-start) path on the same initial data "
                          "and report the speed-up and the rel. L1 error per conserved field")
+    ap.add_argument("--mach", type=float, default=None,
+                    help="lowmach: start in convective rolls at this RMS Mach number, "
+                         "heating = mach^3")
+    ap.add_argument("--mlp-box", choices=("state", "table"), default="state",
+                    help="surrogate training box: around the problem's states, or the "
+                         "whole table range")
     ap.add_argument("--mlp-samples", type=int, default=1 << 22)
     ap.add_argument("--mlp-epochs", type=int, default=8)
     args = ap.parse_args()
@@ -292,6 +320,12 @@ def main():
                    "plain_eos": base.get("eos"), "steps_compared": args.warmup + args.steps}
         log(f"speed-up {compare['speedup']:.3f}x; rel L1 " +
             " ".join(f"{k} {v:.2e}" for k, v in rel.items()))
+        if "eq_rho_cc" in pb.profiles:
+            # the WB deviations and the velocity: L1 of the difference over
+            # L1 of the FP64 field itself
+            dev = deviation_errors(res["state"], base["state"], pb)
+            compare["rel_l1_deviations"] = dev
+            log("deviations rel L1 " + " ".join(f"{k} {v:.2e}" for k, v in dev.items()))
 
     cpu = None
     if not args.no_cpu_baseline:

@@ -410,14 +410,23 @@ def state_box(pb: Problem, dex: float = 0.5) -> tuple:
 # --------------------------------------------------------------------- C4
 def lowmach_box(n: int = 384, heating: float = 1e-6, sdims: int = 3, recon: int = LIM2ND_REC,
                 flux: int = LHLLC_FLUX, precision: int = 0, seed: int = SEED,
-                noise: float = 1e-6, mach_cutoff: float = 1e-5) -> Problem:
+                noise: float = 1e-6, mach_cutoff: float = 1e-5,
+                mach: Optional[float] = None) -> Problem:
     """Low-Mach convection box: ideal gas gamma = 5/3, constant gravity
     g = -1 in x2, the C1 polytrope rho = (1 - 0.4 y)^1.5, p = rho^gamma on
     the unit cube (periodic x1/x3, reflecting x2), heated at the bottom by
     edot = heating * sin(8 pi y) for y < 1/8 (pattern of
     tests/andrassy2022/app.F90:132-167), seeded density noise.  USE_WB +
     USE_GRAVITY + EVOLVE_ETOT + USE_EDOT, LHLLC, RK2, CFL 0.8.  The flow
-    speed grows with the heating (Mach ~ heating^(1/3) once convective)."""
+    speed grows with the heating (Mach ~ heating^(1/3) once convective).
+
+    ``mach`` starts the box in that convective state instead of at rest
+    (reaching it from rest takes ~1/mach sound crossings): divergence-free
+    rolls v1 = pi sin(2 pi x) cos(pi y) cos(2 pi z) A, v2 = -2 pi cos(2 pi x)
+    sin(pi y) cos(2 pi z) A (v.n = 0 on the walls), A set for an RMS Mach
+    number ``mach``, and heating = mach^3."""
+    if mach is not None:
+        heating = mach ** 3
     cfg = PhlxConfig(sdims=sdims, nx1=n, nx2=n, nx3=n if sdims == 3 else 1,
                      recon=recon, flux=flux, rk_stages=2, cfl=0.8, mach_cutoff=mach_cutoff,
                      bc=(BC_PERIODIC, BC_PERIODIC, BC_REFLECTIVE, BC_REFLECTIVE,
@@ -438,6 +447,18 @@ def lowmach_box(n: int = 384, heating: float = 1e-6, sdims: int = 3, recon: int 
     prim = np.zeros((cfg.nvars,) + shape)
     prim[0] = rho_cc[None, :, None] * (1.0 + noise * u)
     prim[cfg.sdims + 1] = np.broadcast_to(p_cc[None, :, None], shape)
+    if mach is not None:
+        x = centres(cfg, 0)[:, None, None]
+        y = yc[None, :, None]
+        z = centres(cfg, 2)[None, None, :] if sdims == 3 else np.zeros((1, 1, 1))
+        cz = np.cos(2.0 * math.pi * z)
+        v1 = math.pi * np.sin(2.0 * math.pi * x) * np.cos(math.pi * y) * cz
+        v2 = -2.0 * math.pi * np.cos(2.0 * math.pi * x) * np.sin(math.pi * y) * cz
+        v1, v2 = np.broadcast_to(v1, shape), np.broadcast_to(v2, shape)
+        cs = np.sqrt(gm * prim[cfg.sdims + 1] / prim[0])
+        amp = mach / np.sqrt(np.mean((v1 ** 2 + v2 ** 2) / cs ** 2))
+        prim[1] = amp * v1
+        prim[2] = amp * v2
     prof = dict(eq_rho_cc=rho_cc, eq_p_cc=p_cc, eq_rho_f=rho_f, eq_p_f=p_f,
                 grav_cc=np.full(n, -1.0), phi_cc=yc.copy(), phi_f=yf.copy(), edot_cc=edot)
     return Problem("lowmach_box", cfg, prim, None, prof)
