@@ -22,6 +22,33 @@ __device__ __forceinline__ bool stencil_inside(const Grid &g, int i, int j, int 
   return x - O >= 0 && x + NS - O - 1 < n;
 }
 
+// This thread's face (i, j, k) of a sweep along DIR; false past the end.
+// x1 faces (n1 + 1 per row) are numbered along the flattened (i, j) plane of
+// each k with 256-thread blocks: a 64-wide mapping of rows of n1 + 1 faces
+// would add a wave with a single live lane to every row, and the sweep is
+// VALU bound.  x2 / x3 faces use the 2D (64, 4) mapping.
+// (i, j) of this thread in a flattened e1 x e2 plane of 256-thread blocks
+// (blockDim (64, 4)); false past the plane's end.
+__device__ __forceinline__ bool plane_coords(int e1, int e2, int &i, int &j) {
+  const int t = blockIdx.x * 256 + threadIdx.y * blockDim.x + threadIdx.x;
+  int q = (int)((double)t * (1.0 / (double)e1));
+  int r = t - q * e1;
+  while (r < 0) { q--; r += e1; }
+  while (r >= e1) { q++; r -= e1; }
+  i = r;
+  j = q;
+  return j < e2;
+}
+
+template <int DIR>
+__device__ __forceinline__ bool face_coords(const Grid &g, int &i, int &j, int &k) {
+  k = blockIdx.z;
+  if (DIR == 0) return plane_coords(g.n1 + 1, g.n2, i, j);
+  i = blockIdx.x * blockDim.x + threadIdx.x;
+  j = blockIdx.y * blockDim.y + threadIdx.y;
+  return i < g.n1 && j < g.n2 + (DIR == 1);
+}
+
 // ---------------------------------------------------------------- flux
 // One thread per face along DIR.  Loads the four cells f-2..f+1, builds the
 // left state (right edge of cell f-1) and right state (left edge of cell
@@ -33,11 +60,8 @@ __global__ void __launch_bounds__(256) k_flux(Grid g) {
   constexpr int NV = SD + 2 + NAS, IE = SD + 1;
   constexpr int NS = R5 ? 6 : 4, O = R5 ? 3 : 2;  // window: cells f-O .. f+NS-O-1
   if (!g.sc->active) return;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y * blockDim.y + threadIdx.y;
-  const int k = blockIdx.z;
-  const int e1 = g.n1 + (DIR == 0), e2 = g.n2 + (DIR == 1);
-  if (i >= e1 || j >= e2) return;
+  int i, j, k;
+  if (!face_coords<DIR>(g, i, j, k)) return;
 
   double q[NS][NV], b[NS][3], ga[NS][2];
   if (__all(stencil_inside<DIR, NS, O>(g, i, j, k))) {
@@ -161,11 +185,8 @@ __global__ void __launch_bounds__(256) k_flux_mx(Grid g) {
   constexpr int NV = SD + 2 + NAS, IE = SD + 1;
   constexpr int NS = R5 ? 6 : 4, O = R5 ? 3 : 2;
   if (!g.sc->active) return;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y * blockDim.y + threadIdx.y;
-  const int k = blockIdx.z;
-  const int e1 = g.n1 + (DIR == 0), e2 = g.n2 + (DIR == 1);
-  if (i >= e1 || j >= e2) return;
+  int i, j, k;
+  if (!face_coords<DIR>(g, i, j, k)) return;
 
   float q[NS][NV], b[NS][3], ga[NS][2];
   const bool fast = __all(stencil_inside<DIR, NS, O>(g, i, j, k));
@@ -352,10 +373,9 @@ __device__ __forceinline__ double corner_emf(double eS, double eW, double eN, do
 template <int SD>
 __global__ void __launch_bounds__(256) k_corner(Grid g) {
   if (!g.sc->active) return;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  int i, j;
   const int k = blockIdx.z;
-  if (i > g.n1 || j > g.n2) return;
+  if (!plane_coords(g.n1 + 1, g.n2 + 1, i, j)) return;
   double eN, eE, eS, eW, sS, sN, sW, sE, t;
   if (k < g.n3) {  // x3-edges (emfx3_cor)
     face_emf(g, 0, 2, i, j, k, eN, sN);
@@ -401,10 +421,9 @@ template <int SD>
 __global__ void __launch_bounds__(256) k_faceb(Grid g, int first, double a1, double a2,
                                                double c3) {
   if (!g.sc->active) return;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  int i, j;
   const int k = blockIdx.z;
-  if (i > g.n1 || j > g.n2) return;
+  if (!plane_coords(g.n1 + 1, g.n2 + 1, i, j)) return;
   const double a3 = c3 * g.sc->dt;
   const double i1 = g.inv_dx[0], i2 = g.inv_dx[1], i3 = g.inv_dx[2];
   const double *C1 = g.cor[0], *C2 = g.cor[1], *C3 = g.cor[2];
@@ -848,12 +867,19 @@ static dim3 grid_for(int e1, int e2, int e3, dim3 blk) {
   return dim3((e1 + blk.x - 1) / blk.x, (e2 + blk.y - 1) / blk.y, e3);
 }
 
+// flattened e1 x e2 planes, 256 threads per block (plane_coords)
+static dim3 grid_plane(int e1, int e2, int e3) {
+  const long long nplane = (long long)e1 * e2;
+  return dim3((unsigned)((nplane + 255) / 256), 1, e3);
+}
+static dim3 grid_x1_faces(const Grid &g) { return grid_plane(g.n1 + 1, g.n2, g.n3); }
+
 template <int SD, int NAS, bool MHD, bool HELM, bool MIX, bool R5>
 static void launch_flux(const Grid &g, int dir, hipStream_t st) {
   const dim3 blk(64, 4, 1);
   if (MIX) {
     if (dir == 0)
-      k_flux_mx<0, SD, NAS, MHD, HELM, R5><<<grid_for(g.n1 + 1, g.n2, g.n3, blk), blk, 0, st>>>(g);
+      k_flux_mx<0, SD, NAS, MHD, HELM, R5><<<grid_x1_faces(g), blk, 0, st>>>(g);
     else if (dir == 1)
       k_flux_mx<1, SD, NAS, MHD, HELM, R5><<<grid_for(g.n1, g.n2 + 1, g.n3, blk), blk, 0, st>>>(g);
     else if (SD == 3)
@@ -861,7 +887,7 @@ static void launch_flux(const Grid &g, int dir, hipStream_t st) {
     return;
   }
   if (dir == 0)
-    k_flux<0, SD, NAS, MHD, HELM, R5><<<grid_for(g.n1 + 1, g.n2, g.n3, blk), blk, 0, st>>>(g);
+    k_flux<0, SD, NAS, MHD, HELM, R5><<<grid_x1_faces(g), blk, 0, st>>>(g);
   else if (dir == 1)
     k_flux<1, SD, NAS, MHD, HELM, R5><<<grid_for(g.n1, g.n2 + 1, g.n3, blk), blk, 0, st>>>(g);
   else if (SD == 3)
@@ -875,11 +901,11 @@ struct Kern {
   }
   static void corner(const Grid &g, hipStream_t st) {
     const dim3 blk(64, 4, 1);
-    k_corner<SD><<<grid_for(g.n1 + 1, g.n2 + 1, SD == 3 ? g.n3 + 1 : 1, blk), blk, 0, st>>>(g);
+    k_corner<SD><<<grid_plane(g.n1 + 1, g.n2 + 1, SD == 3 ? g.n3 + 1 : 1), blk, 0, st>>>(g);
   }
   static void faceb(const Grid &g, int first, double a1, double a2, double c3, hipStream_t st) {
     const dim3 blk(64, 4, 1);
-    k_faceb<SD><<<grid_for(g.n1 + 1, g.n2 + 1, SD == 3 ? g.n3 + 1 : 1, blk), blk, 0, st>>>(
+    k_faceb<SD><<<grid_plane(g.n1 + 1, g.n2 + 1, SD == 3 ? g.n3 + 1 : 1), blk, 0, st>>>(
         g, first, a1, a2, c3);
   }
   static void update(const Grid &g, int first, int last, double a1, double a2, double c3,
