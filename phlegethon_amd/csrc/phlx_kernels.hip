@@ -297,6 +297,8 @@ __global__ void __launch_bounds__(256) k_flux_mx(Grid g) {
 // Face-centred EMF component `comp` (0..2 = E1..E3) of the face array of
 // direction fd at a possibly-ghost transverse position; ghosts follow the
 // EMF halo copy and EMF boundary conditions (source/source.F90:11370-11964).
+// FAST: the caller guarantees an interior position (no map).
+template <bool FAST = false>
 __device__ __forceinline__ void face_emf(const Grid &g, int fd, int comp, int i, int j,
                                          int k, double &e, double &s) {
   int ii[3] = {i, j, k};
@@ -304,7 +306,7 @@ __device__ __forceinline__ void face_emf(const Grid &g, int fd, int comp, int i,
   double fac = 1.0;
 #pragma unroll
   for (int a = 0; a < 3; a++) {
-    if (a == fd || a >= g.sd) continue;
+    if (FAST || a == fd || a >= g.sd) continue;
     int c = ii[a];
     if (c < 0 || c >= n[a]) {
       const int side = c < 0 ? 0 : 1;
@@ -342,8 +344,9 @@ __device__ __forceinline__ double emf_of(int comp, const double *v, const double
 // holds exactly one axis-a factor), and IEEE negation commutes with
 // rounding, so this equals the value formed from the flipped ghost state
 // bit for bit.
-template <int SD>
+template <int SD, bool FAST = false>
 __device__ __forceinline__ double cell_emf(const Grid &g, int comp, int i, int j, int k) {
+  if (FAST) return g.ecell[(long long)comp * g.N + cix(g, i, j, k)];
   int si, sj, sk = 0;
   const int mi = map_axis(g, 0, i, g.n1, si);
   const int mj = map_axis(g, 1, j, g.n2, sj);
@@ -368,51 +371,63 @@ __device__ __forceinline__ double corner_emf(double eS, double eW, double eN, do
   return 0.25 * (eS + eW + eN + eE + dS - dN + dW - dE);
 }
 
+// The three edge EMFs whose lower end is node (i, j, k) (FAST: all eight
+// surrounding cells interior).
+template <int SD, bool FAST>
+__device__ __forceinline__ void corner_node(const Grid &g, int i, int j, int k) {
+  double eN, eE, eS, eW, sS, sN, sW, sE;
+  if (k < g.n3) {  // x3-edges (emfx3_cor)
+    face_emf<FAST>(g, 0, 2, i, j, k, eN, sN);
+    face_emf<FAST>(g, 1, 2, i, j, k, eE, sE);
+    face_emf<FAST>(g, 0, 2, i, j - 1, k, eS, sS);
+    face_emf<FAST>(g, 1, 2, i - 1, j, k, eW, sW);
+    const double eSW = cell_emf<SD, FAST>(g, 2, i - 1, j - 1, k);
+    const double eNW = cell_emf<SD, FAST>(g, 2, i - 1, j, k);
+    const double eNE = cell_emf<SD, FAST>(g, 2, i, j, k);
+    const double eSE = cell_emf<SD, FAST>(g, 2, i, j - 1, k);
+    g.cor[2][kix(g, 2, i, j, k)] = corner_emf(eS, eW, eN, eE, eSW, eNW, eNE, eSE, sS, sN, sW, sE);
+  }
+  if (SD == 3) {
+    if (i < g.n1) {  // x1-edges (emfx1_cor, source/source.F90:12050-12107)
+      face_emf<FAST>(g, 1, 0, i, j, k, eN, sN);
+      face_emf<FAST>(g, 2, 0, i, j, k, eE, sE);
+      face_emf<FAST>(g, 1, 0, i, j, k - 1, eS, sS);
+      face_emf<FAST>(g, 2, 0, i, j - 1, k, eW, sW);
+      const double eSW = cell_emf<SD, FAST>(g, 0, i, j - 1, k - 1);
+      const double eNW = cell_emf<SD, FAST>(g, 0, i, j - 1, k);
+      const double eNE = cell_emf<SD, FAST>(g, 0, i, j, k);
+      const double eSE = cell_emf<SD, FAST>(g, 0, i, j, k - 1);
+      g.cor[0][kix(g, 0, i, j, k)] = corner_emf(eS, eW, eN, eE, eSW, eNW, eNE, eSE, sS, sN, sW, sE);
+    }
+    if (j < g.n2) {  // x2-edges (emfx2_cor, source/source.F90:12128-12185)
+      face_emf<FAST>(g, 2, 1, i, j, k, eN, sN);
+      face_emf<FAST>(g, 0, 1, i, j, k, eE, sE);
+      face_emf<FAST>(g, 2, 1, i - 1, j, k, eS, sS);
+      face_emf<FAST>(g, 0, 1, i, j, k - 1, eW, sW);
+      const double eSW = cell_emf<SD, FAST>(g, 1, i - 1, j, k - 1);
+      const double eNW = cell_emf<SD, FAST>(g, 1, i, j, k - 1);
+      const double eNE = cell_emf<SD, FAST>(g, 1, i, j, k);
+      const double eSE = cell_emf<SD, FAST>(g, 1, i - 1, j, k);
+      g.cor[1][kix(g, 1, i, j, k)] = corner_emf(eS, eW, eN, eE, eSW, eNW, eNE, eSE, sS, sN, sW, sE);
+    }
+  }
+}
+
 // One thread per grid node (i,j,k) in [0,n1]x[0,n2]x[0,n3]: the three edge
-// EMFs whose lower end is that node.
+// EMFs whose lower end is that node.  A wave whose nodes all have their
+// eight surrounding cells inside the grid reads without the boundary map.
 template <int SD>
 __global__ void __launch_bounds__(256) k_corner(Grid g) {
   if (!g.sc->active) return;
   int i, j;
   const int k = blockIdx.z;
   if (!plane_coords(g.n1 + 1, g.n2 + 1, i, j)) return;
-  double eN, eE, eS, eW, sS, sN, sW, sE, t;
-  if (k < g.n3) {  // x3-edges (emfx3_cor)
-    face_emf(g, 0, 2, i, j, k, eN, sN);
-    face_emf(g, 1, 2, i, j, k, eE, sE);
-    face_emf(g, 0, 2, i, j - 1, k, eS, sS);
-    face_emf(g, 1, 2, i - 1, j, k, eW, sW);
-    const double eSW = cell_emf<SD>(g, 2, i - 1, j - 1, k);
-    const double eNW = cell_emf<SD>(g, 2, i - 1, j, k);
-    const double eNE = cell_emf<SD>(g, 2, i, j, k);
-    const double eSE = cell_emf<SD>(g, 2, i, j - 1, k);
-    g.cor[2][kix(g, 2, i, j, k)] = corner_emf(eS, eW, eN, eE, eSW, eNW, eNE, eSE, sS, sN, sW, sE);
-  }
-  if (SD == 3) {
-    if (i < g.n1) {  // x1-edges (emfx1_cor, source/source.F90:12050-12107)
-      face_emf(g, 1, 0, i, j, k, eN, sN);
-      face_emf(g, 2, 0, i, j, k, eE, sE);
-      face_emf(g, 1, 0, i, j, k - 1, eS, sS);
-      face_emf(g, 2, 0, i, j - 1, k, eW, sW);
-      const double eSW = cell_emf<SD>(g, 0, i, j - 1, k - 1);
-      const double eNW = cell_emf<SD>(g, 0, i, j - 1, k);
-      const double eNE = cell_emf<SD>(g, 0, i, j, k);
-      const double eSE = cell_emf<SD>(g, 0, i, j, k - 1);
-      g.cor[0][kix(g, 0, i, j, k)] = corner_emf(eS, eW, eN, eE, eSW, eNW, eNE, eSE, sS, sN, sW, sE);
-    }
-    if (j < g.n2) {  // x2-edges (emfx2_cor, source/source.F90:12128-12185)
-      face_emf(g, 2, 1, i, j, k, eN, sN);
-      face_emf(g, 0, 1, i, j, k, eE, sE);
-      face_emf(g, 2, 1, i - 1, j, k, eS, sS);
-      face_emf(g, 0, 1, i, j, k - 1, eW, sW);
-      const double eSW = cell_emf<SD>(g, 1, i - 1, j, k - 1);
-      const double eNW = cell_emf<SD>(g, 1, i, j, k - 1);
-      const double eNE = cell_emf<SD>(g, 1, i, j, k);
-      const double eSE = cell_emf<SD>(g, 1, i - 1, j, k);
-      g.cor[1][kix(g, 1, i, j, k)] = corner_emf(eS, eW, eN, eE, eSW, eNW, eNE, eSE, sS, sN, sW, sE);
-    }
-  }
-  (void)t;
+  const bool inside = i >= 1 && i < g.n1 && j >= 1 && j < g.n2 &&
+                      (SD == 2 || (k >= 1 && k < g.n3));
+  if (__all(inside))
+    corner_node<SD, true>(g, i, j, k);
+  else
+    corner_node<SD, false>(g, i, j, k);
 }
 
 // Face-B update b = a1*b0 + a2*b + a3*dt*curl(E) in FP64 (source/source.F90
