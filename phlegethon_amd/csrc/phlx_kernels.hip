@@ -50,46 +50,17 @@ __device__ __forceinline__ bool face_coords(const Grid &g, int &i, int &j, int &
 }
 
 // ---------------------------------------------------------------- flux
-// One thread per face along DIR.  Loads the four cells f-2..f+1, builds the
-// left state (right edge of cell f-1) and right state (left edge of cell
-// f), rotates into the face frame, solves, rotates back and applies the WB
-// flux correction and the ETOT potential flux.  MHD faces also store the
-// two tangential EMF components and the contact-upwind sign used by CT.
-template <int DIR, int SD, int NAS, bool MHD, bool HELM, bool R5>
-__global__ void __launch_bounds__(256) k_flux(Grid g) {
+// The face's solve from its reconstructed states (grid frame; transverse B
+// and the FASTEOS gammas reconstructed, the normal B read here): rotation
+// into the face frame, WB add-back, Riemann solve, rotation back, EMF and
+// contact sign (MHD), WB flux correction, ETOT potential flux, store.
+template <int DIR, int SD, int NAS, bool MHD, bool HELM>
+__device__ __forceinline__ void flux_face(const Grid &g, int i, int j, int k, double *qL,
+                                          double *qR, double *bL, double *bR, double *gL,
+                                          double *gR) {
   constexpr int NV = SD + 2 + NAS, IE = SD + 1;
-  constexpr int NS = R5 ? 6 : 4, O = R5 ? 3 : 2;  // window: cells f-O .. f+NS-O-1
-  if (!g.sc->active) return;
-  int i, j, k;
-  if (!face_coords<DIR>(g, i, j, k)) return;
-
-  double q[NS][NV], b[NS][3], ga[NS][2];
-  if (__all(stencil_inside<DIR, NS, O>(g, i, j, k))) {
-#pragma unroll
-    for (int s = 0; s < NS; s++) {
-      const int o = s - O;
-      load_cell<SD, NV, MHD, HELM, true>(g, i + (DIR == 0 ? o : 0), j + (DIR == 1 ? o : 0),
-                                         k + (DIR == 2 ? o : 0), q[s], b[s], ga[s]);
-    }
-  } else {
-#pragma unroll
-    for (int s = 0; s < NS; s++) {
-      const int o = s - O;
-      load_cell<SD, NV, MHD, HELM>(g, i + (DIR == 0 ? o : 0), j + (DIR == 1 ? o : 0),
-                                   k + (DIR == 2 ? o : 0), q[s], b[s], ga[s]);
-    }
-  }
-  double qL[NV], qR[NV], bL[3] = {0.0, 0.0, 0.0}, bR[3] = {0.0, 0.0, 0.0};
-  // scalars take the limited parabola under PPH (source/source.F90:8194-8243)
-#pragma unroll
-  for (int v = 0; v < NV; v++) face_states<R5>(g.recon, v > IE, &q[0][v], NV, qL[v], qR[v]);
   const long long f = fix(g, DIR, i, j, k);
   if (MHD) {
-#pragma unroll
-    for (int c = 0; c < SD; c++) {
-      if (c == DIR) continue;
-      face_states<R5>(g.recon, false, &b[0][c], 3, bL[c], bR[c]);
-    }
     const double bn = g.bf[DIR][f];
     bL[DIR] = bn;
     bR[DIR] = bn;
@@ -119,18 +90,11 @@ __global__ void __launch_bounds__(256) k_flux(Grid g) {
     qL[IE] = qL[IE] + peq;
     qR[IE] = qR[IE] + peq;
   }
-  // FASTEOS: gamma_e / gamma_c face values, unlimited PPH or the LIM2ND
-  // slope (source/source.F90:8497-8713), WB add-back of the face gamma_e
-  // (:8715-8727)
-  double gL[2] = {0.0, 0.0}, gR[2] = {0.0, 0.0};
-  if (HELM) {
-#pragma unroll
-    for (int c = 0; c < 2; c++) face_states<R5>(g.recon, false, &ga[0][c], 2, gL[c], gR[c]);
-    if (g.wb) {
-      const double geq = (DIR == 1) ? g.eq_ge_f[jy] : g.eq_ge_cc[jy];
-      gL[0] = gL[0] + geq;
-      gR[0] = gR[0] + geq;
-    }
+  // FASTEOS: WB add-back of the face gamma_e (source/source.F90:8715-8727)
+  if (HELM && g.wb) {
+    const double geq = (DIR == 1) ? g.eq_ge_f[jy] : g.eq_ge_cc[jy];
+    gL[0] = gL[0] + geq;
+    gR[0] = gR[0] + geq;
   }
   double F[NV], Fb[3] = {0.0, 0.0, 0.0};
   if (MHD)
@@ -172,6 +136,165 @@ __global__ void __launch_bounds__(256) k_flux(Grid g) {
   }
 #pragma unroll
   for (int v = 0; v < NV; v++) g.F[DIR][v * nf + f] = F[v];
+}
+
+// One thread per face along DIR.  Loads the four cells f-2..f+1, builds the
+// left state (right edge of cell f-1) and right state (left edge of cell
+// f), rotates into the face frame, solves, rotates back and applies the WB
+// flux correction and the ETOT potential flux.  MHD faces also store the
+// two tangential EMF components and the contact-upwind sign used by CT.
+template <int DIR, int SD, int NAS, bool MHD, bool HELM, bool R5>
+__global__ void __launch_bounds__(256) k_flux(Grid g) {
+  constexpr int NV = SD + 2 + NAS, IE = SD + 1;
+  constexpr int NS = R5 ? 6 : 4, O = R5 ? 3 : 2;  // window: cells f-O .. f+NS-O-1
+  if (!g.sc->active) return;
+  int i, j, k;
+  if (!face_coords<DIR>(g, i, j, k)) return;
+
+  double q[NS][NV], b[NS][3], ga[NS][2];
+  if (__all(stencil_inside<DIR, NS, O>(g, i, j, k))) {
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+      const int o = s - O;
+      load_cell<SD, NV, MHD, HELM, true>(g, i + (DIR == 0 ? o : 0), j + (DIR == 1 ? o : 0),
+                                         k + (DIR == 2 ? o : 0), q[s], b[s], ga[s]);
+    }
+  } else {
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+      const int o = s - O;
+      load_cell<SD, NV, MHD, HELM>(g, i + (DIR == 0 ? o : 0), j + (DIR == 1 ? o : 0),
+                                   k + (DIR == 2 ? o : 0), q[s], b[s], ga[s]);
+    }
+  }
+  double qL[NV], qR[NV], bL[3] = {0.0, 0.0, 0.0}, bR[3] = {0.0, 0.0, 0.0};
+  // scalars take the limited parabola under PPH (source/source.F90:8194-8243)
+#pragma unroll
+  for (int v = 0; v < NV; v++) face_states<R5>(g.recon, v > IE, &q[0][v], NV, qL[v], qR[v]);
+  if (MHD) {
+#pragma unroll
+    for (int c = 0; c < SD; c++) {
+      if (c == DIR) continue;
+      face_states<R5>(g.recon, false, &b[0][c], 3, bL[c], bR[c]);
+    }
+  }
+  // FASTEOS: gamma_e / gamma_c face values, unlimited PPH or the LIM2ND
+  // slope (source/source.F90:8497-8713)
+  double gL[2] = {0.0, 0.0}, gR[2] = {0.0, 0.0};
+  if (HELM) {
+#pragma unroll
+    for (int c = 0; c < 2; c++) face_states<R5>(g.recon, false, &ga[0][c], 2, gL[c], gR[c]);
+  }
+  flux_face<DIR, SD, NAS, MHD, HELM>(g, i, j, k, qL, qR, bL, bR, gL, gR);
+}
+
+// Both edges of one cell from its window (qm, q0, qp) under the 2nd-order
+// reconstructions (LIM2ND, PPH; PPH limits the parabola of scalars).
+__device__ __forceinline__ void cell_edges(int recon, bool limited_scalar, double qm, double q0,
+                                           double qp, double &l, double &r) {
+  if (recon == REC_LIM2ND)
+    rec_lim2nd(qm, q0, qp, l, r);
+  else if (!limited_scalar)
+    rec_pph(qm, q0, qp, l, r);
+  else
+    rec_pph_lim(qm, q0, qp, l, r);
+}
+
+struct CellW {
+  double q[8], b[3], ga[2];
+};
+
+// x2 / x3 sweeps with the 2nd-order reconstructions: one thread per
+// (x1, transverse) column walks kSegFaces consecutive faces along DIR,
+// loading each cell once and reconstructing it once; the right edges of
+// cell f-1 are carried to face f.  k_flux builds every face from four cell
+// loads and two reconstructions.  Each face's arithmetic is that of k_flux.
+constexpr int kSegFaces = 32;
+#ifndef PHLX_FLUX_SEG
+#define PHLX_FLUX_SEG 1
+#endif
+
+template <int DIR, int SD, int NAS, bool MHD, bool HELM, bool FAST>
+__device__ __forceinline__ void seg_walk(const Grid &g, int i, int o, int f0, int f1) {
+  constexpr int NV = SD + 2 + NAS, IE = SD + 1;
+  static_assert(NV <= 8, "cell window size");
+  auto load = [&](int c, CellW &w, int ii, int oo) {
+    load_cell<SD, NV, MHD, HELM, FAST>(g, ii, DIR == 1 ? c : oo, DIR == 2 ? c : oo, w.q, w.b,
+                                       w.ga);
+  };
+  CellW cm, c0, cp;
+  load(f0 - 2, cm, i, o);
+  load(f0 - 1, c0, i, o);
+  load(f0, cp, i, o);
+  // right edges of cell f0 - 1
+  double rq[NV], rb[3] = {0.0, 0.0, 0.0}, rg[2] = {0.0, 0.0};
+  {
+    double l;
+#pragma unroll
+    for (int v = 0; v < NV; v++) cell_edges(g.recon, v > IE, cm.q[v], c0.q[v], cp.q[v], l, rq[v]);
+    if (MHD) {
+#pragma unroll
+      for (int c = 0; c < SD; c++)
+        if (c != DIR) cell_edges(g.recon, false, cm.b[c], c0.b[c], cp.b[c], l, rb[c]);
+    }
+    if (HELM) {
+#pragma unroll
+      for (int c = 0; c < 2; c++) cell_edges(g.recon, false, cm.ga[c], c0.ga[c], cp.ga[c], l, rg[c]);
+    }
+  }
+  cm = c0;
+  c0 = cp;
+#pragma unroll 1
+  for (int f = f0; f < f1; f++) {
+    // the column indices are re-materialised per face: hoisting every
+    // address and profile value of the body out of the loop costs ~100 VGPRs
+    int ii = i, oo = o;
+    asm volatile("" : "+v"(ii), "+v"(oo));
+    load(f + 1, cp, ii, oo);
+    double qL[NV], qR[NV], bL[3] = {0.0, 0.0, 0.0}, bR[3] = {0.0, 0.0, 0.0};
+    double gL[2] = {0.0, 0.0}, gR[2] = {0.0, 0.0};
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+      qL[v] = rq[v];
+      cell_edges(g.recon, v > IE, cm.q[v], c0.q[v], cp.q[v], qR[v], rq[v]);
+    }
+    if (MHD) {
+#pragma unroll
+      for (int c = 0; c < SD; c++) {
+        if (c == DIR) continue;
+        bL[c] = rb[c];
+        cell_edges(g.recon, false, cm.b[c], c0.b[c], cp.b[c], bR[c], rb[c]);
+      }
+    }
+    if (HELM) {
+#pragma unroll
+      for (int c = 0; c < 2; c++) {
+        gL[c] = rg[c];
+        cell_edges(g.recon, false, cm.ga[c], c0.ga[c], cp.ga[c], gR[c], rg[c]);
+      }
+    }
+    flux_face<DIR, SD, NAS, MHD, HELM>(g, ii, DIR == 1 ? f : oo, DIR == 2 ? f : oo, qL, qR, bL,
+                                       bR, gL, gR);
+    cm = c0;
+    c0 = cp;
+  }
+}
+
+template <int DIR, int SD, int NAS, bool MHD, bool HELM>
+__global__ void __launch_bounds__(256) k_flux_seg(Grid g) {
+  static_assert(DIR == 1 || DIR == 2, "x2 / x3 sweeps");
+  if (!g.sc->active) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int o = blockIdx.y * blockDim.y + threadIdx.y;  // x3 cell (DIR 1) / x2 cell (DIR 2)
+  const int n = (DIR == 1) ? g.n2 : g.n3;                // cells along DIR
+  if (i >= g.n1 || o >= ((DIR == 1) ? g.n3 : g.n2)) return;
+  const int f0 = blockIdx.z * kSegFaces;
+  const int f1 = min(f0 + kSegFaces, n + 1);
+  // cells f0 - 2 .. f1 are read
+  if (__all(f0 - 2 >= 0 && f1 <= n - 1))
+    seg_walk<DIR, SD, NAS, MHD, HELM, true>(g, i, o, f0, f1);
+  else
+    seg_walk<DIR, SD, NAS, MHD, HELM, false>(g, i, o, f0, f1);
 }
 
 // ------------------------------------------------------- flux, mixed precision
@@ -901,12 +1024,21 @@ static void launch_flux(const Grid &g, int dir, hipStream_t st) {
       k_flux_mx<2, SD, NAS, MHD, HELM, R5><<<grid_for(g.n1, g.n2, g.n3 + 1, blk), blk, 0, st>>>(g);
     return;
   }
-  if (dir == 0)
+  if (dir == 0) {
     k_flux<0, SD, NAS, MHD, HELM, R5><<<grid_x1_faces(g), blk, 0, st>>>(g);
-  else if (dir == 1)
+  } else if (!R5 && PHLX_FLUX_SEG) {
+    // column segments: (x1 cells, transverse cells, segments of kSegFaces)
+    const int n = dir == 1 ? g.n2 : g.n3, no = dir == 1 ? g.n3 : g.n2;
+    const dim3 grd((g.n1 + 63) / 64, (no + 3) / 4, (n + 1 + kSegFaces - 1) / kSegFaces);
+    if (dir == 1)
+      k_flux_seg<1, SD, NAS, MHD, HELM><<<grd, blk, 0, st>>>(g);
+    else if (SD == 3)
+      k_flux_seg<2, SD, NAS, MHD, HELM><<<grd, blk, 0, st>>>(g);
+  } else if (dir == 1) {
     k_flux<1, SD, NAS, MHD, HELM, R5><<<grid_for(g.n1, g.n2 + 1, g.n3, blk), blk, 0, st>>>(g);
-  else if (SD == 3)
+  } else if (SD == 3) {
     k_flux<2, SD, NAS, MHD, HELM, R5><<<grid_for(g.n1, g.n2, g.n3 + 1, blk), blk, 0, st>>>(g);
+  }
 }
 
 template <int SD, int NAS, bool MHD, bool HELM, bool MIX, bool R5 = false>
