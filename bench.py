@@ -46,9 +46,9 @@ def kernel_bytes(cfg, name: str) -> float:
     mhd = cfg.mhd
     nf = [(nx + 1) * ny * nz, nx * (ny + 1) * nz, nx * ny * (nz + 1)]
     cell_in = 8 * (nv + (sd if mhd else 0))           # prim (+ b_cc) per cell
-    helm = cfg.eos == 1                                 # + gamma_e, gamma_c per cell
-    if helm:
-        cell_in += 16
+    helm = cfg.eos == 1
+    if helm:  # + gamma_e, gamma_c per cell (FASTEOS) or T (face Newton guess)
+        cell_in += 16 if cfg.fasteos else 8
     if name.startswith("flux_x"):
         d = int(name[-1]) - 1
         per_face_out = 8 * nv + (8 * 2 + 1 if mhd else 0)  # fluxes (+2 EMFs + sign)
@@ -60,9 +60,9 @@ def kernel_bytes(cfg, name: str) -> float:
     if name == "update":   # fluxes, prim, U, U0 (or eint, b_cc), faces in; prim, U, eint, b_cc out
         rd = sum(nf[:sd]) * 8 * nv + N * 8 * (3 * nv) + (sum(nf[:sd]) * 8 if mhd else 0)
         wr = N * 8 * (2 * nv + 1 + (sd if mhd else 0))
-        if helm:  # T in; T, gamma_e, gamma_c out
+        if helm:  # T in; T (+ gamma_e, gamma_c with FASTEOS) out
             rd += N * 8
-            wr += N * 24
+            wr += N * (24 if cfg.fasteos else 8)
         return rd + wr
     return 0.0
 
@@ -72,7 +72,7 @@ def b_zu(cfg) -> int:
     8 * (5 n_q + 4 n_e) with n_q evolved fields (conserved + face B) and n_e
     carried EoS fields (T, gamma_e, gamma_c for Helmholtz FASTEOS)."""
     n_q = cfg.nvars + (cfg.sdims if cfg.mhd else 0)
-    n_e = 3 if cfg.eos == 1 else 0
+    n_e = (3 if cfg.fasteos else 1) if cfg.eos == 1 else 0
     stages = cfg.rk_stages
     if stages == 2:
         return 8 * (5 * n_q + 4 * n_e)
@@ -108,9 +108,12 @@ def build_problem(args):
         log(f"building the Helmholtz slab {n}^3 (table {table.nrho}x{table.nT}, "
             f"MHD beta {args.mhd_beta})")
         eos = HelmEoS(table)
-        pb = problems.helm_slab(eos, table, n=n, mhd_beta=args.mhd_beta)
+        pb = problems.helm_slab(eos, table, n=n, mhd_beta=args.mhd_beta,
+                                fasteos=not args.no_fasteos)
         eos.close()
-        desc = (f"3D stratified Helmholtz slab, {n}^3, USE_FASTEOS + USE_WB + EVOLVE_ETOT + "
+        eosdesc = ("per-face T(rho,P) Newton (no USE_FASTEOS)" if args.no_fasteos
+                   else "USE_FASTEOS")
+        desc = (f"3D stratified Helmholtz slab, {n}^3, {eosdesc} + USE_WB + EVOLVE_ETOT + "
                 f"ADVECT_YE_IABAR, {'LHLLD + CT, beta ' + format(args.mhd_beta, 'g') if mhd else 'LHLLC'}"
                 f" + LIM2ND, RK2, CFL 0.8")
         data = ("synthetic (plane-parallel rho/T/abar profiles, seeded 1e-4 density noise"
@@ -251,6 +254,9 @@ def main():
     ap.add_argument("--compare", action="store_true",
                     help="also run the plain FP64 (warm-start) path on the same initial data "
                          "and report the speed-up and the rel. L1 error per conserved field")
+    ap.add_argument("--no-fasteos", action="store_true",
+                    help="helm_slab: Helmholtz without USE_FASTEOS (two T(rho, P) Newton "
+                         "solves per face in the Riemann solvers)")
     ap.add_argument("--mach", type=float, default=None,
                     help="lowmach: start in convective rolls at this RMS Mach number, "
                          "heating = mach^3")

@@ -512,7 +512,23 @@ void orc_reconstruct(int scheme, int limited_scalar, int n, const double *q,
 typedef struct {
   int sd, nv, nas, flux, all_mach, fasteos;
   double gm, igmm1, mach_cutoff;
+  const orc_helm *helm; /* Helmholtz without FASTEOS: per-face T(rho, P) */
 } rparams;
+
+/* Helmholtz without FASTEOS: rho*e and the sound speed of one face state
+ * from the Newton T(rho, P) started at the face guess Tg (HLLC
+ * source.F90:15776-15780, :15891-15895; HLLD :14172-14176 and the others
+ * alike); ye and abar from the state's advected scalars. */
+static void face_helm(const rparams *P, const double *q, double Tg, double *rhoe, double *sound) {
+  const int ia = 2 + P->sd, ie = P->sd + 1;
+  const double rho = q[0], p = q[ie];
+  const double ye = q[ia];
+  const double abar = 1.0 / q[ia + 1];
+  const double zbar = ye * abar;
+  double T = Tg, eint;
+  orc_helm_rhoP_given(P->helm, rho, p, abar, zbar, &T, &eint, sound, 1);
+  *rhoe = eint * rho;
+}
 
 static rparams make_rp(const orc_config *c) {
   rparams p;
@@ -525,13 +541,14 @@ static rparams make_rp(const orc_config *c) {
   p.gm = c->gamma;
   p.igmm1 = 1.0 / (c->gamma - 1.0);
   p.mach_cutoff = c->mach_cutoff;
+  p.helm = NULL;
   return p;
 }
 
 /* HLLC / LHLLC, Cartesian (source.F90:15611-16260). q = (rho, v1, v2, [v3],
  * p, scalars); gL/gR = (gamma_e, gamma_c) face values for FASTEOS. */
 static void riemann_hydro_face(const rparams *P, const double *qL, const double *qR,
-                               const double *gL, const double *gR, double *flux) {
+                               const double *gL, const double *gR, double Tg, double *flux) {
   const int sd = P->sd, nv = P->nv;
   const int im1 = 1, im2 = 2, im3 = (sd == 3) ? 3 : 2, ie = im3 + 1, ia = 2 + sd;
   double rhoL = qL[0], inv_rhoL = 1.0 / rhoL;
@@ -541,6 +558,8 @@ static void riemann_hydro_face(const rparams *P, const double *qL, const double 
   if (P->fasteos) {
     eint = pL / (gL[0] - 1.0);
     cL = sqrt(gL[1] * pL * inv_rhoL);
+  } else if (P->helm) {
+    face_helm(P, qL, Tg, &eint, &cL);
   } else {
     eint = P->igmm1 * pL;
     cL = sqrt(P->gm * pL * inv_rhoL);
@@ -553,6 +572,8 @@ static void riemann_hydro_face(const rparams *P, const double *qL, const double 
   if (P->fasteos) {
     eint = pR / (gR[0] - 1.0);
     cR = sqrt(gR[1] * pR * inv_rhoR);
+  } else if (P->helm) {
+    face_helm(P, qR, Tg, &eint, &cR);
   } else {
     eint = P->igmm1 * pR;
     cR = sqrt(P->gm * pR * inv_rhoR);
@@ -647,7 +668,7 @@ static void riemann_hydro_face(const rparams *P, const double *qL, const double 
 /* HLLD / LHLLD (source.F90:13969-15024).  b = (b_normal, b_t1, [b_t2]). */
 static void riemann_mhd_face(const rparams *P, const double *qL, const double *qR,
                              const double *bL, const double *bR, const double *gL,
-                             const double *gR, double *flux, double *fluxb) {
+                             const double *gR, double Tg, double *flux, double *fluxb) {
   const int sd = P->sd, nv = P->nv;
   const int im1 = 1, im2 = 2, im3 = (sd == 3) ? 3 : 2, ie = im3 + 1, ia = 2 + sd;
   double rhoL = qL[0], inv_rhoL = 1.0 / rhoL;
@@ -657,6 +678,10 @@ static void riemann_mhd_face(const rparams *P, const double *qL, const double *q
   if (P->fasteos) {
     eint = pL / (gL[0] - 1.0);
     c2L = gL[1] * pL * inv_rhoL;
+  } else if (P->helm) {
+    double sound;
+    face_helm(P, qL, Tg, &eint, &sound);
+    c2L = sound * sound;
   } else {
     eint = pL * P->igmm1;
     c2L = P->gm * pL * inv_rhoL;
@@ -670,6 +695,10 @@ static void riemann_mhd_face(const rparams *P, const double *qL, const double *q
   if (P->fasteos) {
     eint = pR / (gR[0] - 1.0);
     c2R = gR[1] * pR * inv_rhoR;
+  } else if (P->helm) {
+    double sound;
+    face_helm(P, qR, Tg, &eint, &sound);
+    c2R = sound * sound;
   } else {
     eint = pR * P->igmm1;
     c2R = P->gm * pR * inv_rhoR;
@@ -858,7 +887,7 @@ void orc_riemann_hydro(const orc_config *cfg, int n, const double *qL,
                        const double *qR, double *flux) {
   rparams P = make_rp(cfg);
   for (int i = 0; i < n; i++)
-    riemann_hydro_face(&P, qL + (long)i * P.nv, qR + (long)i * P.nv, NULL, NULL,
+    riemann_hydro_face(&P, qL + (long)i * P.nv, qR + (long)i * P.nv, NULL, NULL, 0.0,
                        flux + (long)i * P.nv);
 }
 
@@ -868,7 +897,7 @@ void orc_riemann_mhd(const orc_config *cfg, int n, const double *qL,
   rparams P = make_rp(cfg);
   for (int i = 0; i < n; i++)
     riemann_mhd_face(&P, qL + (long)i * P.nv, qR + (long)i * P.nv, bL + (long)i * P.sd,
-                     bR + (long)i * P.sd, NULL, NULL, flux + (long)i * P.nv,
+                     bR + (long)i * P.sd, NULL, NULL, 0.0, flux + (long)i * P.nv,
                      fluxb + (long)i * P.sd);
 }
 
@@ -886,6 +915,8 @@ static void sweep(orc_grid *g, int d) {
   const int fasteos = g->c.fasteos;
   const double inv_dl = g->inv_dx[d];
   rparams P = make_rp(&g->c);
+  const int helm_face = g->c.eos == ORC_EOS_HELMHOLTZ && !fasteos;
+  if (helm_face) P.helm = g->helm;
   long stride = (d == 0) ? 1 : (d == 1) ? g->e1 : (long)g->e1 * g->e2;
   const int ip = g->ip, ias1 = g->ias1;
   const int imd = g->iv[d];
@@ -1012,15 +1043,19 @@ static void sweep(orc_grid *g, int d) {
             gR[(long)f * 2] = gR[(long)f * 2] + ge_eq;
           }
         }
-      /* Riemann */
+      /* Riemann; without FASTEOS the face T guess is the mean of the two
+       * cells' T (x1 :8488-8494, x2 :9766-9772, x3 :10908-10914) */
       for (int f = 0; f <= n; f++) {
+        double Tg = 0.0;
+        if (helm_face)
+          Tg = 0.5 * (g->temp[c0 + (long)(f - 1) * stride] + g->temp[c0 + (long)f * stride]);
         if (mhd)
           riemann_mhd_face(&P, qL + (long)f * nv, qR + (long)f * nv, bL + (long)f * 3,
-                           bR + (long)f * 3, gL + (long)f * 2, gR + (long)f * 2,
+                           bR + (long)f * 3, gL + (long)f * 2, gR + (long)f * 2, Tg,
                            fx + (long)f * nv, fb + (long)f * 3);
         else
           riemann_hydro_face(&P, qL + (long)f * nv, qR + (long)f * nv, gL + (long)f * 2,
-                             gR + (long)f * 2, fx + (long)f * nv);
+                             gR + (long)f * 2, Tg, fx + (long)f * nv);
       }
       for (int f = 0; f <= n; f++) {
         double *F = fx + (long)f * nv, *FB = fb + (long)f * 3;
@@ -1335,14 +1370,19 @@ int orc_hydro_step(orc_grid *g) {
           p[2] = m2 * inv_rho;
           if (sd == 3) p[3] = m3 * inv_rho;
           if (g->c.eos == ORC_EOS_HELMHOLTZ) {
-            /* ADVECT_YE_IABAR + HELMHOLTZ_EOS + USE_FASTEOS (:13573-13578, :13648-13660) */
+            /* ADVECT_YE_IABAR + HELMHOLTZ_EOS (:13573-13578); USE_FASTEOS
+             * keeps gamma_e / gamma_c (:13648-13653), without it the final
+             * evaluation is pressure only (:13654-13655) */
             double ye = S[g->ias1] * inv_rho;
             double abar = rho / S[g->ias1 + 1];
             double zbar = ye * abar;
             double T = g->temp[c], pr, sound;
-            orc_helm_rhoe_given(g->helm, rho, eint * inv_rho, abar, zbar, &T, &pr, &sound, 1);
-            g->gamf[c * 2] = pr / eint + 1.0;
-            g->gamf[c * 2 + 1] = sound * sound * rho / pr;
+            orc_helm_rhoe_given(g->helm, rho, eint * inv_rho, abar, zbar, &T, &pr, &sound,
+                                g->c.fasteos);
+            if (g->c.fasteos) {
+              g->gamf[c * 2] = pr / eint + 1.0;
+              g->gamf[c * 2 + 1] = sound * sound * rho / pr;
+            }
             p[ip] = pr;
             g->temp[c] = T;
           } else {
@@ -1410,15 +1450,18 @@ int orc_init(orc_grid *g) {
         long c = cidx(g, i, j, k), q = iidx(g, i, j, k);
         double rho = g->prim[c * nv], p = g->prim[c * nv + g->ip];
         if (g->c.eos == ORC_EOS_HELMHOLTZ) {
-          /* step-0 Helmholtz FASTEOS init (source.F90:6279-6285, :6351-6357) */
+          /* step-0 Helmholtz init (source.F90:6279-6285; FASTEOS
+           * :6351-6357, otherwise energy only :6358-6361) */
           double abar = 1.0 / g->prim[c * nv + g->ias1 + 1];
           double ye = g->prim[c * nv + g->ias1];
           double zbar = abar * ye;
           double T = g->temp[c], eint, sound;
-          orc_helm_rhoP_given(g->helm, rho, p, abar, zbar, &T, &eint, &sound, 1);
+          orc_helm_rhoP_given(g->helm, rho, p, abar, zbar, &T, &eint, &sound, g->c.fasteos);
           eint = rho * eint;
-          g->gamf[c * 2] = p / eint + 1.0;
-          g->gamf[c * 2 + 1] = sound * sound * rho / p;
+          if (g->c.fasteos) {
+            g->gamf[c * 2] = p / eint + 1.0;
+            g->gamf[c * 2 + 1] = sound * sound * rho / p;
+          }
           g->eint[q] = eint;
           g->temp[c] = T;
         } else {

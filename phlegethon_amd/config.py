@@ -146,9 +146,12 @@ class PhlxConfig:
                 raise ConfigError("DIODE boundaries are not supported with USE_MHD")
         if self.fasteos and self.eos == EOS_IDEAL:
             raise ConfigError("USE_FASTEOS requires a complex EoS (Make.checks:221-226)")
-        if self.eos == EOS_HELMHOLTZ and not (self.fasteos and self.nas == 2):
-            raise ConfigError("HELMHOLTZ_EOS runs here with USE_FASTEOS and "
-                              "ADVECT_YE_IABAR (nas_make=2: Ye, 1/abar)")
+        if self.eos == EOS_HELMHOLTZ and self.nas != 2:
+            raise ConfigError("HELMHOLTZ_EOS runs here with ADVECT_YE_IABAR "
+                              "(nas_make=2: Ye, 1/abar)")
+        if self.eos == EOS_HELMHOLTZ and not self.fasteos and self.precision != PREC_FP64:
+            raise ConfigError("HELMHOLTZ_EOS without USE_FASTEOS (a T(rho, P) Newton per "
+                              "face state) runs in FP64 only")
         if self.use_wb and not self.use_gravity:
             raise ConfigError("USE_WB requires USE_GRAVITY")
         if self.cfl <= 0 or self.gamma <= 1 or self.mu <= 0:

@@ -103,10 +103,13 @@ static int validate(const phlx_config *c) {
     if ((lo == PHLX_BC_PERIODIC) != (hi == PHLX_BC_PERIODIC)) return PHLX_E_CONFIG;
     if (mhd && (lo == PHLX_BC_DIODE || hi == PHLX_BC_DIODE)) return PHLX_E_CONFIG;
   }
-  // Helmholtz runs use USE_FASTEOS with ADVECT_YE_IABAR (Ye, 1/abar as the
-  // two advected scalars); FASTEOS needs a complex EoS (Make.checks)
+  // Helmholtz runs carry Ye and 1/abar (ADVECT_YE_IABAR) as the two
+  // advected scalars; without USE_FASTEOS the Riemann solvers make a
+  // T(rho, P) Newton per face state (FP64 only); FASTEOS needs a complex EoS
+  // (Make.checks)
   if (c->eos == PHLX_EOS_HELMHOLTZ) {
-    if (!c->fasteos || c->nas != 2) return PHLX_E_CONFIG;
+    if (c->nas != 2) return PHLX_E_CONFIG;
+    if (!c->fasteos && c->precision != PHLX_PREC_FP64) return PHLX_E_CONFIG;
   } else if (c->eos == PHLX_EOS_IDEAL) {
     if (c->fasteos) return PHLX_E_CONFIG;
   } else {
@@ -194,8 +197,9 @@ extern "C" int phlx_create(const phlx_config *cfg, phlx_ctx **out) {
     ctx->rk[2][0] = 1.0 / 3.0; ctx->rk[2][1] = 2.0 / 3.0; ctx->rk[2][2] = -(2.0 / 3.0);
   }
   const bool helm = cfg->eos == PHLX_EOS_HELMHOLTZ;
-  if (!select_kernels(g.sd, g.nas, g.mhd, helm, cfg->precision == PHLX_PREC_MIXED,
-                      cfg->recon == PHLX_LIM5TH_REC, ctx->kt)) {
+  if (!select_kernels(g.sd, g.nas, g.mhd, helm, cfg->fasteos != 0,
+                      cfg->precision == PHLX_PREC_MIXED, cfg->recon == PHLX_LIM5TH_REC,
+                      ctx->kt)) {
     phlx_destroy(ctx);
     return PHLX_E_CONFIG;
   }

@@ -260,19 +260,42 @@ __device__ __forceinline__ void face_states(int recon, bool limited_scalar, cons
 // ------------------------------------------------------------ Riemann: hydro
 // HLLC / LHLLC in the face frame (slot 1 = normal velocity).
 // source/source.F90:15611-16260.  g_e/g_c: FASTEOS gamma_e, gamma_c.
-template <int SD, int NAS, bool FE = false>
+// EoS kind of the face states: EK_IDEAL, EK_FASTEOS (gamma_e / gamma_c face
+// values), EK_HELM_FACE (Helmholtz without FASTEOS: a Newton T(rho, P) per
+// face state from the face guess Tg).
+enum { EK_IDEAL = 0, EK_FASTEOS = 1, EK_HELM_FACE = 2 };
+
+// rho*e and the sound speed of one face state by helm_rhoP_given from Tg
+// (HLLC source/source.F90:15776-15780, :15891-15895; HLLD :14172-14176 and
+// the others alike); Ye and abar from the advected scalars (:15706-15725).
+template <int SD>
+__device__ __forceinline__ void face_helm(const Grid &g, const double *q, double Tg,
+                                          double &rhoe, double &sound) {
+  constexpr int IE = SD + 1, IA = SD + 2;
+  const double rho = q[0], p = q[IE];
+  const double ye = q[IA];
+  const double abar = 1.0 / q[IA + 1];
+  const double zbar = ye * abar;
+  double T = Tg, eint;
+  helm_rhoP_given(g.helm, rho, p, abar, zbar, T, eint, sound, true);
+  rhoe = eint * rho;
+}
+
+template <int SD, int NAS, int EK = EK_IDEAL>
 __device__ __forceinline__ void riemann_hydro(const Grid &g, const double *qL,
                                               const double *qR, const double *gL,
-                                              const double *gR, double *F) {
+                                              const double *gR, double Tg, double *F) {
   constexpr int IE = SD + 1, IA = SD + 2, NV = SD + 2 + NAS;
   const double rhoL = qL[0], irL = 1.0 / rhoL;
   const double uL = qL[1], vL = qL[2], wL = (SD == 3) ? qL[3] : 0.0;
   const double v2L = uL * uL + vL * vL + wL * wL;
   const double pL = qL[IE];
   double eL, cL, eR, cR;
-  if (FE) {
+  if (EK == EK_FASTEOS) {
     eL = pL / (gL[0] - 1.0);
     cL = sqrt(gL[1] * pL * irL);
+  } else if (EK == EK_HELM_FACE) {
+    face_helm<SD>(g, qL, Tg, eL, cL);
   } else {
     eL = g.igmm1 * pL;
     cL = sqrt(g.gm * pL * irL);
@@ -282,9 +305,11 @@ __device__ __forceinline__ void riemann_hydro(const Grid &g, const double *qL,
   const double uR = qR[1], vR = qR[2], wR = (SD == 3) ? qR[3] : 0.0;
   const double v2R = uR * uR + vR * vR + wR * wR;
   const double pR = qR[IE];
-  if (FE) {
+  if (EK == EK_FASTEOS) {
     eR = pR / (gR[0] - 1.0);
     cR = sqrt(gR[1] * pR * irR);
+  } else if (EK == EK_HELM_FACE) {
+    face_helm<SD>(g, qR, Tg, eR, cR);
   } else {
     eR = g.igmm1 * pR;
     cR = sqrt(g.gm * pR * irR);
@@ -374,20 +399,25 @@ __device__ __forceinline__ void riemann_hydro(const Grid &g, const double *qL,
 // -------------------------------------------------------------- Riemann: MHD
 // HLLD / LHLLD in the face frame; b = (b_n, b_t1, b_t2).  Returns the
 // induction fluxes Fb (Fb[0] = 0).  source/source.F90:13969-15024.
-template <int SD, int NAS, bool FE = false>
+template <int SD, int NAS, int EK = EK_IDEAL>
 __device__ __forceinline__ void riemann_mhd(const Grid &g, const double *qL,
                                             const double *qR, const double *bL,
                                             const double *bR, const double *gL,
-                                            const double *gR, double *F, double *Fb) {
+                                            const double *gR, double Tg, double *F,
+                                            double *Fb) {
   constexpr int IE = SD + 1, IA = SD + 2, NV = SD + 2 + NAS;
   const double rhoL = qL[0], irL = 1.0 / rhoL;
   const double uL = qL[1], vL = qL[2], wL = (SD == 3) ? qL[3] : 0.0;
   const double v2L = uL * uL + vL * vL + wL * wL;
   const double pL = qL[IE];
   double e, c2L, c2R;
-  if (FE) {
+  if (EK == EK_FASTEOS) {
     e = pL / (gL[0] - 1.0);
     c2L = gL[1] * pL * irL;
+  } else if (EK == EK_HELM_FACE) {
+    double sound;
+    face_helm<SD>(g, qL, Tg, e, sound);
+    c2L = sound * sound;
   } else {
     e = pL * g.igmm1;
     c2L = g.gm * pL * irL;
@@ -398,9 +428,13 @@ __device__ __forceinline__ void riemann_mhd(const Grid &g, const double *qL,
   const double uR = qR[1], vR = qR[2], wR = (SD == 3) ? qR[3] : 0.0;
   const double v2R = uR * uR + vR * vR + wR * wR;
   const double pR = qR[IE];
-  if (FE) {
+  if (EK == EK_FASTEOS) {
     e = pR / (gR[0] - 1.0);
     c2R = gR[1] * pR * irR;
+  } else if (EK == EK_HELM_FACE) {
+    double sound;
+    face_helm<SD>(g, qR, Tg, e, sound);
+    c2R = sound * sound;
   } else {
     e = pR * g.igmm1;
     c2R = g.gm * pR * irR;

@@ -19,7 +19,8 @@ struct KernelTable {
   void (*divb)(const Grid &, hipStream_t);
 };
 
-bool select_kernels(int sd, int nas, bool mhd, bool helm, bool mixed, bool r5, KernelTable &t);
+bool select_kernels(int sd, int nas, bool mhd, bool helm, bool fasteos, bool mixed, bool r5,
+                    KernelTable &t);
 void launch_step_begin(Scalars *sc, hipStream_t st);
 void launch_step_end(const Grid &g, int advance, hipStream_t st);
 void launch_reduce_to(const Grid &g, double *out, hipStream_t st);

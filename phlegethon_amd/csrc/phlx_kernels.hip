@@ -50,14 +50,36 @@ __device__ __forceinline__ bool face_coords(const Grid &g, int &i, int &j, int &
 }
 
 // ---------------------------------------------------------------- flux
+// Carried T of a (possibly ghost) cell: ghosts copy the mapped interior
+// cell's T (the boundary fill flips no sign of T).
+template <bool FAST>
+__device__ __forceinline__ double cell_temp(const Grid &g, int i, int j, int k) {
+  if (FAST) return g.temp[cix(g, i, j, k)];
+  int s;
+  const int mi = map_axis(g, 0, i, g.n1, s);
+  const int mj = map_axis(g, 1, j, g.n2, s);
+  const int mk = (g.sd == 3) ? map_axis(g, 2, k, g.n3, s) : k;
+  return g.temp[cix(g, mi, mj, mk)];
+}
+
+// Helmholtz without FASTEOS: the face's Newton guess, the mean of the T of
+// the two cells beside face (i, j, k) along DIR (source/source.F90:8488-8494,
+// :9766-9772, :10908-10914).
+template <int DIR, bool FAST>
+__device__ __forceinline__ double face_temp_guess(const Grid &g, int i, int j, int k) {
+  const double tm = cell_temp<FAST>(g, i - (DIR == 0), j - (DIR == 1), k - (DIR == 2));
+  const double tc = cell_temp<FAST>(g, i, j, k);
+  return 0.5 * (tm + tc);
+}
+
 // The face's solve from its reconstructed states (grid frame; transverse B
 // and the FASTEOS gammas reconstructed, the normal B read here): rotation
 // into the face frame, WB add-back, Riemann solve, rotation back, EMF and
 // contact sign (MHD), WB flux correction, ETOT potential flux, store.
-template <int DIR, int SD, int NAS, bool MHD, bool HELM>
+template <int DIR, int SD, int NAS, bool MHD, int EK>
 __device__ __forceinline__ void flux_face(const Grid &g, int i, int j, int k, double *qL,
                                           double *qR, double *bL, double *bR, double *gL,
-                                          double *gR) {
+                                          double *gR, double Tg) {
   constexpr int NV = SD + 2 + NAS, IE = SD + 1;
   const long long f = fix(g, DIR, i, j, k);
   if (MHD) {
@@ -91,16 +113,16 @@ __device__ __forceinline__ void flux_face(const Grid &g, int i, int j, int k, do
     qR[IE] = qR[IE] + peq;
   }
   // FASTEOS: WB add-back of the face gamma_e (source/source.F90:8715-8727)
-  if (HELM && g.wb) {
+  if (EK == EK_FASTEOS && g.wb) {
     const double geq = (DIR == 1) ? g.eq_ge_f[jy] : g.eq_ge_cc[jy];
     gL[0] = gL[0] + geq;
     gR[0] = gR[0] + geq;
   }
   double F[NV], Fb[3] = {0.0, 0.0, 0.0};
   if (MHD)
-    riemann_mhd<SD, NAS, HELM>(g, qL, qR, bL, bR, gL, gR, F, Fb);
+    riemann_mhd<SD, NAS, EK>(g, qL, qR, bL, bR, gL, gR, Tg, F, Fb);
   else
-    riemann_hydro<SD, NAS, HELM>(g, qL, qR, gL, gR, F);
+    riemann_hydro<SD, NAS, EK>(g, qL, qR, gL, gR, Tg, F);
   // back to the grid frame
   if (DIR == 1) {
     double t = F[1]; F[1] = -F[2]; F[2] = t;
@@ -143,29 +165,36 @@ __device__ __forceinline__ void flux_face(const Grid &g, int i, int j, int k, do
 // f), rotates into the face frame, solves, rotates back and applies the WB
 // flux correction and the ETOT potential flux.  MHD faces also store the
 // two tangential EMF components and the contact-upwind sign used by CT.
-template <int DIR, int SD, int NAS, bool MHD, bool HELM, bool R5>
+// EK: EoS kind of the face states (EK_IDEAL, EK_FASTEOS, EK_HELM_FACE).
+template <int DIR, int SD, int NAS, bool MHD, int EK, bool R5>
 __global__ void __launch_bounds__(256) k_flux(Grid g) {
   constexpr int NV = SD + 2 + NAS, IE = SD + 1;
   constexpr int NS = R5 ? 6 : 4, O = R5 ? 3 : 2;  // window: cells f-O .. f+NS-O-1
+  constexpr bool GAM = EK == EK_FASTEOS;
   if (!g.sc->active) return;
   int i, j, k;
   if (!face_coords<DIR>(g, i, j, k)) return;
 
   double q[NS][NV], b[NS][3], ga[NS][2];
+  double Tg = 0.0;
   if (__all(stencil_inside<DIR, NS, O>(g, i, j, k))) {
 #pragma unroll
     for (int s = 0; s < NS; s++) {
       const int o = s - O;
-      load_cell<SD, NV, MHD, HELM, true>(g, i + (DIR == 0 ? o : 0), j + (DIR == 1 ? o : 0),
-                                         k + (DIR == 2 ? o : 0), q[s], b[s], ga[s]);
+      load_cell<SD, NV, MHD, GAM, true>(g, i + (DIR == 0 ? o : 0), j + (DIR == 1 ? o : 0),
+                                        k + (DIR == 2 ? o : 0), q[s], b[s], ga[s]);
     }
+    if (EK == EK_HELM_FACE)
+      Tg = face_temp_guess<DIR, true>(g, i, j, k);
   } else {
 #pragma unroll
     for (int s = 0; s < NS; s++) {
       const int o = s - O;
-      load_cell<SD, NV, MHD, HELM>(g, i + (DIR == 0 ? o : 0), j + (DIR == 1 ? o : 0),
-                                   k + (DIR == 2 ? o : 0), q[s], b[s], ga[s]);
+      load_cell<SD, NV, MHD, GAM>(g, i + (DIR == 0 ? o : 0), j + (DIR == 1 ? o : 0),
+                                  k + (DIR == 2 ? o : 0), q[s], b[s], ga[s]);
     }
+    if (EK == EK_HELM_FACE)
+      Tg = face_temp_guess<DIR, false>(g, i, j, k);
   }
   double qL[NV], qR[NV], bL[3] = {0.0, 0.0, 0.0}, bR[3] = {0.0, 0.0, 0.0};
   // scalars take the limited parabola under PPH (source/source.F90:8194-8243)
@@ -181,11 +210,11 @@ __global__ void __launch_bounds__(256) k_flux(Grid g) {
   // FASTEOS: gamma_e / gamma_c face values, unlimited PPH or the LIM2ND
   // slope (source/source.F90:8497-8713)
   double gL[2] = {0.0, 0.0}, gR[2] = {0.0, 0.0};
-  if (HELM) {
+  if (GAM) {
 #pragma unroll
     for (int c = 0; c < 2; c++) face_states<R5>(g.recon, false, &ga[0][c], 2, gL[c], gR[c]);
   }
-  flux_face<DIR, SD, NAS, MHD, HELM>(g, i, j, k, qL, qR, bL, bR, gL, gR);
+  flux_face<DIR, SD, NAS, MHD, EK>(g, i, j, k, qL, qR, bL, bR, gL, gR, Tg);
 }
 
 // Both edges of one cell from its window (qm, q0, qp) under the 2nd-order
@@ -201,7 +230,7 @@ __device__ __forceinline__ void cell_edges(int recon, bool limited_scalar, doubl
 }
 
 struct CellW {
-  double q[8], b[3], ga[2];
+  double q[8], b[3], ga[2], t;
 };
 
 // x2 / x3 sweeps with the 2nd-order reconstructions: one thread per
@@ -214,13 +243,15 @@ constexpr int kSegFaces = 32;
 #define PHLX_FLUX_SEG 1
 #endif
 
-template <int DIR, int SD, int NAS, bool MHD, bool HELM, bool FAST>
+template <int DIR, int SD, int NAS, bool MHD, int EK, bool FAST>
 __device__ __forceinline__ void seg_walk(const Grid &g, int i, int o, int f0, int f1) {
   constexpr int NV = SD + 2 + NAS, IE = SD + 1;
+  constexpr bool HELM = EK == EK_FASTEOS;  // gamma_e / gamma_c carried per cell
   static_assert(NV <= 8, "cell window size");
   auto load = [&](int c, CellW &w, int ii, int oo) {
-    load_cell<SD, NV, MHD, HELM, FAST>(g, ii, DIR == 1 ? c : oo, DIR == 2 ? c : oo, w.q, w.b,
-                                       w.ga);
+    const int jj = DIR == 1 ? c : oo, kk = DIR == 2 ? c : oo;
+    load_cell<SD, NV, MHD, HELM, FAST>(g, ii, jj, kk, w.q, w.b, w.ga);
+    if (EK == EK_HELM_FACE) w.t = cell_temp<FAST>(g, ii, jj, kk);
   };
   CellW cm, c0, cp;
   load(f0 - 2, cm, i, o);
@@ -273,14 +304,15 @@ __device__ __forceinline__ void seg_walk(const Grid &g, int i, int o, int f0, in
         cell_edges(g.recon, false, cm.ga[c], c0.ga[c], cp.ga[c], gR[c], rg[c]);
       }
     }
-    flux_face<DIR, SD, NAS, MHD, HELM>(g, ii, DIR == 1 ? f : oo, DIR == 2 ? f : oo, qL, qR, bL,
-                                       bR, gL, gR);
+    const double Tg = (EK == EK_HELM_FACE) ? 0.5 * (cm.t + c0.t) : 0.0;
+    flux_face<DIR, SD, NAS, MHD, EK>(g, ii, DIR == 1 ? f : oo, DIR == 2 ? f : oo, qL, qR, bL,
+                                     bR, gL, gR, Tg);
     cm = c0;
     c0 = cp;
   }
 }
 
-template <int DIR, int SD, int NAS, bool MHD, bool HELM>
+template <int DIR, int SD, int NAS, bool MHD, int EK>
 __global__ void __launch_bounds__(256) k_flux_seg(Grid g) {
   static_assert(DIR == 1 || DIR == 2, "x2 / x3 sweeps");
   if (!g.sc->active) return;
@@ -292,9 +324,9 @@ __global__ void __launch_bounds__(256) k_flux_seg(Grid g) {
   const int f1 = min(f0 + kSegFaces, n + 1);
   // cells f0 - 2 .. f1 are read
   if (__all(f0 - 2 >= 0 && f1 <= n - 1))
-    seg_walk<DIR, SD, NAS, MHD, HELM, true>(g, i, o, f0, f1);
+    seg_walk<DIR, SD, NAS, MHD, EK, true>(g, i, o, f0, f1);
   else
-    seg_walk<DIR, SD, NAS, MHD, HELM, false>(g, i, o, f0, f1);
+    seg_walk<DIR, SD, NAS, MHD, EK, false>(g, i, o, f0, f1);
 }
 
 // ------------------------------------------------------- flux, mixed precision
@@ -303,43 +335,14 @@ __global__ void __launch_bounds__(256) k_flux_seg(Grid g) {
 // Riemann solve run in FP32 (deviation form, phlx_device.hpp); the face
 // equilibrium is added back in FP64 before the cast; fluxes are stored as
 // FP32 and accumulated in FP64 by k_update.
-template <int DIR, int SD, int NAS, bool MHD, bool HELM, bool R5>
-__global__ void __launch_bounds__(256) k_flux_mx(Grid g) {
+// The mixed-precision face solve from its FP32 face states (as flux_face).
+template <int DIR, int SD, int NAS, bool MHD, bool HELM>
+__device__ __forceinline__ void flux_face_mx(const Grid &g, int i, int j, int k, float *qL,
+                                             float *qR, float *bL, float *bR, float *gL,
+                                             float *gR) {
   constexpr int NV = SD + 2 + NAS, IE = SD + 1;
-  constexpr int NS = R5 ? 6 : 4, O = R5 ? 3 : 2;
-  if (!g.sc->active) return;
-  int i, j, k;
-  if (!face_coords<DIR>(g, i, j, k)) return;
-
-  float q[NS][NV], b[NS][3], ga[NS][2];
-  const bool fast = __all(stencil_inside<DIR, NS, O>(g, i, j, k));
-#pragma unroll
-  for (int s = 0; s < NS; s++) {
-    const int o = s - O;
-    double qd[NV], bd[3] = {0.0, 0.0, 0.0}, gd[2] = {0.0, 0.0};
-    if (fast)
-      load_cell<SD, NV, MHD, HELM, true>(g, i + (DIR == 0 ? o : 0), j + (DIR == 1 ? o : 0),
-                                         k + (DIR == 2 ? o : 0), qd, bd, gd);
-    else
-      load_cell<SD, NV, MHD, HELM>(g, i + (DIR == 0 ? o : 0), j + (DIR == 1 ? o : 0),
-                                   k + (DIR == 2 ? o : 0), qd, bd, gd);
-#pragma unroll
-    for (int v = 0; v < NV; v++) q[s][v] = (float)qd[v];
-#pragma unroll
-    for (int v = 0; v < 3; v++) b[s][v] = (float)bd[v];
-    ga[s][0] = (float)gd[0];
-    ga[s][1] = (float)gd[1];
-  }
-  float qL[NV], qR[NV], bL[3] = {0.0f, 0.0f, 0.0f}, bR[3] = {0.0f, 0.0f, 0.0f};
-#pragma unroll
-  for (int v = 0; v < NV; v++) mx::face_states<R5>(g.recon, v > IE, &q[0][v], NV, qL[v], qR[v]);
   const long long f = fix(g, DIR, i, j, k);
   if (MHD) {
-#pragma unroll
-    for (int c = 0; c < SD; c++) {
-      if (c == DIR) continue;
-      mx::face_states<R5>(g.recon, false, &b[0][c], 3, bL[c], bR[c]);
-    }
     const float bn = (float)g.bf[DIR][f];
     bL[DIR] = bn;
     bR[DIR] = bn;
@@ -371,15 +374,10 @@ __global__ void __launch_bounds__(256) k_flux_mx(Grid g) {
     pfL = (float)(peq + (double)qL[IE]);
     pfR = (float)(peq + (double)qR[IE]);
   }
-  float gL[2] = {0.0f, 0.0f}, gR[2] = {0.0f, 0.0f};
-  if (HELM) {
-#pragma unroll
-    for (int c = 0; c < 2; c++) mx::face_states<R5>(g.recon, false, &ga[0][c], 2, gL[c], gR[c]);
-    if (g.wb) {
-      const double geq = (DIR == 1) ? g.eq_ge_f[jy] : g.eq_ge_cc[jy];
-      gL[0] = (float)(geq + (double)gL[0]);
-      gR[0] = (float)(geq + (double)gR[0]);
-    }
+  if (HELM && g.wb) {
+    const double geq = (DIR == 1) ? g.eq_ge_f[jy] : g.eq_ge_cc[jy];
+    gL[0] = (float)(geq + (double)gL[0]);
+    gR[0] = (float)(geq + (double)gR[0]);
   }
   float F[NV], Fb[3] = {0.0f, 0.0f, 0.0f};
   if (MHD)
@@ -414,6 +412,154 @@ __global__ void __launch_bounds__(256) k_flux_mx(Grid g) {
   }
 #pragma unroll
   for (int v = 0; v < NV; v++) g.Ff[DIR][v * nf + f] = F[v];
+}
+
+// Mixed-precision column segments (as k_flux_seg): cells are loaded and the
+// WB deviations formed in FP64, then cast; reconstruction in FP32.
+struct CellWf {
+  float q[8], b[3], ga[2];
+};
+
+__device__ __forceinline__ void cell_edges_mx(int recon, bool limited_scalar, float qm, float q0,
+                                              float qp, float &l, float &r) {
+  if (recon == REC_LIM2ND)
+    mx::rec_lim2nd(qm, q0, qp, l, r);
+  else if (!limited_scalar)
+    mx::rec_pph(qm, q0, qp, l, r);
+  else
+    mx::rec_pph_lim(qm, q0, qp, l, r);
+}
+
+template <int DIR, int SD, int NAS, bool MHD, bool HELM, bool FAST>
+__device__ __forceinline__ void seg_walk_mx(const Grid &g, int i, int o, int f0, int f1) {
+  constexpr int NV = SD + 2 + NAS, IE = SD + 1;
+  static_assert(NV <= 8, "cell window size");
+  auto load = [&](int c, CellWf &w, int ii, int oo) {
+    double qd[NV], bd[3] = {0.0, 0.0, 0.0}, gd[2] = {0.0, 0.0};
+    load_cell<SD, NV, MHD, HELM, FAST>(g, ii, DIR == 1 ? c : oo, DIR == 2 ? c : oo, qd, bd, gd);
+#pragma unroll
+    for (int v = 0; v < NV; v++) w.q[v] = (float)qd[v];
+#pragma unroll
+    for (int v = 0; v < 3; v++) w.b[v] = (float)bd[v];
+    w.ga[0] = (float)gd[0];
+    w.ga[1] = (float)gd[1];
+  };
+  CellWf cm, c0, cp;
+  load(f0 - 2, cm, i, o);
+  load(f0 - 1, c0, i, o);
+  load(f0, cp, i, o);
+  float rq[NV], rb[3] = {0.0f, 0.0f, 0.0f}, rg[2] = {0.0f, 0.0f};
+  {
+    float l;
+#pragma unroll
+    for (int v = 0; v < NV; v++)
+      cell_edges_mx(g.recon, v > IE, cm.q[v], c0.q[v], cp.q[v], l, rq[v]);
+    if (MHD) {
+#pragma unroll
+      for (int c = 0; c < SD; c++)
+        if (c != DIR) cell_edges_mx(g.recon, false, cm.b[c], c0.b[c], cp.b[c], l, rb[c]);
+    }
+    if (HELM) {
+#pragma unroll
+      for (int c = 0; c < 2; c++)
+        cell_edges_mx(g.recon, false, cm.ga[c], c0.ga[c], cp.ga[c], l, rg[c]);
+    }
+  }
+  cm = c0;
+  c0 = cp;
+#pragma unroll 1
+  for (int f = f0; f < f1; f++) {
+    int ii = i, oo = o;  // re-materialised per face (see seg_walk)
+    asm volatile("" : "+v"(ii), "+v"(oo));
+    load(f + 1, cp, ii, oo);
+    float qL[NV], qR[NV], bL[3] = {0.0f, 0.0f, 0.0f}, bR[3] = {0.0f, 0.0f, 0.0f};
+    float gL[2] = {0.0f, 0.0f}, gR[2] = {0.0f, 0.0f};
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+      qL[v] = rq[v];
+      cell_edges_mx(g.recon, v > IE, cm.q[v], c0.q[v], cp.q[v], qR[v], rq[v]);
+    }
+    if (MHD) {
+#pragma unroll
+      for (int c = 0; c < SD; c++) {
+        if (c == DIR) continue;
+        bL[c] = rb[c];
+        cell_edges_mx(g.recon, false, cm.b[c], c0.b[c], cp.b[c], bR[c], rb[c]);
+      }
+    }
+    if (HELM) {
+#pragma unroll
+      for (int c = 0; c < 2; c++) {
+        gL[c] = rg[c];
+        cell_edges_mx(g.recon, false, cm.ga[c], c0.ga[c], cp.ga[c], gR[c], rg[c]);
+      }
+    }
+    flux_face_mx<DIR, SD, NAS, MHD, HELM>(g, ii, DIR == 1 ? f : oo, DIR == 2 ? f : oo, qL, qR,
+                                          bL, bR, gL, gR);
+    cm = c0;
+    c0 = cp;
+  }
+}
+
+template <int DIR, int SD, int NAS, bool MHD, bool HELM>
+__global__ void __launch_bounds__(256) k_flux_seg_mx(Grid g) {
+  static_assert(DIR == 1 || DIR == 2, "x2 / x3 sweeps");
+  if (!g.sc->active) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int o = blockIdx.y * blockDim.y + threadIdx.y;
+  const int n = (DIR == 1) ? g.n2 : g.n3;
+  if (i >= g.n1 || o >= ((DIR == 1) ? g.n3 : g.n2)) return;
+  const int f0 = blockIdx.z * kSegFaces;
+  const int f1 = min(f0 + kSegFaces, n + 1);
+  if (__all(f0 - 2 >= 0 && f1 <= n - 1))
+    seg_walk_mx<DIR, SD, NAS, MHD, HELM, true>(g, i, o, f0, f1);
+  else
+    seg_walk_mx<DIR, SD, NAS, MHD, HELM, false>(g, i, o, f0, f1);
+}
+
+template <int DIR, int SD, int NAS, bool MHD, bool HELM, bool R5>
+__global__ void __launch_bounds__(256) k_flux_mx(Grid g) {
+  constexpr int NV = SD + 2 + NAS, IE = SD + 1;
+  constexpr int NS = R5 ? 6 : 4, O = R5 ? 3 : 2;
+  if (!g.sc->active) return;
+  int i, j, k;
+  if (!face_coords<DIR>(g, i, j, k)) return;
+
+  float q[NS][NV], b[NS][3], ga[NS][2];
+  const bool fast = __all(stencil_inside<DIR, NS, O>(g, i, j, k));
+#pragma unroll
+  for (int s = 0; s < NS; s++) {
+    const int o = s - O;
+    double qd[NV], bd[3] = {0.0, 0.0, 0.0}, gd[2] = {0.0, 0.0};
+    if (fast)
+      load_cell<SD, NV, MHD, HELM, true>(g, i + (DIR == 0 ? o : 0), j + (DIR == 1 ? o : 0),
+                                         k + (DIR == 2 ? o : 0), qd, bd, gd);
+    else
+      load_cell<SD, NV, MHD, HELM>(g, i + (DIR == 0 ? o : 0), j + (DIR == 1 ? o : 0),
+                                   k + (DIR == 2 ? o : 0), qd, bd, gd);
+#pragma unroll
+    for (int v = 0; v < NV; v++) q[s][v] = (float)qd[v];
+#pragma unroll
+    for (int v = 0; v < 3; v++) b[s][v] = (float)bd[v];
+    ga[s][0] = (float)gd[0];
+    ga[s][1] = (float)gd[1];
+  }
+  float qL[NV], qR[NV], bL[3] = {0.0f, 0.0f, 0.0f}, bR[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+  for (int v = 0; v < NV; v++) mx::face_states<R5>(g.recon, v > IE, &q[0][v], NV, qL[v], qR[v]);
+  if (MHD) {
+#pragma unroll
+    for (int c = 0; c < SD; c++) {
+      if (c == DIR) continue;
+      mx::face_states<R5>(g.recon, false, &b[0][c], 3, bL[c], bR[c]);
+    }
+  }
+  float gL[2] = {0.0f, 0.0f}, gR[2] = {0.0f, 0.0f};
+  if (HELM) {
+#pragma unroll
+    for (int c = 0; c < 2; c++) mx::face_states<R5>(g.recon, false, &ga[0][c], 2, gL[c], gR[c]);
+  }
+  flux_face_mx<DIR, SD, NAS, MHD, HELM>(g, i, j, k, qL, qR, bL, bR, gL, gR);
 }
 
 // ------------------------------------------------------------------ CT
@@ -845,7 +991,7 @@ __global__ void __launch_bounds__(256)
                              fb);
         } else {
           it = helm_rhoe_guess(g.helm, mode, tg, g.mlp.resid_tol, rho, eint * ir, abar, zbar, T,
-                               pr, sound, true, fb);
+                               pr, sound, g.fasteos != 0, fb);
         }
         if (g.eos_count) {  // uniform across the grid
           const int tid = threadIdx.y * blockDim.x + threadIdx.x;
@@ -865,12 +1011,15 @@ __global__ void __launch_bounds__(256)
         int fb;
         helm_rhoe_lin(g.helm, g.eos_lin_tol, rho, eint * ir, abar, zbar, T, pr, sound, fb);
       } else {
-        helm_rhoe_given(g.helm, rho, eint * ir, abar, zbar, T, pr, sound, true);
+        // without FASTEOS the last evaluation is pressure only (:13654-13655)
+        helm_rhoe_given(g.helm, rho, eint * ir, abar, zbar, T, pr, sound, g.fasteos != 0);
       }
       helm_pin(pr);
       helm_pin(sound);
-      g.gam[c] = pr / eint + 1.0;
-      g.gam[N + c] = sound * sound * rho / pr;
+      if (g.fasteos) {
+        g.gam[c] = pr / eint + 1.0;
+        g.gam[N + c] = sound * sound * rho / pr;
+      }
       g.temp[c] = T;
       g.P[IE * N + c] = pr;
       p[IE] = pr;
@@ -912,17 +1061,19 @@ __global__ void __launch_bounds__(256) k_init(Grid g) {
   if (i >= g.n1 || j >= g.n2) return;
   const long long N = g.N, c = cix(g, i, j, k);
   if (HELM) {
-    // T(rho, p) Newton from the uploaded T, eint, gamma_e, gamma_c
-    // (source/source.F90:6279-6285, :6351-6357)
+    // T(rho, p) Newton from the uploaded T, eint (+ gamma_e, gamma_c with
+    // FASTEOS) (source/source.F90:6279-6285, :6351-6361)
     const double rho = g.P[c], p = g.P[(SD + 1) * N + c];
     const double abar = 1.0 / g.P[(SD + 3) * N + c];
     const double ye = g.P[(SD + 2) * N + c];
     const double zbar = abar * ye;
     double T = g.temp[c], eint, sound;
-    helm_rhoP_given(g.helm, rho, p, abar, zbar, T, eint, sound, true);
+    helm_rhoP_given(g.helm, rho, p, abar, zbar, T, eint, sound, g.fasteos != 0);
     eint = rho * eint;
-    g.gam[c] = p / eint + 1.0;
-    g.gam[N + c] = sound * sound * rho / p;
+    if (g.fasteos) {
+      g.gam[c] = p / eint + 1.0;
+      g.gam[N + c] = sound * sound * rho / p;
+    }
     g.eint[c] = eint;
     g.temp[c] = T;
   } else {
@@ -1012,39 +1163,50 @@ static dim3 grid_plane(int e1, int e2, int e3) {
 }
 static dim3 grid_x1_faces(const Grid &g) { return grid_plane(g.n1 + 1, g.n2, g.n3); }
 
-template <int SD, int NAS, bool MHD, bool HELM, bool MIX, bool R5>
+template <int SD, int NAS, bool MHD, int EK, bool MIX, bool R5>
 static void launch_flux(const Grid &g, int dir, hipStream_t st) {
+  constexpr bool HELM = EK != EK_IDEAL;
   const dim3 blk(64, 4, 1);
   if (MIX) {
-    if (dir == 0)
+    static_assert(!MIX || EK != EK_HELM_FACE, "mixed precision needs FASTEOS");
+    if (dir == 0) {
       k_flux_mx<0, SD, NAS, MHD, HELM, R5><<<grid_x1_faces(g), blk, 0, st>>>(g);
-    else if (dir == 1)
+    } else if (!R5 && PHLX_FLUX_SEG) {
+      const int n = dir == 1 ? g.n2 : g.n3, no = dir == 1 ? g.n3 : g.n2;
+      const dim3 grd((g.n1 + 63) / 64, (no + 3) / 4, (n + 1 + kSegFaces - 1) / kSegFaces);
+      if (dir == 1)
+        k_flux_seg_mx<1, SD, NAS, MHD, HELM><<<grd, blk, 0, st>>>(g);
+      else if (SD == 3)
+        k_flux_seg_mx<2, SD, NAS, MHD, HELM><<<grd, blk, 0, st>>>(g);
+    } else if (dir == 1) {
       k_flux_mx<1, SD, NAS, MHD, HELM, R5><<<grid_for(g.n1, g.n2 + 1, g.n3, blk), blk, 0, st>>>(g);
-    else if (SD == 3)
+    } else if (SD == 3) {
       k_flux_mx<2, SD, NAS, MHD, HELM, R5><<<grid_for(g.n1, g.n2, g.n3 + 1, blk), blk, 0, st>>>(g);
+    }
     return;
   }
   if (dir == 0) {
-    k_flux<0, SD, NAS, MHD, HELM, R5><<<grid_x1_faces(g), blk, 0, st>>>(g);
+    k_flux<0, SD, NAS, MHD, EK, R5><<<grid_x1_faces(g), blk, 0, st>>>(g);
   } else if (!R5 && PHLX_FLUX_SEG) {
     // column segments: (x1 cells, transverse cells, segments of kSegFaces)
     const int n = dir == 1 ? g.n2 : g.n3, no = dir == 1 ? g.n3 : g.n2;
     const dim3 grd((g.n1 + 63) / 64, (no + 3) / 4, (n + 1 + kSegFaces - 1) / kSegFaces);
     if (dir == 1)
-      k_flux_seg<1, SD, NAS, MHD, HELM><<<grd, blk, 0, st>>>(g);
+      k_flux_seg<1, SD, NAS, MHD, EK><<<grd, blk, 0, st>>>(g);
     else if (SD == 3)
-      k_flux_seg<2, SD, NAS, MHD, HELM><<<grd, blk, 0, st>>>(g);
+      k_flux_seg<2, SD, NAS, MHD, EK><<<grd, blk, 0, st>>>(g);
   } else if (dir == 1) {
-    k_flux<1, SD, NAS, MHD, HELM, R5><<<grid_for(g.n1, g.n2 + 1, g.n3, blk), blk, 0, st>>>(g);
+    k_flux<1, SD, NAS, MHD, EK, R5><<<grid_for(g.n1, g.n2 + 1, g.n3, blk), blk, 0, st>>>(g);
   } else if (SD == 3) {
-    k_flux<2, SD, NAS, MHD, HELM, R5><<<grid_for(g.n1, g.n2, g.n3 + 1, blk), blk, 0, st>>>(g);
+    k_flux<2, SD, NAS, MHD, EK, R5><<<grid_for(g.n1, g.n2, g.n3 + 1, blk), blk, 0, st>>>(g);
   }
 }
 
-template <int SD, int NAS, bool MHD, bool HELM, bool MIX, bool R5 = false>
+template <int SD, int NAS, bool MHD, bool HELM, bool MIX, bool R5 = false,
+          int EK = HELM ? EK_FASTEOS : EK_IDEAL>
 struct Kern {
   static void flux(const Grid &g, int dir, hipStream_t st) {
-    launch_flux<SD, NAS, MHD, HELM, MIX, R5>(g, dir, st);
+    launch_flux<SD, NAS, MHD, EK, MIX, R5>(g, dir, st);
   }
   static void corner(const Grid &g, hipStream_t st) {
     const dim3 blk(64, 4, 1);
@@ -1081,9 +1243,10 @@ struct Kern {
   }
 };
 
-template <int SD, int NAS, bool MHD, bool HELM, bool MIX, bool R5>
+template <int SD, int NAS, bool MHD, bool HELM, bool MIX, bool R5,
+          int EK = HELM ? EK_FASTEOS : EK_IDEAL>
 static KernelTable make_table() {
-  using K = Kern<SD, NAS, MHD, HELM, MIX, R5>;
+  using K = Kern<SD, NAS, MHD, HELM, MIX, R5, EK>;
   KernelTable t;
   t.flux = &K::flux;
   t.corner = &K::corner;
@@ -1097,10 +1260,20 @@ static KernelTable make_table() {
 }
 
 template <bool MIX, bool R5>
-static bool select_prec(int sd, int nas, bool mhd, bool helm, KernelTable &t) {
+static bool select_prec(int sd, int nas, bool mhd, bool helm, bool fasteos, KernelTable &t) {
   if (helm) {
     // Helmholtz runs carry Ye and 1/abar as the two advected scalars
     if (nas != 2) return false;
+    if (!fasteos) {
+      // per-face T(rho, P) Newton in the Riemann solvers: FP64 only
+      if (MIX) return false;
+      constexpr int EK = EK_HELM_FACE;
+      if (sd == 2 && !mhd) t = make_table<2, 2, false, true, false, R5, EK>();
+      else if (sd == 3 && !mhd) t = make_table<3, 2, false, true, false, R5, EK>();
+      else if (sd == 3 && mhd) t = make_table<3, 2, true, true, false, R5, EK>();
+      else return false;
+      return true;
+    }
     if (sd == 2 && !mhd) t = make_table<2, 2, false, true, MIX, R5>();
     else if (sd == 3 && !mhd) t = make_table<3, 2, false, true, MIX, R5>();
     else if (sd == 3 && mhd) t = make_table<3, 2, true, true, MIX, R5>();
@@ -1118,12 +1291,13 @@ static bool select_prec(int sd, int nas, bool mhd, bool helm, KernelTable &t) {
   return true;
 }
 
-bool select_kernels(int sd, int nas, bool mhd, bool helm, bool mixed, bool r5, KernelTable &t) {
+bool select_kernels(int sd, int nas, bool mhd, bool helm, bool fasteos, bool mixed, bool r5,
+                    KernelTable &t) {
   if (mixed)
-    return r5 ? select_prec<true, true>(sd, nas, mhd, helm, t)
-              : select_prec<true, false>(sd, nas, mhd, helm, t);
-  return r5 ? select_prec<false, true>(sd, nas, mhd, helm, t)
-            : select_prec<false, false>(sd, nas, mhd, helm, t);
+    return r5 ? select_prec<true, true>(sd, nas, mhd, helm, fasteos, t)
+              : select_prec<true, false>(sd, nas, mhd, helm, fasteos, t);
+  return r5 ? select_prec<false, true>(sd, nas, mhd, helm, fasteos, t)
+            : select_prec<false, false>(sd, nas, mhd, helm, fasteos, t);
 }
 
 void launch_step_begin(Scalars *sc, hipStream_t st) { k_step_begin<<<1, 1, 0, st>>>(sc); }

@@ -331,7 +331,7 @@ def slab_profiles(y: np.ndarray, L: float = SLAB_L):
 def helm_slab(eos, table, n: int = 256, ny: Optional[int] = None, sdims: int = 3,
               recon: int = LIM2ND_REC, flux: int = LHLLC_FLUX, seed: int = SEED,
               noise: float = 1e-4, mhd_beta: Optional[float] = None,
-              rk_stages: int = 2) -> Problem:
+              rk_stages: int = 2, fasteos: bool = True) -> Problem:
     """Stratified Helmholtz slab: plane-parallel equilibrium rho(y), T(y),
     abar(y) with P and e from the EoS, gravity from the discrete balance
     (p_f[j+1] - p_f[j]) / dy = rho_cc[j] g_j and the potential integrated
@@ -343,7 +343,9 @@ def helm_slab(eos, table, n: int = 256, ny: Optional[int] = None, sdims: int = 3
     ``eos`` evaluates the table (an object with ``full(rho, T, abar, zbar)``
     returning P, E, cs, cv): the GPU ``runtime.HelmEoS`` or, in CPU-only
     tests, the CPU restatement.  With ``mhd_beta`` set, LHLLD MHD with a weak
-    random divergence-free field of plasma beta ~ mhd_beta."""
+    random divergence-free field of plasma beta ~ mhd_beta.  ``fasteos``
+    False: no USE_FASTEOS (the Riemann solvers make a T(rho, P) Newton per
+    face state from the mean T of the two cells)."""
     ny = ny or n
     mhd = mhd_beta is not None
     cfg = PhlxConfig(sdims=sdims, nx1=n, nx2=ny, nx3=n if sdims == 3 else 1, nas=2,
@@ -354,7 +356,7 @@ def helm_slab(eos, table, n: int = 256, ny: Optional[int] = None, sdims: int = 3
                      bc=(BC_PERIODIC, BC_PERIODIC, BC_REFLECTIVE, BC_REFLECTIVE,
                          BC_PERIODIC, BC_PERIODIC),
                      use_wb=True, use_gravity=True, evolve_etot=True,
-                     eos=EOS_HELMHOLTZ, fasteos=True).validate()
+                     eos=EOS_HELMHOLTZ, fasteos=fasteos).validate()
     yc = centres(cfg, 1)
     yf = nodes(cfg, 1)
     dy = cfg.dx[1]

@@ -71,6 +71,9 @@ def test_from_opts_matches_reference_flags():
     dict(fasteos=True),                               # FASTEOS needs a complex EoS
     dict(use_wb=True),                                # WB needs gravity
     dict(nas=1),
+    dict(eos=C.EOS_HELMHOLTZ, nas=0, fasteos=True),   # Helmholtz carries Ye, 1/abar
+    dict(eos=C.EOS_HELMHOLTZ, nas=2, fasteos=False,   # per-face Newton: FP64 only
+         precision=C.PREC_MIXED),
 ])
 def test_validation_rejects(bad):
     with pytest.raises(C.ConfigError):

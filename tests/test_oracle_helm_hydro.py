@@ -1,6 +1,7 @@
-"""CPU pins of the oracle's Helmholtz FASTEOS hydro path (stratified slab,
-USE_WB + EVOLVE_ETOT + ADVECT_YE_IABAR; source/source.F90 :6279-6357,
-:7160-7162, :8497-8727, :13573-13660, :13801-13847).
+"""CPU pins of the oracle's Helmholtz hydro path (stratified slab, USE_WB +
+EVOLVE_ETOT + ADVECT_YE_IABAR; source/source.F90 :6279-6361, :7160-7162,
+:8488-8727, :13573-13660, :13801-13847, :15776-15780), with USE_FASTEOS and
+without it (a T(rho, P) Newton per Riemann face state).
 
 - conservation: with periodic x1/x3 and reflecting x2 walls the wall mass
   flux vanishes identically, so total mass, Ye mass and 1/abar mass are
@@ -36,8 +37,9 @@ def _run(oracle_mod, pb, nsteps):
     return st0, st
 
 
-def test_conservation(oracle_mod, table):
-    pb = P.helm_slab(oracle_mod.OracleHelm(table), table, n=10, sdims=3)
+@pytest.mark.parametrize("fasteos", [True, False])
+def test_conservation(oracle_mod, table, fasteos):
+    pb = P.helm_slab(oracle_mod.OracleHelm(table), table, n=10, sdims=3, fasteos=fasteos)
     st0, st = _run(oracle_mod, pb, 8)
     for name, f in (("mass", lambda s: s["prim"][0]),
                     ("Ye mass", lambda s: s["prim"][0] * s["prim"][5]),
@@ -46,8 +48,10 @@ def test_conservation(oracle_mod, table):
         assert abs(m1 / m0 - 1.0) < 1e-13, (name, m1 / m0 - 1.0)
 
 
-def test_well_balanced_rest(oracle_mod, table):
-    pb = P.helm_slab(oracle_mod.OracleHelm(table), table, n=8, ny=16, sdims=3, noise=0.0)
+@pytest.mark.parametrize("fasteos", [True, False])
+def test_well_balanced_rest(oracle_mod, table, fasteos):
+    pb = P.helm_slab(oracle_mod.OracleHelm(table), table, n=8, ny=16, sdims=3, noise=0.0,
+                     fasteos=fasteos)
     st0, st = _run(oracle_mod, pb, 10)
     v = np.abs(st["prim"][1:4]).max()
     # sound speed ~1e8 cm/s; a non-balanced scheme moves the gas at ~g dt
@@ -74,3 +78,21 @@ def test_init_and_gammas(oracle_mod, table):
     np.testing.assert_allclose(st["gammaf"][0], p / st["eint"] + 1.0, rtol=1e-15)
     np.testing.assert_allclose(st["gammaf"][1].ravel(), csf ** 2 * rho.ravel() / p.ravel(),
                                rtol=1e-12)
+
+
+def test_face_newton_close_to_fasteos(oracle_mod, table):
+    """Both EoS treatments of the face states describe the same gas: on the
+    perturbed slab they differ by the face-state EoS approximation only (the
+    FASTEOS gammas are reconstructed, the per-face Newton is exact), far below
+    the perturbation's own signal."""
+    orc = oracle_mod.OracleHelm(table)
+    a = P.helm_slab(orc, table, n=8, sdims=3, fasteos=True)
+    b = P.helm_slab(orc, table, n=8, sdims=3, fasteos=False)
+    _, sa = _run(oracle_mod, a, 4)
+    _, sb = _run(oracle_mod, b, 4)
+    for v in (0, 4, 5, 6):
+        d = np.abs(sa["prim"][v] - sb["prim"][v]).sum() / np.abs(sa["prim"][v]).sum()
+        assert d < 1e-6, (v, d)
+    # the velocities develop from the noise alike
+    va, vb = sa["prim"][2], sb["prim"][2]
+    assert np.abs(va - vb).sum() < 0.05 * np.abs(va).sum()
