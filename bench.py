@@ -119,6 +119,12 @@ def build_problem(args):
         data = ("synthetic (plane-parallel rho/T/abar profiles, seeded 1e-4 density noise"
                 f"{', seeded random-phase divergence-free field' if mhd else ''}; generated "
                 f"{table.nrho}x{table.nT} Helmholtz table)")
+    elif args.workload == "hotbubble":
+        log(f"building the 2D hot bubble {n}^2")
+        pb = problems.hot_bubble(n=n)
+        desc = (f"2D hot bubble, {n}^2, ideal gas gamma 5/3 + gravity, USE_WB, LHLLC + LIM2ND, "
+                f"RK2, CFL 0.8")
+        data = "synthetic (polytrope + Gaussian entropy bump, seeded 1e-6 velocity noise)"
     else:
         log(f"building the low-Mach heated box {n}^3 (Mach {args.mach})")
         pb = problems.lowmach_box(n=n, heating=args.heating, mach=args.mach)
@@ -243,7 +249,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=("ot3d", "helm_slab", "lowmach"), default="ot3d")
+    ap.add_argument("--workload", choices=("ot3d", "helm_slab", "lowmach", "hotbubble"),
+                    default="ot3d")
     ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64")
     ap.add_argument("--eos-guess", choices=("warm", "mlp"), default="warm",
                     help="Helmholtz: Newton start (carried T, or the trained surrogate)")

@@ -114,6 +114,13 @@ int phlx_download_gammaf(phlx_ctx *ctx, double *gammaf);
  * b_cc from the face field, eint and T from (rho, p), then
  * compute_hyperbolic_dt.  Sets time = 0, step = 0. */
 int phlx_init(phlx_ctx *ctx);
+/* Resume from a checkpoint, the read_restart path (source/source.F90
+ * :2761-2897) plus time_loop's step /= 0 branch (:6413-6500): after
+ * phlx_upload_state of the restart file's prim, T and face fields, rebuilds
+ * b_cc, eint (Helmholtz: a full evaluation at (rho, T); ideal: p/(gamma-1))
+ * and the FASTEOS gammas, and takes time, dt, smax and step as given (dt is
+ * not recomputed).  dt, smax > 0, step >= 0. */
+int phlx_resume(phlx_ctx *ctx, double time, double dt, double smax, int64_t step);
 /* hydro_step(mgrid, lgrid) (source/source.F90:6847-13701) with the current
  * dt; ghost zones are produced on the device (no halo exchange). */
 int phlx_hydro_step(phlx_ctx *ctx);

@@ -75,6 +75,9 @@ int orc_get_gammaf(const orc_grid *g, double *out);
 /* time_loop step-0 initialisation: b_cc from faces, eint/T from (rho,p),
  * compute_hyperbolic_dt (:6012-6062, :6271-6409). */
 int orc_init(orc_grid *g);
+/* Resume after read_restart: eint (+ FASTEOS gammas) rebuilt from (rho, T)
+ * or p, scalars from the file (:2761-2897, :6413-6500). */
+int orc_resume(orc_grid *g, double time, double dt, double smax, long step);
 /* nsteps passes of the time_loop body: dt clip, hydro_step, time += dt,
  * compute_hyperbolic_dt, step++ (:6662-6720). */
 int orc_run(orc_grid *g, int nsteps);

@@ -63,6 +63,8 @@ def load() -> ctypes.CDLL:
     lib.orc_set_helm.argtypes = [P, P]
     lib.orc_get_gammaf.argtypes = [P, P]
     lib.orc_init.argtypes = [P]
+    lib.orc_resume.argtypes = [P, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                               ctypes.c_long]
     lib.orc_run.argtypes = [P, ctypes.c_int]
     lib.orc_hydro_step.argtypes = [P]
     lib.orc_compute_dt.argtypes = [P]
@@ -157,6 +159,10 @@ class OracleRun:
 
     def init(self):
         self.lib.orc_init(self.g)
+
+    def resume(self, time: float, dt: float, smax: float, step: int):
+        """Continue from a restart file's state (already set) and scalars."""
+        self.lib.orc_resume(self.g, time, dt, smax, step)
 
     def run(self, n: int):
         self.lib.orc_run(self.g, int(n))

@@ -1475,6 +1475,43 @@ int orc_init(orc_grid *g) {
   return orc_compute_dt(g);
 }
 
+/* Resume from a restart file: read_restart (source.F90:2761-2897) restores
+ * prim, the face fields, temp and time, dt, smax, step; time_loop's step /= 0
+ * branch (:6413-6500) then rebuilds eint (and the FASTEOS gammas) from
+ * (rho, T) by a full Helmholtz evaluation (:6476-6483; the EoS pressure only
+ * feeds the gammas) or eint = p / (gamma - 1) (:6492); dt is the file's. */
+int orc_resume(orc_grid *g, double time, double dt, double smax, long step) {
+  const int nv = g->nv;
+  if (g->mhd) bcc_from_faces(g);
+  for (int k = 0; k < g->n[2]; k++)
+    for (int j = 0; j < g->n[1]; j++)
+      for (int i = 0; i < g->n[0]; i++) {
+        long c = cidx(g, i, j, k), q = iidx(g, i, j, k);
+        double rho = g->prim[c * nv], p = g->prim[c * nv + g->ip];
+        double eint;
+        if (g->c.eos == ORC_EOS_HELMHOLTZ) {
+          double abar = 1.0 / g->prim[c * nv + g->ias1 + 1];
+          double ye = g->prim[c * nv + g->ias1];
+          double zbar = abar * ye;
+          double T = g->temp[c], pe, sound, cv;
+          orc_helm_full(g->helm, rho, T, abar, zbar, &pe, &eint, &sound, &cv);
+          eint = rho * eint;
+          if (g->c.fasteos) {
+            g->gamf[c * 2] = pe / eint + 1.0;
+            g->gamf[c * 2 + 1] = sound * sound * rho / pe;
+          }
+        } else {
+          eint = p * g->igmm1;
+        }
+        g->eint[q] = eint;
+      }
+  g->time = time;
+  g->dt = dt;
+  g->smax = smax;
+  g->step = step;
+  return 0;
+}
+
 int orc_run(orc_grid *g, int nsteps) {
   for (int s = 0; s < nsteps; s++) {
     if (!(g->time < g->c.tmax)) break;                             /* :6525 */

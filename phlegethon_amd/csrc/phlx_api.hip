@@ -587,11 +587,32 @@ extern "C" int phlx_init(phlx_ctx *ctx) {
   s0.cfl = ctx->cfg.cfl;
   s0.active = 1;
   HIPCHK(hipMemcpyAsync(ctx->g.sc, &s0, sizeof(s0), hipMemcpyHostToDevice, ctx->stream));
-  ctx->kt.init(ctx->g, ctx->stream);
+  ctx->kt.init(ctx->g, 0, ctx->stream);
   int rc = check_launch();
   if (rc) return rc;
   ctx->initialized = true;
   return phlx_compute_dt(ctx, nullptr, nullptr);
+}
+
+extern "C" int phlx_resume(phlx_ctx *ctx, double time, double dt, double smax, int64_t step) {
+  if (!ctx || !(dt > 0.0) || !(smax > 0.0) || step < 0) return PHLX_E_ARG;
+  if (ctx->cfg.eos == PHLX_EOS_HELMHOLTZ && !ctx->helm_loaded) return PHLX_E_TABLE;
+  HIPCHK(hipSetDevice(ctx->device));
+  Scalars s0;
+  std::memset(&s0, 0, sizeof(s0));
+  s0.time = time;
+  s0.dt = dt;
+  s0.smax = smax;
+  s0.step = step;
+  s0.tmax = ctx->cfg.tmax;
+  s0.cfl = ctx->cfg.cfl;
+  s0.active = time < ctx->cfg.tmax ? 1 : 0;
+  HIPCHK(hipMemcpyAsync(ctx->g.sc, &s0, sizeof(s0), hipMemcpyHostToDevice, ctx->stream));
+  ctx->kt.init(ctx->g, 1, ctx->stream);
+  int rc = check_launch();
+  if (rc) return rc;
+  ctx->initialized = true;
+  return PHLX_OK;
 }
 
 extern "C" int phlx_compute_dt(phlx_ctx *ctx, double *dt_out, double *smax_out) {

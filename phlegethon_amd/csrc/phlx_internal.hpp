@@ -15,7 +15,7 @@ struct KernelTable {
   void (*update)(const Grid &, int, int, double, double, double, hipStream_t);
   void (*update_guess)(const Grid &, int, int, double, double, double, hipStream_t);
   void (*cfl)(const Grid &, hipStream_t);
-  void (*init)(const Grid &, hipStream_t);
+  void (*init)(const Grid &, int restart, hipStream_t);
   void (*divb)(const Grid &, hipStream_t);
 };
 

@@ -1054,13 +1054,29 @@ __global__ void __launch_bounds__(256) k_cfl(Grid g) {
 // Step-0 initialisation: b_cc from faces and the ideal-gas eint
 // (source/source.F90:6012-6062, :6373-6376).
 template <int SD, bool MHD, bool HELM>
-__global__ void __launch_bounds__(256) k_init(Grid g) {
+__global__ void __launch_bounds__(256) k_init(Grid g, int restart) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y * blockDim.y + threadIdx.y;
   const int k = blockIdx.z;
   if (i >= g.n1 || j >= g.n2) return;
   const long long N = g.N, c = cix(g, i, j, k);
-  if (HELM) {
+  if (HELM && restart) {
+    // resume: eint (+ gamma_e, gamma_c) from a full evaluation at the
+    // restored (rho, T); its pressure only feeds the gammas
+    // (source/source.F90:6413-6500)
+    const double rho = g.P[c];
+    const double abar = 1.0 / g.P[(SD + 3) * N + c];
+    const double ye = g.P[(SD + 2) * N + c];
+    const double zbar = abar * ye;
+    double pe, eint, sound, cv;
+    helm_full(g.helm, rho, g.temp[c], abar, zbar, pe, eint, sound, cv);
+    eint = rho * eint;
+    if (g.fasteos) {
+      g.gam[c] = pe / eint + 1.0;
+      g.gam[N + c] = sound * sound * rho / pe;
+    }
+    g.eint[c] = eint;
+  } else if (HELM) {
     // T(rho, p) Newton from the uploaded T, eint (+ gamma_e, gamma_c with
     // FASTEOS) (source/source.F90:6279-6285, :6351-6361)
     const double rho = g.P[c], p = g.P[(SD + 1) * N + c];
@@ -1233,9 +1249,9 @@ struct Kern {
     const dim3 blk(64, 4, 1);
     k_cfl<SD, MHD, HELM><<<grid_for(g.n1, g.n2, g.n3, blk), blk, 0, st>>>(g);
   }
-  static void init(const Grid &g, hipStream_t st) {
+  static void init(const Grid &g, int restart, hipStream_t st) {
     const dim3 blk(64, 4, 1);
-    k_init<SD, MHD, HELM><<<grid_for(g.n1, g.n2, g.n3, blk), blk, 0, st>>>(g);
+    k_init<SD, MHD, HELM><<<grid_for(g.n1, g.n2, g.n3, blk), blk, 0, st>>>(g, restart);
   }
   static void divb(const Grid &g, hipStream_t st) {
     const dim3 blk(64, 4, 1);

@@ -63,6 +63,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.phlx_synchronize.argtypes = [P]
     lib.phlx_get_scalars.argtypes = [P, DP, DP, DP, ctypes.POINTER(ctypes.c_int64)]
     lib.phlx_set_scalars.argtypes = [P, D, D, D, ctypes.c_int64]
+    lib.phlx_resume.argtypes = [P, D, D, D, ctypes.c_int64]
     lib.phlx_stage_begin.argtypes = [P, I]
     lib.phlx_sweep.argtypes = [P, I]
     lib.phlx_ct.argtypes = [P, I, D, D, D]
@@ -204,6 +205,23 @@ class Simulation:
     # -- the reference's entry points ------------------------------------
     def init(self):
         _chk(self.lib.phlx_init(self._ctx), "phlx_init")
+
+    def resume(self, time: float, dt: float, smax: float, step: int):
+        """Continue from a checkpoint (uploaded prim, T, faces) with its
+        scalars: the reference's read_restart + step /= 0 initialisation."""
+        _chk(self.lib.phlx_resume(self._ctx, float(time), float(dt), float(smax), int(step)),
+             "phlx_resume")
+
+    def checkpoint(self) -> Dict[str, np.ndarray]:
+        """What the reference's write_restart stores (source/source.F90
+        :2507-2616): prim, T, the face fields and time, dt, smax, step."""
+        st = self.download()
+        out = {"prim": st["prim"], "temp": st["temp"]}
+        for d in range(3):
+            if f"b_x{d+1}" in st:
+                out[f"b_x{d+1}"] = st[f"b_x{d+1}"]
+        out.update(self.scalars())
+        return out
 
     def hydro_step(self):
         _chk(self.lib.phlx_hydro_step(self._ctx), "phlx_hydro_step")
