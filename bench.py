@@ -343,10 +343,11 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        log(f"cpu baseline: oracle on {threads} threads, {args.cpu_steps} steps of {n}^3")
+        shape = "x".join(str(v) for v in reversed(pb.cfg.shape) if v > 1)
+        log(f"cpu baseline: oracle on {threads} threads, {args.cpu_steps} steps of {shape}")
         rate, wall = cpu_baseline(pb, args.cpu_steps, threads)
         cpu = {"value": rate, "unit": "zone-updates/s", "cores": threads, "kind": "port",
-               "sample": f"{args.cpu_steps} full steps of the same {n}^3 workload "
+               "sample": f"{args.cpu_steps} full steps of the same {shape} workload "
                          f"({wall:.1f} s wall), C FP64 restatement with OpenMP"}
         log(f"cpu baseline {rate:.4e} zone-updates/s")
 

@@ -14,6 +14,9 @@ module phlx_iface
                                PHLX_HLLD_FLUX = 2, PHLX_LHLLD_FLUX = 3
   integer(c_int), parameter :: PHLX_BC_PERIODIC = 0, PHLX_BC_REFLECTIVE = 1, &
                                PHLX_BC_OUTFLOW = 2, PHLX_BC_DIODE = 3
+  integer(c_int), parameter :: PHLX_EOS_IDEAL = 0, PHLX_EOS_HELMHOLTZ = 1
+  integer(c_int), parameter :: PHLX_PREC_FP64 = 0, PHLX_PREC_MIXED = 1
+  integer(c_int), parameter :: PHLX_GUESS_WARM = 0, PHLX_GUESS_MLP = 1, PHLX_GUESS_BRACKET = 2
 
   type, bind(c) :: phlx_config
     integer(c_int32_t) :: sdims, nx1, nx2, nx3, nas
@@ -94,6 +97,101 @@ module phlx_iface
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: ctx
       real(c_double), intent(out) :: divb
+    end function
+    integer(c_int) function phlx_download_state(ctx, prim, eint, temp, b1, b2, b3, bcc) &
+        bind(c, name="phlx_download_state")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, prim, eint, temp, b1, b2, b3, bcc
+    end function
+    ! restart: upload the restart file's prim, temp, faces, then resume with
+    ! its time, dt, smax, step (read_restart + time_loop's step /= 0 branch)
+    integer(c_int) function phlx_resume(ctx, time, dt, smax, step) bind(c, name="phlx_resume")
+      import :: c_int, c_ptr, c_double, c_int64_t
+      type(c_ptr), value :: ctx
+      real(c_double), value :: time, dt, smax
+      integer(c_int64_t), value :: step
+    end function
+    ! ---- Helmholtz EoS (load_helm_table and the per-zone / batched calls)
+    ! f9 = helm_table_var(9, nrho, nT), c4 = dpdrho_table_var(4, nrho, nT)
+    integer(c_int) function phlx_load_helm_table(ctx, f9, c4, nrho, nt, ldlo, ldhi, ltlo, lthi) &
+        bind(c, name="phlx_load_helm_table")
+      import :: c_int, c_ptr, c_int32_t, c_double
+      type(c_ptr), value :: ctx, f9, c4
+      integer(c_int32_t), value :: nrho, nt
+      real(c_double), value :: ldlo, ldhi, ltlo, lthi
+    end function
+    integer(c_int) function phlx_set_eos_profiles(ctx, eq_gammae_cc, eq_gammae_f) &
+        bind(c, name="phlx_set_eos_profiles")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, eq_gammae_cc, eq_gammae_f
+    end function
+    integer(c_int) function phlx_download_gammaf(ctx, gammaf) bind(c, name="phlx_download_gammaf")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, gammaf
+    end function
+    ! batched helm_rhoT_given_full / helm_rhoe_given / helm_rhoP_given on
+    ! device arrays of n points
+    integer(c_int) function phlx_helm_rhoT_full_batch(ctx, n, rho, t, abar, zbar, p, e, sound, cv) &
+        bind(c, name="phlx_helm_rhoT_full_batch")
+      import :: c_int, c_ptr, c_int64_t
+      type(c_ptr), value :: ctx, rho, t, abar, zbar, p, e, sound, cv
+      integer(c_int64_t), value :: n
+    end function
+    integer(c_int) function phlx_helm_rhoe_given_batch(ctx, n, rho, e, abar, zbar, t, p, sound, &
+        return_sound, iters) bind(c, name="phlx_helm_rhoe_given_batch")
+      import :: c_int, c_ptr, c_int64_t, c_int32_t
+      type(c_ptr), value :: ctx, rho, e, abar, zbar, t, p, sound, iters
+      integer(c_int64_t), value :: n
+      integer(c_int32_t), value :: return_sound
+    end function
+    integer(c_int) function phlx_helm_rhoP_given_batch(ctx, n, rho, p, abar, zbar, t, e, sound, &
+        return_sound, iters) bind(c, name="phlx_helm_rhoP_given_batch")
+      import :: c_int, c_ptr, c_int64_t, c_int32_t
+      type(c_ptr), value :: ctx, rho, p, abar, zbar, t, e, sound, iters
+      integer(c_int64_t), value :: n
+      integer(c_int32_t), value :: return_sound
+    end function
+    integer(c_int) function phlx_helm_rhoe_given_guess_batch(ctx, n, rho, e, abar, zbar, t, p, &
+        sound, return_sound, guess_mode, iters, fallback) &
+        bind(c, name="phlx_helm_rhoe_given_guess_batch")
+      import :: c_int, c_ptr, c_int64_t, c_int32_t
+      type(c_ptr), value :: ctx, rho, e, abar, zbar, t, p, sound, iters, fallback
+      integer(c_int64_t), value :: n
+      integer(c_int32_t), value :: return_sound, guess_mode
+    end function
+    ! ---- T(rho, e) start inside the update, surrogate, mixed-precision EoS step
+    integer(c_int) function phlx_set_eos_guess(ctx, mode, resid_tol, count_iterations) &
+        bind(c, name="phlx_set_eos_guess")
+      import :: c_int, c_ptr, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: mode, count_iterations
+      real(c_double), value :: resid_tol
+    end function
+    integer(c_int) function phlx_set_eos_lin_tol(ctx, lin_tol) bind(c, name="phlx_set_eos_lin_tol")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: lin_tol
+    end function
+    integer(c_int) function phlx_eos_stats(ctx, newton_iters, solves, fallbacks, reset) &
+        bind(c, name="phlx_eos_stats")
+      import :: c_int, c_ptr, c_int64_t, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), intent(out) :: newton_iters, solves, fallbacks
+      integer(c_int32_t), value :: reset
+    end function
+    ! box(8): log10(rho*Ye) lo, hi, log10 T lo, hi, abar lo, hi, Ye lo, hi
+    integer(c_int) function phlx_mlp_set_train_box(ctx, box) bind(c, name="phlx_mlp_set_train_box")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, box
+    end function
+    integer(c_int) function phlx_mlp_train(ctx, nsamples, epochs, batch, lr, seed, train_loss, &
+        val_rms_logt, val_rms_loge) bind(c, name="phlx_mlp_train")
+      import :: c_int, c_ptr, c_int64_t, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), value :: nsamples, seed
+      integer(c_int32_t), value :: epochs, batch
+      real(c_double), value :: lr
+      real(c_double), intent(out) :: train_loss, val_rms_logt, val_rms_loge
     end function
   end interface
 end module phlx_iface
