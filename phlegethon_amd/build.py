@@ -42,5 +42,31 @@ def build(verbose: bool = False) -> str:
     return OUT
 
 
+H5_OUT = os.path.join(OUT_DIR, "libphlx_h5.so")
+H5_PREFIX = os.environ.get("PHLX_HDF5_PREFIX", "/opt/conda")
+
+
+def build_h5(verbose: bool = False):
+    """Host-only library of the reference-layout HDF5 output (phlx_h5.cpp),
+    built with g++ against the image's HDF5 1.10; None when HDF5 is absent
+    (the output module then reports it at use)."""
+    inc = os.path.join(H5_PREFIX, "include")
+    lib = os.path.join(H5_PREFIX, "lib")
+    if not os.path.exists(os.path.join(inc, "hdf5.h")):
+        return None
+    os.makedirs(OUT_DIR, exist_ok=True)
+    tmp = H5_OUT + ".tmp"
+    cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", f"-I{inc}", "-o", tmp,
+           os.path.join(CSRC, "phlx_h5.cpp"), f"-L{lib}", "-lhdf5", f"-Wl,-rpath,{lib}"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("g++ failed building libphlx_h5.so")
+    if verbose:
+        sys.stderr.write(r.stderr)
+    os.replace(tmp, H5_OUT)
+    return H5_OUT
+
+
 if __name__ == "__main__":
     print(build(verbose=True))

@@ -223,6 +223,13 @@ class Simulation:
         out.update(self.scalars())
         return out
 
+    def write_output(self, directory: str, profiles=None) -> str:
+        """write_output of the reference: <directory>/grid_nNNNNN.h5 in its
+        HDF5 layout (phlegethon_amd.output); ``profiles``: the problem's x2
+        profiles for the step-0 statics (grav, phi_cc, edot)."""
+        from . import output
+        return output.write_grid(directory, self.cfg, self.scalars(), self.download(), profiles)
+
     def hydro_step(self):
         _chk(self.lib.phlx_hydro_step(self._ctx), "phlx_hydro_step")
 
