@@ -247,6 +247,16 @@ int phlx_set_eos_guess(phlx_ctx *ctx, int32_t mode, double resid_tol, int32_t co
 int phlx_set_eos_lin_tol(phlx_ctx *ctx, double lin_tol);
 int phlx_eos_stats(phlx_ctx *ctx, int64_t *newton_iters, int64_t *solves, int64_t *fallbacks,
                    int32_t reset);
+/* USE_SHOCK_FLATTENING (a compile-time OPTS switch with eps_sf_make in the
+ * reference; source/source.F90:8073-8131 detection, :8731-8740 and
+ * :8743-8869 use).  At the start of every stage a cell whose summed relative
+ * pressure jump sum_d |p(c+e_d) - p(c-e_d)| / p(c) exceeds eps_sf (the
+ * reference's default 0.25) flags its 3^sdims neighbourhood; a flagged cell
+ * is reconstructed with the LIM2ND slope, and a face beside one takes the
+ * two-wave HLL flux (HLLC :15929-16003, HLLD :14386-14486).  enable 0 turns
+ * it off.  FP64 contexts only (PHLX_E_CONFIG otherwise), and every axis
+ * holds at least 3 cells.  May be called before or after phlx_init. */
+int phlx_set_shock_flattening(phlx_ctx *ctx, int32_t enable, double eps_sf);
 
 /* Diagnostics: max over cells of |sum_d (b_d(+) - b_d(-))/dx_d|. */
 int phlx_divb_max(phlx_ctx *ctx, double *out);

@@ -179,6 +179,14 @@ module phlx_iface
       integer(c_int64_t), intent(out) :: newton_iters, solves, fallbacks
       integer(c_int32_t), value :: reset
     end function
+    ! USE_SHOCK_FLATTENING with eps_sf_make (FP64 contexts)
+    integer(c_int) function phlx_set_shock_flattening(ctx, enable, eps_sf) &
+        bind(c, name="phlx_set_shock_flattening")
+      import :: c_int, c_ptr, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: enable
+      real(c_double), value :: eps_sf
+    end function
     ! box(8): log10(rho*Ye) lo, hi, log10 T lo, hi, abar lo, hi, Ye lo, hi
     integer(c_int) function phlx_mlp_set_train_box(ctx, box) bind(c, name="phlx_mlp_set_train_box")
       import :: c_int, c_ptr

@@ -41,6 +41,8 @@ typedef struct {
   int use_wb, use_gravity, evolve_etot, use_edot;
   int eos, fasteos;
   int nthreads;           /* OpenMP threads for the baseline (0 = default) */
+  int shock_flattening;   /* USE_SHOCK_FLATTENING */
+  double eps_sf;          /* eps_sf_make: relative pressure jump that flattens */
 } orc_config;
 
 typedef struct orc_grid orc_grid;
@@ -77,6 +79,9 @@ int orc_get_gammaf(const orc_grid *g, double *out);
 int orc_init(orc_grid *g);
 /* Resume after read_restart: eint (+ FASTEOS gammas) rebuilt from (rho, T)
  * or p, scalars from the file (:2761-2897, :6413-6500). */
+/* USE_SHOCK_FLATTENING flags of the current state on cells -1..n per axis
+ * (x3 only in 3D), x1 fastest; -1 when flattening is off. */
+int orc_shock_flags(orc_grid *g, unsigned char *out);
 int orc_resume(orc_grid *g, double time, double dt, double smax, long step);
 /* nsteps passes of the time_loop body: dt clip, hydro_step, time += dt,
  * compute_hyperbolic_dt, step++ (:6662-6720). */

@@ -34,6 +34,7 @@ class OrcConfig(ctypes.Structure):
         ("evolve_etot", ctypes.c_int), ("use_edot", ctypes.c_int),
         ("eos", ctypes.c_int), ("fasteos", ctypes.c_int),
         ("nthreads", ctypes.c_int),
+        ("shock_flattening", ctypes.c_int), ("eps_sf", ctypes.c_double),
     ]
 
 
@@ -66,6 +67,7 @@ def load() -> ctypes.CDLL:
     lib.orc_resume.argtypes = [P, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                ctypes.c_long]
     lib.orc_run.argtypes = [P, ctypes.c_int]
+    lib.orc_shock_flags.argtypes = [P, P]
     lib.orc_hydro_step.argtypes = [P]
     lib.orc_compute_dt.argtypes = [P]
     lib.orc_get_scalar.restype = ctypes.c_double
@@ -166,6 +168,18 @@ class OracleRun:
 
     def run(self, n: int):
         self.lib.orc_run(self.g, int(n))
+
+    def shock_flags(self) -> np.ndarray:
+        """USE_SHOCK_FLATTENING flags of the current state on cells -1..n of
+        each axis, shape (nz, ny, nx) with the ghost layer (flag[.., 1, 1] is
+        cell (0, 0))."""
+        cfg = self.cfg
+        nz = cfg.nx3 + 2 if cfg.sdims == 3 else 1
+        out = np.zeros((nz, cfg.nx2 + 2, cfg.nx1 + 2), dtype=np.uint8)
+        rc = self.lib.orc_shock_flags(self.g, out.ctypes.data_as(ctypes.c_void_p))
+        if rc != 0:
+            raise ValueError("shock flattening is off")
+        return out
 
     def hydro_step(self):
         self.lib.orc_hydro_step(self.g)

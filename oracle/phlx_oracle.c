@@ -39,6 +39,7 @@ struct orc_grid {
   double dx[3], inv_dx[3];
   double gm, gmm1, igmm1, inv_mu;
   double *prim, *bcc, *temp, *gamf; /* ghosted, AoS (nv / 3 / 1 / 2 per cell) */
+  unsigned char *flat;               /* ghosted: USE_SHOCK_FLATTENING cell flags */
   double *eint;                     /* interior */
   double *state, *state0, *res;     /* interior AoS [cell][nv] */
   double *bf[3], *b0f[3];           /* face B, interior faces */
@@ -85,7 +86,10 @@ orc_grid *orc_create(const orc_config *cfg) {
   g->c = *cfg;
   g->sd = cfg->sdims;
   g->nv = cfg->sdims + 2 + cfg->nas;            /* source.F90:472-477 */
-  g->ng = (cfg->recon == ORC_LIM5TH) ? 3 : 2; /* LIM5TH reads i +- 2 of cells lx1-1..ux1+1 */
+  /* LIM5TH reads i +- 2 of cells lx1-1..ux1+1; the shock detector reads
+   * i +- 1 of cells lx1-2..ux1+2 (with the reference's ngc = 2 its outer
+   * ring reads past the ghosts; here the ghosts are filled one deeper) */
+  g->ng = (cfg->recon == ORC_LIM5TH || cfg->shock_flattening) ? 3 : 2;
   g->mhd = (cfg->flux == ORC_HLLD || cfg->flux == ORC_LHLLD);
   g->n[0] = cfg->nx1;
   g->n[1] = cfg->nx2;
@@ -115,6 +119,7 @@ orc_grid *orc_create(const orc_config *cfg) {
   g->bcc = (double *)calloc(nc * 3, sizeof(double));
   g->temp = (double *)calloc(nc, sizeof(double));
   g->gamf = (double *)calloc(nc * 2, sizeof(double));
+  g->flat = (unsigned char *)calloc(nc, 1);
   g->eint = (double *)calloc(ni, sizeof(double));
   g->state = (double *)calloc(ni * g->nv, sizeof(double));
   g->state0 = (double *)calloc(ni * g->nv, sizeof(double));
@@ -150,7 +155,7 @@ orc_grid *orc_create(const orc_config *cfg) {
 
 void orc_destroy(orc_grid *g) {
   if (!g) return;
-  free(g->prim); free(g->bcc); free(g->temp); free(g->gamf); free(g->eint);
+  free(g->prim); free(g->bcc); free(g->temp); free(g->gamf); free(g->eint); free(g->flat);
   free(g->state); free(g->state0); free(g->res);
   for (int d = 0; d < 3; d++) {
     free(g->bf[d]); free(g->b0f[d]); free(g->emf[d]); free(g->sgn[d]); free(g->cor[d]);
@@ -548,7 +553,8 @@ static rparams make_rp(const orc_config *c) {
 /* HLLC / LHLLC, Cartesian (source.F90:15611-16260). q = (rho, v1, v2, [v3],
  * p, scalars); gL/gR = (gamma_e, gamma_c) face values for FASTEOS. */
 static void riemann_hydro_face(const rparams *P, const double *qL, const double *qR,
-                               const double *gL, const double *gR, double Tg, double *flux) {
+                               const double *gL, const double *gR, double Tg, int hll,
+                               double *flux) {
   const int sd = P->sd, nv = P->nv;
   const int im1 = 1, im2 = 2, im3 = (sd == 3) ? 3 : 2, ie = im3 + 1, ia = 2 + sd;
   double rhoL = qL[0], inv_rhoL = 1.0 / rhoL;
@@ -588,6 +594,33 @@ static void riemann_hydro_face(const rparams *P, const double *qL, const double 
   double rhoestarL = rhostarL * (rhoeL * inv_rhoL + (ustar - vx1L) * (ustar + pL * inv_rhoL / dsuL));
   double rhostarR = rhoR * (dsuR / (sR - ustar));
   double rhoestarR = rhostarR * (rhoeR * inv_rhoR + (ustar - vx1R) * (ustar + pR * inv_rhoR / dsuR));
+
+  if (hll) {
+    /* USE_SHOCK_FLATTENING: two-wave HLL flux on a flattened face
+     * (source.F90:15929-16003) */
+    double UL[16], UR[16], fL[16], fR[16];
+    sL = fmin(0.0, sL);
+    sR = fmax(0.0, sR);
+    UL[0] = rhoL; UL[im1] = rhoL * vx1L; UL[im2] = rhoL * vx2L;
+    if (sd == 3) UL[im3] = rhoL * vx3L;
+    UL[ie] = rhoeL;
+    for (int v = ia; v < nv; v++) UL[v] = rhoL * qL[v];
+    UR[0] = rhoR; UR[im1] = rhoR * vx1R; UR[im2] = rhoR * vx2R;
+    if (sd == 3) UR[im3] = rhoR * vx3R;
+    UR[ie] = rhoeR;
+    for (int v = ia; v < nv; v++) UR[v] = rhoR * qR[v];
+    fL[0] = rhoL * vx1L; fL[im1] = rhoL * vx1L * vx1L + pL; fL[im2] = rhoL * vx1L * vx2L;
+    if (sd == 3) fL[im3] = rhoL * vx1L * vx3L;
+    fL[ie] = (rhoeL + pL) * vx1L;
+    for (int v = ia; v < nv; v++) fL[v] = UL[v] * vx1L;
+    fR[0] = rhoR * vx1R; fR[im1] = rhoR * vx1R * vx1R + pR; fR[im2] = rhoR * vx1R * vx2R;
+    if (sd == 3) fR[im3] = rhoR * vx1R * vx3R;
+    fR[ie] = (rhoeR + pR) * vx1R;
+    for (int v = ia; v < nv; v++) fR[v] = UR[v] * vx1R;
+    double dummy = 1.0 / (sR - sL), phi = sL * sR;
+    for (int v = 0; v < nv; v++) flux[v] = dummy * (sR * fL[v] - sL * fR[v] + phi * (UR[v] - UL[v]));
+    return;
+  }
 
   if (P->flux == ORC_LHLLC) {
     /* Minoshima & Miyoshi low-dissipation pressure (source.F90:16014-16030) */
@@ -668,7 +701,8 @@ static void riemann_hydro_face(const rparams *P, const double *qL, const double 
 /* HLLD / LHLLD (source.F90:13969-15024).  b = (b_normal, b_t1, [b_t2]). */
 static void riemann_mhd_face(const rparams *P, const double *qL, const double *qR,
                              const double *bL, const double *bR, const double *gL,
-                             const double *gR, double Tg, double *flux, double *fluxb) {
+                             const double *gR, double Tg, int hll, double *flux,
+                             double *fluxb) {
   const int sd = P->sd, nv = P->nv;
   const int im1 = 1, im2 = 2, im3 = (sd == 3) ? 3 : 2, ie = im3 + 1, ia = 2 + sd;
   double rhoL = qL[0], inv_rhoL = 1.0 / rhoL;
@@ -741,6 +775,42 @@ static void riemann_mhd_face(const rparams *P, const double *qL, const double *q
     } else {
       phi = 1.0;
     }
+  }
+  if (hll) {
+    /* USE_SHOCK_FLATTENING: two-wave HLL flux on a flattened face
+     * (source.F90:14386-14486) */
+    double UL[16], UR[16], fL[16], fR[16], UbL[3], UbR[3], fbL[3], fbR[3];
+    sL = fmin(0.0, sL);
+    sR = fmax(0.0, sR);
+    UL[0] = rhoL; UL[im1] = rhoL * vx1L; UL[im2] = rhoL * vx2L;
+    if (sd == 3) UL[im3] = rhoL * vx3L;
+    UL[ie] = rhoeL;
+    for (int v = ia; v < nv; v++) UL[v] = rhoL * qL[v];
+    UbL[0] = bx1; UbL[1] = bx2L; UbL[2] = bx3L;
+    UR[0] = rhoR; UR[im1] = rhoR * vx1R; UR[im2] = rhoR * vx2R;
+    if (sd == 3) UR[im3] = rhoR * vx3R;
+    UR[ie] = rhoeR;
+    for (int v = ia; v < nv; v++) UR[v] = rhoR * qR[v];
+    UbR[0] = bx1; UbR[1] = bx2R; UbR[2] = bx3R;
+    fL[0] = rhoL * vx1L;
+    fL[im1] = rhoL * vx1L * vx1L + ptL - bx1_2;
+    fL[im2] = rhoL * vx1L * vx2L - bx1 * bx2L;
+    if (sd == 3) fL[im3] = rhoL * vx1L * vx3L - bx1 * bx3L;
+    fL[ie] = (rhoeL + ptL) * vx1L - bx1 * (bx1 * vx1L + bx2L * vx2L + bx3L * vx3L);
+    for (int v = ia; v < nv; v++) fL[v] = UL[v] * vx1L;
+    fbL[0] = 0.0; fbL[1] = vx1L * bx2L - vx2L * bx1; fbL[2] = vx1L * bx3L - vx3L * bx1;
+    fR[0] = rhoR * vx1R;
+    fR[im1] = rhoR * vx1R * vx1R + ptR - bx1_2;
+    fR[im2] = rhoR * vx1R * vx2R - bx1 * bx2R;
+    if (sd == 3) fR[im3] = rhoR * vx1R * vx3R - bx1 * bx3R;
+    fR[ie] = (rhoeR + ptR) * vx1R - bx1 * (bx1 * vx1R + bx2R * vx2R + bx3R * vx3R);
+    for (int v = ia; v < nv; v++) fR[v] = UR[v] * vx1R;
+    fbR[0] = 0.0; fbR[1] = vx1R * bx2R - vx2R * bx1; fbR[2] = vx1R * bx3R - vx3R * bx1;
+    double dm = 1.0 / (sR - sL), ph = sL * sR;
+    for (int v = 0; v < nv; v++) flux[v] = dm * (sR * fL[v] - sL * fR[v] + ph * (UR[v] - UL[v]));
+    for (int v = 0; v < sd; v++)
+      fluxb[v] = dm * (sR * fbL[v] - sL * fbR[v] + ph * (UbR[v] - UbL[v]));
+    return;
   }
   double pstar = (dsuR * rhoR * ptL - dsuL * rhoL * ptR - phi * rhoL * rhoR * dsuR * dsuL * (vx1L - vx1R)) * inv_temp1;
   double rhostarL = rhoL * dsuL / dsumL;
@@ -887,7 +957,7 @@ void orc_riemann_hydro(const orc_config *cfg, int n, const double *qL,
                        const double *qR, double *flux) {
   rparams P = make_rp(cfg);
   for (int i = 0; i < n; i++)
-    riemann_hydro_face(&P, qL + (long)i * P.nv, qR + (long)i * P.nv, NULL, NULL, 0.0,
+    riemann_hydro_face(&P, qL + (long)i * P.nv, qR + (long)i * P.nv, NULL, NULL, 0.0, 0,
                        flux + (long)i * P.nv);
 }
 
@@ -897,11 +967,106 @@ void orc_riemann_mhd(const orc_config *cfg, int n, const double *qL,
   rparams P = make_rp(cfg);
   for (int i = 0; i < n; i++)
     riemann_mhd_face(&P, qL + (long)i * P.nv, qR + (long)i * P.nv, bL + (long)i * P.sd,
-                     bR + (long)i * P.sd, NULL, NULL, 0.0, flux + (long)i * P.nv,
+                     bR + (long)i * P.sd, NULL, NULL, 0.0, 0, flux + (long)i * P.nv,
                      fluxb + (long)i * P.sd);
 }
 
 /* -------------------------------------------------------------- sweeps */
+
+/* Equilibrium pressure of a (possibly ghost) x2 row: the row the boundary
+ * map sends it to (periodic wrap or mirror). */
+static double eq_p_row(const orc_grid *g, int j) {
+  const int n = g->n[1];
+  if (j < 0) j = (g->c.bc[2] == ORC_BC_PERIODIC) ? j + n : -1 - j;
+  else if (j >= n) j = (g->c.bc[3] == ORC_BC_PERIODIC) ? j - n : 2 * n - 1 - j;
+  return g->eq_p_cc[j];
+}
+
+/* USE_SHOCK_FLATTENING detector (source.F90:8073-8131): a cell whose summed
+ * relative pressure jump sum_d |p(c+e_d) - p(c-e_d)| / p(c) exceeds eps_sf
+ * flags itself and its 3^sdims neighbourhood; flags cover cells -1..n.  With
+ * USE_WB the full pressure is prim + eq; the equilibrium of a ghost row is
+ * that of the row its boundary map copies (parity unpinned there: the
+ * reference's ghost-row equilibrium comes from the application). */
+static void shock_flags(orc_grid *g) {
+  const int sd = g->sd, nv = g->nv, ip = g->ip;
+  const int n1 = g->n[0], n2 = g->n[1], n3 = g->n[2];
+  const int k0 = (sd == 3) ? -1 : 0, k1 = (sd == 3) ? n3 : 0;
+  for (int k = k0; k <= k1; k++)
+    for (int j = -1; j <= n2; j++)
+      for (int i = -1; i <= n1; i++) g->flat[cidx(g, i, j, k)] = 0;
+  const int kd0 = (sd == 3) ? -2 : 0, kd1 = (sd == 3) ? n3 + 1 : 0;
+  const int wb = g->c.use_wb;
+  for (int k = kd0; k <= kd1; k++)
+    for (int j = -2; j <= n2 + 1; j++)
+      for (int i = -2; i <= n1 + 1; i++) {
+        double pc = g->prim[cidx(g, i, j, k) * nv + ip];
+        double pjp = g->prim[cidx(g, i, j + 1, k) * nv + ip];
+        double pjm = g->prim[cidx(g, i, j - 1, k) * nv + ip];
+        double pip = g->prim[cidx(g, i + 1, j, k) * nv + ip];
+        double pim = g->prim[cidx(g, i - 1, j, k) * nv + ip];
+        double pkp = 0.0, pkm = 0.0;
+        if (sd == 3) {
+          pkp = g->prim[cidx(g, i, j, k + 1) * nv + ip];
+          pkm = g->prim[cidx(g, i, j, k - 1) * nv + ip];
+        }
+        if (wb) {
+          const double e = eq_p_row(g, j);
+          pc = pc + e;
+          pjp = pjp + eq_p_row(g, j + 1);
+          pjm = pjm + eq_p_row(g, j - 1);
+          pip = pip + e;
+          pim = pim + e;
+          if (sd == 3) {
+            pkp = pkp + e;
+            pkm = pkm + e;
+          }
+        }
+        double tmp;
+        if (sd == 3)
+          tmp = (fabs(pkp - pkm) + fabs(pjp - pjm) + fabs(pip - pim)) / pc;
+        else
+          tmp = (fabs(pjp - pjm) + fabs(pip - pim)) / pc;
+        if (tmp > g->c.eps_sf) {
+          for (int dk = (sd == 3 ? -1 : 0); dk <= (sd == 3 ? 1 : 0); dk++)
+            for (int dj = -1; dj <= 1; dj++)
+              for (int di = -1; di <= 1; di++) g->flat[cidx(g, i + di, j + dj, k + dk)] = 1;
+        }
+      }
+}
+
+/* The shock-flattening flags of the current primitive state, as a stage
+ * computes them (WB deviation, halos, boundaries, detector), written to out
+ * as the flag box of cells -1..n on each axis (x3 only in 3D), x1 fastest.
+ * The primitive array is restored afterwards.  Test accessor. */
+int orc_shock_flags(orc_grid *g, unsigned char *out) {
+  if (!g->c.shock_flattening) return -1;
+  const int sd = g->sd, nv = g->nv, ip = g->ip;
+  const int n1 = g->n[0], n2 = g->n[1], n3 = g->n[2];
+  const size_t bytes = sizeof(double) * (size_t)g->e1 * g->e2 * g->e3 * nv;
+  double *save = (double *)malloc(bytes);
+  if (!save) return -2;
+  memcpy(save, g->prim, bytes);
+  if (g->c.use_wb)
+    for (int k = 0; k < n3; k++)
+      for (int j = 0; j < n2; j++)
+        for (int i = 0; i < n1; i++) {
+          double *p = g->prim + cidx(g, i, j, k) * nv;
+          p[0] = p[0] - g->eq_rho_cc[j];
+          p[ip] = p[ip] - g->eq_p_cc[j];
+        }
+  halo_periodic(g, g->prim, nv);
+  physical_bc(g);
+  shock_flags(g);
+  const int k0 = (sd == 3) ? -1 : 0, k1 = (sd == 3) ? n3 : 0;
+  long o = 0;
+  for (int k = k0; k <= k1; k++)
+    for (int j = -1; j <= n2; j++)
+      for (int i = -1; i <= n1; i++) out[o++] = g->flat[cidx(g, i, j, k)];
+  memcpy(g->prim, save, bytes);
+  free(save);
+  return 0;
+}
 
 /* One directional sweep over all interior pencils (x1 :7960-9105, x2
  * :9107-10301, x3 :10303-11364): reconstruct, add the face equilibrium
@@ -945,14 +1110,18 @@ static void sweep(orc_grid *g, int d) {
       for (int c = -1; c <= n; c++) {
         long cm = c0 + (long)(c - 1) * stride, cc = c0 + (long)c * stride, cp = c0 + (long)(c + 1) * stride;
         /* LIM5TH: two more cells (ng = 3); the indices stay inside the ghosts */
-        const int r5 = g->c.recon == ORC_LIM5TH;
+        /* a flattened cell takes the LIM2ND slope whatever the scheme
+         * (x1 :8743-8869, x2 :9635-9761, x3 :10765-10891) */
+        const int fl = g->c.shock_flattening && g->flat[cc];
+        const int r5 = g->c.recon == ORC_LIM5TH && !fl;
+        const int l2 = g->c.recon == ORC_LIM2ND || fl;
         long cmm = r5 ? c0 + (long)(c - 2) * stride : cm, cpp = r5 ? c0 + (long)(c + 2) * stride : cp;
         double l, r;
         for (int v = 0; v < nv; v++) {
           double qm = pr[cm * nv + v], qc = pr[cc * nv + v], qp = pr[cp * nv + v];
           if (r5)
             rec_lim5(pr[cmm * nv + v], qm, qc, qp, pr[cpp * nv + v], &l, &r);
-          else if (g->c.recon == ORC_LIM2ND)
+          else if (l2)
             rec_lim2nd(qm, qc, qp, &l, &r);
           else if (v <= ip)
             rec_pph(qm, qc, qp, &l, &r);
@@ -967,7 +1136,7 @@ static void sweep(orc_grid *g, int d) {
             double qm = bc[cm * 3 + comp], qc = bc[cc * 3 + comp], qp = bc[cp * 3 + comp];
             if (r5)
               rec_lim5(bc[cmm * 3 + comp], qm, qc, qp, bc[cpp * 3 + comp], &l, &r);
-            else if (g->c.recon == ORC_LIM2ND)
+            else if (l2)
               rec_lim2nd(qm, qc, qp, &l, &r);
             else
               rec_pph(qm, qc, qp, &l, &r);
@@ -980,7 +1149,7 @@ static void sweep(orc_grid *g, int d) {
             double qm = gf[cm * 2 + comp], qc = gf[cc * 2 + comp], qp = gf[cp * 2 + comp];
             if (r5)
               rec_lim5(gf[cmm * 2 + comp], qm, qc, qp, gf[cpp * 2 + comp], &l, &r);
-            else if (g->c.recon == ORC_LIM2ND)
+            else if (l2)
               rec_lim2nd(qm, qc, qp, &l, &r);
             else
               rec_pph(qm, qc, qp, &l, &r);
@@ -1049,13 +1218,16 @@ static void sweep(orc_grid *g, int d) {
         double Tg = 0.0;
         if (helm_face)
           Tg = 0.5 * (g->temp[c0 + (long)(f - 1) * stride] + g->temp[c0 + (long)f * stride]);
+        /* a face beside a flattened cell takes the HLL flux (x1 :8731-8740) */
+        const int hll = g->c.shock_flattening &&
+                        (g->flat[c0 + (long)(f - 1) * stride] || g->flat[c0 + (long)f * stride]);
         if (mhd)
           riemann_mhd_face(&P, qL + (long)f * nv, qR + (long)f * nv, bL + (long)f * 3,
-                           bR + (long)f * 3, gL + (long)f * 2, gR + (long)f * 2, Tg,
+                           bR + (long)f * 3, gL + (long)f * 2, gR + (long)f * 2, Tg, hll,
                            fx + (long)f * nv, fb + (long)f * 3);
         else
           riemann_hydro_face(&P, qL + (long)f * nv, qR + (long)f * nv, gL + (long)f * 2,
-                             gR + (long)f * 2, Tg, fx + (long)f * nv);
+                             gR + (long)f * 2, Tg, hll, fx + (long)f * nv);
       }
       for (int f = 0; f <= n; f++) {
         double *F = fx + (long)f * nv, *FB = fb + (long)f * 3;
@@ -1331,6 +1503,7 @@ int orc_hydro_step(orc_grid *g) {
     if (g->c.eos == ORC_EOS_HELMHOLTZ) halo_periodic(g, g->temp, 1);
     if (g->c.fasteos) halo_periodic(g, g->gamf, 2);
     physical_bc(g);
+    if (g->c.shock_flattening) shock_flags(g);
     for (int d = 0; d < sd; d++) sweep(g, d);
     double a1 = rk[irk][0], a2 = rk[irk][1], a3 = rk[irk][2] * g->dt;
     if (mhd) {

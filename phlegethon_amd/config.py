@@ -90,6 +90,10 @@ class PhlxConfig:
     fasteos: bool = False
     precision: int = PREC_FP64
     device: int = 0
+    # USE_SHOCK_FLATTENING / eps_sf_make (default 0.25, source.F90:177-181,
+    # :8073-8131); applied through phlx_set_shock_flattening, not the C struct
+    shock_flattening: bool = False
+    eps_sf: float = 0.25
 
     # ------------------------------------------------------------------
     @property
@@ -154,6 +158,13 @@ class PhlxConfig:
                               "face state) runs in FP64 only")
         if self.use_wb and not self.use_gravity:
             raise ConfigError("USE_WB requires USE_GRAVITY")
+        if self.shock_flattening:
+            if self.precision != PREC_FP64:
+                raise ConfigError("USE_SHOCK_FLATTENING runs in FP64 only")
+            if min(self.nx1, self.nx2) < 3 or (self.sdims == 3 and self.nx3 < 3):
+                raise ConfigError("USE_SHOCK_FLATTENING needs at least 3 cells per axis")
+            if not self.eps_sf > 0:
+                raise ConfigError("eps_sf_make must be positive")
         if self.cfl <= 0 or self.gamma <= 1 or self.mu <= 0:
             raise ConfigError("cfl, gamma-1 and mu must be positive")
         return self
@@ -184,10 +195,11 @@ class PhlxConfig:
                 key = {"sdims_make": "sdims", "nx1_make": "nx1", "nx2_make": "nx2",
                        "nx3_make": "nx3", "nas_make": "nas", "cfl_make": "cfl",
                        "Mach_cutoff_make": "mach_cutoff", "Mach_ct_make": "mach_ct",
-                       "tmax_make": "tmax"}.get(k)
+                       "tmax_make": "tmax", "eps_sf_make": "eps_sf"}.get(k)
                 if key is None:
                     continue
-                d[key] = float(v) if key in ("cfl", "mach_cutoff", "mach_ct", "tmax") else int(v)
+                d[key] = float(v) if key in ("cfl", "mach_cutoff", "mach_ct", "tmax",
+                                             "eps_sf") else int(v)
             elif o in _RECON:
                 d["recon"] = _RECON[o]
             elif o in _FLUX:
@@ -198,6 +210,8 @@ class PhlxConfig:
                 d["rk_stages"] = 3
             elif o == "FLUX_IS_ALL_MACH":
                 d["all_mach"] = True
+            elif o == "USE_SHOCK_FLATTENING":
+                d["shock_flattening"] = True
             elif o in ("USE_WB", "USE_GRAVITY", "EVOLVE_ETOT", "USE_EDOT"):
                 d[o.lower()] = True
             elif o[:3] in sides and o[3:4] == "_" and o[4:] in _BC:

@@ -517,10 +517,18 @@ static int check_launch() {
 
 extern "C" int phlx_stage_begin(phlx_ctx *ctx, int32_t irk) {
   // state0 / b0 capture is fused into the first stage's update and face-B
-  // kernels; nothing to launch here beyond argument checks.
+  // kernels; the stage starts with the shock-flattening flags of the
+  // current state (USE_SHOCK_FLATTENING), else launches nothing.
   if (!ctx) return PHLX_E_ARG;
   if (irk < 0 || irk >= ctx->stages) return PHLX_E_ARG;
   if (!ctx->initialized) return PHLX_E_STATE;
+  if (ctx->g.sflat) {
+    ProfRec r;
+    int on = kbegin(ctx, KID_MISC, r);
+    launch_shock_flags(ctx->g, ctx->stream);
+    kend(ctx, on, r);
+    return check_launch();
+  }
   return PHLX_OK;
 }
 
@@ -1154,6 +1162,37 @@ extern "C" int phlx_set_eos_guess(phlx_ctx *ctx, int32_t mode, double resid_tol,
 extern "C" int phlx_set_eos_lin_tol(phlx_ctx *ctx, double lin_tol) {
   if (!ctx || !(lin_tol >= 0.0) || lin_tol > 1e-2) return PHLX_E_ARG;
   ctx->g.eos_lin_tol = lin_tol;
+  return PHLX_OK;
+}
+
+// USE_SHOCK_FLATTENING with eps_sf_make (source/source.F90:8073-8131): the
+// detector needs three ghost cells, so every axis holds at least 3 cells;
+// FP64 only.  The flag arrays are made on the first enable.
+extern "C" int phlx_set_shock_flattening(phlx_ctx *ctx, int32_t enable, double eps_sf) {
+  if (!ctx || (enable != 0 && enable != 1)) return PHLX_E_ARG;
+  Grid &g = ctx->g;
+  if (!enable) {
+    g.sflat = 0;
+    return PHLX_OK;
+  }
+  if (!(eps_sf > 0.0) || !std::isfinite(eps_sf)) return PHLX_E_ARG;
+  if (ctx->cfg.precision != PHLX_PREC_FP64) return PHLX_E_CONFIG;
+  if (g.n1 < 3 || g.n2 < 3 || (g.sd == 3 && g.n3 < 3)) return PHLX_E_CONFIG;
+  HIPCHK(hipSetDevice(ctx->device));
+  if (!g.sf_flag) {
+    const long long e3 = (g.sd == 3) ? g.n3 + 4 : 1, f3 = (g.sd == 3) ? g.n3 + 2 : 1;
+    const size_t nd = (size_t)(g.n1 + 4) * (g.n2 + 4) * e3;
+    const size_t nfl = (size_t)(g.n1 + 2) * (g.n2 + 2) * f3;
+    unsigned char *det = nullptr, *flag = nullptr;
+    int rc = dalloc(ctx, &det, nd);
+    if (rc) return rc;
+    rc = dalloc(ctx, &flag, nfl);
+    if (rc) return rc;
+    g.sf_det = det;
+    g.sf_flag = flag;
+  }
+  g.eps_sf = eps_sf;
+  g.sflat = 1;
   return PHLX_OK;
 }
 

@@ -67,6 +67,11 @@ struct Grid {
   double eos_lin_tol;      // mixed precision: residual below which the EoS step is linear
   unsigned int *eos_part;
   float *mlp_y;            // network output at each zone's last solve (NaN: none yet)
+  // USE_SHOCK_FLATTENING: detector bytes on cells -2 .. n+1 and flags on
+  // cells -1 .. n of every axis (x3 only in 3D); sflat 0 = off
+  int sflat;
+  double eps_sf;
+  unsigned char *sf_det, *sf_flag;
   double *red;       // per-block partial maxima of the cell-grid reductions
   int nred;          // number of partials (blocks of a cell-grid launch)
   Scalars *sc;
@@ -85,6 +90,17 @@ __device__ __forceinline__ long long kix(const Grid &g, int d, int i, int j, int
   if (d == 2) return ((long long)k * (g.n2 + 1) + j) * (g.n1 + 1) + i;
   if (d == 0) return ((long long)k * (g.n2 + 1) + j) * g.n1 + i;
   return ((long long)k * g.n2 + j) * (g.n1 + 1) + i;
+}
+
+// Shock-flattening arrays: flag box of cells -1 .. n, detector box of cells
+// -2 .. n+1 on each axis (the x3 extension only in 3D).
+__device__ __forceinline__ long long sfix(const Grid &g, int i, int j, int k) {
+  const int k0 = (g.sd == 3) ? 1 : 0;
+  return ((long long)(k + k0) * (g.n2 + 2) + (j + 1)) * (g.n1 + 2) + (i + 1);
+}
+__device__ __forceinline__ long long sdix(const Grid &g, int i, int j, int k) {
+  const int k0 = (g.sd == 3) ? 2 : 0;
+  return ((long long)(k + k0) * (g.n2 + 4) + (j + 2)) * (g.n1 + 4) + (i + 2);
 }
 
 // Map a cell-centred index along axis a into the interior.  side = -1/+1
@@ -257,6 +273,39 @@ __device__ __forceinline__ void face_states(int recon, bool limited_scalar, cons
   }
 }
 
+// As face_states with USE_SHOCK_FLATTENING: a flattened cell (flm: cell
+// f-1, flc: cell f) takes the LIM2ND slope whatever the scheme (x1
+// source/source.F90:8743-8869, x2 :9635-9761, x3 :10765-10891).
+template <bool R5>
+__device__ __forceinline__ void face_states_sf(int recon, bool limited_scalar, const double *w,
+                                               int stride, bool flm, bool flc, double &qL,
+                                               double &qR) {
+  // a[0] = cell f-2, a[s] = f-1, a[2s] = f, a[3s] = f+1
+  const double *a = w + (R5 ? stride : 0);
+  const int s = stride;
+  double l, r;
+  if (flm)
+    rec_lim2nd(a[0], a[s], a[2 * s], l, qL);
+  else if (R5)
+    rec_lim5(w[0], w[s], w[2 * s], w[3 * s], w[4 * s], l, qL);
+  else if (recon == REC_LIM2ND)
+    rec_lim2nd(a[0], a[s], a[2 * s], l, qL);
+  else if (!limited_scalar)
+    rec_pph(a[0], a[s], a[2 * s], l, qL);
+  else
+    rec_pph_lim(a[0], a[s], a[2 * s], l, qL);
+  if (flc)
+    rec_lim2nd(a[s], a[2 * s], a[3 * s], qR, r);
+  else if (R5)
+    rec_lim5(w[s], w[2 * s], w[3 * s], w[4 * s], w[5 * s], qR, r);
+  else if (recon == REC_LIM2ND)
+    rec_lim2nd(a[s], a[2 * s], a[3 * s], qR, r);
+  else if (!limited_scalar)
+    rec_pph(a[s], a[2 * s], a[3 * s], qR, r);
+  else
+    rec_pph_lim(a[s], a[2 * s], a[3 * s], qR, r);
+}
+
 // ------------------------------------------------------------ Riemann: hydro
 // HLLC / LHLLC in the face frame (slot 1 = normal velocity).
 // source/source.F90:15611-16260.  g_e/g_c: FASTEOS gamma_e, gamma_c.
@@ -281,10 +330,13 @@ __device__ __forceinline__ void face_helm(const Grid &g, const double *q, double
   rhoe = eint * rho;
 }
 
-template <int SD, int NAS, int EK = EK_IDEAL>
+// SFL: the face may take the two-wave HLL flux of USE_SHOCK_FLATTENING
+// (hll true: a cell beside the face is flattened).
+template <int SD, int NAS, int EK = EK_IDEAL, bool SFL = false>
 __device__ __forceinline__ void riemann_hydro(const Grid &g, const double *qL,
                                               const double *qR, const double *gL,
-                                              const double *gR, double Tg, double *F) {
+                                              const double *gR, double Tg, double *F,
+                                              bool hll = false) {
   constexpr int IE = SD + 1, IA = SD + 2, NV = SD + 2 + NAS;
   const double rhoL = qL[0], irL = 1.0 / rhoL;
   const double uL = qL[1], vL = qL[2], wL = (SD == 3) ? qL[3] : 0.0;
@@ -318,6 +370,34 @@ __device__ __forceinline__ void riemann_hydro(const Grid &g, const double *qL,
   const double cmax = fmax(cL, cR);
   const double sL = fmin(uL, uR) - cmax;
   const double sR = fmax(uL, uR) + cmax;
+  if (SFL && hll) {
+    // HLL flux with the outer speeds clipped at zero (source/source.F90:15929-16003)
+    const double s0 = fmin(0.0, sL), s1 = fmax(0.0, sR);
+    double UL[NV], UR[NV], fL[NV], fR[NV];
+    UL[0] = rhoL; UL[1] = rhoL * uL; UL[2] = rhoL * vL;
+    if (SD == 3) UL[3] = rhoL * wL;
+    UL[IE] = EL;
+    UR[0] = rhoR; UR[1] = rhoR * uR; UR[2] = rhoR * vR;
+    if (SD == 3) UR[3] = rhoR * wR;
+    UR[IE] = ER;
+    fL[0] = rhoL * uL; fL[1] = rhoL * uL * uL + pL; fL[2] = rhoL * uL * vL;
+    if (SD == 3) fL[3] = rhoL * uL * wL;
+    fL[IE] = (EL + pL) * uL;
+    fR[0] = rhoR * uR; fR[1] = rhoR * uR * uR + pR; fR[2] = rhoR * uR * vR;
+    if (SD == 3) fR[3] = rhoR * uR * wR;
+    fR[IE] = (ER + pR) * uR;
+#pragma unroll
+    for (int v = IA; v < NV; v++) {
+      UL[v] = rhoL * qL[v];
+      UR[v] = rhoR * qR[v];
+      fL[v] = UL[v] * uL;
+      fR[v] = UR[v] * uR;
+    }
+    const double idn = 1.0 / (s1 - s0), ss = s0 * s1;
+#pragma unroll
+    for (int v = 0; v < NV; v++) F[v] = idn * (s1 * fL[v] - s0 * fR[v] + ss * (UR[v] - UL[v]));
+    return;
+  }
   const double dL = sL - uL, dR = sR - uR;
   const double us = (pR - pL + rhoL * uL * dL - rhoR * uR * dR) / (rhoL * dL - rhoR * dR);
   const double rsL = rhoL * (dL / (sL - us));
@@ -399,12 +479,12 @@ __device__ __forceinline__ void riemann_hydro(const Grid &g, const double *qL,
 // -------------------------------------------------------------- Riemann: MHD
 // HLLD / LHLLD in the face frame; b = (b_n, b_t1, b_t2).  Returns the
 // induction fluxes Fb (Fb[0] = 0).  source/source.F90:13969-15024.
-template <int SD, int NAS, int EK = EK_IDEAL>
+template <int SD, int NAS, int EK = EK_IDEAL, bool SFL = false>
 __device__ __forceinline__ void riemann_mhd(const Grid &g, const double *qL,
                                             const double *qR, const double *bL,
                                             const double *bR, const double *gL,
                                             const double *gR, double Tg, double *F,
-                                            double *Fb) {
+                                            double *Fb, bool hll = false) {
   constexpr int IE = SD + 1, IA = SD + 2, NV = SD + 2 + NAS;
   const double rhoL = qL[0], irL = 1.0 / rhoL;
   const double uL = qL[1], vL = qL[2], wL = (SD == 3) ? qL[3] : 0.0;
@@ -457,6 +537,44 @@ __device__ __forceinline__ void riemann_mhd(const Grid &g, const double *qL,
   const double cmax = fmax(cL, cR);
   const double sL = fmin(uL, uR) - cmax;
   const double sR = fmax(uL, uR) + cmax;
+  if (SFL && hll) {
+    // HLL flux with the outer speeds clipped at zero (source/source.F90:14386-14486)
+    const double s0 = fmin(0.0, sL), s1 = fmax(0.0, sR);
+    double UL[NV], UR[NV], fL[NV], fR[NV];
+    UL[0] = rhoL; UL[1] = rhoL * uL; UL[2] = rhoL * vL;
+    if (SD == 3) UL[3] = rhoL * wL;
+    UL[IE] = EL;
+    UR[0] = rhoR; UR[1] = rhoR * uR; UR[2] = rhoR * vR;
+    if (SD == 3) UR[3] = rhoR * wR;
+    UR[IE] = ER;
+    fL[0] = rhoL * uL;
+    fL[1] = rhoL * uL * uL + ptL - bx2;
+    fL[2] = rhoL * uL * vL - bx * byL;
+    if (SD == 3) fL[3] = rhoL * uL * wL - bx * bzL;
+    fL[IE] = (EL + ptL) * uL - bx * (bx * uL + byL * vL + bzL * wL);
+    fR[0] = rhoR * uR;
+    fR[1] = rhoR * uR * uR + ptR - bx2;
+    fR[2] = rhoR * uR * vR - bx * byR;
+    if (SD == 3) fR[3] = rhoR * uR * wR - bx * bzR;
+    fR[IE] = (ER + ptR) * uR - bx * (bx * uR + byR * vR + bzR * wR);
+#pragma unroll
+    for (int v = IA; v < NV; v++) {
+      UL[v] = rhoL * qL[v];
+      UR[v] = rhoR * qR[v];
+      fL[v] = UL[v] * uL;
+      fR[v] = UR[v] * uR;
+    }
+    const double UbL[3] = {bx, byL, bzL}, UbR[3] = {bx, byR, bzR};
+    const double fbL[3] = {0.0, uL * byL - vL * bx, uL * bzL - wL * bx};
+    const double fbR[3] = {0.0, uR * byR - vR * bx, uR * bzR - wR * bx};
+    const double idn = 1.0 / (s1 - s0), ss = s0 * s1;
+#pragma unroll
+    for (int v = 0; v < NV; v++) F[v] = idn * (s1 * fL[v] - s0 * fR[v] + ss * (UR[v] - UL[v]));
+#pragma unroll
+    for (int v = 0; v < SD; v++)
+      Fb[v] = idn * (s1 * fbL[v] - s0 * fbR[v] + ss * (UbR[v] - UbL[v]));
+    return;
+  }
   const double dL = sL - uL, dR = sR - uR;
   const double it1 = 1.0 / (dR * rhoR - dL * rhoL);
   const double us = (dR * rhoR * uR - dL * rhoL * uL - ptR + ptL) * it1;

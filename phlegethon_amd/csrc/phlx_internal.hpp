@@ -26,6 +26,8 @@ void launch_step_end(const Grid &g, int advance, hipStream_t st);
 void launch_reduce_to(const Grid &g, double *out, hipStream_t st);
 int cell_grid_blocks(const Grid &g);
 void launch_stream_copy(const void *src, void *dst, long long n16, hipStream_t st);
+// USE_SHOCK_FLATTENING flags of the current primitive state (g.sf_flag)
+void launch_shock_flags(const Grid &g, hipStream_t st);
 
 // Helmholtz EoS (phlx_helm.hip)
 void launch_helm_pack(long long np, const double *f9, const double *c4, double *tab,

@@ -76,10 +76,10 @@ __device__ __forceinline__ double face_temp_guess(const Grid &g, int i, int j, i
 // and the FASTEOS gammas reconstructed, the normal B read here): rotation
 // into the face frame, WB add-back, Riemann solve, rotation back, EMF and
 // contact sign (MHD), WB flux correction, ETOT potential flux, store.
-template <int DIR, int SD, int NAS, bool MHD, int EK>
+template <int DIR, int SD, int NAS, bool MHD, int EK, bool SFL = false>
 __device__ __forceinline__ void flux_face(const Grid &g, int i, int j, int k, double *qL,
                                           double *qR, double *bL, double *bR, double *gL,
-                                          double *gR, double Tg) {
+                                          double *gR, double Tg, bool hll = false) {
   constexpr int NV = SD + 2 + NAS, IE = SD + 1;
   const long long f = fix(g, DIR, i, j, k);
   if (MHD) {
@@ -120,9 +120,9 @@ __device__ __forceinline__ void flux_face(const Grid &g, int i, int j, int k, do
   }
   double F[NV], Fb[3] = {0.0, 0.0, 0.0};
   if (MHD)
-    riemann_mhd<SD, NAS, EK>(g, qL, qR, bL, bR, gL, gR, Tg, F, Fb);
+    riemann_mhd<SD, NAS, EK, SFL>(g, qL, qR, bL, bR, gL, gR, Tg, F, Fb, hll);
   else
-    riemann_hydro<SD, NAS, EK>(g, qL, qR, gL, gR, Tg, F);
+    riemann_hydro<SD, NAS, EK, SFL>(g, qL, qR, gL, gR, Tg, F, hll);
   // back to the grid frame
   if (DIR == 1) {
     double t = F[1]; F[1] = -F[2]; F[2] = t;
@@ -166,7 +166,9 @@ __device__ __forceinline__ void flux_face(const Grid &g, int i, int j, int k, do
 // flux correction and the ETOT potential flux.  MHD faces also store the
 // two tangential EMF components and the contact-upwind sign used by CT.
 // EK: EoS kind of the face states (EK_IDEAL, EK_FASTEOS, EK_HELM_FACE).
-template <int DIR, int SD, int NAS, bool MHD, int EK, bool R5>
+// SF: USE_SHOCK_FLATTENING -- the flags of cells f-1 and f (k_sf_dilate)
+// select LIM2ND for a flattened cell and the HLL flux for the face.
+template <int DIR, int SD, int NAS, bool MHD, int EK, bool R5, bool SF = false>
 __global__ void __launch_bounds__(256) k_flux(Grid g) {
   constexpr int NV = SD + 2 + NAS, IE = SD + 1;
   constexpr int NS = R5 ? 6 : 4, O = R5 ? 3 : 2;  // window: cells f-O .. f+NS-O-1
@@ -197,6 +199,30 @@ __global__ void __launch_bounds__(256) k_flux(Grid g) {
       Tg = face_temp_guess<DIR, false>(g, i, j, k);
   }
   double qL[NV], qR[NV], bL[3] = {0.0, 0.0, 0.0}, bR[3] = {0.0, 0.0, 0.0};
+  double gL[2] = {0.0, 0.0}, gR[2] = {0.0, 0.0};
+  if (SF) {
+    const bool flm = g.sf_flag[sfix(g, i - (DIR == 0), j - (DIR == 1), k - (DIR == 2))] != 0;
+    const bool flc = g.sf_flag[sfix(g, i, j, k)] != 0;
+#pragma unroll
+    for (int v = 0; v < NV; v++)
+      face_states_sf<R5>(g.recon, v > IE, &q[0][v], NV, flm, flc, qL[v], qR[v]);
+    if (MHD) {
+#pragma unroll
+      for (int c = 0; c < SD; c++) {
+        if (c == DIR) continue;
+        face_states_sf<R5>(g.recon, false, &b[0][c], 3, flm, flc, bL[c], bR[c]);
+      }
+    }
+    if (GAM) {
+#pragma unroll
+      for (int c = 0; c < 2; c++)
+        face_states_sf<R5>(g.recon, false, &ga[0][c], 2, flm, flc, gL[c], gR[c]);
+    }
+    // a face beside a flattened cell takes the HLL flux
+    // (source/source.F90:8731-8740, :9626-9630, :10756-10760)
+    flux_face<DIR, SD, NAS, MHD, EK, true>(g, i, j, k, qL, qR, bL, bR, gL, gR, Tg, flm || flc);
+    return;
+  }
   // scalars take the limited parabola under PPH (source/source.F90:8194-8243)
 #pragma unroll
   for (int v = 0; v < NV; v++) face_states<R5>(g.recon, v > IE, &q[0][v], NV, qL[v], qR[v]);
@@ -209,7 +235,6 @@ __global__ void __launch_bounds__(256) k_flux(Grid g) {
   }
   // FASTEOS: gamma_e / gamma_c face values, unlimited PPH or the LIM2ND
   // slope (source/source.F90:8497-8713)
-  double gL[2] = {0.0, 0.0}, gR[2] = {0.0, 0.0};
   if (GAM) {
 #pragma unroll
     for (int c = 0; c < 2; c++) face_states<R5>(g.recon, false, &ga[0][c], 2, gL[c], gR[c]);
@@ -1167,6 +1192,62 @@ void launch_stream_copy(const void *src, void *dst, long long n16, hipStream_t s
   k_stream_copy<<<blocks, 256, 0, st>>>((const double2 *)src, (double2 *)dst, n16);
 }
 
+// ------------------------------------------------------- shock flattening
+// USE_SHOCK_FLATTENING detector (source/source.F90:8073-8131).  k_sf_detect:
+// one thread per cell of -2 .. n+1 on each axis marks a cell whose summed
+// relative pressure jump sum_d |p(c+e_d) - p(c-e_d)| / p(c) exceeds eps_sf;
+// k_sf_dilate: a cell of -1 .. n is flattened when a marked cell lies in
+// its 3^sdims neighbourhood (the reference's scatter written as a gather, so
+// no flag array has to be cleared first).  Ghost pressures come through the
+// boundary map (pressure carries no sign flip); with USE_WB the full
+// pressure is the stored deviation plus the mapped row's equilibrium.
+template <int SD>
+__device__ __forceinline__ double sf_pressure(const Grid &g, int i, int j, int k) {
+  int s;
+  const int mi = map_axis(g, 0, i, g.n1, s);
+  const int mj = map_axis(g, 1, j, g.n2, s);
+  const int mk = (SD == 3) ? map_axis(g, 2, k, g.n3, s) : k;
+  const double p = g.P[(long long)(SD + 1) * g.N + cix(g, mi, mj, mk)];
+  if (!g.wb) return p;
+  const double e = g.eq_p_cc[mj];
+  return (p - e) + e;
+}
+
+template <int SD>
+__global__ void __launch_bounds__(256) k_sf_detect(Grid g) {
+  if (!g.sc->active) return;
+  int a, b;
+  if (!plane_coords(g.n1 + 4, g.n2 + 4, a, b)) return;
+  const int i = a - 2, j = b - 2, k = (SD == 3) ? (int)blockIdx.z - 2 : 0;
+  const double pc = sf_pressure<SD>(g, i, j, k);
+  const double dj = fabs(sf_pressure<SD>(g, i, j + 1, k) - sf_pressure<SD>(g, i, j - 1, k));
+  const double di = fabs(sf_pressure<SD>(g, i + 1, j, k) - sf_pressure<SD>(g, i - 1, j, k));
+  double tmp;
+  if (SD == 3) {
+    const double dk = fabs(sf_pressure<SD>(g, i, j, k + 1) - sf_pressure<SD>(g, i, j, k - 1));
+    tmp = (dk + dj + di) / pc;
+  } else {
+    tmp = (dj + di) / pc;
+  }
+  g.sf_det[sdix(g, i, j, k)] = (tmp > g.eps_sf) ? 1 : 0;
+}
+
+template <int SD>
+__global__ void __launch_bounds__(256) k_sf_dilate(Grid g) {
+  if (!g.sc->active) return;
+  int a, b;
+  if (!plane_coords(g.n1 + 2, g.n2 + 2, a, b)) return;
+  const int i = a - 1, j = b - 1, k = (SD == 3) ? (int)blockIdx.z - 1 : 0;
+  int any = 0;
+#pragma unroll
+  for (int dk = (SD == 3 ? -1 : 0); dk <= (SD == 3 ? 1 : 0); dk++)
+#pragma unroll
+    for (int dj = -1; dj <= 1; dj++)
+#pragma unroll
+      for (int di = -1; di <= 1; di++) any |= g.sf_det[sdix(g, i + di, j + dj, k + dk)];
+  g.sf_flag[sfix(g, i, j, k)] = (unsigned char)(any != 0);
+}
+
 // -------------------------------------------------------------- launchers
 static dim3 grid_for(int e1, int e2, int e3, dim3 blk) {
   return dim3((e1 + blk.x - 1) / blk.x, (e2 + blk.y - 1) / blk.y, e3);
@@ -1218,10 +1299,40 @@ static void launch_flux(const Grid &g, int dir, hipStream_t st) {
   }
 }
 
+void launch_shock_flags(const Grid &g, hipStream_t st) {
+  const dim3 blk(64, 4, 1);
+  if (g.sd == 3) {
+    k_sf_detect<3><<<grid_plane(g.n1 + 4, g.n2 + 4, g.n3 + 4), blk, 0, st>>>(g);
+    k_sf_dilate<3><<<grid_plane(g.n1 + 2, g.n2 + 2, g.n3 + 2), blk, 0, st>>>(g);
+  } else {
+    k_sf_detect<2><<<grid_plane(g.n1 + 4, g.n2 + 4, 1), blk, 0, st>>>(g);
+    k_sf_dilate<2><<<grid_plane(g.n1 + 2, g.n2 + 2, 1), blk, 0, st>>>(g);
+  }
+}
+
+// USE_SHOCK_FLATTENING sweeps (FP64): one thread per face along every
+// direction, the flags of the two cells beside the face read per face.
+template <int SD, int NAS, bool MHD, int EK, bool R5>
+static void launch_flux_sf(const Grid &g, int dir, hipStream_t st) {
+  const dim3 blk(64, 4, 1);
+  if (dir == 0)
+    k_flux<0, SD, NAS, MHD, EK, R5, true><<<grid_x1_faces(g), blk, 0, st>>>(g);
+  else if (dir == 1)
+    k_flux<1, SD, NAS, MHD, EK, R5, true><<<grid_for(g.n1, g.n2 + 1, g.n3, blk), blk, 0, st>>>(g);
+  else if (SD == 3)
+    k_flux<2, SD, NAS, MHD, EK, R5, true><<<grid_for(g.n1, g.n2, g.n3 + 1, blk), blk, 0, st>>>(g);
+}
+
 template <int SD, int NAS, bool MHD, bool HELM, bool MIX, bool R5 = false,
           int EK = HELM ? EK_FASTEOS : EK_IDEAL>
 struct Kern {
   static void flux(const Grid &g, int dir, hipStream_t st) {
+    if constexpr (!MIX) {
+      if (g.sflat) {
+        launch_flux_sf<SD, NAS, MHD, EK, R5>(g, dir, st);
+        return;
+      }
+    }
     launch_flux<SD, NAS, MHD, EK, MIX, R5>(g, dir, st);
   }
   static void corner(const Grid &g, hipStream_t st) {

@@ -212,6 +212,30 @@ def sod_tube(n: int = 200, flux: int = HLLC_FLUX, recon: int = LIM2ND_REC,
     return Problem("sod", cfg, prim, None, {})
 
 
+def blast(cfg: PhlxConfig, seed: int = SEED, p_ratio: float = 20.0,
+          radius: float = 0.2) -> Problem:
+    """Pressure blast for the shock-flattening cases: p = p_ratio inside a
+    sphere (circle) of the given radius around the box centre, 1 outside;
+    rho = 1 + 5 % noise, small random velocities; a random divergence-free
+    field of strength ~0.3 under MHD."""
+    X, Y, Z = mesh(cfg)
+    N = X.size
+    u = lambda s: (2.0 * uniform01(seed, s, N) - 1.0).reshape(X.shape)
+    c = [0.5 * (cfg.x1l + cfg.x1u), 0.5 * (cfg.x2l + cfg.x2u), 0.5 * (cfg.x3l + cfg.x3u)]
+    r2 = (X - c[0]) ** 2 + (Y - c[1]) ** 2
+    if cfg.sdims == 3:
+        r2 = r2 + (Z - c[2]) ** 2
+    prim = np.zeros((cfg.nvars,) + cfg.shape)
+    prim[0] = 1.0 + 0.05 * u(1)
+    for d in range(cfg.sdims):
+        prim[1 + d] = 0.05 * u(2 + d)
+    prim[cfg.sdims + 1] = np.where(r2 < radius * radius, p_ratio, 1.0)
+    for s in range(cfg.sdims + 2, cfg.nvars):
+        prim[s] = 0.5 + 0.2 * u(10 + s)
+    bf = _random_div_free(cfg, seed) if cfg.mhd else None
+    return Problem("blast", cfg, prim, bf, {})
+
+
 def sound_wave(n: int = 64, amp: float = 1e-6, flux: int = HLLC_FLUX,
                recon: int = LIM2ND_REC, rk_stages: int = 2) -> Problem:
     """Right-going linear sound wave along x1 on a periodic box; rho0 = 1,

@@ -88,6 +88,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.phlx_mlp_set_train_box.argtypes = [P, P]
     lib.phlx_set_eos_guess.argtypes = [P, I, D, I]
     lib.phlx_set_eos_lin_tol.argtypes = [P, D]
+    lib.phlx_set_shock_flattening.argtypes = [P, I, D]
     I64P = ctypes.POINTER(ctypes.c_int64)
     lib.phlx_eos_stats.argtypes = [P, I64P, I64P, I64P, I]
     _lib = lib
@@ -122,6 +123,8 @@ class Simulation:
         self._ctx = ctypes.c_void_p()
         c = self.cfg.to_c()
         _chk(self.lib.phlx_create(ctypes.byref(c), ctypes.byref(self._ctx)), "phlx_create")
+        if self.cfg.shock_flattening:
+            self.set_shock_flattening(True, self.cfg.eps_sf)
 
     # -- lifecycle -------------------------------------------------------
     def close(self):
@@ -343,6 +346,12 @@ class Simulation:
         per-zone T(rho, e) solve takes its Newton step linearly after one
         FP64 evaluation (0: always the reference Newton)."""
         _chk(self.lib.phlx_set_eos_lin_tol(self._ctx, float(lin_tol)), "phlx_set_eos_lin_tol")
+
+    def set_shock_flattening(self, enable: bool = True, eps_sf: float = 0.25):
+        """USE_SHOCK_FLATTENING with eps_sf_make: flagged cells take LIM2ND,
+        faces beside them the HLL flux (FP64 contexts)."""
+        _chk(self.lib.phlx_set_shock_flattening(self._ctx, int(bool(enable)), float(eps_sf)),
+             "phlx_set_shock_flattening")
 
     def eos_stats(self, reset: bool = False):
         it, n, fb = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
