@@ -64,6 +64,12 @@ def kernel_bytes(cfg, name: str) -> float:
             rd += N * 8
             wr += N * (24 if cfg.fasteos else 8)
         return rd + wr
+    if name == "misc" and cfg.shock_flattening:
+        # shock-flattening flags: pressure in, detector bytes out and in
+        # (cells -2..n+1), flag bytes out (cells -1..n)
+        e = [nx + 4, ny + 4, nz + 4 if sd == 3 else 1]
+        f = [nx + 2, ny + 2, nz + 2 if sd == 3 else 1]
+        return N * 8 + 2 * e[0] * e[1] * e[2] + f[0] * f[1] * f[2]
     return 0.0
 
 
@@ -226,7 +232,7 @@ def run_variant(pb, cfg, args, guess="warm", keep_state=False, profile=True):
         sim.run(args.profile_steps)
         sim.synchronize()
         kern = {}
-        for name in ("flux_x1", "flux_x2", "flux_x3", "corner", "faceb", "update"):
+        for name in ("flux_x1", "flux_x2", "flux_x3", "corner", "faceb", "update", "misc"):
             ms, cnt = sim.profile_read(name)
             if cnt:
                 kern[name] = {"total_ms": ms, "launches": cnt, "avg_ms": ms / cnt,
@@ -270,6 +276,8 @@ def main():
     ap.add_argument("--mlp-box", choices=("state", "table"), default="state",
                     help="surrogate training box: around the problem's states, or the "
                          "whole table range")
+    ap.add_argument("--shock-flattening", type=float, default=None, metavar="EPS_SF",
+                    help="USE_SHOCK_FLATTENING at this eps_sf (FP64)")
     ap.add_argument("--mlp-samples", type=int, default=1 << 22)
     ap.add_argument("--mlp-epochs", type=int, default=8)
     args = ap.parse_args()
@@ -283,6 +291,9 @@ def main():
     from phlegethon_amd.config import PREC_MIXED
 
     pb, desc, data = build_problem(args)
+    if args.shock_flattening is not None:
+        pb.cfg = pb.cfg.with_(shock_flattening=True, eps_sf=args.shock_flattening)
+        desc += f" | shock flattening eps_sf={args.shock_flattening:g}"
     n = args.n
     zones = pb.cfg.nx1 * pb.cfg.nx2 * (pb.cfg.nx3 if pb.cfg.sdims == 3 else 1)
     cfg = pb.cfg
@@ -302,6 +313,7 @@ def main():
     traffic = None
     pmc = os.path.join(ROOT, "profiles", "ot3d256_fp64_pmc.json")
     if (n == 256 and args.workload == "ot3d" and args.precision == "fp64"
+            and args.shock_flattening is None
             and os.path.exists(pmc)):
         # HBM bytes per dispatch measured with rocprofv3 --pmc FETCH_SIZE /
         # WRITE_SIZE on this workload, scaled by a copy kernel of known size

@@ -256,6 +256,7 @@ __device__ __forceinline__ void cell_edges(int recon, bool limited_scalar, doubl
 
 struct CellW {
   double q[8], b[3], ga[2], t;
+  int fl;  // shock-flattening flag (SF walks)
 };
 
 // x2 / x3 sweeps with the 2nd-order reconstructions: one thread per
@@ -268,7 +269,9 @@ constexpr int kSegFaces = 32;
 #define PHLX_FLUX_SEG 1
 #endif
 
-template <int DIR, int SD, int NAS, bool MHD, int EK, bool FAST>
+// SF: USE_SHOCK_FLATTENING -- a flagged cell takes the LIM2ND edges, a face
+// beside one the HLL flux (as k_flux<..., SF>).
+template <int DIR, int SD, int NAS, bool MHD, int EK, bool FAST, bool SF = false>
 __device__ __forceinline__ void seg_walk(const Grid &g, int i, int o, int f0, int f1) {
   constexpr int NV = SD + 2 + NAS, IE = SD + 1;
   constexpr bool HELM = EK == EK_FASTEOS;  // gamma_e / gamma_c carried per cell
@@ -277,6 +280,13 @@ __device__ __forceinline__ void seg_walk(const Grid &g, int i, int o, int f0, in
     const int jj = DIR == 1 ? c : oo, kk = DIR == 2 ? c : oo;
     load_cell<SD, NV, MHD, HELM, FAST>(g, ii, jj, kk, w.q, w.b, w.ga);
     if (EK == EK_HELM_FACE) w.t = cell_temp<FAST>(g, ii, jj, kk);
+    if (SF) {
+      // the walk also loads cells -2 and n+1, whose flags it never uses;
+      // the flag box holds cells -1 .. n, so their reads are clamped into it
+      const int n = DIR == 1 ? g.n2 : g.n3;
+      const int cf = min(max(c, -1), n);
+      w.fl = g.sf_flag[sfix(g, ii, DIR == 1 ? cf : oo, DIR == 2 ? cf : oo)];
+    }
   };
   CellW cm, c0, cp;
   load(f0 - 2, cm, i, o);
@@ -286,16 +296,17 @@ __device__ __forceinline__ void seg_walk(const Grid &g, int i, int o, int f0, in
   double rq[NV], rb[3] = {0.0, 0.0, 0.0}, rg[2] = {0.0, 0.0};
   {
     double l;
+    const int rec = (SF && c0.fl) ? REC_LIM2ND : g.recon;
 #pragma unroll
-    for (int v = 0; v < NV; v++) cell_edges(g.recon, v > IE, cm.q[v], c0.q[v], cp.q[v], l, rq[v]);
+    for (int v = 0; v < NV; v++) cell_edges(rec, v > IE, cm.q[v], c0.q[v], cp.q[v], l, rq[v]);
     if (MHD) {
 #pragma unroll
       for (int c = 0; c < SD; c++)
-        if (c != DIR) cell_edges(g.recon, false, cm.b[c], c0.b[c], cp.b[c], l, rb[c]);
+        if (c != DIR) cell_edges(rec, false, cm.b[c], c0.b[c], cp.b[c], l, rb[c]);
     }
     if (HELM) {
 #pragma unroll
-      for (int c = 0; c < 2; c++) cell_edges(g.recon, false, cm.ga[c], c0.ga[c], cp.ga[c], l, rg[c]);
+      for (int c = 0; c < 2; c++) cell_edges(rec, false, cm.ga[c], c0.ga[c], cp.ga[c], l, rg[c]);
     }
   }
   cm = c0;
@@ -309,35 +320,36 @@ __device__ __forceinline__ void seg_walk(const Grid &g, int i, int o, int f0, in
     load(f + 1, cp, ii, oo);
     double qL[NV], qR[NV], bL[3] = {0.0, 0.0, 0.0}, bR[3] = {0.0, 0.0, 0.0};
     double gL[2] = {0.0, 0.0}, gR[2] = {0.0, 0.0};
+    const int rec = (SF && c0.fl) ? REC_LIM2ND : g.recon;
 #pragma unroll
     for (int v = 0; v < NV; v++) {
       qL[v] = rq[v];
-      cell_edges(g.recon, v > IE, cm.q[v], c0.q[v], cp.q[v], qR[v], rq[v]);
+      cell_edges(rec, v > IE, cm.q[v], c0.q[v], cp.q[v], qR[v], rq[v]);
     }
     if (MHD) {
 #pragma unroll
       for (int c = 0; c < SD; c++) {
         if (c == DIR) continue;
         bL[c] = rb[c];
-        cell_edges(g.recon, false, cm.b[c], c0.b[c], cp.b[c], bR[c], rb[c]);
+        cell_edges(rec, false, cm.b[c], c0.b[c], cp.b[c], bR[c], rb[c]);
       }
     }
     if (HELM) {
 #pragma unroll
       for (int c = 0; c < 2; c++) {
         gL[c] = rg[c];
-        cell_edges(g.recon, false, cm.ga[c], c0.ga[c], cp.ga[c], gR[c], rg[c]);
+        cell_edges(rec, false, cm.ga[c], c0.ga[c], cp.ga[c], gR[c], rg[c]);
       }
     }
     const double Tg = (EK == EK_HELM_FACE) ? 0.5 * (cm.t + c0.t) : 0.0;
-    flux_face<DIR, SD, NAS, MHD, EK>(g, ii, DIR == 1 ? f : oo, DIR == 2 ? f : oo, qL, qR, bL,
-                                     bR, gL, gR, Tg);
+    flux_face<DIR, SD, NAS, MHD, EK, SF>(g, ii, DIR == 1 ? f : oo, DIR == 2 ? f : oo, qL, qR, bL,
+                                         bR, gL, gR, Tg, SF && (cm.fl || c0.fl));
     cm = c0;
     c0 = cp;
   }
 }
 
-template <int DIR, int SD, int NAS, bool MHD, int EK>
+template <int DIR, int SD, int NAS, bool MHD, int EK, bool SF = false>
 __global__ void __launch_bounds__(256) k_flux_seg(Grid g) {
   static_assert(DIR == 1 || DIR == 2, "x2 / x3 sweeps");
   if (!g.sc->active) return;
@@ -349,9 +361,9 @@ __global__ void __launch_bounds__(256) k_flux_seg(Grid g) {
   const int f1 = min(f0 + kSegFaces, n + 1);
   // cells f0 - 2 .. f1 are read
   if (__all(f0 - 2 >= 0 && f1 <= n - 1))
-    seg_walk<DIR, SD, NAS, MHD, EK, true>(g, i, o, f0, f1);
+    seg_walk<DIR, SD, NAS, MHD, EK, true, SF>(g, i, o, f0, f1);
   else
-    seg_walk<DIR, SD, NAS, MHD, EK, false>(g, i, o, f0, f1);
+    seg_walk<DIR, SD, NAS, MHD, EK, false, SF>(g, i, o, f0, f1);
 }
 
 // ------------------------------------------------------- flux, mixed precision
@@ -1213,23 +1225,58 @@ __device__ __forceinline__ double sf_pressure(const Grid &g, int i, int j, int k
   return (p - e) + e;
 }
 
+// The detector of cell (i, j, k) from the full pressures of its stencil.
+template <int SD>
+__device__ __forceinline__ unsigned char sf_test(const Grid &g, double pc, double pim,
+                                                 double pip, double pjm, double pjp, double pkm,
+                                                 double pkp) {
+  const double dj = fabs(pjp - pjm), di = fabs(pip - pim);
+  double tmp;
+  if (SD == 3)
+    tmp = (fabs(pkp - pkm) + dj + di) / pc;
+  else
+    tmp = (dj + di) / pc;
+  return (tmp > g.eps_sf) ? 1 : 0;
+}
+
 template <int SD>
 __global__ void __launch_bounds__(256) k_sf_detect(Grid g) {
   if (!g.sc->active) return;
   int a, b;
   if (!plane_coords(g.n1 + 4, g.n2 + 4, a, b)) return;
   const int i = a - 2, j = b - 2, k = (SD == 3) ? (int)blockIdx.z - 2 : 0;
-  const double pc = sf_pressure<SD>(g, i, j, k);
-  const double dj = fabs(sf_pressure<SD>(g, i, j + 1, k) - sf_pressure<SD>(g, i, j - 1, k));
-  const double di = fabs(sf_pressure<SD>(g, i + 1, j, k) - sf_pressure<SD>(g, i - 1, j, k));
-  double tmp;
-  if (SD == 3) {
-    const double dk = fabs(sf_pressure<SD>(g, i, j, k + 1) - sf_pressure<SD>(g, i, j, k - 1));
-    tmp = (dk + dj + di) / pc;
+  const bool inside = i >= 1 && i <= g.n1 - 2 && j >= 1 && j <= g.n2 - 2 &&
+                      (SD == 2 || (k >= 1 && k <= g.n3 - 2));
+  unsigned char d;
+  if (__all(inside)) {
+    // every stencil cell of the wave is interior: direct loads
+    const double *p = g.P + (long long)(SD + 1) * g.N;
+    const long long c = cix(g, i, j, k), sj = g.n1, sk = (long long)g.n1 * g.n2;
+    double pc = p[c], pim = p[c - 1], pip = p[c + 1], pjm = p[c - sj], pjp = p[c + sj];
+    double pkm = 0.0, pkp = 0.0;
+    if (SD == 3) {
+      pkm = p[c - sk];
+      pkp = p[c + sk];
+    }
+    if (g.wb) {
+      const double e = g.eq_p_cc[j], em = g.eq_p_cc[j - 1], ep = g.eq_p_cc[j + 1];
+      pc = (pc - e) + e;
+      pim = (pim - e) + e;
+      pip = (pip - e) + e;
+      pkm = (pkm - e) + e;
+      pkp = (pkp - e) + e;
+      pjm = (pjm - em) + em;
+      pjp = (pjp - ep) + ep;
+    }
+    d = sf_test<SD>(g, pc, pim, pip, pjm, pjp, pkm, pkp);
   } else {
-    tmp = (dj + di) / pc;
+    d = sf_test<SD>(g, sf_pressure<SD>(g, i, j, k), sf_pressure<SD>(g, i - 1, j, k),
+                    sf_pressure<SD>(g, i + 1, j, k), sf_pressure<SD>(g, i, j - 1, k),
+                    sf_pressure<SD>(g, i, j + 1, k),
+                    SD == 3 ? sf_pressure<SD>(g, i, j, k - 1) : 0.0,
+                    SD == 3 ? sf_pressure<SD>(g, i, j, k + 1) : 0.0);
   }
-  g.sf_det[sdix(g, i, j, k)] = (tmp > g.eps_sf) ? 1 : 0;
+  g.sf_det[sdix(g, i, j, k)] = d;
 }
 
 template <int SD>
@@ -1310,14 +1357,21 @@ void launch_shock_flags(const Grid &g, hipStream_t st) {
   }
 }
 
-// USE_SHOCK_FLATTENING sweeps (FP64): one thread per face along every
-// direction, the flags of the two cells beside the face read per face.
+// USE_SHOCK_FLATTENING sweeps (FP64): as launch_flux with the flag-reading
+// variants (per-face x1 and LIM5TH sweeps, column segments otherwise).
 template <int SD, int NAS, bool MHD, int EK, bool R5>
 static void launch_flux_sf(const Grid &g, int dir, hipStream_t st) {
   const dim3 blk(64, 4, 1);
-  if (dir == 0)
+  if (dir == 0) {
     k_flux<0, SD, NAS, MHD, EK, R5, true><<<grid_x1_faces(g), blk, 0, st>>>(g);
-  else if (dir == 1)
+  } else if (!R5 && PHLX_FLUX_SEG) {
+    const int n = dir == 1 ? g.n2 : g.n3, no = dir == 1 ? g.n3 : g.n2;
+    const dim3 grd((g.n1 + 63) / 64, (no + 3) / 4, (n + 1 + kSegFaces - 1) / kSegFaces);
+    if (dir == 1)
+      k_flux_seg<1, SD, NAS, MHD, EK, true><<<grd, blk, 0, st>>>(g);
+    else if (SD == 3)
+      k_flux_seg<2, SD, NAS, MHD, EK, true><<<grd, blk, 0, st>>>(g);
+  } else if (dir == 1)
     k_flux<1, SD, NAS, MHD, EK, R5, true><<<grid_for(g.n1, g.n2 + 1, g.n3, blk), blk, 0, st>>>(g);
   else if (SD == 3)
     k_flux<2, SD, NAS, MHD, EK, R5, true><<<grid_for(g.n1, g.n2, g.n3 + 1, blk), blk, 0, st>>>(g);
