@@ -247,6 +247,33 @@ int phlx_set_eos_guess(phlx_ctx *ctx, int32_t mode, double resid_tol, int32_t co
 int phlx_set_eos_lin_tol(phlx_ctx *ctx, double lin_tol);
 int phlx_eos_stats(phlx_ctx *ctx, int64_t *newton_iters, int64_t *solves, int64_t *fallbacks,
                    int32_t reset);
+/* THERMAL_DIFFUSION_STS (compile-time OPTS in the reference, with
+ * EVALUATE_PARABOLIC_TIMESTEP and FIX_TEMPERATURE_AT_X1L..X3U; time_loop
+ * source/source.F90:6672-6688, thermal_diffusion_step :19949-20269).
+ * Radiative diffusion of the internal energy, flux (4/3) a c K_int dT/dx
+ * with K = T^3 / (rho kappa) (harmonic mean on faces), by RKL2
+ * super-time-stepping: phlx_run then runs diffusion over dt/2, hydro_step,
+ * diffusion over dt/2, and the hyperbolic dt of the result.  evaluate_dt 1:
+ * dt_par = 1 / max(2 c / (3 kappa rho) sum 1/dx^2) every step, 0: once in
+ * phlx_init (set kappa first).  fix_t: NULL or six wall temperatures
+ * X1L X1U X2L X2U X3L X3U, NaN for the zero-gradient ghost; a fixed one
+ * sets the ghost to 2 T_wall - T (non-periodic sides only).  The ideal gas
+ * needs nas 0 (PHLX_E_CONFIG otherwise); diffusion evolves eint (no
+ * STS_EVOLVE_TEMP), Cartesian, no USE_TIMMES_KAPPA, no internal boundaries. */
+int phlx_set_thermal_diffusion(phlx_ctx *ctx, int32_t enable, int32_t evaluate_dt,
+                               const double *fix_t);
+/* Opacity on cells -1..n of each axis (one ghost layer; the x3 layer only in
+ * 3D), x1 fastest: (nx1+2)*(nx2+2)*(nx3+2 or 1) positive doubles, as the
+ * application fills lgrid%kappa (source/source.F90:1523).  After
+ * phlx_set_thermal_diffusion. */
+int phlx_set_kappa(phlx_ctx *ctx, const double *kappa);
+/* compute_parabolic_dt (source/source.F90:19804-19947); dt_par may be NULL. */
+int phlx_parabolic_dt(phlx_ctx *ctx, double *dt_par);
+/* thermal_diffusion_step(strang) over half the current dt (after
+ * phlx_init), for callers that drive the step themselves. */
+int phlx_diffusion_step(phlx_ctx *ctx, int32_t strang);
+/* RKL2 stage count of the last diffusion step (the reference prints it). */
+int phlx_diffusion_stages(phlx_ctx *ctx, int32_t *stages);
 /* USE_SHOCK_FLATTENING (a compile-time OPTS switch with eps_sf_make in the
  * reference; source/source.F90:8073-8131 detection, :8731-8740 and
  * :8743-8869 use).  At the start of every stage a cell whose summed relative
@@ -262,7 +289,8 @@ int phlx_set_shock_flattening(phlx_ctx *ctx, int32_t enable, double eps_sf);
 int phlx_divb_max(phlx_ctx *ctx, double *out);
 /* Per-kernel device time accumulated while profiling is on (HIP events on
  * the context stream).  kernel ids: 0 flux x1, 1 flux x2, 2 flux x3,
- * 3 corner EMF, 4 face-B, 5 update, 6 misc. */
+ * 3 corner EMF, 4 face-B, 5 update, 6 misc (shock flags), 7 thermal
+ * diffusion. */
 int phlx_profile_enable(phlx_ctx *ctx, int32_t on);
 int phlx_profile_read(phlx_ctx *ctx, int32_t kernel, double *total_ms, int64_t *launches);
 /* Bandwidth probe: copies nbytes between two fresh device buffers `reps`

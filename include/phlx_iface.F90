@@ -179,6 +179,36 @@ module phlx_iface
       integer(c_int64_t), intent(out) :: newton_iters, solves, fallbacks
       integer(c_int32_t), value :: reset
     end function
+    ! THERMAL_DIFFUSION_STS; fix_t(6) wall temperatures (NaN: zero gradient)
+    integer(c_int) function phlx_set_thermal_diffusion(ctx, enable, evaluate_dt, fix_t) &
+        bind(c, name="phlx_set_thermal_diffusion")
+      import :: c_int, c_ptr, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: enable, evaluate_dt
+      real(c_double), intent(in) :: fix_t(6)
+    end function
+    ! kappa(nx1+2, nx2+2, nx3+2): the opacity with one ghost layer
+    integer(c_int) function phlx_set_kappa(ctx, kappa) bind(c, name="phlx_set_kappa")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: kappa(*)
+    end function
+    integer(c_int) function phlx_parabolic_dt(ctx, dt_par) bind(c, name="phlx_parabolic_dt")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: dt_par
+    end function
+    integer(c_int) function phlx_diffusion_step(ctx, strang) bind(c, name="phlx_diffusion_step")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: strang
+    end function
+    integer(c_int) function phlx_diffusion_stages(ctx, stages) &
+        bind(c, name="phlx_diffusion_stages")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), intent(out) :: stages
+    end function
     ! USE_SHOCK_FLATTENING with eps_sf_make (FP64 contexts)
     integer(c_int) function phlx_set_shock_flattening(ctx, enable, eps_sf) &
         bind(c, name="phlx_set_shock_flattening")

@@ -43,6 +43,9 @@ typedef struct {
   int nthreads;           /* OpenMP threads for the baseline (0 = default) */
   int shock_flattening;   /* USE_SHOCK_FLATTENING */
   double eps_sf;          /* eps_sf_make: relative pressure jump that flattens */
+  int thermal_diffusion;  /* THERMAL_DIFFUSION_STS (RKL2, Strang-split around hydro_step) */
+  int eval_parabolic_dt;  /* EVALUATE_PARABOLIC_TIMESTEP: dt_par every step (else at step 0) */
+  double fix_t[6];        /* FIX_TEMPERATURE_AT_X1L..X3U wall temperature, NaN = not fixed */
 } orc_config;
 
 typedef struct orc_grid orc_grid;
@@ -79,12 +82,20 @@ int orc_get_gammaf(const orc_grid *g, double *out);
 int orc_init(orc_grid *g);
 /* Resume after read_restart: eint (+ FASTEOS gammas) rebuilt from (rho, T)
  * or p, scalars from the file (:2761-2897, :6413-6500). */
+int orc_resume(orc_grid *g, double time, double dt, double smax, long step);
 /* USE_SHOCK_FLATTENING flags of the current state on cells -1..n per axis
  * (x3 only in 3D), x1 fastest; -1 when flattening is off. */
 int orc_shock_flags(orc_grid *g, unsigned char *out);
-int orc_resume(orc_grid *g, double time, double dt, double smax, long step);
-/* nsteps passes of the time_loop body: dt clip, hydro_step, time += dt,
- * compute_hyperbolic_dt, step++ (:6662-6720). */
+/* Opacity kappa on cells -1..n of each axis (one ghost layer; x3 only in
+ * 3D), x1 fastest, as the application fills lgrid%kappa. */
+int orc_set_kappa(orc_grid *g, const double *kappa);
+/* compute_parabolic_dt (:19804-19947); returns dt_par. */
+double orc_parabolic_dt(orc_grid *g);
+/* thermal_diffusion_step (:19949-20269) over half of the current dt. */
+int orc_diffusion_step(orc_grid *g, int strang);
+/* nsteps passes of the time_loop body: dt clip, [dt_par, diffusion 1],
+ * hydro_step, [diffusion 2], time += dt, compute_hyperbolic_dt, step++
+ * (:6662-6720). */
 int orc_run(orc_grid *g, int nsteps);
 /* One hydro_step with the current dt (no dt update). */
 int orc_hydro_step(orc_grid *g);

@@ -35,6 +35,8 @@ class OrcConfig(ctypes.Structure):
         ("eos", ctypes.c_int), ("fasteos", ctypes.c_int),
         ("nthreads", ctypes.c_int),
         ("shock_flattening", ctypes.c_int), ("eps_sf", ctypes.c_double),
+        ("thermal_diffusion", ctypes.c_int), ("eval_parabolic_dt", ctypes.c_int),
+        ("fix_t", ctypes.c_double * 6),
     ]
 
 
@@ -68,6 +70,10 @@ def load() -> ctypes.CDLL:
                                ctypes.c_long]
     lib.orc_run.argtypes = [P, ctypes.c_int]
     lib.orc_shock_flags.argtypes = [P, P]
+    lib.orc_set_kappa.argtypes = [P, P]
+    lib.orc_parabolic_dt.restype = ctypes.c_double
+    lib.orc_parabolic_dt.argtypes = [P]
+    lib.orc_diffusion_step.argtypes = [P, ctypes.c_int]
     lib.orc_hydro_step.argtypes = [P]
     lib.orc_compute_dt.argtypes = [P]
     lib.orc_get_scalar.restype = ctypes.c_double
@@ -93,13 +99,16 @@ def load() -> ctypes.CDLL:
 def to_orc(cfg, nthreads: int = 0) -> OrcConfig:
     c = OrcConfig()
     for name, _ in OrcConfig._fields_:
-        if name in ("bc", "nthreads"):
+        if name in ("bc", "nthreads", "fix_t"):
             continue
         v = getattr(cfg, name)
         setattr(c, name, int(v) if isinstance(v, bool) else v)
     for i, v in enumerate(tuple(cfg.bc)):
         c.bc[i] = int(v)
     c.nthreads = nthreads
+    fix = getattr(cfg, "fix_temperature", None) or (None,) * 6
+    for i, v in enumerate(fix):
+        c.fix_t[i] = float("nan") if v is None else float(v)
     return c
 
 
@@ -144,6 +153,10 @@ class OracleRun:
                 np.ascontiguousarray(problem.profiles[n], dtype=np.float64) for n in names]
         self._keep = arrs
         self.lib.orc_set_profiles(self.g, *[_p(a) for a in arrs])
+        kap = getattr(problem, "kappa", None)
+        if kap is not None:
+            self._kappa = np.ascontiguousarray(kap, dtype=np.float64)
+            self.lib.orc_set_kappa(self.g, _p(self._kappa))
 
     def close(self):
         if self.g:
@@ -168,6 +181,13 @@ class OracleRun:
 
     def run(self, n: int):
         self.lib.orc_run(self.g, int(n))
+
+    def parabolic_dt(self) -> float:
+        return self.lib.orc_parabolic_dt(self.g)
+
+    def diffusion_step(self, strang: int = 1):
+        if self.lib.orc_diffusion_step(self.g, int(strang)) != 0:
+            raise ValueError("no kappa set")
 
     def shock_flags(self) -> np.ndarray:
         """USE_SHOCK_FLATTENING flags of the current state on cells -1..n of

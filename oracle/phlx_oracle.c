@@ -49,6 +49,10 @@ struct orc_grid {
   double *eq_rho_cc, *eq_p_cc, *eq_rho_f, *eq_p_f, *grav, *phi_cc, *phi_f, *edot;
   double *eq_ge_cc, *eq_ge_f;       /* FASTEOS WB: equilibrium gamma_e */
   const orc_helm *helm;             /* Helmholtz table (not owned) */
+  /* THERMAL_DIFFUSION_STS: kappa on cells -1..n (one ghost layer), RKL2
+   * registers e_0, e_jm2, Me_0 and the residual Me (interior) */
+  double *kappa, *e0, *ejm2, *me0, *me;
+  double dt_par;
   double time, dt, smax;
   long step;
 };
@@ -156,6 +160,7 @@ orc_grid *orc_create(const orc_config *cfg) {
 void orc_destroy(orc_grid *g) {
   if (!g) return;
   free(g->prim); free(g->bcc); free(g->temp); free(g->gamf); free(g->eint); free(g->flat);
+  free(g->kappa); free(g->e0); free(g->ejm2); free(g->me0); free(g->me);
   free(g->state); free(g->state0); free(g->res);
   for (int d = 0; d < 3; d++) {
     free(g->bf[d]); free(g->b0f[d]); free(g->emf[d]); free(g->sgn[d]); free(g->cor[d]);
@@ -1645,6 +1650,9 @@ int orc_init(orc_grid *g) {
       }
   g->time = 0.0;
   g->step = 0;
+  /* without EVALUATE_PARABOLIC_TIMESTEP dt_par is computed once, at step 0
+   * (:6504-6512) */
+  if (g->c.thermal_diffusion && g->kappa && !g->c.eval_parabolic_dt) orc_parabolic_dt(g);
   return orc_compute_dt(g);
 }
 
@@ -1685,11 +1693,240 @@ int orc_resume(orc_grid *g, double time, double dt, double smax, long step) {
   return 0;
 }
 
+/* ------------------------------------------------ thermal diffusion (STS) */
+
+#define CONST_RAD 7.565767381646406e-15 /* source.F90:614 */
+#define CONST_C 2.99792458e10           /* source.F90:622 */
+
+static inline long kapidx(const orc_grid *g, int i, int j, int k) {
+  const int k0 = (g->sd == 3) ? 1 : 0;
+  return ((long)(k + k0) * (g->n[1] + 2) + (j + 1)) * (g->n[0] + 2) + (i + 1);
+}
+
+int orc_set_kappa(orc_grid *g, const double *kappa) {
+  const long n = (long)(g->n[0] + 2) * (g->n[1] + 2) * ((g->sd == 3) ? g->n[2] + 2 : 1);
+  const long ni = (long)g->n[0] * g->n[1] * g->n[2];
+  if (!g->kappa) {
+    g->kappa = (double *)calloc(n, sizeof(double));
+    g->e0 = (double *)calloc(ni, sizeof(double));
+    g->ejm2 = (double *)calloc(ni, sizeof(double));
+    g->me0 = (double *)calloc(ni, sizeof(double));
+    g->me = (double *)calloc(ni, sizeof(double));
+    if (!g->kappa || !g->e0 || !g->ejm2 || !g->me0 || !g->me) return -1;
+  }
+  memcpy(g->kappa, kappa, n * sizeof(double));
+  return 0;
+}
+
+/* compute_parabolic_dt (source.F90:19804-19947), Cartesian, without
+ * USE_TIMMES_KAPPA: D = c / (3 kappa rho), dt_par = 1 / max 2 D sum 1/dx^2. */
+double orc_parabolic_dt(orc_grid *g) {
+  const int nv = g->nv;
+  const double i1 = g->inv_dx[0], i2 = g->inv_dx[1], i3 = (g->sd == 3) ? g->inv_dx[2] : 0.0;
+  const double inv_dl2 = i1 * i1 + i2 * i2 + i3 * i3;
+  double m = 0.0;
+  for (int k = 0; k < g->n[2]; k++)
+    for (int j = 0; j < g->n[1]; j++)
+      for (int i = 0; i < g->n[0]; i++) {
+        const double rho = g->prim[cidx(g, i, j, k) * nv];
+        const double drad = (1.0 / 3.0) * CONST_C / (g->kappa[kapidx(g, i, j, k)] * rho);
+        const double v = 2.0 * drad * inv_dl2;
+        m = (m > v) ? m : v;  /* Fortran max(m, v) */
+      }
+  g->dt_par = 1.0 / m;
+  return g->dt_par;
+}
+
+/* bcs_thermal_diffusion (source.F90:20837-21068) without internal
+ * boundaries: periodic copies of temp; at a physical wall the density
+ * ghost is extrapolated linearly (2 rho(lx) - rho(lx+1)) and the
+ * temperature ghost copies the wall cell, or with FIX_TEMPERATURE_AT_* is
+ * 2 T_wall - T(lx) (the wall value on the face).  One ghost layer, interior
+ * transverse ranges. */
+static void diff_bcs(orc_grid *g) {
+  const int nv = g->nv;
+  halo_periodic(g, g->temp, 1);
+  for (int d = 0; d < g->sd; d++) {
+    if (g->c.bc[2 * d] == ORC_BC_PERIODIC) continue;
+    const int t1 = (d == 0) ? 1 : 0, t2 = (d == 2) ? 1 : 2;
+    const int n = g->n[d];
+    for (int side = 0; side < 2; side++) {
+      const double tw = g->c.fix_t[2 * d + side];
+      for (int b = 0; b < g->n[t2]; b++)
+        for (int a = 0; a < g->n[t1]; a++) {
+          int id[3], i0[3], i1[3];
+          id[t1] = i0[t1] = i1[t1] = a;
+          id[t2] = i0[t2] = i1[t2] = b;
+          id[d] = side == 0 ? -1 : n;
+          i0[d] = side == 0 ? 0 : n - 1;
+          i1[d] = side == 0 ? 1 : n - 2;
+          const long cg = cidx(g, id[0], id[1], id[2]);
+          const long c0 = cidx(g, i0[0], i0[1], i0[2]), c1 = cidx(g, i1[0], i1[1], i1[2]);
+          g->prim[cg * nv] = 2.0 * g->prim[c0 * nv] - g->prim[c1 * nv];
+          if (tw == tw)
+            g->temp[cg] = 2.0 * tw - g->temp[c0];
+          else
+            g->temp[cg] = g->temp[c0];
+        }
+    }
+  }
+}
+
+/* One face's radiative flux Krad K_int grad T between cells m and p
+ * (source.F90:20488-20509): K = T^3 / (rho kappa), harmonic mean. */
+static inline double diff_flux(const orc_grid *g, long cm, long cp, long km, long kp,
+                               double inv_dl) {
+  const double krad = (4.0 / 3.0) * CONST_RAD * CONST_C;
+  const double tm = g->temp[cm], tp = g->temp[cp];
+  const double kmn = tm * tm * tm / (g->prim[cm * g->nv] * g->kappa[km]);
+  const double kpl = tp * tp * tp / (g->prim[cp * g->nv] * g->kappa[kp]);
+  const double kint = 2.0 * (kmn * kpl) / (kmn + kpl);
+  const double grad = (tp - tm) * inv_dl;
+  return krad * kint * grad;
+}
+
+/* residuals_thermal_diffusion (source.F90:20394-20835), Cartesian: me =
+ * sum_d (fe(+) - fe(-)) / dx_d, x1 first. */
+static void diff_residual(orc_grid *g) {
+  const int sd = g->sd;
+#pragma omp parallel for schedule(static)
+  for (int k = 0; k < g->n[2]; k++)
+    for (int j = 0; j < g->n[1]; j++)
+      for (int i = 0; i < g->n[0]; i++) {
+        const long q = iidx(g, i, j, k);
+        const long c = cidx(g, i, j, k), kc = kapidx(g, i, j, k);
+        double m;
+        {
+          const double fm = diff_flux(g, cidx(g, i - 1, j, k), c, kapidx(g, i - 1, j, k), kc,
+                                      g->inv_dx[0]);
+          const double fp = diff_flux(g, c, cidx(g, i + 1, j, k), kc, kapidx(g, i + 1, j, k),
+                                      g->inv_dx[0]);
+          m = (fp - fm) * g->inv_dx[0];
+        }
+        {
+          const double fm = diff_flux(g, cidx(g, i, j - 1, k), c, kapidx(g, i, j - 1, k), kc,
+                                      g->inv_dx[1]);
+          const double fp = diff_flux(g, c, cidx(g, i, j + 1, k), kc, kapidx(g, i, j + 1, k),
+                                      g->inv_dx[1]);
+          m = m + (fp - fm) * g->inv_dx[1];
+        }
+        if (sd == 3) {
+          const double fm = diff_flux(g, cidx(g, i, j, k - 1), c, kapidx(g, i, j, k - 1), kc,
+                                      g->inv_dx[2]);
+          const double fp = diff_flux(g, c, cidx(g, i, j, k + 1), kc, kapidx(g, i, j, k + 1),
+                                      g->inv_dx[2]);
+          m = m + (fp - fm) * g->inv_dx[2];
+        }
+        g->me[q] = m;
+      }
+}
+
+/* eint_to_temp (source.F90:20271-20392): ideal gas closed form, Helmholtz
+ * helm_rhoe_given from the carried T (energy only). */
+static void eint_to_temp(orc_grid *g) {
+  const int nv = g->nv;
+#pragma omp parallel for schedule(static)
+  for (int k = 0; k < g->n[2]; k++)
+    for (int j = 0; j < g->n[1]; j++)
+      for (int i = 0; i < g->n[0]; i++) {
+        const long c = cidx(g, i, j, k), q = iidx(g, i, j, k);
+        const double rho = g->prim[c * nv], eint = g->eint[q];
+        if (g->c.eos == ORC_EOS_HELMHOLTZ) {
+          const double ye = g->prim[c * nv + g->ias1];
+          const double abar = 1.0 / g->prim[c * nv + g->ias1 + 1];
+          const double zbar = ye * abar;
+          double T = g->temp[c], pr, sound;
+          orc_helm_rhoe_given(g->helm, rho, eint / rho, abar, zbar, &T, &pr, &sound, 0);
+          g->temp[c] = T;
+        } else {
+          g->temp[c] = g->gmm1 * eint / (g->inv_mu * rho * CONST_RGAS);
+        }
+      }
+}
+
+/* thermal_diffusion_step (source.F90:19949-20269), evolving eint (no
+ * STS_EVOLVE_TEMP): RKL2 super-time-stepping over dt/2 with
+ * s = max(3, 1 + floor((sqrt(9 + 16 dt/dt_par) - 1) / 2)) stages
+ * (Meyer, Balsara & Aslam 2014), then T from eint and p (Helmholtz: eint,
+ * p and the FASTEOS gammas) from (rho, T). */
+int orc_diffusion_step(orc_grid *g, int strang) {
+  (void)strang;
+  if (!g->kappa) return -1;
+  const int nv = g->nv, ip = g->ip;
+  const long ni = (long)g->n[0] * g->n[1] * g->n[2];
+  const double fac = g->dt / g->dt_par;
+  const double half_dt = 0.5 * g->dt;
+  int s = 1 + (int)floor(0.5 * (sqrt(9.0 + 16.0 * fac) - 1.0));
+  if (s < 3) s = 3;
+  const double w1 = 4.0 / ((double)(s * s + s) - 2.0);
+  const double b1 = 1.0 / 3.0, mut1 = b1 * w1;
+  for (long q = 0; q < ni; q++) g->e0[q] = g->eint[q];
+  halo_periodic(g, g->prim, nv);
+  diff_bcs(g);
+  diff_residual(g);
+  for (long q = 0; q < ni; q++) {
+    g->me0[q] = g->me[q];
+    const double e1 = g->e0[q] + half_dt * mut1 * g->me0[q];
+    g->ejm2[q] = g->e0[q];
+    g->eint[q] = e1;
+  }
+  double bjm2 = 1.0 / 3.0, bjm1 = 1.0 / 3.0;
+  for (int j = 2; j <= s; j++) {
+    const double bj = ((double)(j * j + j) - 2.0) / (2.0 * j * (j + 1.0));
+    const double muj = ((2.0 * j - 1.0) / j) * bj / bjm1;
+    const double mutj = muj * w1;
+    const double nuj = -(((j - 1.0) / j) * bj / bjm2);
+    const double ajm1 = 1.0 - bjm1;
+    const double gammatj = -ajm1 * mutj;
+    eint_to_temp(g);
+    diff_bcs(g);
+    diff_residual(g);
+    for (long q = 0; q < ni; q++) {
+      const double ejm1 = g->eint[q];
+      g->eint[q] = muj * ejm1 + nuj * g->ejm2[q] + (1.0 - muj - nuj) * g->e0[q] +
+                   half_dt * mutj * g->me[q] + half_dt * gammatj * g->me0[q];
+      g->ejm2[q] = ejm1;
+    }
+    bjm2 = bjm1;
+    bjm1 = bj;
+  }
+  eint_to_temp(g);
+  for (int k = 0; k < g->n[2]; k++)
+    for (int j = 0; j < g->n[1]; j++)
+      for (int i = 0; i < g->n[0]; i++) {
+        const long c = cidx(g, i, j, k), q = iidx(g, i, j, k);
+        const double rho = g->prim[c * nv], T = g->temp[c];
+        if (g->c.eos == ORC_EOS_HELMHOLTZ) {
+          const double ye = g->prim[c * nv + g->ias1];
+          const double abar = 1.0 / g->prim[c * nv + g->ias1 + 1];
+          const double zbar = abar * ye;
+          double pr, e, sound, cv;
+          orc_helm_full(g->helm, rho, T, abar, zbar, &pr, &e, &sound, &cv);
+          const double eint = rho * e;
+          g->prim[c * nv + ip] = pr;
+          g->eint[q] = eint;
+          if (g->c.fasteos) {
+            g->gamf[c * 2] = pr / eint + 1.0;
+            g->gamf[c * 2 + 1] = sound * sound * rho / pr;
+          }
+        } else {
+          g->prim[c * nv + ip] = CONST_RGAS * rho * T * g->inv_mu;
+        }
+      }
+  return 0;
+}
+
 int orc_run(orc_grid *g, int nsteps) {
+  const int diff = g->c.thermal_diffusion && g->kappa;
   for (int s = 0; s < nsteps; s++) {
     if (!(g->time < g->c.tmax)) break;                             /* :6525 */
     if (g->time + g->dt > g->c.tmax) g->dt = g->c.tmax - g->time; /* :6662-6664 */
+    if (diff) {                                                    /* :6672-6679 */
+      if (g->c.eval_parabolic_dt) orc_parabolic_dt(g);
+      orc_diffusion_step(g, 1);
+    }
     orc_hydro_step(g);
+    if (diff) orc_diffusion_step(g, 2);                             /* :6685-6687 */
     g->time = g->time + g->dt;
     orc_compute_dt(g);
     g->step++;

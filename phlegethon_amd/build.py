@@ -14,7 +14,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "_lib")
 OUT = os.path.join(OUT_DIR, "libphlx.so")
-SOURCES = ["phlx_kernels.hip", "phlx_helm.hip", "phlx_mlp.hip", "phlx_api.hip"]
+SOURCES = ["phlx_kernels.hip", "phlx_helm.hip", "phlx_mlp.hip", "phlx_diffusion.hip",
+           "phlx_api.hip"]
 ARCH = os.environ.get("PHLX_OFFLOAD_ARCH", "gfx950")
 
 

@@ -10,7 +10,7 @@ from __future__ import annotations
 
 import ctypes
 from dataclasses import dataclass, field, replace
-from typing import Sequence
+from typing import Optional, Sequence
 
 LIM2ND_REC, PPH_REC, LIM5TH_REC = 0, 1, 2
 HLLC_FLUX, LHLLC_FLUX, HLLD_FLUX, LHLLD_FLUX = 0, 1, 2, 3
@@ -94,6 +94,12 @@ class PhlxConfig:
     # :8073-8131); applied through phlx_set_shock_flattening, not the C struct
     shock_flattening: bool = False
     eps_sf: float = 0.25
+    # THERMAL_DIFFUSION_STS (RKL2 around hydro_step), EVALUATE_PARABOLIC_TIMESTEP
+    # and FIX_TEMPERATURE_AT_X1L..X3U (wall temperature per side, None = the
+    # zero-gradient ghost); applied through phlx_set_thermal_diffusion
+    thermal_diffusion: bool = False
+    eval_parabolic_dt: bool = True
+    fix_temperature: Optional[Sequence[Optional[float]]] = None
 
     # ------------------------------------------------------------------
     @property
@@ -165,6 +171,17 @@ class PhlxConfig:
                 raise ConfigError("USE_SHOCK_FLATTENING needs at least 3 cells per axis")
             if not self.eps_sf > 0:
                 raise ConfigError("eps_sf_make must be positive")
+        if self.thermal_diffusion:
+            if self.eos == EOS_IDEAL and self.nas != 0:
+                raise ConfigError("THERMAL_DIFFUSION_STS with the ideal gas runs with "
+                                  "nas_make=0 (a constant mean molecular weight)")
+            if self.fix_temperature is not None:
+                if len(tuple(self.fix_temperature)) != 6:
+                    raise ConfigError("fix_temperature needs six entries X1L X1U X2L X2U X3L X3U")
+                for d, t in enumerate(self.fix_temperature):
+                    if t is not None and (not t > 0 or tuple(self.bc)[d] == BC_PERIODIC):
+                        raise ConfigError("FIX_TEMPERATURE_AT_* needs a positive wall "
+                                          "temperature on a non-periodic side")
         if self.cfl <= 0 or self.gamma <= 1 or self.mu <= 0:
             raise ConfigError("cfl, gamma-1 and mu must be positive")
         return self
@@ -212,6 +229,11 @@ class PhlxConfig:
                 d["all_mach"] = True
             elif o == "USE_SHOCK_FLATTENING":
                 d["shock_flattening"] = True
+            elif o == "THERMAL_DIFFUSION_STS":
+                d["thermal_diffusion"] = True
+                d.setdefault("eval_parabolic_dt", False)
+            elif o == "EVALUATE_PARABOLIC_TIMESTEP":
+                d["eval_parabolic_dt"] = True
             elif o in ("USE_WB", "USE_GRAVITY", "EVOLVE_ETOT", "USE_EDOT"):
                 d[o.lower()] = True
             elif o[:3] in sides and o[3:4] == "_" and o[4:] in _BC:

@@ -48,6 +48,12 @@ struct phlx_ctx {
   double mlp_box[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   bool mlp_has_box = false;
   bool eos_stats = false;
+  // THERMAL_DIFFUSION_STS: on / dt_par every step (else at step 0), the
+  // host copy of dt_par, kappa uploaded, the ideal gas' diffusion T current
+  // (false after any hydro update: the next step recomputes it from p)
+  bool diff_on = false, diff_eval = true, kappa_set = false, tdiff_valid = false;
+  double dt_par = 0.0;
+  int diff_stages = 0;  // stages of the last diffusion step
   // profiling
   bool prof = false;
   std::vector<ProfRec> recs;
@@ -339,6 +345,9 @@ extern "C" int phlx_download_state(phlx_ctx *ctx, double *prim, double *eint, do
   if (eint) rc |= copy_out(ctx, eint, g.eint, sizeof(double) * g.N);
   if (temp && g.temp) {
     rc |= copy_out(ctx, temp, g.temp, sizeof(double) * g.N);
+  } else if (temp && g.tdiff && ctx->tdiff_valid) {
+    // the ideal gas after a thermal-diffusion step: its eint_to_temp T
+    rc |= copy_out(ctx, temp, g.tdiff, sizeof(double) * g.N);
   } else if (temp) {
     // T = p / (rho R / mu) for the ideal gas (source/source.F90:13670-13674)
     std::vector<double> p(2 * g.N);
@@ -557,6 +566,7 @@ extern "C" int phlx_ct(phlx_ctx *ctx, int32_t irk, double a1, double a2, double 
 extern "C" int phlx_rk_update(phlx_ctx *ctx, int32_t irk, double a1, double a2, double a3,
                               int32_t last_stage) {
   if (!ctx) return PHLX_E_ARG;
+  ctx->tdiff_valid = false;
   ProfRec r;
   int on = kbegin(ctx, KID_UPDATE, r);
   const bool guess = ctx->kt.update_guess && (ctx->g.eos_guess != GUESS_WARM || ctx->eos_stats);
@@ -567,6 +577,9 @@ extern "C" int phlx_rk_update(phlx_ctx *ctx, int32_t irk, double a1, double a2, 
   kend(ctx, on, r);
   return check_launch();
 }
+
+static int diffusion_step(phlx_ctx *ctx, int strang);
+static int parabolic_dt(phlx_ctx *ctx);
 
 static int do_hydro_step(phlx_ctx *ctx, bool with_cfl) {
   for (int irk = 0; irk < ctx->stages; irk++) {
@@ -599,6 +612,10 @@ extern "C" int phlx_init(phlx_ctx *ctx) {
   int rc = check_launch();
   if (rc) return rc;
   ctx->initialized = true;
+  ctx->tdiff_valid = false;
+  // without EVALUATE_PARABOLIC_TIMESTEP dt_par is computed once, at step 0
+  // (source/source.F90:6504-6512)
+  if (ctx->diff_on && !ctx->diff_eval && (rc = parabolic_dt(ctx))) return rc;
   return phlx_compute_dt(ctx, nullptr, nullptr);
 }
 
@@ -652,10 +669,22 @@ extern "C" int phlx_run(phlx_ctx *ctx, int32_t nsteps) {
   if (!ctx || nsteps < 0) return PHLX_E_ARG;
   if (!ctx->initialized) return PHLX_E_STATE;
   HIPCHK(hipSetDevice(ctx->device));
+  const bool diff = ctx->diff_on;
   for (int s = 0; s < nsteps; s++) {
     launch_step_begin(ctx->g.sc, ctx->stream);
-    int rc = do_hydro_step(ctx, true);
+    int rc;
+    if (diff) {
+      // THERMAL_DIFFUSION_STS around hydro_step (source/source.F90:6672-6688);
+      // the hyperbolic dt follows the second half (:6716)
+      if (ctx->diff_eval && (rc = parabolic_dt(ctx))) return rc;
+      if ((rc = diffusion_step(ctx, 1))) return rc;
+    }
+    rc = do_hydro_step(ctx, !diff);
     if (rc) return rc;
+    if (diff) {
+      if ((rc = diffusion_step(ctx, 2))) return rc;
+      ctx->kt.cfl(ctx->g, ctx->stream);
+    }
     launch_step_end(ctx->g, 1, ctx->stream);
     rc = check_launch();
     if (rc) return rc;
@@ -1162,6 +1191,154 @@ extern "C" int phlx_set_eos_guess(phlx_ctx *ctx, int32_t mode, double resid_tol,
 extern "C" int phlx_set_eos_lin_tol(phlx_ctx *ctx, double lin_tol) {
   if (!ctx || !(lin_tol >= 0.0) || lin_tol > 1e-2) return PHLX_E_ARG;
   ctx->g.eos_lin_tol = lin_tol;
+  return PHLX_OK;
+}
+
+// ------------------------------------------------------- thermal diffusion
+
+// THERMAL_DIFFUSION_STS (source/source.F90:19804-21068).  The ideal gas needs
+// a constant mean molecular weight (nas 0); Helmholtz diffuses with its
+// carried T.  fix_t: six wall temperatures (NaN: zero-gradient ghost), only
+// on non-periodic sides.
+extern "C" int phlx_set_thermal_diffusion(phlx_ctx *ctx, int32_t enable, int32_t evaluate_dt,
+                                          const double *fix_t) {
+  if (!ctx || (enable != 0 && enable != 1) || (evaluate_dt != 0 && evaluate_dt != 1))
+    return PHLX_E_ARG;
+  Grid &g = ctx->g;
+  if (!enable) {
+    ctx->diff_on = false;
+    return PHLX_OK;
+  }
+  const bool helm = ctx->cfg.eos == PHLX_EOS_HELMHOLTZ;
+  if (!helm && g.nas != 0) return PHLX_E_CONFIG;
+  double ft[6];
+  for (int i = 0; i < 6; i++) {
+    ft[i] = fix_t ? fix_t[i] : std::nan("");
+    if (ft[i] == ft[i]) {
+      const int d = i / 2;
+      if (d >= g.sd || g.bc[i] == PHLX_BC_PERIODIC || !(ft[i] > 0.0) || !std::isfinite(ft[i]))
+        return PHLX_E_ARG;
+    }
+  }
+  HIPCHK(hipSetDevice(ctx->device));
+  if (!g.e0) {
+    const long long k3 = (g.sd == 3) ? g.n3 + 2 : 1;
+    const size_t nk = (size_t)(g.n1 + 2) * (g.n2 + 2) * k3;
+    int rc = dalloc(ctx, &g.kappa, nk);
+    if (!rc) rc = dalloc(ctx, &g.e0, (size_t)g.N);
+    if (!rc) rc = dalloc(ctx, &g.ejm2, (size_t)g.N);
+    if (!rc) rc = dalloc(ctx, &g.me0, (size_t)g.N);
+    if (!rc && !helm) rc = dalloc(ctx, &g.tdiff, (size_t)g.N);
+    if (rc) return rc;
+    if (helm) g.tdiff = g.temp;
+  }
+  for (int i = 0; i < 6; i++) g.fix_t[i] = ft[i];
+  ctx->diff_eval = evaluate_dt != 0;
+  ctx->diff_on = true;
+  return PHLX_OK;
+}
+
+// kappa on cells -1 .. n of each axis (one ghost layer, x3 only in 3D),
+// x1 fastest, as the application fills lgrid%kappa.
+extern "C" int phlx_set_kappa(phlx_ctx *ctx, const double *kappa) {
+  if (!ctx || !kappa) return PHLX_E_ARG;
+  Grid &g = ctx->g;
+  if (!g.kappa) return PHLX_E_STATE;
+  const long long k3 = (g.sd == 3) ? g.n3 + 2 : 1;
+  const size_t nk = (size_t)(g.n1 + 2) * (g.n2 + 2) * k3;
+  for (size_t i = 0; i < nk; i++)
+    if (!(kappa[i] > 0.0) || !std::isfinite(kappa[i])) return PHLX_E_ARG;
+  HIPCHK(hipSetDevice(ctx->device));
+  HIPCHK(hipMemcpyAsync(g.kappa, kappa, nk * sizeof(double), hipMemcpyHostToDevice,
+                        ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  ctx->kappa_set = true;
+  return PHLX_OK;
+}
+
+static int parabolic_dt(phlx_ctx *ctx) {
+  if (!ctx->kappa_set) return PHLX_E_STATE;
+  const Grid &g = ctx->g;
+  const double i1 = g.inv_dx[0], i2 = g.inv_dx[1], i3 = (g.sd == 3) ? g.inv_dx[2] : 0.0;
+  ProfRec r;
+  int on = kbegin(ctx, KID_DIFF, r);
+  launch_parabolic(g, i1 * i1 + i2 * i2 + i3 * i3, ctx->stream);
+  launch_reduce_to(g, ctx->scratch_val, ctx->stream);
+  kend(ctx, on, r);
+  int rc = check_launch();
+  if (rc) return rc;
+  double m = 0.0;
+  HIPCHK(hipMemcpyAsync(&m, ctx->scratch_val, sizeof(m), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  ctx->dt_par = 1.0 / m;
+  return PHLX_OK;
+}
+
+extern "C" int phlx_parabolic_dt(phlx_ctx *ctx, double *dt_par) {
+  if (!ctx) return PHLX_E_ARG;
+  if (!ctx->diff_on) return PHLX_E_CONFIG;
+  HIPCHK(hipSetDevice(ctx->device));
+  int rc = parabolic_dt(ctx);
+  if (rc) return rc;
+  if (dt_par) *dt_par = ctx->dt_par;
+  return PHLX_OK;
+}
+
+// thermal_diffusion_step over dt/2 (source/source.F90:19949-20269): the
+// stage count and the RKL2 coefficients on the host from the step's dt.
+static int diffusion_step(phlx_ctx *ctx, int strang) {
+  (void)strang;
+  if (!ctx->kappa_set || !(ctx->dt_par > 0.0)) return PHLX_E_STATE;
+  Scalars sc;
+  HIPCHK(hipMemcpyAsync(&sc, ctx->g.sc, sizeof(sc), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  if (!sc.active) return PHLX_OK;
+  const Grid &g = ctx->g;
+  const bool helm = ctx->cfg.eos == PHLX_EOS_HELMHOLTZ;
+  const double fac = sc.dt / ctx->dt_par, half_dt = 0.5 * sc.dt;
+  const double sr = std::floor(0.5 * (std::sqrt(9.0 + 16.0 * fac) - 1.0));
+  if (!(sr < 1e6)) return PHLX_E_ARG;  // dt / dt_par beyond any sensible stage count
+  int s = 1 + (int)sr;
+  if (s < 3) s = 3;
+  const double w1 = 4.0 / ((double)(s * s + s) - 2.0);
+  const double mut1 = (1.0 / 3.0) * w1;
+  ProfRec r;
+  int on = kbegin(ctx, KID_DIFF, r);
+  if (!helm && !ctx->tdiff_valid) launch_diff_t_from_p(g, ctx->stream);
+  launch_diff_stage(g, true, 0.0, 0.0, 0.0, half_dt * mut1, 0.0, ctx->stream);
+  double bjm2 = 1.0 / 3.0, bjm1 = 1.0 / 3.0;
+  for (int j = 2; j <= s; j++) {
+    const double bj = ((double)(j * j + j) - 2.0) / (2.0 * j * (j + 1.0));
+    const double muj = ((2.0 * j - 1.0) / j) * bj / bjm1;
+    const double mutj = muj * w1;
+    const double nuj = -(((j - 1.0) / j) * bj / bjm2);
+    const double gammatj = -(1.0 - bjm1) * mutj;
+    launch_diff_temp(g, helm, ctx->stream);
+    launch_diff_stage(g, false, muj, nuj, 1.0 - muj - nuj, half_dt * mutj, half_dt * gammatj,
+                      ctx->stream);
+    bjm2 = bjm1;
+    bjm1 = bj;
+  }
+  launch_diff_final(g, helm, ctx->stream);
+  kend(ctx, on, r);
+  int rc = check_launch();
+  if (rc) return rc;
+  ctx->tdiff_valid = true;
+  ctx->diff_stages = s;
+  return PHLX_OK;
+}
+
+extern "C" int phlx_diffusion_step(phlx_ctx *ctx, int32_t strang) {
+  if (!ctx || (strang != 1 && strang != 2)) return PHLX_E_ARG;
+  if (!ctx->initialized) return PHLX_E_STATE;
+  if (!ctx->diff_on) return PHLX_E_CONFIG;
+  HIPCHK(hipSetDevice(ctx->device));
+  return diffusion_step(ctx, strang);
+}
+
+extern "C" int phlx_diffusion_stages(phlx_ctx *ctx, int32_t *stages) {
+  if (!ctx || !stages) return PHLX_E_ARG;
+  *stages = ctx->diff_stages;
   return PHLX_OK;
 }
 

@@ -72,6 +72,12 @@ struct Grid {
   int sflat;
   double eps_sf;
   unsigned char *sf_det, *sf_flag;
+  // THERMAL_DIFFUSION_STS: kappa on cells -1 .. n (one ghost layer), the
+  // temperature the residual reads (temp with Helmholtz, an own array for
+  // the ideal gas), RKL2 registers e_0, e_jm2, Me_0, wall temperatures of
+  // FIX_TEMPERATURE_AT_* (NaN: zero-gradient ghost)
+  double *kappa, *tdiff, *e0, *ejm2, *me0;
+  double fix_t[6];
   double *red;       // per-block partial maxima of the cell-grid reductions
   int nred;          // number of partials (blocks of a cell-grid launch)
   Scalars *sc;

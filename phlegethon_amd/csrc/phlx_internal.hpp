@@ -29,6 +29,14 @@ void launch_stream_copy(const void *src, void *dst, long long n16, hipStream_t s
 // USE_SHOCK_FLATTENING flags of the current primitive state (g.sf_flag)
 void launch_shock_flags(const Grid &g, hipStream_t st);
 
+// THERMAL_DIFFUSION_STS (phlx_diffusion.hip)
+void launch_diff_t_from_p(const Grid &g, hipStream_t st);
+void launch_diff_stage(const Grid &g, bool first, double mu, double nu, double c0, double hm,
+                       double hg, hipStream_t st);
+void launch_diff_temp(const Grid &g, bool helm, hipStream_t st);
+void launch_diff_final(const Grid &g, bool helm, hipStream_t st);
+void launch_parabolic(const Grid &g, double inv_dl2, hipStream_t st);
+
 // Helmholtz EoS (phlx_helm.hip)
 void launch_helm_pack(long long np, const double *f9, const double *c4, double *tab,
                       hipStream_t st);
@@ -62,6 +70,9 @@ void launch_helm_rhoe_guess(const HelmDev &h, const MlpDev &m, int mode, long lo
                             const double *zbar, double *T, double *P, double *cs, int ret,
                             int *iters, int *fb, hipStream_t st);
 
-enum { KID_FLUX1 = 0, KID_FLUX2, KID_FLUX3, KID_CORNER, KID_FACEB, KID_UPDATE, KID_MISC, KID_COUNT };
+enum {
+  KID_FLUX1 = 0, KID_FLUX2, KID_FLUX3, KID_CORNER, KID_FACEB, KID_UPDATE, KID_MISC, KID_DIFF,
+  KID_COUNT
+};
 
 }  // namespace phlx

@@ -53,13 +53,15 @@ class Problem:
     profiles: Dict[str, np.ndarray] = field(default_factory=dict)
     temp: Optional[np.ndarray] = None      # initial T (Helmholtz warm start)
     helm: Optional[object] = None          # helmholtz.HelmTable
+    kappa: Optional[np.ndarray] = None     # opacity on cells -1..n (one ghost layer)
 
     def copy(self) -> "Problem":
         return Problem(self.name, self.cfg, self.prim.copy(),
                        None if self.b_faces is None else
                        [None if b is None else b.copy() for b in self.b_faces],
                        {k: v.copy() for k, v in self.profiles.items()},
-                       None if self.temp is None else self.temp.copy(), self.helm)
+                       None if self.temp is None else self.temp.copy(), self.helm,
+                       None if self.kappa is None else self.kappa.copy())
 
 
 def centres(cfg: PhlxConfig, d: int) -> np.ndarray:

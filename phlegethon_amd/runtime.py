@@ -89,6 +89,11 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.phlx_set_eos_guess.argtypes = [P, I, D, I]
     lib.phlx_set_eos_lin_tol.argtypes = [P, D]
     lib.phlx_set_shock_flattening.argtypes = [P, I, D]
+    lib.phlx_set_thermal_diffusion.argtypes = [P, I, I, P]
+    lib.phlx_set_kappa.argtypes = [P, P]
+    lib.phlx_parabolic_dt.argtypes = [P, DP]
+    lib.phlx_diffusion_step.argtypes = [P, I]
+    lib.phlx_diffusion_stages.argtypes = [P, ctypes.POINTER(ctypes.c_int32)]
     I64P = ctypes.POINTER(ctypes.c_int64)
     lib.phlx_eos_stats.argtypes = [P, I64P, I64P, I64P, I]
     _lib = lib
@@ -111,7 +116,7 @@ MLP_NPARAM = 4545
 GUESS_MODES = {"warm": 0, "mlp": 1, "bracket": 2}
 
 KERNELS = {"flux_x1": 0, "flux_x2": 1, "flux_x3": 2, "corner": 3, "faceb": 4,
-           "update": 5, "misc": 6}
+           "update": 5, "misc": 6, "diffusion": 7}
 
 
 class Simulation:
@@ -125,6 +130,9 @@ class Simulation:
         _chk(self.lib.phlx_create(ctypes.byref(c), ctypes.byref(self._ctx)), "phlx_create")
         if self.cfg.shock_flattening:
             self.set_shock_flattening(True, self.cfg.eps_sf)
+        if self.cfg.thermal_diffusion:
+            self.set_thermal_diffusion(True, self.cfg.eval_parabolic_dt,
+                                       self.cfg.fix_temperature)
 
     # -- lifecycle -------------------------------------------------------
     def close(self):
@@ -203,6 +211,8 @@ class Simulation:
         self.upload(problem.prim, problem.b_faces, temp=problem.temp)
         if problem.profiles:
             self.set_profiles(**problem.profiles)
+        if getattr(problem, "kappa", None) is not None:
+            self.set_kappa(problem.kappa)
         return self
 
     # -- the reference's entry points ------------------------------------
@@ -352,6 +362,42 @@ class Simulation:
         faces beside them the HLL flux (FP64 contexts)."""
         _chk(self.lib.phlx_set_shock_flattening(self._ctx, int(bool(enable)), float(eps_sf)),
              "phlx_set_shock_flattening")
+
+    def set_thermal_diffusion(self, enable: bool = True, evaluate_dt: bool = True,
+                              fix_temperature=None):
+        """THERMAL_DIFFUSION_STS (RKL2 around every hydro step);
+        fix_temperature: six wall temperatures or None entries."""
+        ft = None
+        if fix_temperature is not None:
+            ft = np.array([np.nan if v is None else float(v) for v in fix_temperature],
+                          dtype=np.float64)
+            assert ft.size == 6
+        _chk(self.lib.phlx_set_thermal_diffusion(self._ctx, int(bool(enable)),
+                                                 int(bool(evaluate_dt)), _ptr(ft)),
+             "phlx_set_thermal_diffusion")
+        self._keep_ft = ft
+
+    def set_kappa(self, kappa: np.ndarray):
+        """Opacity on cells -1..n per axis: shape (nz+2 or 1, ny+2, nx+2)."""
+        c = self.cfg
+        shp = ((c.nx3 + 2) if c.sdims == 3 else 1, c.nx2 + 2, c.nx1 + 2)
+        k = np.ascontiguousarray(kappa, dtype=np.float64)
+        if k.shape != shp:
+            raise ValueError(f"kappa shape {k.shape}, expected {shp}")
+        _chk(self.lib.phlx_set_kappa(self._ctx, _ptr(k)), "phlx_set_kappa")
+
+    def parabolic_dt(self) -> float:
+        v = ctypes.c_double()
+        _chk(self.lib.phlx_parabolic_dt(self._ctx, ctypes.byref(v)), "phlx_parabolic_dt")
+        return v.value
+
+    def diffusion_step(self, strang: int = 1):
+        _chk(self.lib.phlx_diffusion_step(self._ctx, int(strang)), "phlx_diffusion_step")
+
+    def diffusion_stages(self) -> int:
+        v = ctypes.c_int32()
+        _chk(self.lib.phlx_diffusion_stages(self._ctx, ctypes.byref(v)), "phlx_diffusion_stages")
+        return v.value
 
     def eos_stats(self, reset: bool = False):
         it, n, fb = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()

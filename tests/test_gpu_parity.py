@@ -27,6 +27,8 @@ def gpu_run(pb, nsteps):
     sim.upload(pb.prim, pb.b_faces)
     if pb.profiles:
         sim.set_profiles(**pb.profiles)
+    if pb.kappa is not None:
+        sim.set_kappa(pb.kappa)
     sim.init()
     sim.run(nsteps)
     st = sim.download()
