@@ -73,6 +73,23 @@ def kernel_bytes(cfg, name: str) -> float:
     return 0.0
 
 
+def diffusion_bytes(cfg, stages: int) -> float:
+    """Algorithmic bytes of one thermal-diffusion step (RKL2, s stages):
+    stage 1 reads T, rho, kappa, eint and writes e_0, Me_0, e_jm2, eint;
+    every further stage converts eint to T (eint, rho in, T out; Helmholtz
+    also Ye, 1/abar and the carried T in) and reads T, rho, kappa, eint,
+    e_jm2, e_0, Me_0, writing eint and e_jm2; the end converts once more and
+    writes T and p (Helmholtz also eint and the FASTEOS gammas)."""
+    nz, ny, nx = cfg.shape
+    N = nx * ny * nz
+    helm = cfg.eos == 1
+    to_t = 24 + (24 if helm else 0)
+    stage = 24 + 32 + 16
+    first = 24 + 8 + 32 + (0 if helm else 24)   # + T from p for the ideal gas
+    final = to_t + 8 + (24 if helm and cfg.fasteos else (8 if helm else 0))
+    return N * (first + (stages - 1) * (to_t + stage) + final)
+
+
 def b_zu(cfg) -> int:
     """Algorithmic bytes per zone-update of the whole step, FP64, RK2:
     8 * (5 n_q + 4 n_e) with n_q evolved fields (conserved + face B) and n_e
@@ -180,6 +197,25 @@ def deviation_errors(st, ref, pb):
     return out
 
 
+def add_diffusion(pb, ratio: float):
+    """Turn on THERMAL_DIFFUSION_STS with the uniform kappa whose dt_par is
+    the first hyperbolic dt / ratio (dt_par = 3 kappa rho_min / (2 c sum
+    1/dx^2))."""
+    import numpy as np
+    from phlegethon_amd.runtime import Simulation
+    sim = Simulation(pb.cfg)
+    sim.setup(pb)
+    sim.init()
+    dt = sim.scalars()["dt"]
+    sim.close()
+    cfg = pb.cfg
+    inv_dl2 = sum(1.0 / cfg.dx[d] ** 2 for d in range(cfg.sdims))
+    kap = 2.0 * 2.99792458e10 * inv_dl2 * (dt / ratio) / (3.0 * float(pb.prim[0].min()))
+    nz = cfg.nx3 + 2 if cfg.sdims == 3 else 1
+    pb.kappa = np.full((nz, cfg.nx2 + 2, cfg.nx1 + 2), kap)
+    pb.cfg = cfg.with_(thermal_diffusion=True, eval_parabolic_dt=True)
+
+
 def run_variant(pb, cfg, args, guess="warm", keep_state=False, profile=True):
     """Upload, init, warm up, time args.steps steps; optionally profile."""
     from phlegethon_amd.runtime import Simulation
@@ -232,11 +268,20 @@ def run_variant(pb, cfg, args, guess="warm", keep_state=False, profile=True):
         sim.run(args.profile_steps)
         sim.synchronize()
         kern = {}
-        for name in ("flux_x1", "flux_x2", "flux_x3", "corner", "faceb", "update", "misc"):
+        for name in ("flux_x1", "flux_x2", "flux_x3", "corner", "faceb", "update", "misc",
+                     "diffusion"):
             ms, cnt = sim.profile_read(name)
             if cnt:
                 kern[name] = {"total_ms": ms, "launches": cnt, "avg_ms": ms / cnt,
                               "alg_bytes": kernel_bytes(cfg, name)}
+        if "diffusion" in kern:
+            # per step: two diffusion steps and one dt_par pass (rho, kappa)
+            s = sim.diffusion_stages()
+            nz, ny, nx = cfg.shape
+            per_step = 2 * diffusion_bytes(cfg, s) + 16 * nx * ny * nz
+            kern["diffusion"]["alg_bytes"] = per_step * args.profile_steps / \
+                kern["diffusion"]["launches"]
+            kern["diffusion"]["stages"] = s
         sim.profile(False)
         res["kern"] = kern
         res["copy_gbps"] = sim.stream_copy(1 << 30, 10)
@@ -278,6 +323,9 @@ def main():
                          "whole table range")
     ap.add_argument("--shock-flattening", type=float, default=None, metavar="EPS_SF",
                     help="USE_SHOCK_FLATTENING at this eps_sf (FP64)")
+    ap.add_argument("--diffusion", type=float, default=None, metavar="RATIO",
+                    help="THERMAL_DIFFUSION_STS with a uniform kappa chosen so that the "
+                         "first step's dt / dt_par is RATIO")
     ap.add_argument("--mlp-samples", type=int, default=1 << 22)
     ap.add_argument("--mlp-epochs", type=int, default=8)
     args = ap.parse_args()
@@ -294,6 +342,10 @@ def main():
     if args.shock_flattening is not None:
         pb.cfg = pb.cfg.with_(shock_flattening=True, eps_sf=args.shock_flattening)
         desc += f" | shock flattening eps_sf={args.shock_flattening:g}"
+    if args.diffusion is not None:
+        add_diffusion(pb, args.diffusion)
+        desc += (f" | THERMAL_DIFFUSION_STS (RKL2), uniform kappa, dt/dt_par "
+                 f"{args.diffusion:g} at the start")
     n = args.n
     zones = pb.cfg.nx1 * pb.cfg.nx2 * (pb.cfg.nx3 if pb.cfg.sdims == 3 else 1)
     cfg = pb.cfg
@@ -313,7 +365,7 @@ def main():
     traffic = None
     pmc = os.path.join(ROOT, "profiles", "ot3d256_fp64_pmc.json")
     if (n == 256 and args.workload == "ot3d" and args.precision == "fp64"
-            and args.shock_flattening is None
+            and args.shock_flattening is None and args.diffusion is None
             and os.path.exists(pmc)):
         # HBM bytes per dispatch measured with rocprofv3 --pmc FETCH_SIZE /
         # WRITE_SIZE on this workload, scaled by a copy kernel of known size
