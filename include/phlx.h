@@ -41,7 +41,11 @@ extern "C" {
 enum { PHLX_LIM2ND_REC = 0, PHLX_PPH_REC = 1, PHLX_LIM5TH_REC = 2 };
 enum { PHLX_HLLC_FLUX = 0, PHLX_LHLLC_FLUX = 1, PHLX_HLLD_FLUX = 2, PHLX_LHLLD_FLUX = 3 };
 enum { PHLX_BC_PERIODIC = 0, PHLX_BC_REFLECTIVE = 1, PHLX_BC_OUTFLOW = 2, PHLX_BC_DIODE = 3 };
-enum { PHLX_EOS_IDEAL = 0, PHLX_EOS_HELMHOLTZ = 1 };
+/* PHLX_EOS_PIG: PIG_EOS, a partially-ionised-gas table in the Helmholtz
+ * table layout (pig_rhoT_given_*, source/source.F90:25457-26731), loaded with
+ * phlx_load_pig_table; the hydro path carries the two advected scalars
+ * (nas 2) as Helmholtz does and the EoS ignores them. */
+enum { PHLX_EOS_IDEAL = 0, PHLX_EOS_HELMHOLTZ = 1, PHLX_EOS_PIG = 2 };
 enum { PHLX_PREC_FP64 = 0, PHLX_PREC_MIXED = 1 };
 
 /* Run-time counterpart of the reference's compile-time OPTS subset
@@ -160,6 +164,12 @@ int phlx_rk_update(phlx_ctx *ctx, int32_t irk, double a1, double a2, double a3,
  * T_j = 10**(ltlo + j*(lthi-ltlo)/(nT-1)) (:23094-23104).  The library keeps
  * its own device copy; the host arrays may be freed on return. */
 int phlx_load_helm_table(phlx_ctx *ctx, const double *f9, const double *c4, int32_t nrho,
+                         int32_t nT, double ldlo, double ldhi, double ltlo, double lthi);
+/* load_pig_table (source/source.F90:25407-25455): a PIG_EOS table, same
+ * arguments and column order as phlx_load_helm_table; contexts of eos
+ * PHLX_EOS_PIG take only this one (and Helmholtz contexts only the other),
+ * a batch context of the ideal gas either. */
+int phlx_load_pig_table(phlx_ctx *ctx, const double *f9, const double *c4, int32_t nrho,
                          int32_t nT, double ldlo, double ldhi, double ltlo, double lthi);
 /* Batched helm_rhoT_given_full (source/source.F90:23686-24232): n points,
  * all pointers DEVICE memory of the context's GPU; P, E (specific internal

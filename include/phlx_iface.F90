@@ -120,6 +120,13 @@ module phlx_iface
       integer(c_int32_t), value :: nrho, nt
       real(c_double), value :: ldlo, ldhi, ltlo, lthi
     end function
+    integer(c_int) function phlx_load_pig_table(ctx, f9, c4, nrho, nt, ldlo, ldhi, ltlo, lthi) &
+        bind(c, name="phlx_load_pig_table")
+      import :: c_int, c_ptr, c_int32_t, c_double
+      type(c_ptr), value :: ctx, f9, c4
+      integer(c_int32_t), value :: nrho, nt
+      real(c_double), value :: ldlo, ldhi, ltlo, lthi
+    end function
     integer(c_int) function phlx_set_eos_profiles(ctx, eq_gammae_cc, eq_gammae_f) &
         bind(c, name="phlx_set_eos_profiles")
       import :: c_int, c_ptr

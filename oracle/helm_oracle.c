@@ -37,6 +37,7 @@ struct orc_helm {
   double *c;         /* [j][i][4] internal ids */
   double tl, th;     /* Tl_eos, Th_eos (source.F90:750-751) */
   double lrho1, lT1; /* log10 of the first nodes */
+  int pig;           /* PIG_EOS table: the whole gas in f(rho, T), Ye = 1, no ions */
 };
 
 /* file order: f, fd, ft, fdd, ftt, fdt, fddt, fdtt, fddtt -> internal */
@@ -203,9 +204,101 @@ static double sum16(const orc_helm *h, const cellpos *c) {
   return s;
 }
 
+/* ---------------------------------------------------------------- PIG EoS */
+
+/* pig_rhoT_given_full (source.F90:25457-25900): the table holds the free
+ * energy of the whole partially ionised gas at rho (Ye = 1), radiation is
+ * added; no ion terms. */
+static void pig_full(const orc_helm *h, double rho, double T, double *P, double *E,
+                     double *sound, double *cv) {
+  const double rhos = rho;
+  cellpos c = locate(h, rhos, T);
+  basis r, dr, t, dt, ddt;
+  quintic(c.x, c.drho, &r, &dr, NULL);
+  quintic(c.y, c.dT, &t, &dt, &ddt);
+  const double f = sum36(h, &c, &r, &t);
+  const double df_drho = sum36(h, &c, &dr, &t);
+  const double df_dT = sum36(h, &c, &r, &dt);
+  const double df_drhodT = sum36(h, &c, &dr, &dt);
+  const double df_dT2 = sum36(h, &c, &r, &ddt);
+  const double dP_drho = sum16(h, &c);
+  const double tmp = rhos * rhos;
+  double p = tmp * df_drho;
+  double dP_dT = tmp * df_drhodT;
+  const double s = -df_dT;
+  const double ds_dT = -df_dT2;
+  double e = f + T * s;
+  double dE_dT = T * ds_dT;
+  const double T2 = T * T, T3 = T2 * T, T4 = T3 * T;
+  p = p + CRAD * T4 * (1.0 / 3.0);
+  dP_dT = dP_dT + (4.0 / 3.0) * CRAD * T3;
+  e = e + CRAD * T4 / rho;
+  dE_dT = dE_dT + 4.0 * CRAD * T3 / rho;
+  const double zz = p / rho;
+  const double chiT = T / p * dP_dT;
+  const double chirho = dP_drho / zz;
+  const double x = zz * chiT / (T * dE_dT);
+  const double gam1 = chiT * x + chirho;
+  const double z = 1.0 + (e + CLIGHT2) / zz;
+  *P = p;
+  *E = e;
+  *cv = dE_dT;
+  *sound = CLIGHT * sqrt(gam1 / z);
+}
+
+/* pig_rhoT_given1 (source.F90:26094-26353): P and dP/dT */
+static void pig_given1(const orc_helm *h, double rho, double T, double *P, double *dPdT,
+                       int ret) {
+  const double rhos = rho;
+  cellpos c = locate(h, rhos, T);
+  basis r, dr, t, dt;
+  quintic(c.x, c.drho, &r, &dr, NULL);
+  quintic(c.y, c.dT, &t, &dt, NULL);
+  const double df_drho = sum36(h, &c, &dr, &t);
+  double df_drhodT = 0.0;
+  if (ret) df_drhodT = sum36(h, &c, &dr, &dt);
+  const double tmp = rhos * rhos;
+  double p = tmp * df_drho;
+  double dP_dT = tmp * df_drhodT;
+  const double T2 = T * T, T3 = T2 * T, T4 = T3 * T;
+  p = p + CRAD * T4 * (1.0 / 3.0);
+  dP_dT = dP_dT + (4.0 / 3.0) * CRAD * T3;
+  *P = p;
+  *dPdT = dP_dT;
+}
+
+/* pig_rhoT_given2 (source.F90:26355-26653): E and dE/dT */
+static void pig_given2(const orc_helm *h, double rho, double T, double *E, double *dEdT,
+                       int ret) {
+  const double rhos = rho;
+  cellpos c = locate(h, rhos, T);
+  basis r, t, dt, ddt;
+  quintic(c.x, c.drho, &r, NULL, NULL);
+  quintic(c.y, c.dT, &t, &dt, &ddt);
+  const double f = sum36(h, &c, &r, &t);
+  const double df_dT = sum36(h, &c, &r, &dt);
+  double df_dT2 = 0.0;
+  if (ret) df_dT2 = sum36(h, &c, &r, &ddt);
+  const double s = -df_dT;
+  const double ds_dT = -df_dT2;
+  double e = f + T * s;
+  double dE_dT = T * ds_dT;
+  const double T2 = T * T, T3 = T2 * T, T4 = T3 * T;
+  e = e + CRAD * T4 / rho;
+  dE_dT = dE_dT + 4.0 * CRAD * T3 / rho;
+  *E = e;
+  *dEdT = dE_dT;
+}
+
+void orc_helm_set_pig(orc_helm *h, int pig) { h->pig = pig != 0; }
+
 /* helm_rhoT_given_full (source.F90:23686-24232), no Coulomb corrections */
 void orc_helm_full(const orc_helm *h, double rho, double T, double abar, double zbar,
                    double *P, double *E, double *sound, double *cv) {
+  if (h->pig) {
+    pig_full(h, rho, T, P, E, sound, cv);
+    return;
+  }
   const double Ye = zbar / abar, rhos = rho * Ye;
   cellpos c = locate(h, rhos, T);
   basis r, dr, t, dt, ddt;
@@ -255,6 +348,10 @@ void orc_helm_full(const orc_helm *h, double rho, double T, double abar, double 
 /* helm_rhoT_given1 (source.F90:24234-24569): P and dP/dT */
 void orc_helm_given1(const orc_helm *h, double rho, double T, double abar, double zbar,
                      double *P, double *dPdT, int ret) {
+  if (h->pig) {
+    pig_given1(h, rho, T, P, dPdT, ret);
+    return;
+  }
   const double Ye = zbar / abar, rhos = rho * Ye;
   cellpos c = locate(h, rhos, T);
   basis r, dr, t, dt;
@@ -279,6 +376,10 @@ void orc_helm_given1(const orc_helm *h, double rho, double T, double abar, doubl
 /* helm_rhoT_given2 (source.F90:24571-24941): E and dE/dT */
 void orc_helm_given2(const orc_helm *h, double rho, double T, double abar, double zbar,
                      double *E, double *dEdT, int ret) {
+  if (h->pig) {
+    pig_given2(h, rho, T, E, dEdT, ret);
+    return;
+  }
   const double Ye = zbar / abar, rhos = rho * Ye;
   cellpos c = locate(h, rhos, T);
   basis r, t, dt, ddt;

@@ -29,7 +29,7 @@ extern "C" {
 enum { ORC_LIM2ND = 0, ORC_PPH = 1, ORC_LIM5TH = 2 };
 enum { ORC_HLLC = 0, ORC_LHLLC = 1, ORC_HLLD = 2, ORC_LHLLD = 3 };
 enum { ORC_BC_PERIODIC = 0, ORC_BC_REFLECTIVE = 1, ORC_BC_OUTFLOW = 2, ORC_BC_DIODE = 3 };
-enum { ORC_EOS_IDEAL = 0, ORC_EOS_HELMHOLTZ = 1 };
+enum { ORC_EOS_IDEAL = 0, ORC_EOS_HELMHOLTZ = 1, ORC_EOS_PIG = 2 };
 
 typedef struct {
   int sdims, nx1, nx2, nx3, nas;
@@ -124,6 +124,11 @@ orc_helm *orc_helm_create(int nr, int nt, double ldlo, double ldhi, double ltlo,
                           double lthi, const double *f9_file, const double *c4_file);
 void orc_helm_destroy(orc_helm *h);
 double orc_helm_node(const orc_helm *h, int which, int idx);
+/* PIG_EOS: the table is a partially-ionised-gas table (pig_rhoT_given_*,
+ * source.F90:25457-26731): the same interpolation at rho (Ye = 1) with no
+ * ion terms; abar and zbar are then ignored.  The Newton inversions are
+ * the same (pig_tol = helm_tol). */
+void orc_helm_set_pig(orc_helm *h, int pig);
 void orc_helm_full(const orc_helm *h, double rho, double T, double abar, double zbar,
                    double *P, double *E, double *sound, double *cv);
 void orc_helm_given1(const orc_helm *h, double rho, double T, double abar, double zbar,

@@ -86,6 +86,7 @@ def load() -> ctypes.CDLL:
     lib.orc_helm_create.restype = P
     lib.orc_helm_create.argtypes = [I, I, D, D, D, D, P, P]
     lib.orc_helm_destroy.argtypes = [P]
+    lib.orc_helm_set_pig.argtypes = [P, I]
     lib.orc_helm_node.restype = D
     lib.orc_helm_node.argtypes = [P, I, I]
     lib.orc_helm_full_batch.argtypes = [P, I, P, P, P, P, P, P, P, P]
@@ -289,6 +290,8 @@ class OracleHelm:
         c4 = np.ascontiguousarray(table.c4, dtype=np.float64)
         self.h = self.lib.orc_helm_create(table.nrho, table.nT, table.ldlo, table.ldhi,
                                           table.ltlo, table.lthi, _p(f9), _p(c4))
+        if getattr(table, "kind", "helm") == "pig":
+            self.lib.orc_helm_set_pig(self.h, 1)
 
     def close(self):
         if self.h:

@@ -28,6 +28,9 @@
 #endif
 
 #define CONST_RGAS 8.31446261815324e7 /* source.F90:605 */
+/* HELMHOLTZ_EOS or PIG_EOS: a tabulated EoS with a carried T (the table
+ * object says which interpolation applies) */
+#define TABULATED(g) ((g)->c.eos == ORC_EOS_HELMHOLTZ || (g)->c.eos == ORC_EOS_PIG)
 
 struct orc_grid {
   orc_config c;
@@ -353,7 +356,7 @@ static void physical_bc(orc_grid *g) {
             for (int v = 0; v < g->nv; v++) g->prim[cd * g->nv + v] = fac[v] * g->prim[cr * g->nv + v];
             if (g->mhd)
               for (int v = 0; v < g->sd; v++) g->bcc[cd * 3 + v] = facb[v] * g->bcc[cr * 3 + v];
-            if (g->c.eos == ORC_EOS_HELMHOLTZ) g->temp[cd] = g->temp[cr];
+            if (TABULATED(g)) g->temp[cd] = g->temp[cr];
             if (g->c.fasteos) {
               g->gamf[cd * 2] = g->gamf[cr * 2];
               g->gamf[cd * 2 + 1] = g->gamf[cr * 2 + 1];
@@ -1085,7 +1088,7 @@ static void sweep(orc_grid *g, int d) {
   const int fasteos = g->c.fasteos;
   const double inv_dl = g->inv_dx[d];
   rparams P = make_rp(&g->c);
-  const int helm_face = g->c.eos == ORC_EOS_HELMHOLTZ && !fasteos;
+  const int helm_face = TABULATED(g) && !fasteos;
   if (helm_face) P.helm = g->helm;
   long stride = (d == 0) ? 1 : (d == 1) ? g->e1 : (long)g->e1 * g->e2;
   const int ip = g->ip, ias1 = g->ias1;
@@ -1505,7 +1508,7 @@ int orc_hydro_step(orc_grid *g) {
     }
     halo_periodic(g, g->prim, nv);
     if (mhd) halo_periodic(g, g->bcc, 3);
-    if (g->c.eos == ORC_EOS_HELMHOLTZ) halo_periodic(g, g->temp, 1);
+    if (TABULATED(g)) halo_periodic(g, g->temp, 1);
     if (g->c.fasteos) halo_periodic(g, g->gamf, 2);
     physical_bc(g);
     if (g->c.shock_flattening) shock_flags(g);
@@ -1547,7 +1550,7 @@ int orc_hydro_step(orc_grid *g) {
           p[1] = m1 * inv_rho;
           p[2] = m2 * inv_rho;
           if (sd == 3) p[3] = m3 * inv_rho;
-          if (g->c.eos == ORC_EOS_HELMHOLTZ) {
+          if (TABULATED(g)) {
             /* ADVECT_YE_IABAR + HELMHOLTZ_EOS (:13573-13578); USE_FASTEOS
              * keeps gamma_e / gamma_c (:13648-13653), without it the final
              * evaluation is pressure only (:13654-13655) */
@@ -1588,7 +1591,7 @@ int orc_compute_dt(orc_grid *g) {
         double rho = p[0], vx1 = p[1], vx2 = p[2], vx3 = (sd == 3) ? p[3] : 0.0;
         double pr = p[g->ip];
         double cs;
-        if (g->c.eos == ORC_EOS_HELMHOLTZ) {
+        if (TABULATED(g)) {
           /* :13801-13847: full EoS at the carried T */
           double ye = p[g->ias1];
           double abar = 1.0 / p[g->ias1 + 1];
@@ -1627,7 +1630,7 @@ int orc_init(orc_grid *g) {
       for (int i = 0; i < g->n[0]; i++) {
         long c = cidx(g, i, j, k), q = iidx(g, i, j, k);
         double rho = g->prim[c * nv], p = g->prim[c * nv + g->ip];
-        if (g->c.eos == ORC_EOS_HELMHOLTZ) {
+        if (TABULATED(g)) {
           /* step-0 Helmholtz init (source.F90:6279-6285; FASTEOS
            * :6351-6357, otherwise energy only :6358-6361) */
           double abar = 1.0 / g->prim[c * nv + g->ias1 + 1];
@@ -1670,7 +1673,7 @@ int orc_resume(orc_grid *g, double time, double dt, double smax, long step) {
         long c = cidx(g, i, j, k), q = iidx(g, i, j, k);
         double rho = g->prim[c * nv], p = g->prim[c * nv + g->ip];
         double eint;
-        if (g->c.eos == ORC_EOS_HELMHOLTZ) {
+        if (TABULATED(g)) {
           double abar = 1.0 / g->prim[c * nv + g->ias1 + 1];
           double ye = g->prim[c * nv + g->ias1];
           double zbar = abar * ye;
@@ -1831,7 +1834,7 @@ static void eint_to_temp(orc_grid *g) {
       for (int i = 0; i < g->n[0]; i++) {
         const long c = cidx(g, i, j, k), q = iidx(g, i, j, k);
         const double rho = g->prim[c * nv], eint = g->eint[q];
-        if (g->c.eos == ORC_EOS_HELMHOLTZ) {
+        if (TABULATED(g)) {
           const double ye = g->prim[c * nv + g->ias1];
           const double abar = 1.0 / g->prim[c * nv + g->ias1 + 1];
           const double zbar = ye * abar;
@@ -1896,7 +1899,7 @@ int orc_diffusion_step(orc_grid *g, int strang) {
       for (int i = 0; i < g->n[0]; i++) {
         const long c = cidx(g, i, j, k), q = iidx(g, i, j, k);
         const double rho = g->prim[c * nv], T = g->temp[c];
-        if (g->c.eos == ORC_EOS_HELMHOLTZ) {
+        if (TABULATED(g)) {
           const double ye = g->prim[c * nv + g->ias1];
           const double abar = 1.0 / g->prim[c * nv + g->ias1 + 1];
           const double zbar = abar * ye;

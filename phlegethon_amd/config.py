@@ -15,7 +15,7 @@ from typing import Optional, Sequence
 LIM2ND_REC, PPH_REC, LIM5TH_REC = 0, 1, 2
 HLLC_FLUX, LHLLC_FLUX, HLLD_FLUX, LHLLD_FLUX = 0, 1, 2, 3
 BC_PERIODIC, BC_REFLECTIVE, BC_OUTFLOW, BC_DIODE = 0, 1, 2, 3
-EOS_IDEAL, EOS_HELMHOLTZ = 0, 1
+EOS_IDEAL, EOS_HELMHOLTZ, EOS_PIG = 0, 1, 2
 PREC_FP64, PREC_MIXED = 0, 1
 
 _RECON = {"LIM2ND_REC": LIM2ND_REC, "PPH_REC": PPH_REC, "LIM5TH_REC": LIM5TH_REC}
@@ -156,10 +156,15 @@ class PhlxConfig:
                 raise ConfigError("DIODE boundaries are not supported with USE_MHD")
         if self.fasteos and self.eos == EOS_IDEAL:
             raise ConfigError("USE_FASTEOS requires a complex EoS (Make.checks:221-226)")
+        if self.eos not in (EOS_IDEAL, EOS_HELMHOLTZ, EOS_PIG):
+            raise ConfigError("eos must be the ideal gas, HELMHOLTZ_EOS or PIG_EOS")
         if self.eos == EOS_HELMHOLTZ and self.nas != 2:
             raise ConfigError("HELMHOLTZ_EOS runs here with ADVECT_YE_IABAR "
                               "(nas_make=2: Ye, 1/abar)")
-        if self.eos == EOS_HELMHOLTZ and not self.fasteos and self.precision != PREC_FP64:
+        if self.eos == EOS_PIG and self.nas != 2:
+            raise ConfigError("PIG_EOS runs here on the tabulated-EoS kernels with two "
+                              "advected scalars (nas_make=2, unused by the EoS)")
+        if self.eos != EOS_IDEAL and not self.fasteos and self.precision != PREC_FP64:
             raise ConfigError("HELMHOLTZ_EOS without USE_FASTEOS (a T(rho, P) Newton per "
                               "face state) runs in FP64 only")
         if self.use_wb and not self.use_gravity:
@@ -227,6 +232,12 @@ class PhlxConfig:
                 d["rk_stages"] = 3
             elif o == "FLUX_IS_ALL_MACH":
                 d["all_mach"] = True
+            elif o == "HELMHOLTZ_EOS":
+                d["eos"] = EOS_HELMHOLTZ
+            elif o == "PIG_EOS":
+                d["eos"] = EOS_PIG
+            elif o == "USE_FASTEOS":
+                d["fasteos"] = True
             elif o == "USE_SHOCK_FLATTENING":
                 d["shock_flattening"] = True
             elif o == "THERMAL_DIFFUSION_STS":

@@ -113,7 +113,9 @@ static int validate(const phlx_config *c) {
   // advected scalars; without USE_FASTEOS the Riemann solvers make a
   // T(rho, P) Newton per face state (FP64 only); FASTEOS needs a complex EoS
   // (Make.checks)
-  if (c->eos == PHLX_EOS_HELMHOLTZ) {
+  // PIG_EOS runs on the same tabulated-EoS kernels (its table has Ye = 1
+  // and no ion terms, so the two advected scalars are carried unused)
+  if (c->eos == PHLX_EOS_HELMHOLTZ || c->eos == PHLX_EOS_PIG) {
     if (c->nas != 2) return PHLX_E_CONFIG;
     if (!c->fasteos && c->precision != PHLX_PREC_FP64) return PHLX_E_CONFIG;
   } else if (c->eos == PHLX_EOS_IDEAL) {
@@ -202,7 +204,7 @@ extern "C" int phlx_create(const phlx_config *cfg, phlx_ctx **out) {
     ctx->rk[1][0] = 0.75; ctx->rk[1][1] = 0.25; ctx->rk[1][2] = -0.25;
     ctx->rk[2][0] = 1.0 / 3.0; ctx->rk[2][1] = 2.0 / 3.0; ctx->rk[2][2] = -(2.0 / 3.0);
   }
-  const bool helm = cfg->eos == PHLX_EOS_HELMHOLTZ;
+  const bool helm = cfg->eos == PHLX_EOS_HELMHOLTZ || cfg->eos == PHLX_EOS_PIG;
   if (!select_kernels(g.sd, g.nas, g.mhd, helm, cfg->fasteos != 0,
                       cfg->precision == PHLX_PREC_MIXED, cfg->recon == PHLX_LIM5TH_REC,
                       ctx->kt)) {
@@ -600,7 +602,7 @@ static int do_hydro_step(phlx_ctx *ctx, bool with_cfl) {
 
 extern "C" int phlx_init(phlx_ctx *ctx) {
   if (!ctx) return PHLX_E_ARG;
-  if (ctx->cfg.eos == PHLX_EOS_HELMHOLTZ && !ctx->helm_loaded) return PHLX_E_TABLE;
+  if (ctx->cfg.eos != PHLX_EOS_IDEAL && !ctx->helm_loaded) return PHLX_E_TABLE;
   HIPCHK(hipSetDevice(ctx->device));
   Scalars s0;
   std::memset(&s0, 0, sizeof(s0));
@@ -621,7 +623,7 @@ extern "C" int phlx_init(phlx_ctx *ctx) {
 
 extern "C" int phlx_resume(phlx_ctx *ctx, double time, double dt, double smax, int64_t step) {
   if (!ctx || !(dt > 0.0) || !(smax > 0.0) || step < 0) return PHLX_E_ARG;
-  if (ctx->cfg.eos == PHLX_EOS_HELMHOLTZ && !ctx->helm_loaded) return PHLX_E_TABLE;
+  if (ctx->cfg.eos != PHLX_EOS_IDEAL && !ctx->helm_loaded) return PHLX_E_TABLE;
   HIPCHK(hipSetDevice(ctx->device));
   Scalars s0;
   std::memset(&s0, 0, sizeof(s0));
@@ -779,9 +781,26 @@ extern "C" int phlx_stream_copy(phlx_ctx *ctx, int64_t nbytes, int32_t reps, dou
 
 // ------------------------------------------------------------- Helmholtz EoS
 
+static int load_table(phlx_ctx *ctx, const double *f9, const double *c4, int32_t nrho,
+                      int32_t nT, double ldlo, double ldhi, double ltlo, double lthi, int pig);
+
 extern "C" int phlx_load_helm_table(phlx_ctx *ctx, const double *f9, const double *c4,
                                     int32_t nrho, int32_t nT, double ldlo, double ldhi,
                                     double ltlo, double lthi) {
+  if (ctx && ctx->cfg.eos == PHLX_EOS_PIG) return PHLX_E_CONFIG;
+  return load_table(ctx, f9, c4, nrho, nT, ldlo, ldhi, ltlo, lthi, 0);
+}
+
+// load_pig_table (source/source.F90:25407-25455): the same file columns.
+extern "C" int phlx_load_pig_table(phlx_ctx *ctx, const double *f9, const double *c4,
+                                   int32_t nrho, int32_t nT, double ldlo, double ldhi,
+                                   double ltlo, double lthi) {
+  if (ctx && ctx->cfg.eos == PHLX_EOS_HELMHOLTZ) return PHLX_E_CONFIG;
+  return load_table(ctx, f9, c4, nrho, nT, ldlo, ldhi, ltlo, lthi, 1);
+}
+
+static int load_table(phlx_ctx *ctx, const double *f9, const double *c4, int32_t nrho,
+                      int32_t nT, double ldlo, double ldhi, double ltlo, double lthi, int pig) {
   if (!ctx || !f9 || !c4) return PHLX_E_ARG;
   if (nrho < 2 || nT < 2 || !(ldhi > ldlo) || !(lthi > ltlo)) return PHLX_E_TABLE;
   HIPCHK(hipSetDevice(ctx->device));
@@ -837,6 +856,7 @@ extern "C" int phlx_load_helm_table(phlx_ctx *ctx, const double *f9, const doubl
   h.tab = tab;
   h.rho = nd;
   h.T = nd + nrho;
+  h.pig = pig;
   ctx->helm = h;
   ctx->g.helm = h;
   ctx->helm_loaded = true;
@@ -1021,6 +1041,7 @@ extern "C" int phlx_mlp_train(phlx_ctx *ctx, int64_t nsamples, int32_t epochs, i
       batch > kMlpMaxGradBlocks * 128 || !(lr > 0.0))
     return PHLX_E_ARG;
   if (!ctx->helm_loaded) return PHLX_E_TABLE;
+  if (ctx->helm.pig) return PHLX_E_CONFIG;  // the surrogate's inputs are Helmholtz (abar, zbar)
   HIPCHK(hipSetDevice(ctx->device));
   int rc = mlp_alloc(ctx);
   if (rc) return rc;
@@ -1171,10 +1192,12 @@ extern "C" int phlx_set_eos_guess(phlx_ctx *ctx, int32_t mode, double resid_tol,
                                   int32_t count_iterations) {
   if (!ctx) return PHLX_E_ARG;
   if (mode < GUESS_WARM || mode > GUESS_BRACKET || !(resid_tol > 0.0)) return PHLX_E_ARG;
-  if (ctx->cfg.eos != PHLX_EOS_HELMHOLTZ && !ctx->helm_loaded) return PHLX_E_CONFIG;
+  if (ctx->cfg.eos == PHLX_EOS_IDEAL && !ctx->helm_loaded) return PHLX_E_CONFIG;
+  // the surrogate is trained on Helmholtz states (abar, zbar inputs)
+  if (mode == GUESS_MLP && (ctx->cfg.eos == PHLX_EOS_PIG || ctx->helm.pig)) return PHLX_E_CONFIG;
   if (mode == GUESS_MLP && !ctx->mlp_ready) return PHLX_E_STATE;
   // the guess variant of the update needs every block full of interior cells
-  if (ctx->cfg.eos == PHLX_EOS_HELMHOLTZ && (mode != GUESS_WARM || count_iterations) &&
+  if (ctx->cfg.eos != PHLX_EOS_IDEAL && (mode != GUESS_WARM || count_iterations) &&
       (ctx->g.n1 % 64 || ctx->g.n2 % 4))
     return PHLX_E_CONFIG;
   if (mode == GUESS_MLP && ctx->g.eos_guess != GUESS_MLP) {
@@ -1209,7 +1232,7 @@ extern "C" int phlx_set_thermal_diffusion(phlx_ctx *ctx, int32_t enable, int32_t
     ctx->diff_on = false;
     return PHLX_OK;
   }
-  const bool helm = ctx->cfg.eos == PHLX_EOS_HELMHOLTZ;
+  const bool helm = ctx->cfg.eos != PHLX_EOS_IDEAL;
   if (!helm && g.nas != 0) return PHLX_E_CONFIG;
   double ft[6];
   for (int i = 0; i < 6; i++) {
@@ -1294,7 +1317,7 @@ static int diffusion_step(phlx_ctx *ctx, int strang) {
   HIPCHK(hipStreamSynchronize(ctx->stream));
   if (!sc.active) return PHLX_OK;
   const Grid &g = ctx->g;
-  const bool helm = ctx->cfg.eos == PHLX_EOS_HELMHOLTZ;
+  const bool helm = ctx->cfg.eos != PHLX_EOS_IDEAL;
   const double fac = sc.dt / ctx->dt_par, half_dt = 0.5 * sc.dt;
   const double sr = std::floor(0.5 * (std::sqrt(9.0 + 16.0 * fac) - 1.0));
   if (!(sr < 1e6)) return PHLX_E_ARG;  // dt / dt_par beyond any sensible stage count

@@ -38,6 +38,11 @@ struct HelmDev {
   double lrho1, lT1;  // log10 of the first nodes
   double dr, dt;      // log10 spacings
   double tl, th;      // Tl_eos, Th_eos (source/source.F90:750-751)
+  // PIG_EOS table (pig_rhoT_given_*, source/source.F90:25457-26731): the free
+  // energy of the whole gas at rho, i.e. Ye = 1 and no ion terms; the
+  // evaluations below then take Ye = 1 and 1/abar = 0, which leaves the
+  // reference's PIG arithmetic unchanged bit for bit (x * 1 = x, x + 0 = x)
+  int pig;
 };
 
 struct HelmCell {
@@ -185,7 +190,7 @@ __device__ __forceinline__ double helm_sum16(const HelmDev &h, const HelmCell &c
 __device__ __forceinline__ void helm_full(const HelmDev &h, double rho, double T, double abar,
                                           double zbar, double &P, double &E, double &sound,
                                           double &cv, double *dPdT_out = nullptr) {
-  const double Ye = zbar / abar, rhos = rho * Ye;
+  const double Ye = h.pig ? 1.0 : zbar / abar, rhos = rho * Ye;
   const HelmCell c = helm_locate(h, rhos, T);
   HelmBasis r, dr, t, dt, ddt;
   helm_basis(c.x, c.drho, r);
@@ -209,7 +214,7 @@ __device__ __forceinline__ void helm_full(const HelmDev &h, double rho, double T
   const double ds_dT = -Ye * df_dT2;
   double e = Ye * f + T * s;
   double dE_dT = T * ds_dT;
-  tmp = 1.0 / abar;
+  tmp = h.pig ? 0.0 : 1.0 / abar;
   const double T2 = T * T, T3 = T2 * T, T4 = T3 * T;
   const double pion = kHelmRgas * rho * T * tmp;
   const double dpiondt = kHelmRgas * rho * tmp;
@@ -235,7 +240,7 @@ __device__ __forceinline__ void helm_full(const HelmDev &h, double rho, double T
 // helm_rhoT_given1: P and (when ret) dP/dT.
 __device__ __forceinline__ void helm_given1(const HelmDev &h, double rho, double T, double abar,
                                             double zbar, double &P, double &dPdT, bool ret) {
-  const double Ye = zbar / abar, rhos = rho * Ye;
+  const double Ye = h.pig ? 1.0 : zbar / abar, rhos = rho * Ye;
   const HelmCell c = helm_locate(h, rhos, T);
   HelmBasis dr, t, dt;
   helm_dbasis(c.x, c.drho, dr);
@@ -260,7 +265,7 @@ __device__ __forceinline__ void helm_given1(const HelmDev &h, double rho, double
   double p = tmp * df_drho;
   double dP_dT = tmp * df_drhodT;
   const double T2 = T * T, T3 = T2 * T, T4 = T3 * T;
-  tmp = 1.0 / abar;
+  tmp = h.pig ? 0.0 : 1.0 / abar;
   const double pion = kHelmRgas * rho * T * tmp;
   const double dpiondt = kHelmRgas * rho * tmp;
   p = p + pion + kHelmCrad * T4 * (1.0 / 3.0);
@@ -272,7 +277,7 @@ __device__ __forceinline__ void helm_given1(const HelmDev &h, double rho, double
 // helm_rhoT_given2: E and (when ret) dE/dT.
 __device__ __forceinline__ void helm_given2(const HelmDev &h, double rho, double T, double abar,
                                             double zbar, double &E, double &dEdT, bool ret) {
-  const double Ye = zbar / abar, rhos = rho * Ye;
+  const double Ye = h.pig ? 1.0 : zbar / abar, rhos = rho * Ye;
   const HelmCell c = helm_locate(h, rhos, T);
   HelmBasis r, t, dt, ddt;
   helm_basis(c.x, c.drho, r);
@@ -301,7 +306,7 @@ __device__ __forceinline__ void helm_given2(const HelmDev &h, double rho, double
   double e = Ye * f + T * s;
   double dE_dT = T * ds_dT;
   const double T2 = T * T, T3 = T2 * T, T4 = T3 * T;
-  const double tmp = 1.0 / abar;
+  const double tmp = h.pig ? 0.0 : 1.0 / abar;
   e = e + 1.5 * kHelmRgas * T * tmp + kHelmCrad * T4 / rho;
   dE_dT = dE_dT + 1.5 * kHelmRgas * tmp + 4.0 * kHelmCrad * T3 / rho;
   E = e;

@@ -38,6 +38,7 @@ class HelmTable:
     lthi: float
     f9: np.ndarray      # (nT, nrho, 9), file column order
     c4: np.ndarray      # (nT, nrho, 4)
+    kind: str = "helm"  # "helm": e-e+ free energy at rho*Ye; "pig": the whole gas at rho
 
     @property
     def rho_nodes(self) -> np.ndarray:
@@ -92,13 +93,14 @@ def generate(nrho, nT, ldlo, ldhi, ltlo, lthi, path: str = None, ascii_path: str
 def load(path: str) -> HelmTable:
     with open(path, "rb") as fh:
         magic = fh.read(8)
-        if magic != b"PHLXHLM1":
-            raise ValueError(f"{path}: not a phlx Helmholtz table")
+        if magic not in (b"PHLXHLM1", b"PHLXPIG1"):
+            raise ValueError(f"{path}: not a phlx Helmholtz or PIG table")
         nr, nt = np.frombuffer(fh.read(8), dtype=np.int32)
         box = np.frombuffer(fh.read(32), dtype=np.float64)
         f9 = np.frombuffer(fh.read(8 * 9 * nr * nt), dtype=np.float64).reshape(nt, nr, 9)
         c4 = np.frombuffer(fh.read(8 * 4 * nr * nt), dtype=np.float64).reshape(nt, nr, 4)
-    return HelmTable(int(nr), int(nt), *map(float, box), f9.copy(), c4.copy())
+    return HelmTable(int(nr), int(nt), *map(float, box), f9.copy(), c4.copy(),
+                     kind="pig" if magic == b"PHLXPIG1" else "helm")
 
 
 def default_table() -> HelmTable:

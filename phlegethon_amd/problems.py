@@ -19,7 +19,7 @@ from typing import Dict, List, Optional
 
 import numpy as np
 
-from .config import (BC_PERIODIC, BC_REFLECTIVE, BC_OUTFLOW, EOS_HELMHOLTZ, HLLC_FLUX,
+from .config import (BC_PERIODIC, BC_REFLECTIVE, BC_OUTFLOW, EOS_HELMHOLTZ, EOS_PIG, HLLC_FLUX,
                      LHLLC_FLUX, HLLD_FLUX, LHLLD_FLUX, LIM2ND_REC, PPH_REC, PhlxConfig)
 
 SEED = 20261202
@@ -417,6 +417,45 @@ def helm_slab(eos, table, n: int = 256, ny: Optional[int] = None, sdims: int = 3
         scale = np.sqrt(2.0 * P_cc[ny // 2] / mhd_beta) / bmag
         b_faces = [None if b is None else b * scale for b in b_faces]
     return Problem("helm_slab", cfg, prim, b_faces, prof, temp, table)
+
+
+def pig_box(eos, table, n: int = 32, sdims: int = 2, recon: int = LIM2ND_REC,
+            flux: int = LHLLC_FLUX, seed: int = SEED, mhd_beta: Optional[float] = None,
+            fasteos: bool = True, rho0: float = 1e-7, T0: float = 1.5e4) -> Problem:
+    """Periodic box of partially ionised gas for PIG_EOS: rho and T vary by
+    30 % across the box around (rho0, T0) (hydrogen about half ionised at
+    the defaults), P and the step-0 T from the PIG table, seeded velocities
+    of ~0.1 of the sound speed; two advected scalars (1, 1) that the EoS
+    ignores.  ``eos`` evaluates the table (``full(rho, T, abar, zbar)``)."""
+    mhd = mhd_beta is not None
+    L = 1e9
+    cfg = PhlxConfig(sdims=sdims, nx1=n, nx2=n, nx3=n if sdims == 3 else 1, nas=2,
+                     x1u=L, x2u=L, x3u=L, recon=recon, flux=(LHLLD_FLUX if mhd else flux),
+                     cfl=0.8, mach_cutoff=1e-4, eos=EOS_PIG, fasteos=fasteos).validate()
+    X, Y, Z = mesh(cfg)
+    tp = 2.0 * math.pi / L
+    N = X.size
+    u = lambda s: (2.0 * uniform01(seed, s, N) - 1.0).reshape(X.shape)
+    rho = rho0 * (1.0 + 0.3 * np.sin(tp * X) * np.cos(tp * Y) + 0.02 * u(1))
+    T = T0 * (1.0 + 0.3 * np.cos(tp * (X + Y)) + 0.02 * u(2))
+    one = np.ones(N)
+    P, E, cs, cv = (np.asarray(a).reshape(X.shape)
+                    for a in eos.full(rho.ravel(), T.ravel(), one, one))
+    prim = np.zeros((cfg.nvars,) + cfg.shape)
+    prim[0] = rho
+    for d in range(sdims):
+        prim[1 + d] = 0.1 * cs * (np.sin(tp * (Y if d == 0 else X) + d) + 0.2 * u(3 + d))
+    ip = sdims + 1
+    prim[ip] = P
+    prim[ip + 1] = 1.0
+    prim[ip + 2] = 1.0
+    b_faces = None
+    if mhd:
+        b_faces = _random_div_free(cfg, seed, amp=1.0)
+        bmag = max(np.sqrt(np.mean(b ** 2)) for b in b_faces if b is not None)
+        scale = np.sqrt(2.0 * float(P.mean()) / mhd_beta) / bmag
+        b_faces = [None if b is None else b * scale for b in b_faces]
+    return Problem("pig_box", cfg, prim, b_faces, {}, temp=T.copy(), helm=table)
 
 
 def state_box(pb: Problem, dex: float = 0.5) -> tuple:

@@ -16,7 +16,7 @@ from typing import Dict, Optional
 
 import numpy as np
 
-from .config import EOS_HELMHOLTZ, PhlxConfig
+from .config import EOS_HELMHOLTZ, EOS_IDEAL, PhlxConfig
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "_lib", "libphlx.so")
@@ -75,6 +75,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.phlx_stream.argtypes = [P]
     lib.phlx_stream.restype = P
     lib.phlx_load_helm_table.argtypes = [P, P, P, I, I, D, D, D, D]
+    lib.phlx_load_pig_table.argtypes = [P, P, P, I, I, D, D, D, D]
     lib.phlx_helm_rhoT_full_batch.argtypes = [P, ctypes.c_int64] + [P] * 8
     lib.phlx_helm_rhoe_given_batch.argtypes = [P, ctypes.c_int64] + [P] * 7 + [I, P]
     lib.phlx_helm_rhoP_given_batch.argtypes = [P, ctypes.c_int64] + [P] * 7 + [I, P]
@@ -197,7 +198,7 @@ class Simulation:
         for d in range(3):
             if bs[d] is not None:
                 out[f"b_x{d+1}"] = bs[d]
-        if cfg.eos == EOS_HELMHOLTZ:
+        if cfg.eos != EOS_IDEAL:
             gf = np.empty((2,) + cfg.shape)
             _chk(self.lib.phlx_download_gammaf(self._ctx, _ptr(gf)), "phlx_download_gammaf")
             out["gammaf"] = gf
@@ -300,9 +301,10 @@ class Simulation:
         file column order, [T][rho][column])."""
         f9 = np.ascontiguousarray(table.f9, dtype=np.float64)
         c4 = np.ascontiguousarray(table.c4, dtype=np.float64)
-        _chk(self.lib.phlx_load_helm_table(self._ctx, _ptr(f9), _ptr(c4), table.nrho, table.nT,
-                                           table.ldlo, table.ldhi, table.ltlo, table.lthi),
-             "phlx_load_helm_table")
+        fn = "phlx_load_pig_table" if getattr(table, "kind", "helm") == "pig" else \
+            "phlx_load_helm_table"
+        _chk(getattr(self.lib, fn)(self._ctx, _ptr(f9), _ptr(c4), table.nrho, table.nT,
+                                   table.ldlo, table.ldhi, table.ltlo, table.lthi), fn)
 
     @property
     def ctx(self):
