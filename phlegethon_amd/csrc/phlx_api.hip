@@ -217,6 +217,7 @@ extern "C" int phlx_create(const phlx_config *cfg, phlx_ctx **out) {
   e |= dalloc(ctx, &g.U, nv * N);
   e |= dalloc(ctx, &g.U0, nv * N);
   e |= dalloc(ctx, &g.eint, N);
+  if (cfg->precision == PHLX_PREC_MIXED) e |= dalloc(ctx, &g.Pf, (nv + 5) * N);
   for (int d = 0; d < g.sd; d++) {
     e |= dalloc(ctx, &g.F[d], nv * (size_t)g.nf[d]);
     g.Ff[d] = (float *)g.F[d];  // the mixed path stores FP32 fluxes in the same buffer
@@ -314,6 +315,8 @@ static int copy_out(phlx_ctx *ctx, void *dst, const void *src, size_t bytes) {
   return PHLX_OK;
 }
 
+static int check_launch();
+
 extern "C" int phlx_upload_state(phlx_ctx *ctx, const double *prim, const double *eint,
                                  const double *temp, const double *b1, const double *b2,
                                  const double *b3) {
@@ -333,6 +336,13 @@ extern "C" int phlx_upload_state(phlx_ctx *ctx, const double *prim, const double
       rc |= copy_in(ctx, g.bf[d], bs[d], sizeof(double) * g.nf[d]);
     }
   if (rc) return PHLX_E_HIP;
+  // mixed precision: a state replaced between steps refreshes the sweeps'
+  // FP32 inputs (before phlx_init, init does it)
+  if (ctx->initialized && ctx->kt.pack) {
+    ctx->kt.pack(ctx->g, ctx->stream);
+    rc = check_launch();
+    if (rc) return rc;
+  }
   HIPCHK(hipStreamSynchronize(ctx->stream));
   return PHLX_OK;
 }
@@ -611,6 +621,7 @@ extern "C" int phlx_init(phlx_ctx *ctx) {
   s0.active = 1;
   HIPCHK(hipMemcpyAsync(ctx->g.sc, &s0, sizeof(s0), hipMemcpyHostToDevice, ctx->stream));
   ctx->kt.init(ctx->g, 0, ctx->stream);
+  if (ctx->kt.pack) ctx->kt.pack(ctx->g, ctx->stream);
   int rc = check_launch();
   if (rc) return rc;
   ctx->initialized = true;
@@ -636,6 +647,7 @@ extern "C" int phlx_resume(phlx_ctx *ctx, double time, double dt, double smax, i
   s0.active = time < ctx->cfg.tmax ? 1 : 0;
   HIPCHK(hipMemcpyAsync(ctx->g.sc, &s0, sizeof(s0), hipMemcpyHostToDevice, ctx->stream));
   ctx->kt.init(ctx->g, 1, ctx->stream);
+  if (ctx->kt.pack) ctx->kt.pack(ctx->g, ctx->stream);
   int rc = check_launch();
   if (rc) return rc;
   ctx->initialized = true;
@@ -1343,6 +1355,7 @@ static int diffusion_step(phlx_ctx *ctx, int strang) {
     bjm1 = bj;
   }
   launch_diff_final(g, helm, ctx->stream);
+  if (ctx->kt.pack) ctx->kt.pack(g, ctx->stream);
   kend(ctx, on, r);
   int rc = check_launch();
   if (rc) return rc;

@@ -44,6 +44,11 @@ struct Grid {
   double gm, gmm1, igmm1, inv_mu, mach_cutoff, mach_ct;
   int recon, flux, all_mach, wb, grav, etot, edot, fasteos, mhd;
   double *P, *U, *U0, *eint, *bcc;
+  // mixed precision: the sweeps' inputs in FP32, [slot][cell] with slots
+  // 0..nv-1 the primitives (rho and p as FP64-formed WB deviations), nv..nv+2
+  // the cell-centred B, nv+3 / nv+4 gamma_e (deviation) / gamma_c; written by
+  // the update (and k_pack_f), read by the mixed flux kernels
+  float *Pf;
   double *bf[3], *b0f[3];
   double *F[3];
   float *Ff[3];            // FP32 view of F[d] (mixed precision: fluxes stored as FP32)
@@ -168,6 +173,44 @@ __device__ __forceinline__ void load_cell(const Grid &g, int i, int j, int k, do
   if (MHD) {
 #pragma unroll
     for (int v = 0; v < SD; v++) b[v] = g.bcc[v * N + c];
+  }
+  if (FAST) return;
+  if (si && flips(g, 0, si, cix(g, si < 0 ? 0 : g.n1 - 1, mj, mk))) {
+    q[1] = -q[1];
+    if (MHD) b[0] = -b[0];
+  }
+  if (sj && flips(g, 1, sj, cix(g, mi, sj < 0 ? 0 : g.n2 - 1, mk))) {
+    q[2] = -q[2];
+    if (MHD) b[1] = -b[1];
+  }
+  if (SD == 3 && sk && flips(g, 2, sk, cix(g, mi, mj, sk < 0 ? 0 : g.n3 - 1))) {
+    q[3] = -q[3];
+    if (MHD) b[2] = -b[2];
+  }
+}
+
+// load_cell for the mixed-precision sweeps: the same values from the FP32
+// copy g.Pf (the deviations were formed in FP64 before the cast; the mirror
+// flips are exact in either precision).  The DIODE flip test reads the FP64
+// state, as load_cell does.
+template <int SD, int NV, bool MHD, bool GAM = false, bool FAST = false>
+__device__ __forceinline__ void load_cell_f(const Grid &g, int i, int j, int k, float *q,
+                                            float *b, float *ga = nullptr) {
+  int si = 0, sj = 0, sk = 0;
+  int mi = FAST ? i : map_axis(g, 0, i, g.n1, si);
+  int mj = FAST ? j : map_axis(g, 1, j, g.n2, sj);
+  int mk = (SD == 3 && !FAST) ? map_axis(g, 2, k, g.n3, sk) : k;
+  const long long c = cix(g, mi, mj, mk);
+  const long long N = g.N;
+#pragma unroll
+  for (int v = 0; v < NV; v++) q[v] = g.Pf[v * N + c];
+  if (GAM) {
+    ga[0] = g.Pf[(NV + 3) * N + c];
+    ga[1] = g.Pf[(NV + 4) * N + c];
+  }
+  if (MHD) {
+#pragma unroll
+    for (int v = 0; v < SD; v++) b[v] = g.Pf[(NV + v) * N + c];
   }
   if (FAST) return;
   if (si && flips(g, 0, si, cix(g, si < 0 ? 0 : g.n1 - 1, mj, mk))) {

@@ -17,6 +17,8 @@ struct KernelTable {
   void (*cfl)(const Grid &, hipStream_t);
   void (*init)(const Grid &, int restart, hipStream_t);
   void (*divb)(const Grid &, hipStream_t);
+  // mixed precision: refresh the FP32 sweep inputs from the FP64 state
+  void (*pack)(const Grid &, hipStream_t);
 };
 
 bool select_kernels(int sd, int nas, bool mhd, bool helm, bool fasteos, bool mixed, bool r5,

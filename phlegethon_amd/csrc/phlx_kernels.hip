@@ -472,14 +472,10 @@ __device__ __forceinline__ void seg_walk_mx(const Grid &g, int i, int o, int f0,
   constexpr int NV = SD + 2 + NAS, IE = SD + 1;
   static_assert(NV <= 8, "cell window size");
   auto load = [&](int c, CellWf &w, int ii, int oo) {
-    double qd[NV], bd[3] = {0.0, 0.0, 0.0}, gd[2] = {0.0, 0.0};
-    load_cell<SD, NV, MHD, HELM, FAST>(g, ii, DIR == 1 ? c : oo, DIR == 2 ? c : oo, qd, bd, gd);
-#pragma unroll
-    for (int v = 0; v < NV; v++) w.q[v] = (float)qd[v];
-#pragma unroll
-    for (int v = 0; v < 3; v++) w.b[v] = (float)bd[v];
-    w.ga[0] = (float)gd[0];
-    w.ga[1] = (float)gd[1];
+    w.b[0] = w.b[1] = w.b[2] = 0.0f;
+    w.ga[0] = w.ga[1] = 0.0f;
+    load_cell_f<SD, NV, MHD, HELM, FAST>(g, ii, DIR == 1 ? c : oo, DIR == 2 ? c : oo, w.q, w.b,
+                                         w.ga);
   };
   CellWf cm, c0, cp;
   load(f0 - 2, cm, i, o);
@@ -567,19 +563,14 @@ __global__ void __launch_bounds__(256) k_flux_mx(Grid g) {
 #pragma unroll
   for (int s = 0; s < NS; s++) {
     const int o = s - O;
-    double qd[NV], bd[3] = {0.0, 0.0, 0.0}, gd[2] = {0.0, 0.0};
+    b[s][0] = b[s][1] = b[s][2] = 0.0f;
+    ga[s][0] = ga[s][1] = 0.0f;
     if (fast)
-      load_cell<SD, NV, MHD, HELM, true>(g, i + (DIR == 0 ? o : 0), j + (DIR == 1 ? o : 0),
-                                         k + (DIR == 2 ? o : 0), qd, bd, gd);
+      load_cell_f<SD, NV, MHD, HELM, true>(g, i + (DIR == 0 ? o : 0), j + (DIR == 1 ? o : 0),
+                                           k + (DIR == 2 ? o : 0), q[s], b[s], ga[s]);
     else
-      load_cell<SD, NV, MHD, HELM>(g, i + (DIR == 0 ? o : 0), j + (DIR == 1 ? o : 0),
-                                   k + (DIR == 2 ? o : 0), qd, bd, gd);
-#pragma unroll
-    for (int v = 0; v < NV; v++) q[s][v] = (float)qd[v];
-#pragma unroll
-    for (int v = 0; v < 3; v++) b[s][v] = (float)bd[v];
-    ga[s][0] = (float)gd[0];
-    ga[s][1] = (float)gd[1];
+      load_cell_f<SD, NV, MHD, HELM>(g, i + (DIR == 0 ? o : 0), j + (DIR == 1 ? o : 0),
+                                     k + (DIR == 2 ? o : 0), q[s], b[s], ga[s]);
   }
   float qL[NV], qR[NV], bL[3] = {0.0f, 0.0f, 0.0f}, bR[3] = {0.0f, 0.0f, 0.0f};
 #pragma unroll
@@ -986,6 +977,18 @@ __global__ void __launch_bounds__(256)
       g.U[v * N + c] = S[v];
     }
     g.eint[c] = eint;
+    if constexpr (MIX) {
+      // FP32 copy of the next sweeps' inputs (the WB deviations formed in FP64)
+      g.Pf[c] = (float)(g.wb ? p[0] - g.eq_rho_cc[j] : p[0]);
+#pragma unroll
+      for (int v = 1; v < NV; v++)
+        if (v != IE) g.Pf[v * N + c] = (float)p[v];
+      if (!HELM) g.Pf[IE * N + c] = (float)(g.wb ? p[IE] - g.eq_p_cc[j] : p[IE]);
+      if (MHD) {
+#pragma unroll
+        for (int v = 0; v < SD; v++) g.Pf[(NV + v) * N + c] = (float)b[v];
+      }
+    }
     if (MHD) {
       const double vv[3] = {p[1], p[2], SD == 3 ? p[3] : 0.0};
       g.ecell[2 * N + c] = emf_of<SD>(2, vv, b);
@@ -1061,6 +1064,12 @@ __global__ void __launch_bounds__(256)
       g.P[IE * N + c] = pr;
       p[IE] = pr;
       sound_out = sound;
+      if constexpr (MIX) {
+        g.Pf[IE * N + c] = (float)(g.wb ? pr - g.eq_p_cc[j] : pr);
+        const double ge = pr / eint + 1.0;
+        g.Pf[(NV + 3) * N + c] = (float)(g.wb ? ge - g.eq_ge_cc[j] : ge);
+        g.Pf[(NV + 4) * N + c] = (float)(sound * sound * rho / pr);
+      }
     }
     if (last) smax_local = cell_speed<SD, MHD, HELM, HELM && MIX>(g, p, b, T, sound_out);
   }
@@ -1086,6 +1095,35 @@ __global__ void __launch_bounds__(256) k_cfl(Grid g) {
     s = cell_speed<SD, MHD, HELM>(g, p, b, HELM ? g.temp[c] : 0.0);
   }
   block_max_to(s, g.red);
+}
+
+// The FP32 copy of the sweep inputs from the FP64 state (mixed precision:
+// after init / resume and after a thermal-diffusion step), with the values
+// the update writes.
+template <int SD, int NAS, bool MHD, bool HELM>
+__global__ void __launch_bounds__(256) k_pack_f(Grid g) {
+  constexpr int NV = SD + 2 + NAS, IE = SD + 1;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = blockIdx.z;
+  if (i >= g.n1 || j >= g.n2) return;
+  const long long N = g.N, c = cix(g, i, j, k);
+#pragma unroll
+  for (int v = 0; v < NV; v++) {
+    double x = g.P[v * N + c];
+    if (g.wb && v == 0) x = x - g.eq_rho_cc[j];
+    if (g.wb && v == IE) x = x - g.eq_p_cc[j];
+    g.Pf[v * N + c] = (float)x;
+  }
+  if (MHD) {
+#pragma unroll
+    for (int v = 0; v < SD; v++) g.Pf[(NV + v) * N + c] = (float)g.bcc[v * N + c];
+  }
+  if (HELM) {
+    const double ge = g.gam[c];
+    g.Pf[(NV + 3) * N + c] = (float)(g.wb ? ge - g.eq_ge_cc[j] : ge);
+    g.Pf[(NV + 4) * N + c] = (float)g.gam[N + c];
+  }
 }
 
 // Step-0 initialisation: b_cc from faces and the ideal-gas eint
@@ -1422,6 +1460,12 @@ struct Kern {
     const dim3 blk(64, 4, 1);
     k_divb<SD><<<grid_for(g.n1, g.n2, g.n3, blk), blk, 0, st>>>(g);
   }
+  static void pack(const Grid &g, hipStream_t st) {
+    if constexpr (MIX) {
+      const dim3 blk(64, 4, 1);
+      k_pack_f<SD, NAS, MHD, HELM><<<grid_for(g.n1, g.n2, g.n3, blk), blk, 0, st>>>(g);
+    }
+  }
 };
 
 template <int SD, int NAS, bool MHD, bool HELM, bool MIX, bool R5,
@@ -1437,6 +1481,7 @@ static KernelTable make_table() {
   t.cfl = &K::cfl;
   t.init = &K::init;
   t.divb = &K::divb;
+  t.pack = MIX ? &K::pack : nullptr;
   return t;
 }
 

@@ -41,6 +41,15 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         raise RuntimeError(
             f"{path} not found: build the HIP library first (python -c "
             "'import __graft_entry__ as g; g.build()')")
+    # torch ships its own HIP runtime under the same soname
+    # (libamdhip64.so.7) as the system one the library is linked against:
+    # whichever loads first serves the whole process, and torch's device
+    # code does not initialise on the system runtime.  Loading torch first
+    # makes both use its runtime (the library's calls are plain HIP API).
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = ctypes.CDLL(path)
     P, I, D = ctypes.c_void_p, ctypes.c_int32, ctypes.c_double
     DP = ctypes.POINTER(ctypes.c_double)

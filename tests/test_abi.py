@@ -32,7 +32,8 @@ def test_header_declares_core_entry_points():
 def test_library_exports_every_declared_symbol():
     if not os.path.exists(LIB_PATH):
         pytest.skip("libphlx.so not built (run __graft_entry__.build())")
-    lib = ctypes.CDLL(LIB_PATH)
+    from phlegethon_amd.runtime import load_library
+    lib = load_library(LIB_PATH)   # loads torch's HIP runtime first
     missing = [s for s in declared_symbols() if not hasattr(lib, s)]
     assert not missing, missing
     lib.phlx_error_string.restype = ctypes.c_char_p
