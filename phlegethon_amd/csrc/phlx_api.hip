@@ -52,6 +52,9 @@ struct phlx_ctx {
   // host copy of dt_par, kappa uploaded, the ideal gas' diffusion T current
   // (false after any hydro update: the next step recomputes it from p)
   bool diff_on = false, diff_eval = true, kappa_set = false, tdiff_valid = false;
+  // U holds the state of the last completed step (false after init, resume,
+  // uploads and diffusion, which change prim / eint without U)
+  bool u_valid = false;
   double dt_par = 0.0;
   int diff_stages = 0;  // stages of the last diffusion step
   // profiling
@@ -338,6 +341,7 @@ extern "C" int phlx_upload_state(phlx_ctx *ctx, const double *prim, const double
   if (rc) return PHLX_E_HIP;
   // mixed precision: a state replaced between steps refreshes the sweeps'
   // FP32 inputs (before phlx_init, init does it)
+  ctx->u_valid = false;
   if (ctx->initialized && ctx->kt.pack) {
     ctx->kt.pack(ctx->g, ctx->stream);
     rc = check_launch();
@@ -582,10 +586,15 @@ extern "C" int phlx_rk_update(phlx_ctx *ctx, int32_t irk, double a1, double a2, 
   ProfRec r;
   int on = kbegin(ctx, KID_UPDATE, r);
   const bool guess = ctx->kt.update_guess && (ctx->g.eos_guess != GUESS_WARM || ctx->eos_stats);
+  // stage 1 of the mixed path starts from U when it is current (the FP64
+  // conserved state of the last completed step)
+  const int first = irk == 0 ? ((ctx->g.Pf && ctx->u_valid) ? 2 : 1) : 0;
+  const int fin = irk == ctx->stages - 1;
   if (guess)
-    ctx->kt.update_guess(ctx->g, irk == 0, last_stage != 0, a1, a2, a3, ctx->stream);
+    ctx->kt.update_guess(ctx->g, first, last_stage != 0, a1, a2, a3, fin, ctx->stream);
   else
-    ctx->kt.update(ctx->g, irk == 0, last_stage != 0, a1, a2, a3, ctx->stream);
+    ctx->kt.update(ctx->g, first, last_stage != 0, a1, a2, a3, fin, ctx->stream);
+  ctx->u_valid = fin;
   kend(ctx, on, r);
   return check_launch();
 }
@@ -626,6 +635,7 @@ extern "C" int phlx_init(phlx_ctx *ctx) {
   if (rc) return rc;
   ctx->initialized = true;
   ctx->tdiff_valid = false;
+  ctx->u_valid = false;
   // without EVALUATE_PARABOLIC_TIMESTEP dt_par is computed once, at step 0
   // (source/source.F90:6504-6512)
   if (ctx->diff_on && !ctx->diff_eval && (rc = parabolic_dt(ctx))) return rc;
@@ -648,6 +658,7 @@ extern "C" int phlx_resume(phlx_ctx *ctx, double time, double dt, double smax, i
   HIPCHK(hipMemcpyAsync(ctx->g.sc, &s0, sizeof(s0), hipMemcpyHostToDevice, ctx->stream));
   ctx->kt.init(ctx->g, 1, ctx->stream);
   if (ctx->kt.pack) ctx->kt.pack(ctx->g, ctx->stream);
+  ctx->u_valid = false;
   int rc = check_launch();
   if (rc) return rc;
   ctx->initialized = true;
@@ -1356,6 +1367,7 @@ static int diffusion_step(phlx_ctx *ctx, int strang) {
   }
   launch_diff_final(g, helm, ctx->stream);
   if (ctx->kt.pack) ctx->kt.pack(g, ctx->stream);
+  ctx->u_valid = false;
   kend(ctx, on, r);
   int rc = check_launch();
   if (rc) return rc;

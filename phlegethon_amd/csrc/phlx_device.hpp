@@ -189,10 +189,22 @@ __device__ __forceinline__ void load_cell(const Grid &g, int i, int j, int k, do
   }
 }
 
+// flips() on the FP32 copy of the state (the mixed path writes the FP64
+// primitives only at the end of a step): the DIODE test reads the boundary
+// cell's normal velocity from g.Pf.
+__device__ __forceinline__ bool flips_f(const Grid &g, int a, int side, long long cb) {
+  int bc = g.bc[2 * a + (side > 0)];
+  if (bc == BC_REFLECTIVE) return true;
+  if (bc == BC_DIODE) {
+    const float v = g.Pf[(long long)(1 + a) * g.N + cb];
+    return side < 0 ? (v > 0.0f) : (v < 0.0f);
+  }
+  return false;
+}
+
 // load_cell for the mixed-precision sweeps: the same values from the FP32
 // copy g.Pf (the deviations were formed in FP64 before the cast; the mirror
-// flips are exact in either precision).  The DIODE flip test reads the FP64
-// state, as load_cell does.
+// flips are exact in either precision).
 template <int SD, int NV, bool MHD, bool GAM = false, bool FAST = false>
 __device__ __forceinline__ void load_cell_f(const Grid &g, int i, int j, int k, float *q,
                                             float *b, float *ga = nullptr) {
@@ -213,15 +225,15 @@ __device__ __forceinline__ void load_cell_f(const Grid &g, int i, int j, int k, 
     for (int v = 0; v < SD; v++) b[v] = g.Pf[(NV + v) * N + c];
   }
   if (FAST) return;
-  if (si && flips(g, 0, si, cix(g, si < 0 ? 0 : g.n1 - 1, mj, mk))) {
+  if (si && flips_f(g, 0, si, cix(g, si < 0 ? 0 : g.n1 - 1, mj, mk))) {
     q[1] = -q[1];
     if (MHD) b[0] = -b[0];
   }
-  if (sj && flips(g, 1, sj, cix(g, mi, sj < 0 ? 0 : g.n2 - 1, mk))) {
+  if (sj && flips_f(g, 1, sj, cix(g, mi, sj < 0 ? 0 : g.n2 - 1, mk))) {
     q[2] = -q[2];
     if (MHD) b[1] = -b[1];
   }
-  if (SD == 3 && sk && flips(g, 2, sk, cix(g, mi, mj, sk < 0 ? 0 : g.n3 - 1))) {
+  if (SD == 3 && sk && flips_f(g, 2, sk, cix(g, mi, mj, sk < 0 ? 0 : g.n3 - 1))) {
     q[3] = -q[3];
     if (MHD) b[2] = -b[2];
   }

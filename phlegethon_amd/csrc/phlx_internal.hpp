@@ -12,8 +12,8 @@ struct KernelTable {
   void (*flux)(const Grid &, int, hipStream_t);
   void (*corner)(const Grid &, hipStream_t);
   void (*faceb)(const Grid &, int, double, double, double, hipStream_t);
-  void (*update)(const Grid &, int, int, double, double, double, hipStream_t);
-  void (*update_guess)(const Grid &, int, int, double, double, double, hipStream_t);
+  void (*update)(const Grid &, int, int, double, double, double, int, hipStream_t);
+  void (*update_guess)(const Grid &, int, int, double, double, double, int, hipStream_t);
   void (*cfl)(const Grid &, hipStream_t);
   void (*init)(const Grid &, int restart, hipStream_t);
   void (*divb)(const Grid &, hipStream_t);

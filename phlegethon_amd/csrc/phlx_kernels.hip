@@ -872,20 +872,35 @@ template <int SD, int NAS, bool MHD, bool HELM, bool MIX, bool GUESS = false>
 __global__ void __launch_bounds__(256)
     __attribute__((amdgpu_waves_per_eu(HELM ? PHLX_HELM_UPDATE_WAVES : 1)))
     k_update(Grid g, int first, int last, double a1,
-                                                double a2, double c3) {
+                                                double a2, double c3, int fin) {
+  // first: 1 = stage 1, state0 rebuilt from prim + eint (the reference);
+  // 2 = stage 1 of the mixed path with U current: state0 = U.  last: the
+  // CFL maximum of the new state.  fin: the step's last stage.  The mixed
+  // path writes the FP64 primitives, eint, B_cc and the gammas only at
+  // fin (between its stages the sweeps read the FP32 copy g.Pf).
   constexpr int NV = SD + 2 + NAS, IE = SD + 1, IA = SD + 2;
   if (!g.sc->active) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y * blockDim.y + threadIdx.y;
   const int k = blockIdx.z;
   const bool in = (i < g.n1) && (j < g.n2);
+  const bool keep = !MIX || fin;
   double smax_local = 0.0;
   if (in) {
     const long long N = g.N, c = cix(g, i, j, k);
     double S[NV], S0[NV], R[NV], p[NV];
+    if (!MIX || first == 1) {
 #pragma unroll
-    for (int v = 0; v < NV; v++) p[v] = g.P[v * N + c];
-    if (first) {
+      for (int v = 0; v < NV; v++) p[v] = g.P[v * N + c];
+    }
+    if (MIX && first == 2) {
+#pragma unroll
+      for (int v = 0; v < NV; v++) {
+        S0[v] = g.U[v * N + c];
+        S[v] = S0[v];
+        g.U0[v * N + c] = S0[v];
+      }
+    } else if (first) {
       // source/source.F90:7057-7117
       const double rho = p[0], v1 = p[1], v2 = p[2], v3 = (SD == 3) ? p[3] : 0.0;
       S0[0] = rho;
@@ -937,8 +952,10 @@ __global__ void __launch_bounds__(256)
         R[v] = R[v] + (flux_at<MIX>(g, 2, v * nf + fp) - flux_at<MIX>(g, 2, v * nf + fm)) * g.inv_dx[2];
     }
     if (g.grav) {
-      // source/source.F90:12606-12644 on the WB deviation rho'
-      const double rho = g.wb ? (p[0] - g.eq_rho_cc[j]) : p[0];
+      // source/source.F90:12606-12644 on the WB deviation rho' (the stage's
+      // density: prim rho, which equals the state's rho)
+      const double r0 = MIX ? S[0] : p[0];
+      const double rho = g.wb ? (r0 - g.eq_rho_cc[j]) : r0;
       const double gy = g.gravy[j];
       R[2] = R[2] - rho * gy;
       if (!g.etot) R[IE] = R[IE] - S[2] * gy;
@@ -953,8 +970,10 @@ __global__ void __launch_bounds__(256)
       b[0] = 0.5 * (g.bf[0][fix(g, 0, i + 1, j, k)] + g.bf[0][fix(g, 0, i, j, k)]);
       b[1] = 0.5 * (g.bf[1][fix(g, 1, i, j + 1, k)] + g.bf[1][fix(g, 1, i, j, k)]);
       if (SD == 3) b[2] = 0.5 * (g.bf[2][fix(g, 2, i, j, k + 1)] + g.bf[2][fix(g, 2, i, j, k)]);
+      if (keep) {
 #pragma unroll
-      for (int v = 0; v < SD; v++) g.bcc[v * N + c] = b[v];
+        for (int v = 0; v < SD; v++) g.bcc[v * N + c] = b[v];
+      }
     }
     // cons -> prim (source/source.F90:13533-13686).  Everything but the
     // EoS outputs is stored before the Helmholtz solve, so that little state
@@ -973,10 +992,10 @@ __global__ void __launch_bounds__(256)
     if (!HELM) p[IE] = g.gmm1 * eint;
 #pragma unroll
     for (int v = 0; v < NV; v++) {
-      if (!HELM || v != IE) g.P[v * N + c] = p[v];
+      if (keep && (!HELM || v != IE)) g.P[v * N + c] = p[v];
       g.U[v * N + c] = S[v];
     }
-    g.eint[c] = eint;
+    if (keep) g.eint[c] = eint;
     if constexpr (MIX) {
       // FP32 copy of the next sweeps' inputs (the WB deviations formed in FP64)
       g.Pf[c] = (float)(g.wb ? p[0] - g.eq_rho_cc[j] : p[0]);
@@ -1056,12 +1075,12 @@ __global__ void __launch_bounds__(256)
       }
       helm_pin(pr);
       helm_pin(sound);
-      if (g.fasteos) {
+      if (g.fasteos && keep) {
         g.gam[c] = pr / eint + 1.0;
         g.gam[N + c] = sound * sound * rho / pr;
       }
       g.temp[c] = T;
-      g.P[IE * N + c] = pr;
+      if (keep) g.P[IE * N + c] = pr;
       p[IE] = pr;
       sound_out = sound;
       if constexpr (MIX) {
@@ -1437,16 +1456,16 @@ struct Kern {
         g, first, a1, a2, c3);
   }
   static void update(const Grid &g, int first, int last, double a1, double a2, double c3,
-                     hipStream_t st) {
+                     int fin, hipStream_t st) {
     const dim3 blk(64, 4, 1);
     k_update<SD, NAS, MHD, HELM, MIX><<<grid_for(g.n1, g.n2, g.n3, blk), blk, 0, st>>>(
-        g, first, last, a1, a2, c3);
+        g, first, last, a1, a2, c3, fin);
   }
   static void update_guess(const Grid &g, int first, int last, double a1, double a2, double c3,
-                           hipStream_t st) {
+                           int fin, hipStream_t st) {
     const dim3 blk(64, 4, 1);
     k_update<SD, NAS, MHD, HELM, MIX, true><<<grid_for(g.n1, g.n2, g.n3, blk), blk, 0, st>>>(
-        g, first, last, a1, a2, c3);
+        g, first, last, a1, a2, c3, fin);
   }
   static void cfl(const Grid &g, hipStream_t st) {
     const dim3 blk(64, 4, 1);

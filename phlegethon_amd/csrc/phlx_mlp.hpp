@@ -216,7 +216,7 @@ __device__ __forceinline__ int helm_rhoe_lin(const HelmDev &h, double lin_tol, d
                                              double E0, double abar, double zbar, double &T,
                                              double &P, double &sound, int &fallback) {
   double E, cv, dPdT;
-  helm_full(h, rho, T, abar, zbar, P, E, sound, cv, &dPdT);
+  helm_full<true>(h, rho, T, abar, zbar, P, E, sound, cv, &dPdT);
   const double res = E - E0;
   const double dT = -res / cv;
   const double T1 = T + dT;
