@@ -57,13 +57,26 @@ def kernel_bytes(cfg, name: str) -> float:
         return sum(nf[:sd]) * 17 + N * 8 * (2 * sd) + 3 * N * 8
     if name == "faceb":    # b, b0, 4 edge EMFs in; b (+b0 at stage 1) out
         return sum(nf[:sd]) * 8 * 3 + 3 * N * 8
-    if name == "update":   # fluxes, prim, U, U0 (or eint, b_cc), faces in; prim, U, eint, b_cc out
-        rd = sum(nf[:sd]) * 8 * nv + N * 8 * (3 * nv) + (sum(nf[:sd]) * 8 if mhd else 0)
-        wr = N * 8 * (2 * nv + 1 + (sd if mhd else 0))
-        if helm:  # T in; T (+ gamma_e, gamma_c with FASTEOS) out
-            rd += N * 8
-            wr += N * (24 if cfg.fasteos else 8)
-        return rd + wr
+    if name == "update":
+        # compulsory traffic per launch, averaged over the two RK2 stages:
+        # face fluxes (FP32 in the mixed path) and face B in; stage 1 reads
+        # prim + eint + b_cc to rebuild state0 (the mixed path: U) and writes
+        # U0, stage 2 reads U0 + U; both write U and the cell EMFs; the FP64
+        # prim / eint / b_cc (+ gammas) are written at every stage (mixed:
+        # at stage 2 only, plus the FP32 sweep inputs at both); Helmholtz
+        # reads and writes T at both
+        mix = cfg.precision == 1
+        st = nv + 1 + (sd if mhd else 0)                  # prim, eint, b_cc
+        gam = 2 if (helm and cfg.fasteos) else 0
+        ec = sd if mhd else 0
+        rd = sum(nf[:sd]) * (4 if mix else 8) * nv + (sum(nf[:sd]) * 8 if mhd else 0)
+        s1_rd = nv if mix else st
+        s2_rd = 2 * nv
+        s1_wr = 2 * nv + ec + (0 if mix else st + gam)
+        s2_wr = nv + ec + st + gam
+        words = 0.5 * (s1_rd + s2_rd + s1_wr + s2_wr) + (2 if helm else 0)
+        pf = 4 * (nv + 5) if mix else 0
+        return rd + N * (8 * words + pf)
     if name == "misc" and cfg.shock_flattening:
         # shock-flattening flags: pressure in, detector bytes out and in
         # (cells -2..n+1), flag bytes out (cells -1..n)

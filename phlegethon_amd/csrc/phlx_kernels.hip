@@ -889,7 +889,8 @@ __global__ void __launch_bounds__(256)
   if (in) {
     const long long N = g.N, c = cix(g, i, j, k);
     double S[NV], S0[NV], R[NV], p[NV];
-    if (!MIX || first == 1) {
+    // the stored primitives are read only to rebuild state0 (stage 1)
+    if (first == 1) {
 #pragma unroll
       for (int v = 0; v < NV; v++) p[v] = g.P[v * N + c];
     }
@@ -953,8 +954,9 @@ __global__ void __launch_bounds__(256)
     }
     if (g.grav) {
       // source/source.F90:12606-12644 on the WB deviation rho' (the stage's
-      // density: prim rho, which equals the state's rho)
-      const double r0 = MIX ? S[0] : p[0];
+      // prim rho, which is the state's rho bit for bit: cons -> prim stores
+      // rho = S[0])
+      const double r0 = S[0];
       const double rho = g.wb ? (r0 - g.eq_rho_cc[j]) : r0;
       const double gy = g.gravy[j];
       R[2] = R[2] - rho * gy;
