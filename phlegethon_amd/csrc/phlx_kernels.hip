@@ -264,7 +264,10 @@ struct CellW {
 // loading each cell once and reconstructing it once; the right edges of
 // cell f-1 are carried to face f.  k_flux builds every face from four cell
 // loads and two reconstructions.  Each face's arithmetic is that of k_flux.
-constexpr int kSegFaces = 32;
+#ifndef PHLX_SEG_FACES
+#define PHLX_SEG_FACES 32
+#endif
+constexpr int kSegFaces = PHLX_SEG_FACES;
 #ifndef PHLX_FLUX_SEG
 #define PHLX_FLUX_SEG 1
 #endif
