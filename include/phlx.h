@@ -43,8 +43,8 @@ enum { PHLX_HLLC_FLUX = 0, PHLX_LHLLC_FLUX = 1, PHLX_HLLD_FLUX = 2, PHLX_LHLLD_F
 enum { PHLX_BC_PERIODIC = 0, PHLX_BC_REFLECTIVE = 1, PHLX_BC_OUTFLOW = 2, PHLX_BC_DIODE = 3 };
 /* PHLX_EOS_PIG: PIG_EOS, a partially-ionised-gas table in the Helmholtz
  * table layout (pig_rhoT_given_*, source/source.F90:25457-26731), loaded with
- * phlx_load_pig_table; the hydro path carries the two advected scalars
- * (nas 2) as Helmholtz does and the EoS ignores them. */
+ * phlx_load_pig_table; runs carry no advected scalars (nas 0) or two
+ * (nas 2, as Helmholtz) that the EoS ignores. */
 enum { PHLX_EOS_IDEAL = 0, PHLX_EOS_HELMHOLTZ = 1, PHLX_EOS_PIG = 2 };
 enum { PHLX_PREC_FP64 = 0, PHLX_PREC_MIXED = 1 };
 

@@ -535,8 +535,9 @@ typedef struct {
 static void face_helm(const rparams *P, const double *q, double Tg, double *rhoe, double *sound) {
   const int ia = 2 + P->sd, ie = P->sd + 1;
   const double rho = q[0], p = q[ie];
-  const double ye = q[ia];
-  const double abar = 1.0 / q[ia + 1];
+  /* PIG_EOS without advected scalars: the evaluation ignores Ye, abar */
+  const double ye = (P->nv > ia + 1) ? q[ia] : 1.0;
+  const double abar = (P->nv > ia + 1) ? 1.0 / q[ia + 1] : 1.0;
   const double zbar = ye * abar;
   double T = Tg, eint;
   orc_helm_rhoP_given(P->helm, rho, p, abar, zbar, &T, &eint, sound, 1);
@@ -1554,8 +1555,8 @@ int orc_hydro_step(orc_grid *g) {
             /* ADVECT_YE_IABAR + HELMHOLTZ_EOS (:13573-13578); USE_FASTEOS
              * keeps gamma_e / gamma_c (:13648-13653), without it the final
              * evaluation is pressure only (:13654-13655) */
-            double ye = S[g->ias1] * inv_rho;
-            double abar = rho / S[g->ias1 + 1];
+            double ye = nv > g->ias1 + 1 ? S[g->ias1] * inv_rho : 1.0;
+            double abar = nv > g->ias1 + 1 ? rho / S[g->ias1 + 1] : 1.0;
             double zbar = ye * abar;
             double T = g->temp[c], pr, sound;
             orc_helm_rhoe_given(g->helm, rho, eint * inv_rho, abar, zbar, &T, &pr, &sound,
@@ -1593,8 +1594,8 @@ int orc_compute_dt(orc_grid *g) {
         double cs;
         if (TABULATED(g)) {
           /* :13801-13847: full EoS at the carried T */
-          double ye = p[g->ias1];
-          double abar = 1.0 / p[g->ias1 + 1];
+          double ye = nv > g->ias1 + 1 ? p[g->ias1] : 1.0;
+          double abar = nv > g->ias1 + 1 ? 1.0 / p[g->ias1 + 1] : 1.0;
           double zbar = ye * abar, ei, cv;
           orc_helm_full(g->helm, rho, g->temp[c], abar, zbar, &pr, &ei, &cs, &cv);
         } else {
@@ -1633,8 +1634,8 @@ int orc_init(orc_grid *g) {
         if (TABULATED(g)) {
           /* step-0 Helmholtz init (source.F90:6279-6285; FASTEOS
            * :6351-6357, otherwise energy only :6358-6361) */
-          double abar = 1.0 / g->prim[c * nv + g->ias1 + 1];
-          double ye = g->prim[c * nv + g->ias1];
+          double abar = nv > g->ias1 + 1 ? 1.0 / g->prim[c * nv + g->ias1 + 1] : 1.0;
+          double ye = nv > g->ias1 + 1 ? g->prim[c * nv + g->ias1] : 1.0;
           double zbar = abar * ye;
           double T = g->temp[c], eint, sound;
           orc_helm_rhoP_given(g->helm, rho, p, abar, zbar, &T, &eint, &sound, g->c.fasteos);
@@ -1674,8 +1675,8 @@ int orc_resume(orc_grid *g, double time, double dt, double smax, long step) {
         double rho = g->prim[c * nv], p = g->prim[c * nv + g->ip];
         double eint;
         if (TABULATED(g)) {
-          double abar = 1.0 / g->prim[c * nv + g->ias1 + 1];
-          double ye = g->prim[c * nv + g->ias1];
+          double abar = nv > g->ias1 + 1 ? 1.0 / g->prim[c * nv + g->ias1 + 1] : 1.0;
+          double ye = nv > g->ias1 + 1 ? g->prim[c * nv + g->ias1] : 1.0;
           double zbar = abar * ye;
           double T = g->temp[c], pe, sound, cv;
           orc_helm_full(g->helm, rho, T, abar, zbar, &pe, &eint, &sound, &cv);
@@ -1835,8 +1836,8 @@ static void eint_to_temp(orc_grid *g) {
         const long c = cidx(g, i, j, k), q = iidx(g, i, j, k);
         const double rho = g->prim[c * nv], eint = g->eint[q];
         if (TABULATED(g)) {
-          const double ye = g->prim[c * nv + g->ias1];
-          const double abar = 1.0 / g->prim[c * nv + g->ias1 + 1];
+          const double ye = nv > g->ias1 + 1 ? g->prim[c * nv + g->ias1] : 1.0;
+          const double abar = nv > g->ias1 + 1 ? 1.0 / g->prim[c * nv + g->ias1 + 1] : 1.0;
           const double zbar = ye * abar;
           double T = g->temp[c], pr, sound;
           orc_helm_rhoe_given(g->helm, rho, eint / rho, abar, zbar, &T, &pr, &sound, 0);
@@ -1900,8 +1901,8 @@ int orc_diffusion_step(orc_grid *g, int strang) {
         const long c = cidx(g, i, j, k), q = iidx(g, i, j, k);
         const double rho = g->prim[c * nv], T = g->temp[c];
         if (TABULATED(g)) {
-          const double ye = g->prim[c * nv + g->ias1];
-          const double abar = 1.0 / g->prim[c * nv + g->ias1 + 1];
+          const double ye = nv > g->ias1 + 1 ? g->prim[c * nv + g->ias1] : 1.0;
+          const double abar = nv > g->ias1 + 1 ? 1.0 / g->prim[c * nv + g->ias1 + 1] : 1.0;
           const double zbar = abar * ye;
           double pr, e, sound, cv;
           orc_helm_full(g->helm, rho, T, abar, zbar, &pr, &e, &sound, &cv);

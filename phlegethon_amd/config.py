@@ -161,9 +161,8 @@ class PhlxConfig:
         if self.eos == EOS_HELMHOLTZ and self.nas != 2:
             raise ConfigError("HELMHOLTZ_EOS runs here with ADVECT_YE_IABAR "
                               "(nas_make=2: Ye, 1/abar)")
-        if self.eos == EOS_PIG and self.nas != 2:
-            raise ConfigError("PIG_EOS runs here on the tabulated-EoS kernels with two "
-                              "advected scalars (nas_make=2, unused by the EoS)")
+        if self.eos == EOS_PIG and self.nas not in (0, 2):
+            raise ConfigError("PIG_EOS runs with nas_make=0 or 2 (scalars unused by the EoS)")
         if self.eos != EOS_IDEAL and not self.fasteos and self.precision != PREC_FP64:
             raise ConfigError("HELMHOLTZ_EOS without USE_FASTEOS (a T(rho, P) Newton per "
                               "face state) runs in FP64 only")

@@ -119,7 +119,7 @@ static int validate(const phlx_config *c) {
   // PIG_EOS runs on the same tabulated-EoS kernels (its table has Ye = 1
   // and no ion terms, so the two advected scalars are carried unused)
   if (c->eos == PHLX_EOS_HELMHOLTZ || c->eos == PHLX_EOS_PIG) {
-    if (c->nas != 2) return PHLX_E_CONFIG;
+    if (c->nas != 2 && !(c->eos == PHLX_EOS_PIG && c->nas == 0)) return PHLX_E_CONFIG;
     if (!c->fasteos && c->precision != PHLX_PREC_FP64) return PHLX_E_CONFIG;
   } else if (c->eos == PHLX_EOS_IDEAL) {
     if (c->fasteos) return PHLX_E_CONFIG;

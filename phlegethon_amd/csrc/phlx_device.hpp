@@ -103,6 +103,20 @@ __device__ __forceinline__ long long kix(const Grid &g, int d, int i, int j, int
   return ((long long)k * g.n2 + j) * (g.n1 + 1) + i;
 }
 
+// Ye and abar of a cell from the primitive store: the two advected scalars
+// (ADVECT_YE_IABAR, nas 2), or 1 and 1 when the run carries none (PIG_EOS,
+// whose evaluation ignores them).
+__device__ __forceinline__ void cell_ye_abar(const Grid &g, long long c, double &ye,
+                                             double &abar) {
+  if (g.nas >= 2) {
+    ye = g.P[(long long)(g.sd + 2) * g.N + c];
+    abar = 1.0 / g.P[(long long)(g.sd + 3) * g.N + c];
+  } else {
+    ye = 1.0;
+    abar = 1.0;
+  }
+}
+
 // Shock-flattening arrays: flag box of cells -1 .. n, detector box of cells
 // -2 .. n+1 on each axis (the x3 extension only in 3D).
 __device__ __forceinline__ long long sfix(const Grid &g, int i, int j, int k) {
@@ -378,13 +392,16 @@ enum { EK_IDEAL = 0, EK_FASTEOS = 1, EK_HELM_FACE = 2 };
 // rho*e and the sound speed of one face state by helm_rhoP_given from Tg
 // (HLLC source/source.F90:15776-15780, :15891-15895; HLLD :14172-14176 and
 // the others alike); Ye and abar from the advected scalars (:15706-15725).
-template <int SD>
+template <int SD, int NAS>
 __device__ __forceinline__ void face_helm(const Grid &g, const double *q, double Tg,
                                           double &rhoe, double &sound) {
   constexpr int IE = SD + 1, IA = SD + 2;
   const double rho = q[0], p = q[IE];
-  const double ye = q[IA];
-  const double abar = 1.0 / q[IA + 1];
+  double ye = 1.0, abar = 1.0;   // PIG_EOS without scalars ignores both
+  if constexpr (NAS >= 2) {
+    ye = q[IA];
+    abar = 1.0 / q[IA + 1];
+  }
   const double zbar = ye * abar;
   double T = Tg, eint;
   helm_rhoP_given(g.helm, rho, p, abar, zbar, T, eint, sound, true);
@@ -408,7 +425,7 @@ __device__ __forceinline__ void riemann_hydro(const Grid &g, const double *qL,
     eL = pL / (gL[0] - 1.0);
     cL = sqrt(gL[1] * pL * irL);
   } else if (EK == EK_HELM_FACE) {
-    face_helm<SD>(g, qL, Tg, eL, cL);
+    face_helm<SD, NAS>(g, qL, Tg, eL, cL);
   } else {
     eL = g.igmm1 * pL;
     cL = sqrt(g.gm * pL * irL);
@@ -422,7 +439,7 @@ __device__ __forceinline__ void riemann_hydro(const Grid &g, const double *qL,
     eR = pR / (gR[0] - 1.0);
     cR = sqrt(gR[1] * pR * irR);
   } else if (EK == EK_HELM_FACE) {
-    face_helm<SD>(g, qR, Tg, eR, cR);
+    face_helm<SD, NAS>(g, qR, Tg, eR, cR);
   } else {
     eR = g.igmm1 * pR;
     cR = sqrt(g.gm * pR * irR);
@@ -557,7 +574,7 @@ __device__ __forceinline__ void riemann_mhd(const Grid &g, const double *qL,
     c2L = gL[1] * pL * irL;
   } else if (EK == EK_HELM_FACE) {
     double sound;
-    face_helm<SD>(g, qL, Tg, e, sound);
+    face_helm<SD, NAS>(g, qL, Tg, e, sound);
     c2L = sound * sound;
   } else {
     e = pL * g.igmm1;
@@ -574,7 +591,7 @@ __device__ __forceinline__ void riemann_mhd(const Grid &g, const double *qL,
     c2R = gR[1] * pR * irR;
   } else if (EK == EK_HELM_FACE) {
     double sound;
-    face_helm<SD>(g, qR, Tg, e, sound);
+    face_helm<SD, NAS>(g, qR, Tg, e, sound);
     c2R = sound * sound;
   } else {
     e = pR * g.igmm1;

@@ -139,9 +139,8 @@ template <int SD, bool HELM>
 __device__ __forceinline__ double diff_eint_to_temp(const Grid &g, long long c, double rho,
                                                     double eint) {
   if (!HELM) return g.gmm1 * eint / (g.inv_mu * rho * kRgas);
-  constexpr int IA = SD + 2;
-  const double ye = g.P[(long long)IA * g.N + c];
-  const double abar = 1.0 / g.P[(long long)(IA + 1) * g.N + c];
+  double ye, abar;
+  cell_ye_abar(g, c, ye, abar);
   const double zbar = ye * abar;
   double T = g.tdiff[c], pr, sound;
   helm_rhoe_given(g.helm, rho, eint / rho, abar, zbar, T, pr, sound, false);
@@ -164,14 +163,14 @@ __global__ void __launch_bounds__(256) k_diff_final(Grid g) {
   if (!g.sc->active) return;
   int i, j, k;
   if (!diff_coords<SD>(g, i, j, k)) return;
-  constexpr int IP = SD + 1, IA = SD + 2;
+  constexpr int IP = SD + 1;
   const long long c = cix(g, i, j, k), N = g.N;
   const double rho = g.P[c];
   const double T = diff_eint_to_temp<SD, HELM>(g, c, rho, g.eint[c]);
   g.tdiff[c] = T;
   if (HELM) {
-    const double ye = g.P[(long long)IA * N + c];
-    const double abar = 1.0 / g.P[(long long)(IA + 1) * N + c];
+    double ye, abar;
+    cell_ye_abar(g, c, ye, abar);
     const double zbar = abar * ye;
     double pr, e, sound, cv;
     helm_full(g.helm, rho, T, abar, zbar, pr, e, sound, cv);

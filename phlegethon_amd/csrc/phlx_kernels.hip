@@ -821,15 +821,14 @@ __device__ __forceinline__ double reduce_partials(const double *red, int n) {
 // :13891-13944): (|v_d| + c_f) / dx_d with c_f^2 = c^2 + B^2/rho.
 template <int SD, bool MHD, bool HELM, bool KNOWN_CS = false>
 __device__ __forceinline__ double cell_speed(const Grid &g, const double *p, const double *b,
-                                             double T, double cs_known = 0.0) {
+                                             double T, double cs_known = 0.0, double ye = 1.0,
+                                             double abar = 1.0) {
   double c;
   if (KNOWN_CS) {
     // the sound speed of the zone's own EoS solve (mixed precision)
     c = cs_known;
   } else if (HELM) {
     // full Helmholtz evaluation at the carried T (source/source.F90:13801-13847)
-    const double ye = p[SD + 2];
-    const double abar = 1.0 / p[SD + 3];
     const double zbar = ye * abar;
     double pr, ei, cv;
     helm_full(g.helm, p[0], T, abar, zbar, pr, ei, c, cv);
@@ -1021,13 +1020,22 @@ __global__ void __launch_bounds__(256)
         g.ecell[N + c] = emf_of<SD>(1, vv, b);
       }
     }
-    double T = 0.0, sound_out = 0.0;
+    double T = 0.0, sound_out = 0.0, ye_c = 1.0, abar_c = 1.0;
     if (HELM) {
       // Ye / 1/abar advected; T(rho, e) Newton from the carried T, then
       // gamma_e = p/eint + 1 and gamma_c = c^2 rho / p (:13573-13578,
       // :13648-13660)
-      const double ye = p[IA];
-      const double abar = rho / S[IA + 1];
+      double ye = 1.0, abar = 1.0;   // PIG_EOS without scalars ignores both
+      if constexpr (NAS >= 2) {
+        ye = p[IA];
+        abar = rho / S[IA + 1];
+      }
+      if constexpr (NAS >= 2) {
+        // the CFL's evaluation takes abar from the stored prim, 1 / (1/abar)
+        // (compute_hyperbolic_dt, source/source.F90:13801-13847)
+        ye_c = p[IA];
+        abar_c = 1.0 / p[IA + 1];
+      }
       const double zbar = ye * abar;
       double pr, sound;
       T = g.temp[c];
@@ -1095,7 +1103,8 @@ __global__ void __launch_bounds__(256)
         g.Pf[(NV + 4) * N + c] = (float)(sound * sound * rho / pr);
       }
     }
-    if (last) smax_local = cell_speed<SD, MHD, HELM, HELM && MIX>(g, p, b, T, sound_out);
+    if (last)
+      smax_local = cell_speed<SD, MHD, HELM, HELM && MIX>(g, p, b, T, sound_out, ye_c, abar_c);
   }
   if (last) block_max_to(smax_local, g.red);
 }
@@ -1103,7 +1112,7 @@ __global__ void __launch_bounds__(256)
 // Standalone compute_hyperbolic_dt pass over the current prim / b_cc.
 template <int SD, bool MHD, bool HELM>
 __global__ void __launch_bounds__(256) k_cfl(Grid g) {
-  constexpr int NL = HELM ? SD + 4 : SD + 2;  // + Ye, 1/abar
+  constexpr int NL = SD + 2;  // rho, v, p
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y * blockDim.y + threadIdx.y;
   const int k = blockIdx.z;
@@ -1116,7 +1125,9 @@ __global__ void __launch_bounds__(256) k_cfl(Grid g) {
     if (MHD)
 #pragma unroll
       for (int v = 0; v < SD; v++) b[v] = g.bcc[v * N + c];
-    s = cell_speed<SD, MHD, HELM>(g, p, b, HELM ? g.temp[c] : 0.0);
+    double ye = 1.0, abar = 1.0;
+    if (HELM) cell_ye_abar(g, c, ye, abar);
+    s = cell_speed<SD, MHD, HELM>(g, p, b, HELM ? g.temp[c] : 0.0, 0.0, ye, abar);
   }
   block_max_to(s, g.red);
 }
@@ -1164,8 +1175,8 @@ __global__ void __launch_bounds__(256) k_init(Grid g, int restart) {
     // restored (rho, T); its pressure only feeds the gammas
     // (source/source.F90:6413-6500)
     const double rho = g.P[c];
-    const double abar = 1.0 / g.P[(SD + 3) * N + c];
-    const double ye = g.P[(SD + 2) * N + c];
+    double ye, abar;
+    cell_ye_abar(g, c, ye, abar);
     const double zbar = abar * ye;
     double pe, eint, sound, cv;
     helm_full(g.helm, rho, g.temp[c], abar, zbar, pe, eint, sound, cv);
@@ -1179,8 +1190,8 @@ __global__ void __launch_bounds__(256) k_init(Grid g, int restart) {
     // T(rho, p) Newton from the uploaded T, eint (+ gamma_e, gamma_c with
     // FASTEOS) (source/source.F90:6279-6285, :6351-6361)
     const double rho = g.P[c], p = g.P[(SD + 1) * N + c];
-    const double abar = 1.0 / g.P[(SD + 3) * N + c];
-    const double ye = g.P[(SD + 2) * N + c];
+    double ye, abar;
+    cell_ye_abar(g, c, ye, abar);
     const double zbar = abar * ye;
     double T = g.temp[c], eint, sound;
     helm_rhoP_given(g.helm, rho, p, abar, zbar, T, eint, sound, g.fasteos != 0);
@@ -1512,22 +1523,37 @@ static KernelTable make_table() {
 template <bool MIX, bool R5>
 static bool select_prec(int sd, int nas, bool mhd, bool helm, bool fasteos, KernelTable &t) {
   if (helm) {
-    // Helmholtz runs carry Ye and 1/abar as the two advected scalars
-    if (nas != 2) return false;
+    // Helmholtz runs carry Ye and 1/abar as the two advected scalars; PIG
+    // runs may carry them (ignored) or none
+    if (nas != 2 && nas != 0) return false;
     if (!fasteos) {
       // per-face T(rho, P) Newton in the Riemann solvers: FP64 only
       if (MIX) return false;
       constexpr int EK = EK_HELM_FACE;
-      if (sd == 2 && !mhd) t = make_table<2, 2, false, true, false, R5, EK>();
-      else if (sd == 3 && !mhd) t = make_table<3, 2, false, true, false, R5, EK>();
-      else if (sd == 3 && mhd) t = make_table<3, 2, true, true, false, R5, EK>();
-      else return false;
+      if (nas == 2) {
+        if (sd == 2 && !mhd) t = make_table<2, 2, false, true, false, R5, EK>();
+        else if (sd == 3 && !mhd) t = make_table<3, 2, false, true, false, R5, EK>();
+        else if (sd == 3 && mhd) t = make_table<3, 2, true, true, false, R5, EK>();
+        else return false;
+      } else {
+        if (sd == 2 && !mhd) t = make_table<2, 0, false, true, false, R5, EK>();
+        else if (sd == 3 && !mhd) t = make_table<3, 0, false, true, false, R5, EK>();
+        else if (sd == 3 && mhd) t = make_table<3, 0, true, true, false, R5, EK>();
+        else return false;
+      }
       return true;
     }
-    if (sd == 2 && !mhd) t = make_table<2, 2, false, true, MIX, R5>();
-    else if (sd == 3 && !mhd) t = make_table<3, 2, false, true, MIX, R5>();
-    else if (sd == 3 && mhd) t = make_table<3, 2, true, true, MIX, R5>();
-    else return false;
+    if (nas == 2) {
+      if (sd == 2 && !mhd) t = make_table<2, 2, false, true, MIX, R5>();
+      else if (sd == 3 && !mhd) t = make_table<3, 2, false, true, MIX, R5>();
+      else if (sd == 3 && mhd) t = make_table<3, 2, true, true, MIX, R5>();
+      else return false;
+    } else {
+      if (sd == 2 && !mhd) t = make_table<2, 0, false, true, MIX, R5>();
+      else if (sd == 3 && !mhd) t = make_table<3, 0, false, true, MIX, R5>();
+      else if (sd == 3 && mhd) t = make_table<3, 0, true, true, MIX, R5>();
+      else return false;
+    }
     return true;
   }
   if (sd == 2 && nas == 0 && !mhd) t = make_table<2, 0, false, false, MIX, R5>();

@@ -421,15 +421,17 @@ def helm_slab(eos, table, n: int = 256, ny: Optional[int] = None, sdims: int = 3
 
 def pig_box(eos, table, n: int = 32, sdims: int = 2, recon: int = LIM2ND_REC,
             flux: int = LHLLC_FLUX, seed: int = SEED, mhd_beta: Optional[float] = None,
-            fasteos: bool = True, rho0: float = 1e-7, T0: float = 1.5e4) -> Problem:
+            fasteos: bool = True, rho0: float = 1e-7, T0: float = 1.5e4,
+            nas: int = 2) -> Problem:
     """Periodic box of partially ionised gas for PIG_EOS: rho and T vary by
     30 % across the box around (rho0, T0) (hydrogen about half ionised at
     the defaults), P and the step-0 T from the PIG table, seeded velocities
-    of ~0.1 of the sound speed; two advected scalars (1, 1) that the EoS
-    ignores.  ``eos`` evaluates the table (``full(rho, T, abar, zbar)``)."""
+    of ~0.1 of the sound speed; with nas = 2 two advected scalars (1, 1)
+    that the EoS ignores.  ``eos`` evaluates the table (``full(rho, T, abar,
+    zbar)``)."""
     mhd = mhd_beta is not None
     L = 1e9
-    cfg = PhlxConfig(sdims=sdims, nx1=n, nx2=n, nx3=n if sdims == 3 else 1, nas=2,
+    cfg = PhlxConfig(sdims=sdims, nx1=n, nx2=n, nx3=n if sdims == 3 else 1, nas=nas,
                      x1u=L, x2u=L, x3u=L, recon=recon, flux=(LHLLD_FLUX if mhd else flux),
                      cfl=0.8, mach_cutoff=1e-4, eos=EOS_PIG, fasteos=fasteos).validate()
     X, Y, Z = mesh(cfg)
@@ -447,8 +449,9 @@ def pig_box(eos, table, n: int = 32, sdims: int = 2, recon: int = LIM2ND_REC,
         prim[1 + d] = 0.1 * cs * (np.sin(tp * (Y if d == 0 else X) + d) + 0.2 * u(3 + d))
     ip = sdims + 1
     prim[ip] = P
-    prim[ip + 1] = 1.0
-    prim[ip + 2] = 1.0
+    if nas == 2:
+        prim[ip + 1] = 1.0
+        prim[ip + 2] = 1.0
     b_faces = None
     if mhd:
         b_faces = _random_div_free(cfg, seed, amp=1.0)

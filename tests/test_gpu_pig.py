@@ -76,7 +76,11 @@ def _run_gpu(pb, nsteps, cfg=None):
     dict(n=24, sdims=2, recon=PPH_REC, flux=HLLC_FLUX, fasteos=False),
     dict(n=12, sdims=3, recon=LIM2ND_REC, mhd_beta=10.0),
     dict(n=10, sdims=3, recon=LIM2ND_REC, flux=LHLLC_FLUX, fasteos=False),
-], ids=["2d-fasteos", "2d-face-newton", "3d-mhd", "3d-face-newton"])
+    dict(n=24, sdims=2, recon=LIM2ND_REC, flux=LHLLC_FLUX, nas=0),
+    dict(n=10, sdims=3, recon=PPH_REC, flux=HLLC_FLUX, fasteos=False, nas=0),
+    dict(n=10, sdims=3, recon=LIM2ND_REC, mhd_beta=10.0, nas=0),
+], ids=["2d-fasteos", "2d-face-newton", "3d-mhd", "3d-face-newton", "2d-fasteos-nas0",
+        "3d-face-newton-nas0", "3d-mhd-nas0"])
 def test_hydro_parity(table, orc, oracle_mod, case):
     pb = P.pig_box(orc, table, **case)
     nsteps = 6

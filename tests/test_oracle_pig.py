@@ -113,8 +113,9 @@ def test_config():
     c = PhlxConfig.from_opts(["sdims_make=2", "nx1_make=8", "nx2_make=8", "nas_make=2",
                               "PIG_EOS", "USE_FASTEOS"])
     assert c.eos == EOS_PIG and c.fasteos
+    PhlxConfig(sdims=2, nx1=8, nx2=8, eos=EOS_PIG, nas=0, fasteos=True).validate()
     with pytest.raises(ConfigError):
-        PhlxConfig(sdims=2, nx1=8, nx2=8, eos=EOS_PIG, nas=0).validate()
+        PhlxConfig(sdims=2, nx1=8, nx2=8, eos=EOS_PIG, nas=1).validate()
     with pytest.raises(ConfigError):
         PhlxConfig(sdims=2, nx1=8, nx2=8, eos=EOS_PIG, nas=2, precision=1).validate()
 
@@ -124,8 +125,8 @@ def test_hydro_runs_and_conserves(oracle_mod, pig):
     Newton): mass and energy conserved on the periodic box."""
     from phlegethon_amd import problems as P
     tab, o = pig
-    for fasteos in (True, False):
-        pb = P.pig_box(o, tab, n=16, fasteos=fasteos)
+    for fasteos, nas in ((True, 2), (False, 2), (True, 0), (False, 0)):
+        pb = P.pig_box(o, tab, n=16, fasteos=fasteos, nas=nas)
         r = oracle_mod.OracleRun(pb)
         r.init()
         s0 = r.state()
