@@ -816,10 +816,18 @@ __device__ __forceinline__ void riemann_mhd(const Grid &g, const double *qL,
 // ====================================================================
 namespace mx {
 
+// FP32 quotients and square roots on the hardware's reciprocal and square
+// root (v_rcp_f32, v_sqrt_f32: 1 ulp).  The IEEE-rounded FP32 division and
+// sqrt expand to ~10-instruction sequences each, and the mixed sweeps are
+// VALU-issue bound (~25 quotients and 10 roots per face in LIM2ND + HLLD).
+__device__ __forceinline__ float qrcp(float b) { return __builtin_amdgcn_rcpf(b); }
+__device__ __forceinline__ float qdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+__device__ __forceinline__ float qsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+
 __device__ __forceinline__ void rec_lim2nd(float qm, float qc, float qp, float &l, float &r) {
   float sf = qp - qc, sb = qc - qm;
   float prod = sf * sb, slope = 0.0f;
-  if (prod > 0.0f) slope = prod / (sf + sb);
+  if (prod > 0.0f) slope = qdiv(prod, sf + sb);
   float h = 0.5f * slope;
   l = qc - h;
   r = qc + h;
@@ -893,36 +901,36 @@ __device__ __forceinline__ void riemann_hydro(const Grid &g, const float *qL, co
                                               const float *gR, float *F) {
   constexpr int IE = SD + 1, IA = SD + 2, NV = SD + 2 + NAS;
   const float gm = (float)g.gm, igmm1 = (float)g.igmm1;
-  const float rhoL = qL[0], irL = 1.0f / rhoL;
+  const float rhoL = qL[0], irL = qrcp(rhoL);
   const float uL = qL[1], vL = qL[2], wL = (SD == 3) ? qL[3] : 0.0f;
   const float v2L = uL * uL + vL * vL + wL * wL;
-  const float rhoR = qR[0], irR = 1.0f / rhoR;
+  const float rhoR = qR[0], irR = qrcp(rhoR);
   const float uR = qR[1], vR = qR[2], wR = (SD == 3) ? qR[3] : 0.0f;
   const float v2R = uR * uR + vR * vR + wR * wR;
   float eL, cL, eR, cR;
   if (FE) {
-    eL = pfL / (gL[0] - 1.0f);
-    cL = sqrtf(gL[1] * pfL * irL);
-    eR = pfR / (gR[0] - 1.0f);
-    cR = sqrtf(gR[1] * pfR * irR);
+    eL = qdiv(pfL, gL[0] - 1.0f);
+    cL = qsqrt(gL[1] * pfL * irL);
+    eR = qdiv(pfR, gR[0] - 1.0f);
+    cR = qsqrt(gR[1] * pfR * irR);
   } else {
     eL = igmm1 * pfL;
-    cL = sqrtf(gm * pfL * irL);
+    cL = qsqrt(gm * pfL * irL);
     eR = igmm1 * pfR;
-    cR = sqrtf(gm * pfR * irR);
+    cR = qsqrt(gm * pfR * irR);
   }
   const float EL = eL + 0.5f * rhoL * v2L, ER = eR + 0.5f * rhoR * v2R;
   const float dpL = qL[IE], dpR = qR[IE];
   const float cmax = fmaxf(cL, cR);
   const float sL = fminf(uL, uR) - cmax, sR = fmaxf(uL, uR) + cmax;
   const float dL = sL - uL, dR = sR - uR;
-  const float us = (dpR - dpL + rhoL * uL * dL - rhoR * uR * dR) / (rhoL * dL - rhoR * dR);
-  const float rsL = rhoL * (dL / (sL - us));
-  const float EsL = rsL * (EL * irL + (us - uL) * (us + pfL * irL / dL));
-  const float rsR = rhoR * (dR / (sR - us));
-  const float EsR = rsR * (ER * irR + (us - uR) * (us + pfR * irR / dR));
+  const float us = qdiv(dpR - dpL + rhoL * uL * dL - rhoR * uR * dR, rhoL * dL - rhoR * dR);
+  const float rsL = rhoL * qdiv(dL, sL - us);
+  const float EsL = rsL * (EL * irL + (us - uL) * (us + qdiv(pfL * irL, dL)));
+  const float rsR = rhoR * qdiv(dR, sR - us);
+  const float EsR = rsR * (ER * irR + (us - uR) * (us + qdiv(pfR * irR, dR)));
   if (g.flux == FLUX_LHLLC) {
-    float chi = fminf(1.0f, fmaxf(sqrtf(v2L) / cL, sqrtf(v2R) / cR));
+    float chi = fminf(1.0f, fmaxf(qdiv(qsqrt(v2L), cL), qdiv(qsqrt(v2R), cR)));
     chi = fmaxf((float)g.mach_cutoff, chi);
     const float phi = chi * (2.0f - chi);
     const float rbar = 0.5f * (rhoL + rhoR), cbar = 0.5f * (cL + cR);
@@ -981,13 +989,13 @@ __device__ __forceinline__ void riemann_hydro(const Grid &g, const float *qL, co
     for (int v = 0; v < NV; v++) F[v] = f[v];
   } else {
     const float du = us - u;
-    const float D = rho * du / (s - us);
+    const float D = qdiv(rho * du, s - us);
     float dU[NV];
     dU[0] = D;
     dU[1] = D * us + rho * du;
     dU[2] = D * vv;
     if (SD == 3) dU[3] = D * ww;
-    dU[IE] = D * (E / rho) + rs * du * (us + pf / (rho * d));
+    dU[IE] = D * qdiv(E, rho) + rs * du * (us + qdiv(pf, rho * d));
 #pragma unroll
     for (int v = IA; v < NV; v++) dU[v] = D * q[v];
 #pragma unroll
@@ -1004,17 +1012,17 @@ __device__ __forceinline__ void riemann_mhd(const Grid &g, const float *qL, cons
                                             float *F, float *Fb) {
   constexpr int IE = SD + 1, IA = SD + 2, NV = SD + 2 + NAS;
   const float gm = (float)g.gm, igmm1 = (float)g.igmm1;
-  const float rhoL = qL[0], irL = 1.0f / rhoL;
+  const float rhoL = qL[0], irL = qrcp(rhoL);
   const float uL = qL[1], vL = qL[2], wL = (SD == 3) ? qL[3] : 0.0f;
   const float v2L = uL * uL + vL * vL + wL * wL;
-  const float rhoR = qR[0], irR = 1.0f / rhoR;
+  const float rhoR = qR[0], irR = qrcp(rhoR);
   const float uR = qR[1], vR = qR[2], wR = (SD == 3) ? qR[3] : 0.0f;
   const float v2R = uR * uR + vR * vR + wR * wR;
   float eL, eR, c2L, c2R;
   if (FE) {
-    eL = pfL / (gL[0] - 1.0f);
+    eL = qdiv(pfL, gL[0] - 1.0f);
     c2L = gL[1] * pfL * irL;
-    eR = pfR / (gR[0] - 1.0f);
+    eR = qdiv(pfR, gR[0] - 1.0f);
     c2R = gR[1] * pfR * irR;
   } else {
     eL = pfL * igmm1;
@@ -1034,22 +1042,22 @@ __device__ __forceinline__ void riemann_mhd(const Grid &g, const float *qL, cons
   const float a2L = 2.0f * pbL * irL, a2R = 2.0f * pbR * irR;
   const float an2L = bx2 * irL, an2R = bx2 * irR;
   float t = c2L + a2L;
-  const float cL = sqrtf(0.5f * (c2L + a2L + sqrtf(fmaxf(t * t - 4.0f * c2L * an2L, 0.0f))));
+  const float cL = qsqrt(0.5f * (c2L + a2L + qsqrt(fmaxf(t * t - 4.0f * c2L * an2L, 0.0f))));
   t = c2R + a2R;
-  const float cR = sqrtf(0.5f * (c2R + a2R + sqrtf(fmaxf(t * t - 4.0f * c2R * an2R, 0.0f))));
+  const float cR = qsqrt(0.5f * (c2R + a2R + qsqrt(fmaxf(t * t - 4.0f * c2R * an2R, 0.0f))));
   const float cmax = fmaxf(cL, cR);
   const float sL = fminf(uL, uR) - cmax, sR = fmaxf(uL, uR) + cmax;
   const float dL = sL - uL, dR = sR - uR;
-  const float it1 = 1.0f / (dR * rhoR - dL * rhoL);
+  const float it1 = qrcp(dR * rhoR - dL * rhoL);
   const float us = (dR * rhoR * uR - dL * rhoL * uL - pdR + pdL) * it1;
   const float dmL = sL - us, dmR = sR - us;
   float phi = 1.0f;
   if (g.flux == FLUX_LHLLD) {
     t = v2L + a2L;
-    const float cu2L = 0.5f * (v2L + a2L + sqrtf(fmaxf(t * t - 4.0f * v2L * an2L, 0.0f)));
+    const float cu2L = 0.5f * (v2L + a2L + qsqrt(fmaxf(t * t - 4.0f * v2L * an2L, 0.0f)));
     t = v2R + a2R;
-    const float cu2R = 0.5f * (v2R + a2R + sqrtf(fmaxf(t * t - 4.0f * v2R * an2R, 0.0f)));
-    const float mL = sqrtf(cu2L) / cL, mR = sqrtf(cu2R) / cR;
+    const float cu2R = 0.5f * (v2R + a2R + qsqrt(fmaxf(t * t - 4.0f * v2R * an2R, 0.0f)));
+    const float mL = qdiv(qsqrt(cu2L), cL), mR = qdiv(qsqrt(cu2R), cR);
     if (fmaxf(mL, mR) < 0.6f) {
       float chi = fminf(1.0f, fmaxf(mL, mR));
       chi = fmaxf((float)g.mach_cutoff, chi);
@@ -1059,13 +1067,13 @@ __device__ __forceinline__ void riemann_mhd(const Grid &g, const float *qL, cons
   const float diss = phi * rhoL * rhoR * dR * dL * (uL - uR);
   const float psf = (dR * rhoR * ptL - dL * rhoL * ptR - diss) * it1;
   const float psd = (dR * rhoR * pdL - dL * rhoL * pdR - diss) * it1;
-  const float rsL = rhoL * dL / dmL, rsR = rhoR * dR / dmR;
-  const float ia1 = 1.0f / (rhoL * dL * dmL - bx2);
+  const float rsL = qdiv(rhoL * dL, dmL), rsR = qdiv(rhoR * dR, dmR);
+  const float ia1 = qrcp(rhoL * dL * dmL - bx2);
   const float vsL = vL - bx * byL * (us - uL) * ia1;
   const float bysL = byL * (rhoL * dL * dL - bx2) * ia1;
   const float wsL = wL - bx * bzL * (us - uL) * ia1;
   const float bzsL = bzL * (rhoL * dL * dL - bx2) * ia1;
-  const float ia2 = 1.0f / (rhoR * dR * dmR - bx2);
+  const float ia2 = qrcp(rhoR * dR * dmR - bx2);
   const float vsR = vR - bx * byR * (us - uR) * ia2;
   const float bysR = byR * (rhoR * dR * dR - bx2) * ia2;
   const float wsR = wR - bx * bzR * (us - uR) * ia2;
@@ -1074,12 +1082,12 @@ __device__ __forceinline__ void riemann_mhd(const Grid &g, const float *qL, cons
   const float vbR = uR * bx + vR * byR + wR * bzR;
   const float vbsL = us * bx + vsL * bysL + wsL * bzsL;
   const float vbsR = us * bx + vsR * bysR + wsR * bzsR;
-  const float EsL = (dL * EL - ptL * uL + psf * us + bx * (vbL - vbsL)) / dmL;
-  const float EsR = (dR * ER - ptR * uR + psf * us + bx * (vbR - vbsR)) / dmR;
-  const float sqL = sqrtf(rsL), sqR = sqrtf(rsR);
+  const float EsL = qdiv(dL * EL - ptL * uL + psf * us + bx * (vbL - vbsL), dmL);
+  const float EsR = qdiv(dR * ER - ptR * uR + psf * us + bx * (vbR - vbsR), dmR);
+  const float sqL = qsqrt(rsL), sqR = qsqrt(rsR);
   const float abx = fabsf(bx);
-  const float ssL = us - abx / sqL, ssR = us + abx / sqR;
-  const float isq = 1.0f / (sqL + sqR);
+  const float ssL = us - qdiv(abx, sqL), ssR = us + qdiv(abx, sqR);
+  const float isq = qrcp(sqL + sqR);
   const float sg = copysignf(1.0f, bx);
   const float vss = (sqL * vsL + sqR * vsR + (bysR - bysL) * sg) * isq;
   const float byss = (sqL * bysR + sqR * bysL + sqL * sqR * (vsR - vsL) * sg) * isq;
@@ -1155,13 +1163,13 @@ __device__ __forceinline__ void riemann_mhd(const Grid &g, const float *qL, cons
               vbs = left ? vbsL : vbsR, d = left ? dL : dR, dm = left ? dmL : dmR,
               ia = left ? ia1 : ia2;
   const float du = us - u;
-  const float D = rho * du / dm;
+  const float D = qdiv(rho * du, dm);
   float dU[NV], dUb[3];
   dU[0] = D;
   dU[1] = D * us + rho * du;
   dU[2] = D * vs + rho * (-bx * by * du * ia);
   if (SD == 3) dU[3] = D * ws + rho * (-bx * bz * du * ia);
-  dU[IE] = (du * (E + ptf) + (psd - ptd) * us + bx * (vb - vbs)) / dm;
+  dU[IE] = qdiv(du * (E + ptf) + (psd - ptd) * us + bx * (vb - vbs), dm);
 #pragma unroll
   for (int iv = IA; iv < NV; iv++) dU[iv] = D * q[iv];
   dUb[0] = 0.0f;
