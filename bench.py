@@ -60,8 +60,9 @@ def kernel_bytes(cfg, name: str) -> float:
     if name == "update":
         # compulsory traffic per launch, averaged over the two RK2 stages:
         # face fluxes (FP32 in the mixed path) and face B in; stage 1 reads
-        # prim + eint + b_cc to rebuild state0 (the mixed path: U) and writes
-        # U0, stage 2 reads U0 + U; both write U and the cell EMFs; the FP64
+        # prim + eint + b_cc to rebuild state0 and writes it to U0 (the mixed
+        # path reads U and keeps it as state0), stage 2 reads state0 and the
+        # stage state; both write the new state and the cell EMFs; the FP64
         # prim / eint / b_cc (+ gammas) are written at every stage (mixed:
         # at stage 2 only, plus the FP32 sweep inputs at both); Helmholtz
         # reads and writes T at both
@@ -72,7 +73,7 @@ def kernel_bytes(cfg, name: str) -> float:
         rd = sum(nf[:sd]) * (4 if mix else 8) * nv + (sum(nf[:sd]) * 8 if mhd else 0)
         s1_rd = nv if mix else st
         s2_rd = 2 * nv
-        s1_wr = 2 * nv + ec + (0 if mix else st + gam)
+        s1_wr = (nv if mix else 2 * nv) + ec + (0 if mix else st + gam)
         s2_wr = nv + ec + st + gam
         words = 0.5 * (s1_rd + s2_rd + s1_wr + s2_wr) + (2 if helm else 0)
         pf = 4 * (nv + 5) if mix else 0

@@ -62,11 +62,24 @@ struct HelmBasis {
 };
 
 // cell index int((log10(rhoYe) - log10(rho_1)) / dlog) (source/source.F90
-// :23773-23784), kept inside the table.
+// :23773-23784), kept inside the table.  FAST (the mixed-precision EoS step
+// only): the index from the FP32 base-2 logarithm (v_log_f32) instead of two
+// FP64 log10; within ~1e-5 of a node it may pick the neighbouring cell, whose
+// quintic Hermite interpolant matches value, slope and curvature there, and
+// the offsets x, y are still formed in FP64 from the table's nodes.
+template <bool FAST = false>
 __device__ __forceinline__ HelmCell helm_locate(const HelmDev &h, double rhos, double T) {
   HelmCell c;
-  c.ih = (int)((log10(rhos) - h.lrho1) / h.dr);
-  c.jh = (int)((log10(T) - h.lT1) / h.dt);
+  if (FAST) {
+    constexpr float kLog10of2 = 0.30102999566398120f;
+    const float lr = __builtin_amdgcn_logf((float)rhos) * kLog10of2;
+    const float lt = __builtin_amdgcn_logf((float)T) * kLog10of2;
+    c.ih = (int)((lr - (float)h.lrho1) * __builtin_amdgcn_rcpf((float)h.dr));
+    c.jh = (int)((lt - (float)h.lT1) * __builtin_amdgcn_rcpf((float)h.dt));
+  } else {
+    c.ih = (int)((log10(rhos) - h.lrho1) / h.dr);
+    c.jh = (int)((log10(T) - h.lT1) / h.dt);
+  }
   c.ih = c.ih < 0 ? 0 : (c.ih > h.nr - 2 ? h.nr - 2 : c.ih);
   c.jh = c.jh < 0 ? 0 : (c.jh > h.nt - 2 ? h.nt - 2 : c.jh);
   double tmp = h.rho[c.ih];
@@ -212,7 +225,7 @@ __device__ __forceinline__ void helm_full(const HelmDev &h, double rho, double T
                                           double zbar, double &P, double &E, double &sound,
                                           double &cv, double *dPdT_out = nullptr) {
   const double Ye = h.pig ? 1.0 : zbar / abar, rhos = rho * Ye;
-  const HelmCell c = helm_locate(h, rhos, T);
+  const HelmCell c = helm_locate<FMA>(h, rhos, T);
   HelmBasis r, dr, t, dt, ddt;
   helm_basis(c.x, c.drho, r);
   helm_dbasis(c.x, c.drho, dr);

@@ -887,6 +887,12 @@ __global__ void __launch_bounds__(256)
   const int k = blockIdx.z;
   const bool in = (i < g.n1) && (j < g.n2);
   const bool keep = !MIX || fin;
+  // The FP64 path saves state0 in U0 and carries the stage state in U.  The
+  // mixed path keeps the step's start state in U (no copy at stage 1) and
+  // the stage state in U0 until the last stage writes U.
+  double *const s0_buf = MIX ? g.U : g.U0;
+  double *const s_buf = MIX ? g.U0 : g.U;
+  double *const out = (MIX && !fin) ? g.U0 : g.U;
   double smax_local = 0.0;
   if (in) {
     const long long N = g.N, c = cix(g, i, j, k);
@@ -901,7 +907,6 @@ __global__ void __launch_bounds__(256)
       for (int v = 0; v < NV; v++) {
         S0[v] = g.U[v * N + c];
         S[v] = S0[v];
-        g.U0[v * N + c] = S0[v];
       }
     } else if (first) {
       // source/source.F90:7057-7117
@@ -922,13 +927,13 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
       for (int v = 0; v < NV; v++) {
         S[v] = S0[v];
-        g.U0[v * N + c] = S0[v];
+        s0_buf[v * N + c] = S0[v];
       }
     } else {
 #pragma unroll
       for (int v = 0; v < NV; v++) {
-        S0[v] = g.U0[v * N + c];
-        S[v] = g.U[v * N + c];
+        S0[v] = s0_buf[v * N + c];
+        S[v] = s_buf[v * N + c];
       }
     }
     // flux divergence: x1 overwrites, x2/x3 accumulate (source/source.F90
@@ -997,7 +1002,7 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
     for (int v = 0; v < NV; v++) {
       if (keep && (!HELM || v != IE)) g.P[v * N + c] = p[v];
-      g.U[v * N + c] = S[v];
+      out[v * N + c] = S[v];
     }
     if (keep) g.eint[c] = eint;
     if constexpr (MIX) {
