@@ -271,6 +271,15 @@ constexpr int kSegFaces = PHLX_SEG_FACES;
 #ifndef PHLX_FLUX_SEG
 #define PHLX_FLUX_SEG 1
 #endif
+// the mixed-precision sweeps' own choice (their faces cost fewer
+// instructions, which shifts the balance towards more, shorter threads)
+#ifndef PHLX_FLUX_SEG_MX
+#define PHLX_FLUX_SEG_MX PHLX_FLUX_SEG
+#endif
+#ifndef PHLX_SEG_FACES_MX
+#define PHLX_SEG_FACES_MX PHLX_SEG_FACES
+#endif
+constexpr int kSegFacesMx = PHLX_SEG_FACES_MX;
 
 // SF: USE_SHOCK_FLATTENING -- a flagged cell takes the LIM2ND edges, a face
 // beside one the HLL flux (as k_flux<..., SF>).
@@ -545,8 +554,8 @@ __global__ void __launch_bounds__(256) k_flux_seg_mx(Grid g) {
   const int o = blockIdx.y * blockDim.y + threadIdx.y;
   const int n = (DIR == 1) ? g.n2 : g.n3;
   if (i >= g.n1 || o >= ((DIR == 1) ? g.n3 : g.n2)) return;
-  const int f0 = blockIdx.z * kSegFaces;
-  const int f1 = min(f0 + kSegFaces, n + 1);
+  const int f0 = blockIdx.z * kSegFacesMx;
+  const int f1 = min(f0 + kSegFacesMx, n + 1);
   if (__all(f0 - 2 >= 0 && f1 <= n - 1))
     seg_walk_mx<DIR, SD, NAS, MHD, HELM, true>(g, i, o, f0, f1);
   else
@@ -870,9 +879,13 @@ __device__ __forceinline__ double flux_at(const Grid &g, int d, long long idx) {
 #ifndef PHLX_HELM_UPDATE_WAVES
 #define PHLX_HELM_UPDATE_WAVES 1
 #endif
+// The GUESS variants hold the surrogate's 16 accumulation registers beside
+// the EoS state; they are held to 2 waves per SIMD (VGPRs + AGPRs <= 256),
+// which the unconstrained allocation can miss by a few registers and then
+// halve the occupancy.
 template <int SD, int NAS, bool MHD, bool HELM, bool MIX, bool GUESS = false>
 __global__ void __launch_bounds__(256)
-    __attribute__((amdgpu_waves_per_eu(HELM ? PHLX_HELM_UPDATE_WAVES : 1)))
+    __attribute__((amdgpu_waves_per_eu(HELM ? (GUESS ? 2 : PHLX_HELM_UPDATE_WAVES) : 1)))
     k_update(Grid g, int first, int last, double a1,
                                                 double a2, double c3, int fin) {
   // first: 1 = stage 1, state0 rebuilt from prim + eint (the reference);
@@ -1393,9 +1406,9 @@ static void launch_flux(const Grid &g, int dir, hipStream_t st) {
     static_assert(!MIX || EK != EK_HELM_FACE, "mixed precision needs FASTEOS");
     if (dir == 0) {
       k_flux_mx<0, SD, NAS, MHD, HELM, R5><<<grid_x1_faces(g), blk, 0, st>>>(g);
-    } else if (!R5 && PHLX_FLUX_SEG) {
+    } else if (!R5 && PHLX_FLUX_SEG_MX) {
       const int n = dir == 1 ? g.n2 : g.n3, no = dir == 1 ? g.n3 : g.n2;
-      const dim3 grd((g.n1 + 63) / 64, (no + 3) / 4, (n + 1 + kSegFaces - 1) / kSegFaces);
+      const dim3 grd((g.n1 + 63) / 64, (no + 3) / 4, (n + 1 + kSegFacesMx - 1) / kSegFacesMx);
       if (dir == 1)
         k_flux_seg_mx<1, SD, NAS, MHD, HELM><<<grd, blk, 0, st>>>(g);
       else if (SD == 3)
