@@ -858,6 +858,36 @@ __device__ __forceinline__ double cell_speed(const Grid &g, const double *p, con
   return s1 + s2;
 }
 
+// Block order of a cell kernel launched on a flat grid of nbx * nby * nk
+// blocks (x1 tiles of blockDim.x, x2 tiles of blockDim.y, one x3 plane
+// each).  PHLX_XCD_KC = 0: the order of the 3D grid (x1 tiles fastest, then
+// x2 tiles, then planes).  PHLX_XCD_KC = Kc > 0 (nk % Kc == 0, B % 8 == 0):
+// XCD-aware.  Block ids are dealt round-robin to the 8 XCDs (id % 8), each
+// with its own L2; XCD x takes the contiguous range of items [x B/8,
+// (x+1) B/8), ordered (chunk of Kc planes, tile, plane in chunk), so the x3
+// face plane a block reads as its upper face is read next, as the lower face
+// of the plane above, by a block on the same XCD.  Bijective:
+// id -> (id % 8) B/8 + id / 8, then a mixed-radix decode.
+#ifndef PHLX_XCD_KC
+#define PHLX_XCD_KC 0
+#endif
+__device__ __forceinline__ void xcd_cell_block(int nbx, int nby, int nk, int &bx, int &by,
+                                               int &bz) {
+  const unsigned T = (unsigned)nbx * (unsigned)nby, B = T * (unsigned)nk, id = blockIdx.x;
+  constexpr unsigned KC = PHLX_XCD_KC;
+  if (KC == 0 || (B & 7u) || ((unsigned)nk % (KC ? KC : 1u))) {
+    bx = (int)(id % (unsigned)nbx);
+    by = (int)((id / (unsigned)nbx) % (unsigned)nby);
+    bz = (int)(id / T);
+    return;
+  }
+  const unsigned w = (id & 7u) * (B >> 3) + (id >> 3);
+  const unsigned kk = w % KC, rest = w / KC, t = rest % T;
+  bz = (int)((rest / T) * KC + kk);
+  bx = (int)(t % (unsigned)nbx);
+  by = (int)(t / (unsigned)nbx);
+}
+
 // ---------------------------------------------------------------- update
 // One thread per cell: state0 rebuild (first stage), flux divergence,
 // gravity / heating sources, RK combine, b_cc from the updated faces,
@@ -895,9 +925,11 @@ __global__ void __launch_bounds__(256)
   // fin (between its stages the sweeps read the FP32 copy g.Pf).
   constexpr int NV = SD + 2 + NAS, IE = SD + 1, IA = SD + 2;
   if (!g.sc->active) return;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y * blockDim.y + threadIdx.y;
-  const int k = blockIdx.z;
+  int bx, by, k;
+  xcd_cell_block((g.n1 + blockDim.x - 1) / blockDim.x, (g.n2 + blockDim.y - 1) / blockDim.y, g.n3,
+                 bx, by, k);
+  const int i = bx * blockDim.x + threadIdx.x;
+  const int j = by * blockDim.y + threadIdx.y;
   const bool in = (i < g.n1) && (j < g.n2);
   const bool keep = !MIX || fin;
   // The FP64 path saves state0 in U0 and carries the stage state in U.  The
@@ -1397,6 +1429,12 @@ static dim3 grid_plane(int e1, int e2, int e3) {
   return dim3((unsigned)((nplane + 255) / 256), 1, e3);
 }
 static dim3 grid_x1_faces(const Grid &g) { return grid_plane(g.n1 + 1, g.n2, g.n3); }
+// flat grid of the blocks of grid_for, for kernels that place their blocks
+// with xcd_cell_block
+static dim3 grid_flat(int e1, int e2, int e3, dim3 blk) {
+  const dim3 gr = grid_for(e1, e2, e3, blk);
+  return dim3(gr.x * gr.y * gr.z, 1, 1);
+}
 
 template <int SD, int NAS, bool MHD, int EK, bool MIX, bool R5>
 static void launch_flux(const Grid &g, int dir, hipStream_t st) {
@@ -1492,13 +1530,13 @@ struct Kern {
   static void update(const Grid &g, int first, int last, double a1, double a2, double c3,
                      int fin, hipStream_t st) {
     const dim3 blk(64, 4, 1);
-    k_update<SD, NAS, MHD, HELM, MIX><<<grid_for(g.n1, g.n2, g.n3, blk), blk, 0, st>>>(
+    k_update<SD, NAS, MHD, HELM, MIX><<<grid_flat(g.n1, g.n2, g.n3, blk), blk, 0, st>>>(
         g, first, last, a1, a2, c3, fin);
   }
   static void update_guess(const Grid &g, int first, int last, double a1, double a2, double c3,
                            int fin, hipStream_t st) {
     const dim3 blk(64, 4, 1);
-    k_update<SD, NAS, MHD, HELM, MIX, true><<<grid_for(g.n1, g.n2, g.n3, blk), blk, 0, st>>>(
+    k_update<SD, NAS, MHD, HELM, MIX, true><<<grid_flat(g.n1, g.n2, g.n3, blk), blk, 0, st>>>(
         g, first, last, a1, a2, c3, fin);
   }
   static void cfl(const Grid &g, hipStream_t st) {
