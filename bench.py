@@ -53,8 +53,11 @@ def kernel_bytes(cfg, name: str) -> float:
         d = int(name[-1]) - 1
         per_face_out = 8 * nv + (8 * 2 + 1 if mhd else 0)  # fluxes (+2 EMFs + sign)
         return N * cell_in + nf[d] * ((8 if mhd else 0) + per_face_out)
-    if name == "corner":   # 2 EMFs + sign per face in, v & b_cc per cell, 3 edge EMFs out
-        return sum(nf[:sd]) * 17 + N * 8 * (2 * sd) + 3 * N * 8
+    if name == "corner":
+        # 2 EMFs + sign per face and the 3 cell EMFs in (FP32 in the mixed
+        # path), 3 edge EMFs out
+        w = 4 if cfg.precision == 1 else 8
+        return sum(nf[:sd]) * (2 * w + 1) + N * w * 3 + 3 * N * 8
     if name == "faceb":    # b, b0, 4 edge EMFs in; b (+b0 at stage 1) out
         return sum(nf[:sd]) * 8 * 3 + 3 * N * 8
     if name == "update":

@@ -228,10 +228,13 @@ extern "C" int phlx_create(const phlx_config *cfg, phlx_ctx **out) {
   if (g.mhd) {
     e |= dalloc(ctx, &g.bcc, 3 * N);
     e |= dalloc(ctx, &g.ecell, 3 * N);
+    // the mixed path stores the cell and face EMFs as FP32 in the same buffers
+    g.ecellf = cfg->precision == PHLX_PREC_MIXED ? (float *)g.ecell : nullptr;
     for (int d = 0; d < g.sd; d++) {
       e |= dalloc(ctx, &g.bf[d], (size_t)g.nf[d]);
       e |= dalloc(ctx, &g.b0f[d], (size_t)g.nf[d]);
       e |= dalloc(ctx, &g.emf[d], 2 * (size_t)g.nf[d]);
+      g.emff[d] = (float *)g.emf[d];
       e |= dalloc(ctx, &g.sgn[d], (size_t)g.nf[d]);
     }
     const size_t n1 = g.n1, n2 = g.n2, n3 = g.n3;

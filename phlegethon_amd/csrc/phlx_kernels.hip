@@ -442,12 +442,12 @@ __device__ __forceinline__ void flux_face_mx(const Grid &g, int i, int j, int k,
   }
   const long long nf = g.nf[DIR];
   if (MHD) {
-    double e0, e1v;
+    float e0, e1v;
     if (DIR == 0) { e0 = Fb[2]; e1v = -Fb[1]; }
     else if (DIR == 1) { e0 = -Fb[2]; e1v = Fb[0]; }
     else { e0 = Fb[1]; e1v = -Fb[0]; }
-    g.emf[DIR][f] = e0;
-    g.emf[DIR][nf + f] = e1v;
+    g.emff[DIR][f] = e0;     // FP32 like the induction fluxes they are
+    g.emff[DIR][nf + f] = e1v;
     const double den = 0.5 * ((double)qL[0] + (double)qR[0]) * g.sc->smax;
     const double mach = (double)F[0] / den;
     signed char sgn = 0;
@@ -606,8 +606,9 @@ __global__ void __launch_bounds__(256) k_flux_mx(Grid g) {
 // Face-centred EMF component `comp` (0..2 = E1..E3) of the face array of
 // direction fd at a possibly-ghost transverse position; ghosts follow the
 // EMF halo copy and EMF boundary conditions (source/source.F90:11370-11964).
-// FAST: the caller guarantees an interior position (no map).
-template <bool FAST = false>
+// FAST: the caller guarantees an interior position (no map).  F32: the
+// mixed path's FP32 face EMFs (emff).
+template <bool FAST = false, bool F32 = false>
 __device__ __forceinline__ void face_emf(const Grid &g, int fd, int comp, int i, int j,
                                          int k, double &e, double &s) {
   int ii[3] = {i, j, k};
@@ -633,7 +634,7 @@ __device__ __forceinline__ void face_emf(const Grid &g, int fd, int comp, int i,
   if (fd == 0) slot = (comp == 1) ? 0 : 1;
   else if (fd == 1) slot = (comp == 0) ? 0 : 1;
   else slot = (comp == 0) ? 0 : 1;
-  e = fac * g.emf[fd][slot * g.nf[fd] + f];
+  e = fac * (F32 ? (double)g.emff[fd][slot * g.nf[fd] + f] : g.emf[fd][slot * g.nf[fd] + f]);
   s = (double)g.sgn[fd][f];
 }
 
@@ -652,15 +653,18 @@ __device__ __forceinline__ double emf_of(int comp, const double *v, const double
 // mirrored axis a, which negates every component but E_a (each product
 // holds exactly one axis-a factor), and IEEE negation commutes with
 // rounding, so this equals the value formed from the flipped ghost state
-// bit for bit.
-template <int SD, bool FAST = false>
+// bit for bit.  F32: the mixed path's FP32 cell EMFs (ecellf).
+template <int SD, bool FAST = false, bool F32 = false>
 __device__ __forceinline__ double cell_emf(const Grid &g, int comp, int i, int j, int k) {
-  if (FAST) return g.ecell[(long long)comp * g.N + cix(g, i, j, k)];
+  if (FAST)
+    return F32 ? (double)g.ecellf[(long long)comp * g.N + cix(g, i, j, k)]
+               : g.ecell[(long long)comp * g.N + cix(g, i, j, k)];
   int si, sj, sk = 0;
   const int mi = map_axis(g, 0, i, g.n1, si);
   const int mj = map_axis(g, 1, j, g.n2, sj);
   const int mk = (SD == 3) ? map_axis(g, 2, k, g.n3, sk) : k;
-  double e = g.ecell[(long long)comp * g.N + cix(g, mi, mj, mk)];
+  const long long ce = (long long)comp * g.N + cix(g, mi, mj, mk);
+  double e = F32 ? (double)g.ecellf[ce] : g.ecell[ce];
   if (comp != 0 && si && flips(g, 0, si, cix(g, si < 0 ? 0 : g.n1 - 1, mj, mk))) e = -e;
   if (comp != 1 && sj && flips(g, 1, sj, cix(g, mi, sj < 0 ? 0 : g.n2 - 1, mk))) e = -e;
   if (SD == 3 && comp != 2 && sk && flips(g, 2, sk, cix(g, mi, mj, sk < 0 ? 0 : g.n3 - 1)))
@@ -681,42 +685,42 @@ __device__ __forceinline__ double corner_emf(double eS, double eW, double eN, do
 }
 
 // The three edge EMFs whose lower end is node (i, j, k) (FAST: all eight
-// surrounding cells interior).
-template <int SD, bool FAST>
+// surrounding cells interior; F32: FP32 face and cell EMFs, FP64 arithmetic).
+template <int SD, bool FAST, bool F32>
 __device__ __forceinline__ void corner_node(const Grid &g, int i, int j, int k) {
   double eN, eE, eS, eW, sS, sN, sW, sE;
   if (k < g.n3) {  // x3-edges (emfx3_cor)
-    face_emf<FAST>(g, 0, 2, i, j, k, eN, sN);
-    face_emf<FAST>(g, 1, 2, i, j, k, eE, sE);
-    face_emf<FAST>(g, 0, 2, i, j - 1, k, eS, sS);
-    face_emf<FAST>(g, 1, 2, i - 1, j, k, eW, sW);
-    const double eSW = cell_emf<SD, FAST>(g, 2, i - 1, j - 1, k);
-    const double eNW = cell_emf<SD, FAST>(g, 2, i - 1, j, k);
-    const double eNE = cell_emf<SD, FAST>(g, 2, i, j, k);
-    const double eSE = cell_emf<SD, FAST>(g, 2, i, j - 1, k);
+    face_emf<FAST, F32>(g, 0, 2, i, j, k, eN, sN);
+    face_emf<FAST, F32>(g, 1, 2, i, j, k, eE, sE);
+    face_emf<FAST, F32>(g, 0, 2, i, j - 1, k, eS, sS);
+    face_emf<FAST, F32>(g, 1, 2, i - 1, j, k, eW, sW);
+    const double eSW = cell_emf<SD, FAST, F32>(g, 2, i - 1, j - 1, k);
+    const double eNW = cell_emf<SD, FAST, F32>(g, 2, i - 1, j, k);
+    const double eNE = cell_emf<SD, FAST, F32>(g, 2, i, j, k);
+    const double eSE = cell_emf<SD, FAST, F32>(g, 2, i, j - 1, k);
     g.cor[2][kix(g, 2, i, j, k)] = corner_emf(eS, eW, eN, eE, eSW, eNW, eNE, eSE, sS, sN, sW, sE);
   }
   if (SD == 3) {
     if (i < g.n1) {  // x1-edges (emfx1_cor, source/source.F90:12050-12107)
-      face_emf<FAST>(g, 1, 0, i, j, k, eN, sN);
-      face_emf<FAST>(g, 2, 0, i, j, k, eE, sE);
-      face_emf<FAST>(g, 1, 0, i, j, k - 1, eS, sS);
-      face_emf<FAST>(g, 2, 0, i, j - 1, k, eW, sW);
-      const double eSW = cell_emf<SD, FAST>(g, 0, i, j - 1, k - 1);
-      const double eNW = cell_emf<SD, FAST>(g, 0, i, j - 1, k);
-      const double eNE = cell_emf<SD, FAST>(g, 0, i, j, k);
-      const double eSE = cell_emf<SD, FAST>(g, 0, i, j, k - 1);
+      face_emf<FAST, F32>(g, 1, 0, i, j, k, eN, sN);
+      face_emf<FAST, F32>(g, 2, 0, i, j, k, eE, sE);
+      face_emf<FAST, F32>(g, 1, 0, i, j, k - 1, eS, sS);
+      face_emf<FAST, F32>(g, 2, 0, i, j - 1, k, eW, sW);
+      const double eSW = cell_emf<SD, FAST, F32>(g, 0, i, j - 1, k - 1);
+      const double eNW = cell_emf<SD, FAST, F32>(g, 0, i, j - 1, k);
+      const double eNE = cell_emf<SD, FAST, F32>(g, 0, i, j, k);
+      const double eSE = cell_emf<SD, FAST, F32>(g, 0, i, j, k - 1);
       g.cor[0][kix(g, 0, i, j, k)] = corner_emf(eS, eW, eN, eE, eSW, eNW, eNE, eSE, sS, sN, sW, sE);
     }
     if (j < g.n2) {  // x2-edges (emfx2_cor, source/source.F90:12128-12185)
-      face_emf<FAST>(g, 2, 1, i, j, k, eN, sN);
-      face_emf<FAST>(g, 0, 1, i, j, k, eE, sE);
-      face_emf<FAST>(g, 2, 1, i - 1, j, k, eS, sS);
-      face_emf<FAST>(g, 0, 1, i, j, k - 1, eW, sW);
-      const double eSW = cell_emf<SD, FAST>(g, 1, i - 1, j, k - 1);
-      const double eNW = cell_emf<SD, FAST>(g, 1, i, j, k - 1);
-      const double eNE = cell_emf<SD, FAST>(g, 1, i, j, k);
-      const double eSE = cell_emf<SD, FAST>(g, 1, i - 1, j, k);
+      face_emf<FAST, F32>(g, 2, 1, i, j, k, eN, sN);
+      face_emf<FAST, F32>(g, 0, 1, i, j, k, eE, sE);
+      face_emf<FAST, F32>(g, 2, 1, i - 1, j, k, eS, sS);
+      face_emf<FAST, F32>(g, 0, 1, i, j, k - 1, eW, sW);
+      const double eSW = cell_emf<SD, FAST, F32>(g, 1, i - 1, j, k - 1);
+      const double eNW = cell_emf<SD, FAST, F32>(g, 1, i, j, k - 1);
+      const double eNE = cell_emf<SD, FAST, F32>(g, 1, i, j, k);
+      const double eSE = cell_emf<SD, FAST, F32>(g, 1, i - 1, j, k);
       g.cor[1][kix(g, 1, i, j, k)] = corner_emf(eS, eW, eN, eE, eSW, eNW, eNE, eSE, sS, sN, sW, sE);
     }
   }
@@ -725,7 +729,7 @@ __device__ __forceinline__ void corner_node(const Grid &g, int i, int j, int k) 
 // One thread per grid node (i,j,k) in [0,n1]x[0,n2]x[0,n3]: the three edge
 // EMFs whose lower end is that node.  A wave whose nodes all have their
 // eight surrounding cells inside the grid reads without the boundary map.
-template <int SD>
+template <int SD, bool F32>
 __global__ void __launch_bounds__(256) k_corner(Grid g) {
   if (!g.sc->active) return;
   int i, j;
@@ -734,9 +738,9 @@ __global__ void __launch_bounds__(256) k_corner(Grid g) {
   const bool inside = i >= 1 && i < g.n1 && j >= 1 && j < g.n2 &&
                       (SD == 2 || (k >= 1 && k < g.n3));
   if (__all(inside))
-    corner_node<SD, true>(g, i, j, k);
+    corner_node<SD, true, F32>(g, i, j, k);
   else
-    corner_node<SD, false>(g, i, j, k);
+    corner_node<SD, false, F32>(g, i, j, k);
 }
 
 // Face-B update b = a1*b0 + a2*b + a3*dt*curl(E) in FP64 (source/source.F90
@@ -1064,10 +1068,18 @@ __global__ void __launch_bounds__(256)
     }
     if (MHD) {
       const double vv[3] = {p[1], p[2], SD == 3 ? p[3] : 0.0};
-      g.ecell[2 * N + c] = emf_of<SD>(2, vv, b);
-      if (SD == 3) {
-        g.ecell[c] = emf_of<SD>(0, vv, b);
-        g.ecell[N + c] = emf_of<SD>(1, vv, b);
+      if constexpr (MIX) {
+        g.ecellf[2 * N + c] = (float)emf_of<SD>(2, vv, b);
+        if (SD == 3) {
+          g.ecellf[c] = (float)emf_of<SD>(0, vv, b);
+          g.ecellf[N + c] = (float)emf_of<SD>(1, vv, b);
+        }
+      } else {
+        g.ecell[2 * N + c] = emf_of<SD>(2, vv, b);
+        if (SD == 3) {
+          g.ecell[c] = emf_of<SD>(0, vv, b);
+          g.ecell[N + c] = emf_of<SD>(1, vv, b);
+        }
       }
     }
     double T = 0.0, sound_out = 0.0, ye_c = 1.0, abar_c = 1.0;
@@ -1264,10 +1276,18 @@ __global__ void __launch_bounds__(256) k_init(Grid g, int restart) {
     g.bcc[N + c] = b[1];
     if (SD == 3) g.bcc[2 * N + c] = b[2];
     const double vv[3] = {g.P[N + c], g.P[2 * N + c], SD == 3 ? g.P[3 * N + c] : 0.0};
-    g.ecell[2 * N + c] = emf_of<SD>(2, vv, b);
-    if (SD == 3) {
-      g.ecell[c] = emf_of<SD>(0, vv, b);
-      g.ecell[N + c] = emf_of<SD>(1, vv, b);
+    if (g.ecellf) {   // mixed path: FP32 cell EMFs
+      g.ecellf[2 * N + c] = (float)emf_of<SD>(2, vv, b);
+      if (SD == 3) {
+        g.ecellf[c] = (float)emf_of<SD>(0, vv, b);
+        g.ecellf[N + c] = (float)emf_of<SD>(1, vv, b);
+      }
+    } else {
+      g.ecell[2 * N + c] = emf_of<SD>(2, vv, b);
+      if (SD == 3) {
+        g.ecell[c] = emf_of<SD>(0, vv, b);
+        g.ecell[N + c] = emf_of<SD>(1, vv, b);
+      }
     }
   }
 }
@@ -1520,7 +1540,7 @@ struct Kern {
   }
   static void corner(const Grid &g, hipStream_t st) {
     const dim3 blk(64, 4, 1);
-    k_corner<SD><<<grid_plane(g.n1 + 1, g.n2 + 1, SD == 3 ? g.n3 + 1 : 1), blk, 0, st>>>(g);
+    k_corner<SD, MIX><<<grid_plane(g.n1 + 1, g.n2 + 1, SD == 3 ? g.n3 + 1 : 1), blk, 0, st>>>(g);
   }
   static void faceb(const Grid &g, int first, double a1, double a2, double c3, hipStream_t st) {
     const dim3 blk(64, 4, 1);
