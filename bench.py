@@ -57,9 +57,10 @@ def kernel_bytes(cfg, name: str) -> float:
         # 2 EMFs + sign per face and the 3 cell EMFs in (FP32 in the mixed
         # path), 3 edge EMFs out
         w = 4 if cfg.precision == 1 else 8
-        return sum(nf[:sd]) * (2 * w + 1) + N * w * 3 + 3 * N * 8
+        return sum(nf[:sd]) * (2 * w + 1) + N * w * 3 + 3 * N * w
     if name == "faceb":    # b, b0, 4 edge EMFs in; b (+b0 at stage 1) out
-        return sum(nf[:sd]) * 8 * 3 + 3 * N * 8
+        w = 4 if cfg.precision == 1 else 8
+        return sum(nf[:sd]) * 8 * 3 + 3 * N * w
     if name == "update":
         # compulsory traffic per launch, averaged over the two RK2 stages:
         # face fluxes (FP32 in the mixed path) and face B in; stage 1 reads

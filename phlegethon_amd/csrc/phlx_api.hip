@@ -243,6 +243,7 @@ extern "C" int phlx_create(const phlx_config *cfg, phlx_ctx **out) {
       e |= dalloc(ctx, &g.cor[0], n1 * (n2 + 1) * (n3 + 1));
       e |= dalloc(ctx, &g.cor[1], (n1 + 1) * n2 * (n3 + 1));
     }
+    for (int d = 0; d < 3; d++) g.corf[d] = (float *)g.cor[d];   // mixed: FP32 edge EMFs
   }
   if (helm) {
     e |= dalloc(ctx, &g.temp, N);

@@ -56,6 +56,7 @@ struct Grid {
   float *emff[3];          // FP32 view of emf[d] (mixed precision: face EMFs stored as FP32)
   signed char *sgn[3];     // contact-upwind sign per face
   double *cor[3];          // corner (edge) EMFs along x1, x2, x3
+  float *corf[3];          // FP32 view of cor[d] (mixed precision)
   double *ecell;           // cell EMF -(v x B), [comp][cell], written by update / init
   float *ecellf;           // FP32 view of ecell in the mixed path (null in FP64 runs)
   const double *eq_rho_cc, *eq_p_cc, *eq_rho_f, *eq_p_f, *gravy, *phi_cc, *phi_f,

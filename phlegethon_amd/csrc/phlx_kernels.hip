@@ -698,7 +698,11 @@ __device__ __forceinline__ void corner_node(const Grid &g, int i, int j, int k) 
     const double eNW = cell_emf<SD, FAST, F32>(g, 2, i - 1, j, k);
     const double eNE = cell_emf<SD, FAST, F32>(g, 2, i, j, k);
     const double eSE = cell_emf<SD, FAST, F32>(g, 2, i, j - 1, k);
-    g.cor[2][kix(g, 2, i, j, k)] = corner_emf(eS, eW, eN, eE, eSW, eNW, eNE, eSE, sS, sN, sW, sE);
+    {
+      const double ec = corner_emf(eS, eW, eN, eE, eSW, eNW, eNE, eSE, sS, sN, sW, sE);
+      if (F32) g.corf[2][kix(g, 2, i, j, k)] = (float)ec;
+      else g.cor[2][kix(g, 2, i, j, k)] = ec;
+    }
   }
   if (SD == 3) {
     if (i < g.n1) {  // x1-edges (emfx1_cor, source/source.F90:12050-12107)
@@ -710,7 +714,11 @@ __device__ __forceinline__ void corner_node(const Grid &g, int i, int j, int k) 
       const double eNW = cell_emf<SD, FAST, F32>(g, 0, i, j - 1, k);
       const double eNE = cell_emf<SD, FAST, F32>(g, 0, i, j, k);
       const double eSE = cell_emf<SD, FAST, F32>(g, 0, i, j, k - 1);
-      g.cor[0][kix(g, 0, i, j, k)] = corner_emf(eS, eW, eN, eE, eSW, eNW, eNE, eSE, sS, sN, sW, sE);
+      {
+        const double ec = corner_emf(eS, eW, eN, eE, eSW, eNW, eNE, eSE, sS, sN, sW, sE);
+        if (F32) g.corf[0][kix(g, 0, i, j, k)] = (float)ec;
+        else g.cor[0][kix(g, 0, i, j, k)] = ec;
+      }
     }
     if (j < g.n2) {  // x2-edges (emfx2_cor, source/source.F90:12128-12185)
       face_emf<FAST, F32>(g, 2, 1, i, j, k, eN, sN);
@@ -721,7 +729,11 @@ __device__ __forceinline__ void corner_node(const Grid &g, int i, int j, int k) 
       const double eNW = cell_emf<SD, FAST, F32>(g, 1, i, j, k - 1);
       const double eNE = cell_emf<SD, FAST, F32>(g, 1, i, j, k);
       const double eSE = cell_emf<SD, FAST, F32>(g, 1, i - 1, j, k);
-      g.cor[1][kix(g, 1, i, j, k)] = corner_emf(eS, eW, eN, eE, eSW, eNW, eNE, eSE, sS, sN, sW, sE);
+      {
+        const double ec = corner_emf(eS, eW, eN, eE, eSW, eNW, eNE, eSE, sS, sN, sW, sE);
+        if (F32) g.corf[1][kix(g, 1, i, j, k)] = (float)ec;
+        else g.cor[1][kix(g, 1, i, j, k)] = ec;
+      }
     }
   }
 }
@@ -745,7 +757,14 @@ __global__ void __launch_bounds__(256) k_corner(Grid g) {
 
 // Face-B update b = a1*b0 + a2*b + a3*dt*curl(E) in FP64 (source/source.F90
 // :12208-12435); at the first stage the step-start field is kept in b0.
-template <int SD>
+// F32: the mixed path's FP32 edge EMFs (corf), the update itself in FP64.
+template <bool F32>
+struct EdgeEmf {
+  const double *d;
+  const float *f;
+  __device__ double operator[](long long x) const { return F32 ? (double)f[x] : d[x]; }
+};
+template <int SD, bool F32>
 __global__ void __launch_bounds__(256) k_faceb(Grid g, int first, double a1, double a2,
                                                double c3) {
   if (!g.sc->active) return;
@@ -754,7 +773,7 @@ __global__ void __launch_bounds__(256) k_faceb(Grid g, int first, double a1, dou
   if (!plane_coords(g.n1 + 1, g.n2 + 1, i, j)) return;
   const double a3 = c3 * g.sc->dt;
   const double i1 = g.inv_dx[0], i2 = g.inv_dx[1], i3 = g.inv_dx[2];
-  const double *C1 = g.cor[0], *C2 = g.cor[1], *C3 = g.cor[2];
+  const EdgeEmf<F32> C1{g.cor[0], g.corf[0]}, C2{g.cor[1], g.corf[1]}, C3{g.cor[2], g.corf[2]};
   if (j < g.n2 && k < g.n3) {
     double t;
     if (SD == 3)
@@ -1544,7 +1563,7 @@ struct Kern {
   }
   static void faceb(const Grid &g, int first, double a1, double a2, double c3, hipStream_t st) {
     const dim3 blk(64, 4, 1);
-    k_faceb<SD><<<grid_plane(g.n1 + 1, g.n2 + 1, SD == 3 ? g.n3 + 1 : 1), blk, 0, st>>>(
+    k_faceb<SD, MIX><<<grid_plane(g.n1 + 1, g.n2 + 1, SD == 3 ? g.n3 + 1 : 1), blk, 0, st>>>(
         g, first, a1, a2, c3);
   }
   static void update(const Grid &g, int first, int last, double a1, double a2, double c3,
