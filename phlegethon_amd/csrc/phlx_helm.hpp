@@ -55,6 +55,19 @@ struct HelmCell {
 // which holds every table value of the evaluation live at once.
 __device__ __forceinline__ void helm_pin(double &v) { asm volatile("" : "+v"(v)); }
 
+// a / b; FAST (the mixed-precision EoS step only): the hardware reciprocal
+// refined by two Newton steps (~1 ulp) instead of the IEEE-rounded sequence.
+template <bool FAST>
+__device__ __forceinline__ double hdiv(double a, double b) {
+  if (!FAST) return a / b;
+  double r = __builtin_amdgcn_rcp(b);
+  double e = fma(-b, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-b, r, 1.0);
+  r = fma(r, e, r);
+  return a * r;
+}
+
 // Basis values of one coordinate at the cell's low corner (a[0..2]: order
 // 0, 1, 2 of the Hermite family) and high corner (am[0..2]).
 struct HelmBasis {
@@ -84,10 +97,10 @@ __device__ __forceinline__ HelmCell helm_locate(const HelmDev &h, double rhos, d
   c.jh = c.jh < 0 ? 0 : (c.jh > h.nt - 2 ? h.nt - 2 : c.jh);
   double tmp = h.rho[c.ih];
   c.drho = h.rho[c.ih + 1] - tmp;
-  c.x = (rhos - tmp) / c.drho;
+  c.x = hdiv<FAST>(rhos - tmp, c.drho);
   tmp = h.T[c.jh];
   c.dT = h.T[c.jh + 1] - tmp;
-  c.y = (T - tmp) / c.dT;
+  c.y = hdiv<FAST>(T - tmp, c.dT);
   return c;
 }
 
@@ -106,8 +119,9 @@ __device__ __forceinline__ void helm_basis(double x, double d, HelmBasis &v) {
   v.am[2] = (0.5 * (-o5 + 3.0 * o4 - 3.0 * o3 + o2)) * d2;
 }
 
+template <bool FAST = false>
 __device__ __forceinline__ void helm_dbasis(double x, double d, HelmBasis &v) {
-  const double id = 1.0 / d;
+  const double id = hdiv<FAST>(1.0, d);
   const double x2 = x * x, x3 = x2 * x, x4 = x3 * x;
   const double o = 1.0 - x;
   const double o2 = o * o, o3 = o2 * o, o4 = o3 * o;
@@ -119,8 +133,9 @@ __device__ __forceinline__ void helm_dbasis(double x, double d, HelmBasis &v) {
   v.am[2] = -(0.5 * (-5.0 * o4 + 12.0 * o3 - 9.0 * o2 + 2.0 * o)) * d;
 }
 
+template <bool FAST = false>
 __device__ __forceinline__ void helm_ddbasis(double x, double d, HelmBasis &v) {
-  const double id = 1.0 / d, id2 = id * id;
+  const double id = hdiv<FAST>(1.0, d), id2 = id * id;
   const double x2 = x * x, x3 = x2 * x;
   const double o = 1.0 - x;
   const double o2 = o * o, o3 = o2 * o;
@@ -224,14 +239,14 @@ template <bool FMA = false>
 __device__ __forceinline__ void helm_full(const HelmDev &h, double rho, double T, double abar,
                                           double zbar, double &P, double &E, double &sound,
                                           double &cv, double *dPdT_out = nullptr) {
-  const double Ye = h.pig ? 1.0 : zbar / abar, rhos = rho * Ye;
+  const double Ye = h.pig ? 1.0 : hdiv<FMA>(zbar, abar), rhos = rho * Ye;
   const HelmCell c = helm_locate<FMA>(h, rhos, T);
   HelmBasis r, dr, t, dt, ddt;
   helm_basis(c.x, c.drho, r);
-  helm_dbasis(c.x, c.drho, dr);
+  helm_dbasis<FMA>(c.x, c.drho, dr);
   helm_basis(c.y, c.dT, t);
-  helm_dbasis(c.y, c.dT, dt);
-  helm_ddbasis(c.y, c.dT, ddt);
+  helm_dbasis<FMA>(c.y, c.dT, dt);
+  helm_ddbasis<FMA>(c.y, c.dT, ddt);
   double sm[5];
   {
     const HelmBasis *const RR[5] = {&r, &dr, &r, &dr, &r};
@@ -248,7 +263,7 @@ __device__ __forceinline__ void helm_full(const HelmDev &h, double rho, double T
   const double ds_dT = -Ye * df_dT2;
   double e = Ye * f + T * s;
   double dE_dT = T * ds_dT;
-  tmp = h.pig ? 0.0 : 1.0 / abar;
+  tmp = h.pig ? 0.0 : hdiv<FMA>(1.0, abar);
   const double T2 = T * T, T3 = T2 * T, T4 = T3 * T;
   const double pion = kHelmRgas * rho * T * tmp;
   const double dpiondt = kHelmRgas * rho * tmp;
@@ -256,18 +271,18 @@ __device__ __forceinline__ void helm_full(const HelmDev &h, double rho, double T
   p = p + pion + kHelmCrad * T4 * (1.0 / 3.0);
   dP_dT = dP_dT + dpiondt + (4.0 / 3.0) * kHelmCrad * T3;
   dP_drho = dP_drho + dpiondd;
-  e = e + 1.5 * kHelmRgas * T * tmp + kHelmCrad * T4 / rho;
-  dE_dT = dE_dT + 1.5 * kHelmRgas * tmp + 4.0 * kHelmCrad * T3 / rho;
-  const double zz = p / rho;
-  const double chiT = T / p * dP_dT;
-  const double chirho = dP_drho / zz;
-  const double x = zz * chiT / (T * dE_dT);
+  e = e + 1.5 * kHelmRgas * T * tmp + hdiv<FMA>(kHelmCrad * T4, rho);
+  dE_dT = dE_dT + 1.5 * kHelmRgas * tmp + hdiv<FMA>(4.0 * kHelmCrad * T3, rho);
+  const double zz = hdiv<FMA>(p, rho);
+  const double chiT = hdiv<FMA>(T, p) * dP_dT;
+  const double chirho = hdiv<FMA>(dP_drho, zz);
+  const double x = hdiv<FMA>(zz * chiT, T * dE_dT);
   const double gam1 = chiT * x + chirho;
-  const double z = 1.0 + (e + kHelmClight2) / zz;
+  const double z = 1.0 + hdiv<FMA>(e + kHelmClight2, zz);
   P = p;
   E = e;
   cv = dE_dT;
-  sound = kHelmClight * sqrt(gam1 / z);
+  sound = kHelmClight * sqrt(hdiv<FMA>(gam1, z));
   if (dPdT_out) *dPdT_out = dP_dT;
 }
 

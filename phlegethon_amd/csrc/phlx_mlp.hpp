@@ -218,10 +218,10 @@ __device__ __forceinline__ int helm_rhoe_lin(const HelmDev &h, double lin_tol, d
   double E, cv, dPdT;
   helm_full<true>(h, rho, T, abar, zbar, P, E, sound, cv, &dPdT);
   const double res = E - E0;
-  const double dT = -res / cv;
+  const double dT = hdiv<true>(-res, cv);
   const double T1 = T + dT;
   const bool inside = (T1 >= h.tl) && (T1 <= h.th);
-  if (fabs(res / E0) < lin_tol && inside) {
+  if (fabs(hdiv<true>(res, E0)) < lin_tol && inside) {
     T = T1;
     P = P + dPdT * dT;
     fallback = 0;
