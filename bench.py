@@ -68,7 +68,8 @@ def kernel_bytes(cfg, name: str) -> float:
         # path reads U and keeps it as state0), stage 2 reads state0 and the
         # stage state; both write the new state and the cell EMFs; the FP64
         # prim / eint / b_cc (+ gammas) are written at every stage (mixed:
-        # at stage 2 only, plus the FP32 sweep inputs at both); Helmholtz
+        # the gammas and the pressure at stage 2, the FP32 sweep inputs at
+        # both); Helmholtz
         # reads and writes T at both
         mix = cfg.precision == 1
         st = nv + 1 + (sd if mhd else 0)                  # prim, eint, b_cc
@@ -78,7 +79,9 @@ def kernel_bytes(cfg, name: str) -> float:
         s1_rd = nv if mix else st
         s2_rd = 2 * nv
         s1_wr = (nv if mix else 2 * nv) + ec + (0 if mix else st + gam)
-        s2_wr = nv + ec + st + gam
+        # the mixed path leaves the FP64 prim / eint / b_cc to an on-demand
+        # rebuild from U (Helmholtz: it still stores the EoS pressure)
+        s2_wr = nv + ec + gam + ((1 if helm else 0) if mix else st)
         words = 0.5 * (s1_rd + s2_rd + s1_wr + s2_wr) + (2 if helm else 0)
         pf = 4 * (nv + 5) if mix else 0
         return rd + N * (8 * words + pf)

@@ -55,6 +55,8 @@ struct phlx_ctx {
   // U holds the state of the last completed step (false after init, resume,
   // uploads and diffusion, which change prim / eint without U)
   bool u_valid = false;
+  // mixed path: the FP64 prim / eint / b_cc lag U until materialize() runs
+  bool prim_stale = false;
   double dt_par = 0.0;
   int diff_stages = 0;  // stages of the last diffusion step
   // profiling
@@ -323,12 +325,14 @@ static int copy_out(phlx_ctx *ctx, void *dst, const void *src, size_t bytes) {
 }
 
 static int check_launch();
+static int materialize(phlx_ctx *ctx);
 
 extern "C" int phlx_upload_state(phlx_ctx *ctx, const double *prim, const double *eint,
                                  const double *temp, const double *b1, const double *b2,
                                  const double *b3) {
   if (!ctx) return PHLX_E_ARG;
   HIPCHK(hipSetDevice(ctx->device));
+  if (const int m = materialize(ctx)) return m;   // fields not uploaded stay current
   Grid &g = ctx->g;
   int rc = 0;
   if (prim) rc |= copy_in(ctx, g.P, prim, sizeof(double) * g.nv * g.N);
@@ -359,6 +363,7 @@ extern "C" int phlx_download_state(phlx_ctx *ctx, double *prim, double *eint, do
                                    double *b1, double *b2, double *b3, double *bcc) {
   if (!ctx) return PHLX_E_ARG;
   HIPCHK(hipSetDevice(ctx->device));
+  if (const int m = materialize(ctx)) return m;
   Grid &g = ctx->g;
   int rc = 0;
   if (prim) rc |= copy_out(ctx, prim, g.P, sizeof(double) * g.nv * g.N);
@@ -599,12 +604,22 @@ extern "C" int phlx_rk_update(phlx_ctx *ctx, int32_t irk, double a1, double a2, 
   else
     ctx->kt.update(ctx->g, first, last_stage != 0, a1, a2, a3, fin, ctx->stream);
   ctx->u_valid = fin;
+  if (fin && ctx->kt.prims) ctx->prim_stale = true;
   kend(ctx, on, r);
   return check_launch();
 }
 
 static int diffusion_step(phlx_ctx *ctx, int strang);
 static int parabolic_dt(phlx_ctx *ctx);
+
+// Brings the FP64 prim / eint / b_cc of the mixed path up to date with U
+// (a no-op in FP64 runs and when nothing stepped since the last call).
+static int materialize(phlx_ctx *ctx) {
+  if (!ctx->prim_stale) return PHLX_OK;
+  ctx->kt.prims(ctx->g, ctx->stream);
+  ctx->prim_stale = false;
+  return check_launch();
+}
 
 static int do_hydro_step(phlx_ctx *ctx, bool with_cfl) {
   for (int irk = 0; irk < ctx->stages; irk++) {
@@ -633,6 +648,7 @@ extern "C" int phlx_init(phlx_ctx *ctx) {
   s0.cfl = ctx->cfg.cfl;
   s0.active = 1;
   HIPCHK(hipMemcpyAsync(ctx->g.sc, &s0, sizeof(s0), hipMemcpyHostToDevice, ctx->stream));
+  if (const int m = materialize(ctx)) return m;
   ctx->kt.init(ctx->g, 0, ctx->stream);
   if (ctx->kt.pack) ctx->kt.pack(ctx->g, ctx->stream);
   int rc = check_launch();
@@ -660,6 +676,7 @@ extern "C" int phlx_resume(phlx_ctx *ctx, double time, double dt, double smax, i
   s0.cfl = ctx->cfg.cfl;
   s0.active = time < ctx->cfg.tmax ? 1 : 0;
   HIPCHK(hipMemcpyAsync(ctx->g.sc, &s0, sizeof(s0), hipMemcpyHostToDevice, ctx->stream));
+  if (const int m = materialize(ctx)) return m;
   ctx->kt.init(ctx->g, 1, ctx->stream);
   if (ctx->kt.pack) ctx->kt.pack(ctx->g, ctx->stream);
   ctx->u_valid = false;
@@ -672,6 +689,7 @@ extern "C" int phlx_resume(phlx_ctx *ctx, double time, double dt, double smax, i
 extern "C" int phlx_compute_dt(phlx_ctx *ctx, double *dt_out, double *smax_out) {
   if (!ctx) return PHLX_E_ARG;
   if (!ctx->initialized) return PHLX_E_STATE;
+  if (const int m = materialize(ctx)) return m;
   ctx->kt.cfl(ctx->g, ctx->stream);
   launch_step_end(ctx->g, 0, ctx->stream);
   int rc = check_launch();
@@ -1308,6 +1326,7 @@ extern "C" int phlx_set_kappa(phlx_ctx *ctx, const double *kappa) {
 
 static int parabolic_dt(phlx_ctx *ctx) {
   if (!ctx->kappa_set) return PHLX_E_STATE;
+  if (const int m = materialize(ctx)) return m;
   const Grid &g = ctx->g;
   const double i1 = g.inv_dx[0], i2 = g.inv_dx[1], i3 = (g.sd == 3) ? g.inv_dx[2] : 0.0;
   ProfRec r;
@@ -1338,6 +1357,7 @@ extern "C" int phlx_parabolic_dt(phlx_ctx *ctx, double *dt_par) {
 // stage count and the RKL2 coefficients on the host from the step's dt.
 static int diffusion_step(phlx_ctx *ctx, int strang) {
   (void)strang;
+  if (const int m = materialize(ctx)) return m;
   if (!ctx->kappa_set || !(ctx->dt_par > 0.0)) return PHLX_E_STATE;
   Scalars sc;
   HIPCHK(hipMemcpyAsync(&sc, ctx->g.sc, sizeof(sc), hipMemcpyDeviceToHost, ctx->stream));

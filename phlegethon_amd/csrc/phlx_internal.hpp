@@ -19,6 +19,7 @@ struct KernelTable {
   void (*divb)(const Grid &, hipStream_t);
   // mixed precision: refresh the FP32 sweep inputs from the FP64 state
   void (*pack)(const Grid &, hipStream_t);
+  void (*prims)(const Grid &, hipStream_t);   // mixed: FP64 prim / eint / b_cc from U
 };
 
 bool select_kernels(int sd, int nas, bool mhd, bool helm, bool fasteos, bool mixed, bool r5,

@@ -1047,7 +1047,7 @@ __global__ void __launch_bounds__(256)
       b[0] = 0.5 * (g.bf[0][fix(g, 0, i + 1, j, k)] + g.bf[0][fix(g, 0, i, j, k)]);
       b[1] = 0.5 * (g.bf[1][fix(g, 1, i, j + 1, k)] + g.bf[1][fix(g, 1, i, j, k)]);
       if (SD == 3) b[2] = 0.5 * (g.bf[2][fix(g, 2, i, j, k + 1)] + g.bf[2][fix(g, 2, i, j, k)]);
-      if (keep) {
+      if (!MIX) {
 #pragma unroll
         for (int v = 0; v < SD; v++) g.bcc[v * N + c] = b[v];
       }
@@ -1069,10 +1069,10 @@ __global__ void __launch_bounds__(256)
     if (!HELM) p[IE] = g.gmm1 * eint;
 #pragma unroll
     for (int v = 0; v < NV; v++) {
-      if (keep && (!HELM || v != IE)) g.P[v * N + c] = p[v];
+      if (!MIX && (!HELM || v != IE)) g.P[v * N + c] = p[v];
       out[v * N + c] = S[v];
     }
-    if (keep) g.eint[c] = eint;
+    if (!MIX) g.eint[c] = eint;
     if constexpr (MIX) {
       // FP32 copy of the next sweeps' inputs (the WB deviations formed in FP64)
       g.Pf[c] = (float)(g.wb ? p[0] - g.eq_rho_cc[j] : p[0]);
@@ -1188,6 +1188,45 @@ __global__ void __launch_bounds__(256)
       smax_local = cell_speed<SD, MHD, HELM, HELM && MIX>(g, p, b, T, sound_out, ye_c, abar_c);
   }
   if (last) block_max_to(smax_local, g.red);
+}
+
+// The mixed path's FP64 primitives (all but the Helmholtz pressure, which
+// its update stores), eint and b_cc, rebuilt from the conserved state U and
+// the face field by the operations of k_update's cons -> prim, so they equal
+// what that kernel would have stored bit for bit.  The mixed update leaves
+// them to this kernel, which the library runs before anything reads them.
+template <int SD, int NAS, bool MHD, bool HELM>
+__global__ void __launch_bounds__(256) k_prim_from_u(Grid g) {
+  constexpr int NV = SD + 2 + NAS, IE = SD + 1, IA = SD + 2;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = blockIdx.z;
+  if (i >= g.n1 || j >= g.n2) return;
+  const long long N = g.N, c = cix(g, i, j, k);
+  double S[NV];
+#pragma unroll
+  for (int v = 0; v < NV; v++) S[v] = g.U[v * N + c];
+  double b[3] = {0.0, 0.0, 0.0};
+  if (MHD) {
+    b[0] = 0.5 * (g.bf[0][fix(g, 0, i + 1, j, k)] + g.bf[0][fix(g, 0, i, j, k)]);
+    b[1] = 0.5 * (g.bf[1][fix(g, 1, i, j + 1, k)] + g.bf[1][fix(g, 1, i, j, k)]);
+    if (SD == 3) b[2] = 0.5 * (g.bf[2][fix(g, 2, i, j, k + 1)] + g.bf[2][fix(g, 2, i, j, k)]);
+#pragma unroll
+    for (int v = 0; v < SD; v++) g.bcc[v * N + c] = b[v];
+  }
+  const double rho = S[0], m1 = S[1], m2 = S[2], m3 = (SD == 3) ? S[3] : 0.0;
+  const double ir = 1.0 / rho;
+  double eint = S[IE] - 0.5 * (m1 * m1 + m2 * m2 + m3 * m3) * ir;
+  if (MHD) eint = eint - 0.5 * (b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+  if (g.etot) eint = eint - rho * g.phi_cc[j];
+  g.P[c] = rho;
+  g.P[N + c] = m1 * ir;
+  g.P[2 * N + c] = m2 * ir;
+  if (SD == 3) g.P[3 * N + c] = m3 * ir;
+#pragma unroll
+  for (int v = IA; v < NV; v++) g.P[v * N + c] = S[v] * ir;
+  if (!HELM) g.P[IE * N + c] = g.gmm1 * eint;
+  g.eint[c] = eint;
 }
 
 // Standalone compute_hyperbolic_dt pass over the current prim / b_cc.
@@ -1590,6 +1629,12 @@ struct Kern {
     const dim3 blk(64, 4, 1);
     k_divb<SD><<<grid_for(g.n1, g.n2, g.n3, blk), blk, 0, st>>>(g);
   }
+  static void prims(const Grid &g, hipStream_t st) {
+    if constexpr (MIX) {
+      const dim3 blk(64, 4, 1);
+      k_prim_from_u<SD, NAS, MHD, HELM><<<grid_for(g.n1, g.n2, g.n3, blk), blk, 0, st>>>(g);
+    }
+  }
   static void pack(const Grid &g, hipStream_t st) {
     if constexpr (MIX) {
       const dim3 blk(64, 4, 1);
@@ -1612,6 +1657,7 @@ static KernelTable make_table() {
   t.init = &K::init;
   t.divb = &K::divb;
   t.pack = MIX ? &K::pack : nullptr;
+  t.prims = MIX ? &K::prims : nullptr;
   return t;
 }
 
