@@ -40,7 +40,7 @@ class OrcConfig(ctypes.Structure):
     ]
 
 
-_lib = None
+_libs: Dict[str, ctypes.CDLL] = {}
 
 
 def build() -> str:
@@ -48,13 +48,16 @@ def build() -> str:
     return LIB
 
 
-def load() -> ctypes.CDLL:
-    global _lib
-    if _lib is not None:
-        return _lib
-    if not os.path.exists(LIB):
+def load(path: Optional[str] = None) -> ctypes.CDLL:
+    """The oracle library (default: oracle/_build/libphlx_oracle.so); a
+    path selects another build of the same sources (the mutation tests load
+    deliberately broken copies this way)."""
+    path = path or LIB
+    if path in _libs:
+        return _libs[path]
+    if path == LIB and not os.path.exists(LIB):
         build()
-    lib = ctypes.CDLL(LIB)
+    lib = ctypes.CDLL(path)
     P = ctypes.c_void_p
     lib.orc_create.restype = P
     lib.orc_create.argtypes = [P]
@@ -93,7 +96,7 @@ def load() -> ctypes.CDLL:
     lib.orc_helm_rhoe_batch.argtypes = [P, I, P, P, P, P, P, P, P, I, P]
     lib.orc_helm_rhoP_batch.argtypes = [P, I, P, P, P, P, P, P, P, I, P]
     lib.orc_helm_interp.argtypes = [P, D, D, P]
-    _lib = lib
+    _libs[path] = lib
     return lib
 
 
@@ -123,8 +126,8 @@ def _p(a):
 class OracleRun:
     """CPU restatement of time_loop + hydro_step for one problem."""
 
-    def __init__(self, problem, nthreads: int = 1):
-        self.lib = load()
+    def __init__(self, problem, nthreads: int = 1, lib: Optional[str] = None):
+        self.lib = load(lib)
         self.cfg = problem.cfg
         self._c = to_orc(problem.cfg, nthreads)
         self.g = self.lib.orc_create(ctypes.byref(self._c))
@@ -141,7 +144,7 @@ class OracleRun:
                                _p(bs[2]))
         self.helm = None
         if problem.helm is not None:
-            self.helm = OracleHelm(problem.helm)
+            self.helm = OracleHelm(problem.helm, lib)
             self.lib.orc_set_helm(self.g, self.helm.h)
         ge = [None if problem.profiles.get(n) is None else
               np.ascontiguousarray(problem.profiles[n], dtype=np.float64)
@@ -236,8 +239,8 @@ class OracleRun:
         return out
 
 
-def riemann_hydro(cfg, qL: np.ndarray, qR: np.ndarray) -> np.ndarray:
-    lib = load()
+def riemann_hydro(cfg, qL: np.ndarray, qR: np.ndarray, lib: Optional[str] = None) -> np.ndarray:
+    lib = load(lib)
     c = to_orc(cfg)
     qL = np.ascontiguousarray(qL, dtype=np.float64)
     qR = np.ascontiguousarray(qR, dtype=np.float64)
@@ -246,8 +249,8 @@ def riemann_hydro(cfg, qL: np.ndarray, qR: np.ndarray) -> np.ndarray:
     return out
 
 
-def riemann_mhd(cfg, qL, qR, bL, bR):
-    lib = load()
+def riemann_mhd(cfg, qL, qR, bL, bR, lib: Optional[str] = None):
+    lib = load(lib)
     c = to_orc(cfg)
     qL = np.ascontiguousarray(qL, dtype=np.float64)
     qR = np.ascontiguousarray(qR, dtype=np.float64)
@@ -269,8 +272,9 @@ def reconstruct(scheme: int, q: np.ndarray, limited_scalar: bool = False):
     return qm, qp
 
 
-def run_problem(problem, nsteps: int, nthreads: int = 1) -> Dict[str, np.ndarray]:
-    r = OracleRun(problem, nthreads)
+def run_problem(problem, nsteps: int, nthreads: int = 1,
+                lib: Optional[str] = None) -> Dict[str, np.ndarray]:
+    r = OracleRun(problem, nthreads, lib)
     r.init()
     r.run(nsteps)
     st = r.state()
@@ -283,8 +287,8 @@ def run_problem(problem, nsteps: int, nthreads: int = 1) -> Dict[str, np.ndarray
 class OracleHelm:
     """CPU restatement of the Helmholtz EoS on a loaded table (HelmTable)."""
 
-    def __init__(self, table):
-        self.lib = load()
+    def __init__(self, table, lib: Optional[str] = None):
+        self.lib = load(lib)
         self.table = table
         f9 = np.ascontiguousarray(table.f9, dtype=np.float64)
         c4 = np.ascontiguousarray(table.c4, dtype=np.float64)

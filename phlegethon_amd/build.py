@@ -14,8 +14,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "_lib")
 OUT = os.path.join(OUT_DIR, "libphlx.so")
-SOURCES = ["phlx_kernels.hip", "phlx_helm.hip", "phlx_mlp.hip", "phlx_diffusion.hip",
-           "phlx_api.hip"]
+SOURCES = ["phlx_sel_fp64.hip", "phlx_sel_mixed.hip", "phlx_sel_fp64_r5.hip",
+           "phlx_sel_mixed_r5.hip", "phlx_kernels.hip", "phlx_helm.hip", "phlx_mlp.hip",
+           "phlx_diffusion.hip", "phlx_api.hip"]
 ARCH = os.environ.get("PHLX_OFFLOAD_ARCH", "gfx950")
 
 
@@ -26,19 +27,35 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found")
 
 
-def build(verbose: bool = False) -> str:
-    os.makedirs(OUT_DIR, exist_ok=True)
-    tmp = OUT + ".tmp"
-    cmd = [hipcc(), f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
-           "-ffp-contract=off", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-           "-Wall", "-Wno-unused-command-line-argument",
-           "-o", tmp] + [os.path.join(CSRC, s) for s in SOURCES]
+FLAGS = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
+         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+         "-Wall", "-Wno-unused-command-line-argument"]
+OBJ_DIR = os.path.join(HERE, "_build")
+
+
+def _run(cmd, what, verbose):
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
-        raise RuntimeError("hipcc failed building libphlx.so")
+        raise RuntimeError(f"hipcc failed: {what}")
     if verbose:
         sys.stderr.write(r.stderr)
+
+
+def build(verbose: bool = False) -> str:
+    """Compiles each source to an object in parallel (no cross-unit device
+    references, so no relocatable device code) and links libphlx.so."""
+    from concurrent.futures import ThreadPoolExecutor
+    os.makedirs(OUT_DIR, exist_ok=True)
+    os.makedirs(OBJ_DIR, exist_ok=True)
+    objs = [os.path.join(OBJ_DIR, os.path.splitext(s)[0] + ".o") for s in SOURCES]
+    cmds = [[hipcc(), f"--offload-arch={ARCH}"] + FLAGS + ["-c", "-o", o, os.path.join(CSRC, s)]
+            for s, o in zip(SOURCES, objs)]
+    with ThreadPoolExecutor(max_workers=len(cmds)) as ex:
+        list(ex.map(lambda c: _run(c, os.path.basename(c[-1]), verbose), cmds))
+    tmp = OUT + ".tmp"
+    _run([hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs,
+         "link libphlx.so", verbose)
     os.replace(tmp, OUT)
     return OUT
 

@@ -52,10 +52,9 @@ struct phlx_ctx {
   // host copy of dt_par, kappa uploaded, the ideal gas' diffusion T current
   // (false after any hydro update: the next step recomputes it from p)
   bool diff_on = false, diff_eval = true, kappa_set = false, tdiff_valid = false;
-  // U holds the state of the last completed step (false after init, resume,
-  // uploads and diffusion, which change prim / eint without U)
-  bool u_valid = false;
-  // mixed path: the FP64 prim / eint / b_cc lag U until materialize() runs
+  // mixed path: a step was launched since the last rebuild of the FP64
+  // prim / eint / b_cc from U (whether it ran, and so whether they lag U,
+  // the device flags SC_U_CUR / SC_P_STALE say)
   bool prim_stale = false;
   double dt_par = 0.0;
   int diff_stages = 0;  // stages of the last diffusion step
@@ -347,9 +346,10 @@ extern "C" int phlx_upload_state(phlx_ctx *ctx, const double *prim, const double
       rc |= copy_in(ctx, g.bf[d], bs[d], sizeof(double) * g.nf[d]);
     }
   if (rc) return PHLX_E_HIP;
-  // mixed precision: a state replaced between steps refreshes the sweeps'
+  // mixed precision: the uploaded fields are the state now (U is not, and
+  // nothing lags it); a state replaced between steps refreshes the sweeps'
   // FP32 inputs (before phlx_init, init does it)
-  ctx->u_valid = false;
+  if (ctx->kt.prims) launch_flags(g.sc, 0, 0, ctx->stream);
   if (ctx->initialized && ctx->kt.pack) {
     ctx->kt.pack(ctx->g, ctx->stream);
     rc = check_launch();
@@ -595,15 +595,15 @@ extern "C" int phlx_rk_update(phlx_ctx *ctx, int32_t irk, double a1, double a2, 
   ProfRec r;
   int on = kbegin(ctx, KID_UPDATE, r);
   const bool guess = ctx->kt.update_guess && (ctx->g.eos_guess != GUESS_WARM || ctx->eos_stats);
-  // stage 1 of the mixed path starts from U when it is current (the FP64
-  // conserved state of the last completed step)
-  const int first = irk == 0 ? ((ctx->g.Pf && ctx->u_valid) ? 2 : 1) : 0;
+  // stage 1 of the mixed path starts from U when the device flags say it is
+  // current (the FP64 conserved state of the last step that ran); a last
+  // stage that runs sets them, so a step past tmax leaves them unchanged
+  const int first = irk == 0 ? 1 : 0;
   const int fin = irk == ctx->stages - 1;
   if (guess)
     ctx->kt.update_guess(ctx->g, first, last_stage != 0, a1, a2, a3, fin, ctx->stream);
   else
     ctx->kt.update(ctx->g, first, last_stage != 0, a1, a2, a3, fin, ctx->stream);
-  ctx->u_valid = fin;
   if (fin && ctx->kt.prims) ctx->prim_stale = true;
   kend(ctx, on, r);
   return check_launch();
@@ -613,10 +613,12 @@ static int diffusion_step(phlx_ctx *ctx, int strang);
 static int parabolic_dt(phlx_ctx *ctx);
 
 // Brings the FP64 prim / eint / b_cc of the mixed path up to date with U
-// (a no-op in FP64 runs and when nothing stepped since the last call).
+// (a no-op in FP64 runs and when no step was launched since the last call;
+// on the device the rebuild runs only if a step actually ran, SC_P_STALE).
 static int materialize(phlx_ctx *ctx) {
   if (!ctx->prim_stale) return PHLX_OK;
   ctx->kt.prims(ctx->g, ctx->stream);
+  launch_flags(ctx->g.sc, ~SC_P_STALE, 0, ctx->stream);
   ctx->prim_stale = false;
   return check_launch();
 }
@@ -655,7 +657,6 @@ extern "C" int phlx_init(phlx_ctx *ctx) {
   if (rc) return rc;
   ctx->initialized = true;
   ctx->tdiff_valid = false;
-  ctx->u_valid = false;
   // without EVALUATE_PARABOLIC_TIMESTEP dt_par is computed once, at step 0
   // (source/source.F90:6504-6512)
   if (ctx->diff_on && !ctx->diff_eval && (rc = parabolic_dt(ctx))) return rc;
@@ -679,7 +680,6 @@ extern "C" int phlx_resume(phlx_ctx *ctx, double time, double dt, double smax, i
   if (const int m = materialize(ctx)) return m;
   ctx->kt.init(ctx->g, 1, ctx->stream);
   if (ctx->kt.pack) ctx->kt.pack(ctx->g, ctx->stream);
-  ctx->u_valid = false;
   int rc = check_launch();
   if (rc) return rc;
   ctx->initialized = true;
@@ -1390,8 +1390,11 @@ static int diffusion_step(phlx_ctx *ctx, int strang) {
     bjm1 = bj;
   }
   launch_diff_final(g, helm, ctx->stream);
-  if (ctx->kt.pack) ctx->kt.pack(g, ctx->stream);
-  ctx->u_valid = false;
+  if (ctx->kt.pack) {
+    // the primitives moved past U
+    ctx->kt.pack(g, ctx->stream);
+    launch_flags(g.sc, ~SC_U_CUR, 0, ctx->stream);
+  }
   kend(ctx, on, r);
   int rc = check_launch();
   if (rc) return rc;

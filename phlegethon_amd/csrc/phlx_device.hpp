@@ -32,8 +32,14 @@ struct Scalars {
   unsigned long long spare;
   long long step;
   int active;  // 0 once time >= tmax: the step's kernels return at once
-  int pad;
+  int flags;   // mixed precision: SC_U_CUR | SC_P_STALE (set by the kernels that ran)
 };
+// U holds the conserved state of the current step (the mixed path's stage 1
+// starts from it instead of rebuilding state0 from the primitives)
+constexpr int SC_U_CUR = 1;
+// the FP64 primitives, eint and b_cc lag U (a mixed step ran since they were
+// last rebuilt)
+constexpr int SC_P_STALE = 2;
 
 struct Grid {
   int n1, n2, n3, sd, nv, nas, ip;

@@ -24,7 +24,14 @@ struct KernelTable {
 
 bool select_kernels(int sd, int nas, bool mhd, bool helm, bool fasteos, bool mixed, bool r5,
                     KernelTable &t);
+// the four instance families (phlx_sel_*.hip)
+bool select_fp64(int sd, int nas, bool mhd, bool helm, bool fasteos, KernelTable &t);
+bool select_fp64_r5(int sd, int nas, bool mhd, bool helm, bool fasteos, KernelTable &t);
+bool select_mixed(int sd, int nas, bool mhd, bool helm, bool fasteos, KernelTable &t);
+bool select_mixed_r5(int sd, int nas, bool mhd, bool helm, bool fasteos, KernelTable &t);
 void launch_step_begin(Scalars *sc, hipStream_t st);
+// device state flags: flags = (flags & keep) | set
+void launch_flags(Scalars *sc, int keep, int set, hipStream_t st);
 void launch_step_end(const Grid &g, int advance, hipStream_t st);
 void launch_reduce_to(const Grid &g, double *out, hipStream_t st);
 int cell_grid_blocks(const Grid &g);

@@ -196,6 +196,30 @@ def orszag_tang_2d(n: int = 128, flux: int = LHLLD_FLUX, recon: int = LIM2ND_REC
 
 
 # ------------------------------------------------------ validation problems
+def field_loop(n: int = 64, vel=(2.0, 1.0), A0: float = 1e-3, R: float = 0.3,
+               flux: int = LHLLD_FLUX, recon: int = LIM2ND_REC, mach_ct: float = 1e-6,
+               tmax: float = 1e30) -> Problem:
+    """Field-loop advection (Gardiner & Stone 2005, section 5.1): periodic
+    [-1, 1] x [-0.5, 0.5] with 2n x n cells, rho = p = 1, uniform velocity
+    `vel`, face B from the nodal A_z = A0 (R - r) inside r < R (zero
+    outside), so div B = 0 to round-off; with vel = (2, 1) the loop is back
+    at its start at t = 1."""
+    cfg = PhlxConfig(sdims=2, nx1=2 * n, nx2=n, x1l=-1.0, x1u=1.0, x2l=-0.5, x2u=0.5,
+                     recon=recon, flux=flux, cfl=0.8, mach_ct=mach_ct, tmax=tmax,
+                     gamma=5.0 / 3.0).validate()
+    prim = np.empty((cfg.nvars,) + cfg.shape)
+    prim[0] = 1.0
+    prim[1] = vel[0]
+    prim[2] = vel[1]
+    prim[3] = 1.0
+    xn, yn = nodes(cfg, 0), nodes(cfg, 1)
+    r = np.sqrt(xn[None, :] ** 2 + yn[:, None] ** 2)
+    Az = np.where(r < R, A0 * (R - r), 0.0)
+    bx = (Az[1:, :] - Az[:-1, :]) / cfg.dx[1]
+    by = -(Az[:, 1:] - Az[:, :-1]) / cfg.dx[0]
+    return Problem("field_loop", cfg, prim, [bx[None].copy(), by[None].copy(), None], {})
+
+
 def sod_tube(n: int = 200, flux: int = HLLC_FLUX, recon: int = LIM2ND_REC,
              ny: int = 4, axis: int = 0) -> Problem:
     """Sod shock tube along x1 (or x2 with axis=1): (1, 0, 1) | (0.125, 0, 0.1),
@@ -518,9 +542,10 @@ def lowmach_box(n: int = 384, heating: float = 1e-6, sdims: int = 3, recon: int 
     prim[0] = rho_cc[None, :, None] * (1.0 + noise * u)
     prim[cfg.sdims + 1] = np.broadcast_to(p_cc[None, :, None], shape)
     if mach is not None:
-        x = centres(cfg, 0)[:, None, None]
+        # arrays are (z, y, x)
+        x = centres(cfg, 0)[None, None, :]
         y = yc[None, :, None]
-        z = centres(cfg, 2)[None, None, :] if sdims == 3 else np.zeros((1, 1, 1))
+        z = centres(cfg, 2)[:, None, None] if sdims == 3 else np.zeros((1, 1, 1))
         cz = np.cos(2.0 * math.pi * z)
         v1 = math.pi * np.sin(2.0 * math.pi * x) * np.cos(math.pi * y) * cz
         v2 = -2.0 * math.pi * np.cos(2.0 * math.pi * x) * np.sin(math.pi * y) * cz
