@@ -1,21 +1,43 @@
 #!/usr/bin/env python3
 """Benchmark: zone-updates/s of the explicit FV MHD update on one MI355X.
 
-Workload (default): 3D Orszag-Tang ideal MHD with constrained transport,
-256^3 zones, FP64, LHLLD flux + LIM2ND reconstruction, RK2, CFL 0.8,
-periodic unit cube, seeded 1e-4 density noise.  One "step" is one full
-time step (both RK stages, CT, CFL update) over the whole grid; the metric
-follows the reference's "updated cells/s" = steps * zones / loop wall time
-(source/source.F90:6834-6835) with the state resident in HBM.
+Default workload -- the combined configuration the metric is quoted on:
+3D MHD + Helmholtz EoS stratified slab, 512^3 zones (log rho 6 -> 2, log T
+9 -> 7, abar 4 -> 16, seeded 1e-4 noise, seeded random-phase
+divergence-free field at plasma beta 1e5), USE_FASTEOS + USE_WB +
+EVOLVE_ETOT + ADVECT_YE_IABAR, LHLLD + CT + LIM2ND, RK2, CFL 0.8, the
+generated 541 x 271 Helmholtz table.  It runs
 
-Other workload (``--workload helm_slab``): the 3D stratified Helmholtz slab,
-256^3, FP64, USE_FASTEOS + USE_WB + EVOLVE_ETOT + ADVECT_YE_IABAR, LHLLC +
-LIM2ND, RK2 (one T(rho, e) Newton solve per zone per stage).
+  plain FP64     the reference algorithm (T(rho, e) Newton from the carried T);
+  mixed + NN     FP32 reconstruction / Riemann on FP64 deviations, FP64 update,
+                 CT and EoS residual, T(rho, e) Newton started from the
+                 surrogate's corrected warm start (network trained on the GPU
+                 on EoS-generated pairs over the slab's states);
+  mixed + pred   the same with the energy-predictor start;
+  mixed + warm   the same from the carried T;
+
+each from the same initial data: ``--warmup`` untimed steps, ``--steps``
+timed steps (device-synchronised loop, state resident in HBM), then on to
+``--l1-steps`` in total, where every conserved field and each face-B
+component of the mixed runs is compared with the plain FP64 state (the FP64
+GPU path equals the CPU oracle bit for bit on every parity case).  One
+"step" is one full time step (both RK stages, CT, CFL update) over the whole
+grid; the metric follows the reference's "updated cells/s" = steps * zones /
+loop wall time (source/source.F90:6834-6835).  ``value`` is the mixed + NN
+path; the plain path's figures sit beside it.
+
+Other workloads (``--workload``): ot3d (3D Orszag-Tang MHD 256^3), helm_slab
+(Helmholtz slab 256^3, optionally MHD), lowmach (heated box 384^3),
+hotbubble (2D 256^2), each with ``--precision`` fp64 / mixed.
 
 Prints ONE JSON line on stdout (progress goes to stderr).  Includes
-  roofline      -- dominant kernel, algorithmic bytes / average launch time
-                   (HIP events on the library's stream) vs 8 TB/s HBM peak;
-  step_roofline -- whole step: B_zu * zone-updates/s vs the same peak;
+  roofline      -- the plain FP64 path's dominant kernel: its share of the
+                   algorithmic bytes per zone-update B_zu = 8 (5 n_q + 4 n_e)
+                   (B_zu / RK stages per zone and launch) over its average
+                   launch time (HIP events on the library's stream), against
+                   the 8 TB/s HBM peak; per-kernel byte models under
+                   ``kernel_models``;
+  step_roofline -- whole step: B_zu * zone-updates/s against the same peak;
   cpu_baseline  -- the CPU FP64 restatement (oracle/) on the host cores over
                    a bounded sample of the same workload.
 """
@@ -123,7 +145,33 @@ def b_zu(cfg) -> int:
     return 8 * (8 * n_q + 6 * n_e)
 
 
+def host_description(threads: int) -> dict:
+    """The host the CPU baseline ran on: nproc, the lscpu model and socket
+    count, OMP_NUM_THREADS and the thread binding."""
+    import subprocess
+    info = {"nproc": os.cpu_count(), "threads_used": threads,
+            "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS"),
+            "OMP_PROC_BIND": os.environ.get("OMP_PROC_BIND", "unset (no pinning)")}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in ("Model name", "Socket(s)", "Thread(s) per core",
+                             "Core(s) per socket"):
+                info[k.strip()] = v.strip()
+    except Exception as e:  # lscpu missing: report what is known
+        info["lscpu"] = f"unavailable ({e})"
+    try:
+        info["affinity_cpus"] = len(os.sched_getaffinity(0))
+    except Exception:
+        pass
+    return info
+
+
 def cpu_baseline(pb, steps: int, threads: int):
+    """The oracle (FP64 restatement, -O3 -march=x86-64-v3 -ffp-contract=off,
+    OpenMP) timed over `steps` full steps of the same problem, after its own
+    step-0 initialisation."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     r = oracle.OracleRun(pb, threads)
@@ -144,6 +192,19 @@ def build_problem(args):
         pb = problems.orszag_tang_3d(n=n)
         desc = f"3D Orszag-Tang ideal MHD + CT, {n}^3, LHLLD + LIM2ND, RK2, CFL 0.8, periodic"
         data = "synthetic (seeded 3D Orszag-Tang initial data, counter-based PRNG)"
+    elif args.workload == "c5":
+        from phlegethon_amd import helmholtz
+        from phlegethon_amd.runtime import HelmEoS
+        table = helmholtz.default_table()
+        log(f"building the MHD + Helmholtz slab {n}^3 (table {table.nrho}x{table.nT}, beta 1e5)")
+        eos = HelmEoS(table)
+        pb = problems.helm_slab(eos, table, n=n, mhd_beta=1e5)
+        eos.close()
+        desc = (f"3D MHD + Helmholtz stratified slab, {n}^3, USE_FASTEOS + USE_WB + EVOLVE_ETOT"
+                f" + ADVECT_YE_IABAR, LHLLD + CT + LIM2ND, beta 1e5, RK2, CFL 0.8")
+        data = ("synthetic (plane-parallel rho/T/abar profiles, seeded 1e-4 density noise, "
+                f"seeded random-phase divergence-free field; generated {table.nrho}x{table.nT} "
+                "Helmholtz table)")
     elif args.workload == "helm_slab":
         from phlegethon_amd import helmholtz
         from phlegethon_amd.runtime import HelmEoS
@@ -237,25 +298,30 @@ def add_diffusion(pb, ratio: float):
     pb.cfg = cfg.with_(thermal_diffusion=True, eval_parabolic_dt=True)
 
 
-def run_variant(pb, cfg, args, guess="warm", keep_state=False, profile=True):
-    """Upload, init, warm up, time args.steps steps; optionally profile."""
+def run_variant(pb, cfg, args, guess="warm", keep_state=False, profile=True, total_steps=None):
+    """Upload, init, (train the surrogate,) warm up, time args.steps steps,
+    count Newton iterations over two more, optionally profile the kernels
+    for args.profile_steps, then run on to total_steps steps in all and keep
+    the state."""
     from phlegethon_amd.runtime import Simulation
     sim = Simulation(cfg)
     sim.setup(pb)
     res = {}
+    helm = cfg.eos == 1
     if guess == "mlp":
         t0 = time.perf_counter()
         from phlegethon_amd.problems import state_box
         box = state_box(pb) if args.mlp_box == "state" else None
         loss, rmst, rmse = sim.mlp_train(nsamples=args.mlp_samples, epochs=args.mlp_epochs,
                                          batch=4096, lr=5e-3, box=box)
-        res["mlp"] = {"train_s": time.perf_counter() - t0, "samples": args.mlp_samples,
-                      "box": ([round(v, 4) for v in box] if box is not None else "table"),
-                      "epochs": args.mlp_epochs, "final_batch_loss": loss,
-                      "heldout_rms_log10T": rmst, "heldout_rms_log10e": rmse}
-        log(f"surrogate trained in {res['mlp']['train_s']:.2f} s, held-out RMS log10 T "
+        res["surrogate"] = {"train_s": time.perf_counter() - t0, "samples": args.mlp_samples,
+                            "box": ([round(v, 4) for v in box] if box is not None else "table"),
+                            "epochs": args.mlp_epochs, "final_batch_loss": loss,
+                            "heldout_rms_log10T": rmst, "heldout_rms_log10e": rmse}
+        log(f"surrogate trained in {res['surrogate']['train_s']:.2f} s, held-out RMS log10 T "
             f"{rmst:.4f}, implied log10 e {rmse:.4f}")
-        sim.set_eos_guess("mlp", resid_tol=0.25, count=False)
+    if helm:
+        sim.set_eos_guess(guess, resid_tol=0.25, count=False)
     sim.init()
     log(f"warmup {args.warmup} steps")
     sim.run(args.warmup)
@@ -266,17 +332,15 @@ def run_variant(pb, cfg, args, guess="warm", keep_state=False, profile=True):
     sim.synchronize()
     elapsed = time.perf_counter() - t0
     zones = cfg.nx1 * cfg.nx2 * (cfg.nx3 if cfg.sdims == 3 else 1)
-    res.update(value=zones * args.steps / elapsed, ms_per_step=1e3 * elapsed / args.steps,
-               scalars=sim.scalars(), divb=sim.divb_max())
-    log(f"{res['value']:.4e} zone-updates/s, {res['ms_per_step']:.3f} ms/step, "
-        f"t={res['scalars']['time']:.6g}, max|divB|={res['divb']:.2e}")
-    if keep_state:
-        res["state"] = sim.download()
-    if cfg.eos == 1:
+    res.update(value=zones * args.steps / elapsed, ms_per_step=1e3 * elapsed / args.steps)
+    log(f"{res['value']:.4e} zone-updates/s, {res['ms_per_step']:.3f} ms/step")
+    done = args.warmup + args.steps
+    if helm:
         # Newton iterations per solve over two counted steps
         sim.set_eos_guess(guess, resid_tol=0.25, count=True)
         sim.eos_stats(reset=True)
         sim.run(2)
+        done += 2
         st = sim.eos_stats()
         res["eos"] = {"newton_iters_per_solve": st["newton_iters"] / max(st["solves"], 1),
                       "fallback_rate": st["fallbacks"] / max(st["solves"], 1),
@@ -288,6 +352,7 @@ def run_variant(pb, cfg, args, guess="warm", keep_state=False, profile=True):
         sim.profile(True)
         sim.run(args.profile_steps)
         sim.synchronize()
+        done += args.profile_steps
         kern = {}
         for name in ("flux_x1", "flux_x2", "flux_x3", "corner", "faceb", "update", "misc",
                      "diffusion"):
@@ -307,25 +372,89 @@ def run_variant(pb, cfg, args, guess="warm", keep_state=False, profile=True):
         res["kern"] = kern
         res["copy_gbps"] = sim.stream_copy(1 << 30, 10)
         log(f"stream copy (1 GiB x10): {res['copy_gbps']:.0f} GB/s")
+    if total_steps is not None and total_steps > done:
+        log(f"running on to step {total_steps}")
+        sim.run(total_steps - done)
+    res["scalars"] = sim.scalars()
+    res["divb"] = sim.divb_max()
+    if keep_state:
+        res["state"] = sim.download()
     sim.close()
     return res
+
+
+def rel_l1_fields(st, ref, cfg):
+    """rel. L1 of every conserved field and each face-B component."""
+    import numpy as np
+    a, b = conserved(st, cfg), conserved(ref, cfg)
+    if cfg.mhd:
+        for d in range(cfg.sdims):
+            k = f"b_x{d + 1}"
+            a[k], b[k] = st[k], ref[k]
+    return {k: float(np.abs(a[k] - b[k]).sum() / max(np.abs(b[k]).sum(), 1e-300)) for k in b}
+
+
+def roofline_of(cfg, kern, value):
+    """The dominant kernel's share of B_zu per launch over its average
+    launch time; the whole step's B_zu rate; the per-kernel byte models."""
+    dom = max(kern, key=lambda k: kern[k]["total_ms"])
+    d = kern[dom]
+    zones = cfg.nx1 * cfg.nx2 * (cfg.nx3 if cfg.sdims == 3 else 1)
+    bzu = b_zu(cfg)
+    per_launch = bzu / cfg.rk_stages * zones
+    achieved = per_launch / (d["avg_ms"] * 1e-3)
+    roof = {"bound": "hbm", "kernel": dom, "achieved": achieved / 1e9,
+            "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": achieved / HBM_PEAK,
+            "traffic": None, "alg_bytes_per_launch": per_launch,
+            "alg_bytes_rule": f"B_zu / {cfg.rk_stages} stages x {zones} zones, B_zu = {bzu} B",
+            "avg_launch_ms": d["avg_ms"]}
+    step = {"bound": "hbm", "B_zu": bzu, "achieved": bzu * value / 1e9,
+            "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": bzu * value / HBM_PEAK}
+    models = {k: {"avg_ms": v["avg_ms"], "launches": v["launches"],
+                  "model_bytes_per_launch": v["alg_bytes"],
+                  "model_GBps": v["alg_bytes"] / (v["avg_ms"] * 1e-3) / 1e9}
+              for k, v in kern.items()}
+    return roof, step, models
+
+
+def counter_traffic(name: str, kernel: str):
+    """L2-miss bytes per dispatch of `kernel` from a committed rocprofv3
+    --pmc FETCH_SIZE / WRITE_SIZE summary (profiles/<name>.json), with the
+    head of the build it was measured on; None without one."""
+    path = os.path.join(ROOT, "profiles", name + ".json")
+    if not os.path.exists(path):
+        return None, None
+    with open(path) as fh:
+        rec = json.load(fh)
+    k = rec.get("kernels", {}).get(kernel)
+    if not k:
+        return None, None
+    return k.get("hbm_bytes"), {"file": f"profiles/{name}.json", "head": rec.get("head"),
+                                "counter": "FETCH_SIZE + WRITE_SIZE (L2-miss bytes, "
+                                           "Infinity-Cache hits included), copy-calibrated"}
 
 
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--n", type=int, default=256, help="zones per direction")
-    ap.add_argument("--profile-steps", type=int, default=3)
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--n", type=int, default=None, help="zones per direction")
+    ap.add_argument("--profile-steps", type=int, default=2)
+    ap.add_argument("--l1-steps", type=int, default=100,
+                    help="c5: steps after which the mixed runs are compared with plain FP64")
+    ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=("ot3d", "helm_slab", "lowmach", "hotbubble"),
-                    default="ot3d")
+    ap.add_argument("--workload", choices=("c5", "ot3d", "helm_slab", "lowmach", "hotbubble"),
+                    default="c5")
+    ap.add_argument("--variants", default="mlp,pred,warm",
+                    help="c5: the mixed-precision T(rho, e) starts to run (comma list of "
+                         "mlp, pred, warm; the first is the headline value)")
     ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64")
-    ap.add_argument("--eos-guess", choices=("warm", "mlp"), default="warm",
-                    help="Helmholtz: Newton start (carried T, or the trained surrogate)")
+    ap.add_argument("--eos-guess", choices=("warm", "mlp", "pred"), default="warm",
+                    help="Helmholtz: Newton start (carried T, the trained surrogate, or the "
+                         "energy predictor)")
     ap.add_argument("--mhd-beta", type=float, default=None,
                     help="helm_slab: add a random divergence-free field at this plasma beta")
     ap.add_argument("--heating", type=float, default=1e-3,
@@ -355,9 +484,8 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU only; the multi-GPU "
                           "brick decomposition is a design (not built): use --gpus 1"}))
         sys.exit(2)
-
-    import numpy as np
-    from phlegethon_amd.config import PREC_MIXED
+    if args.n is None:
+        args.n = {"c5": 512, "lowmach": 384}.get(args.workload, 256)
 
     pb, desc, data = build_problem(args)
     if args.shock_flattening is not None:
@@ -367,103 +495,131 @@ def main():
         add_diffusion(pb, args.diffusion)
         desc += (f" | THERMAL_DIFFUSION_STS (RKL2), uniform kappa, dt/dt_par "
                  f"{args.diffusion:g} at the start")
-    n = args.n
+    if args.workload == "c5":
+        out = bench_c5(pb, desc, data, args)
+    else:
+        out = bench_single(pb, desc, data, args)
+    out["cpu_baseline"] = None if args.no_cpu_baseline else run_cpu_baseline(pb, args)
+    print(json.dumps(out), flush=True)
+
+
+def run_cpu_baseline(pb, args):
+    threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+    shape = "x".join(str(v) for v in reversed(pb.cfg.shape) if v > 1)
+    log(f"cpu baseline: oracle on {threads} threads, {args.cpu_steps} steps of {shape}")
+    rate, wall = cpu_baseline(pb, args.cpu_steps, threads)
     zones = pb.cfg.nx1 * pb.cfg.nx2 * (pb.cfg.nx3 if pb.cfg.sdims == 3 else 1)
+    # wct/cell/cycle/core in microseconds (source/source.F90:6832-6833)
+    us = wall / args.cpu_steps / zones * 1e6 * threads
+    log(f"cpu baseline {rate:.4e} zone-updates/s ({us:.3f} us/cell/cycle/core)")
+    return {"value": rate, "unit": "zone-updates/s", "cores": threads, "kind": "port",
+            "wct_per_cell_cycle_core_us": us, "host": host_description(threads),
+            "sample": f"{args.cpu_steps} full step(s) of the same {shape} workload "
+                      f"({wall:.1f} s wall) after the oracle's step-0 initialisation; C FP64 "
+                      "restatement with OpenMP (-O3 -march=x86-64-v3 -ffp-contract=off)"}
+
+
+def bench_single(pb, desc, data, args):
+    """One configuration at one precision (the non-default workloads)."""
+    import numpy as np
+    from phlegethon_amd.config import PREC_MIXED
     cfg = pb.cfg
     if args.precision == "mixed":
         cfg = cfg.with_(precision=PREC_MIXED)
         desc += " | mixed precision (FP32 recon/Riemann, FP64 deviations/update/CT/EoS)"
     else:
         desc += " | FP64"
-    if args.eos_guess == "mlp":
-        desc += " | surrogate Newton guess"
+    if args.eos_guess != "warm":
+        desc += f" | T(rho, e) start: {args.eos_guess}"
     res = run_variant(pb, cfg, args, guess=args.eos_guess, keep_state=args.compare)
-    value, ms_per_step = res["value"], res["ms_per_step"]
-    kern = res["kern"]
-    dom = max(kern, key=lambda k: kern[k]["total_ms"])
-    d = kern[dom]
-    achieved = d["alg_bytes"] / (d["avg_ms"] * 1e-3)
-    traffic = None
-    pmc = os.path.join(ROOT, "profiles", "ot3d256_fp64_pmc.json")
-    if (n == 256 and args.workload == "ot3d" and args.precision == "fp64"
-            and args.shock_flattening is None and args.diffusion is None
-            and os.path.exists(pmc)):
-        # HBM bytes per dispatch measured with rocprofv3 --pmc FETCH_SIZE /
-        # WRITE_SIZE on this workload, scaled by a copy kernel of known size
-        with open(pmc) as fh:
-            rec = json.load(fh)["kernels"].get(dom)
-        if rec:
-            traffic = rec["hbm_bytes"]
-    roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved / 1e9,
-                "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": achieved / HBM_PEAK,
-                "traffic": traffic, "alg_bytes_per_launch": d["alg_bytes"],
-                "avg_launch_ms": d["avg_ms"]}
-    bzu = b_zu(cfg)
-    step_roof = {"bound": "hbm", "B_zu": bzu, "achieved": bzu * value / 1e9,
-                 "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": bzu * value / HBM_PEAK,
-                 "stream_copy_GBps": res["copy_gbps"]}
-    for k, v in kern.items():
-        log(f"  {k:8s} avg {v['avg_ms']:.3f} ms  x{v['launches']}  "
-            f"{v['alg_bytes'] / (v['avg_ms'] * 1e-3) / 1e9:.0f} GB/s algorithmic")
-
-    compare = None
-    if args.compare:
-        log("plain FP64 warm-start run on the same initial data")
-        base = run_variant(pb, pb.cfg, args, guess="warm", keep_state=True, profile=False)
-        cv, cb = conserved(res["state"], pb.cfg), conserved(base["state"], pb.cfg)
-        rel = {k: float(np.abs(cv[k] - cb[k]).sum() / max(np.abs(cb[k]).sum(), 1e-300))
-               for k in cb}
-        compare = {"plain_value": base["value"], "plain_ms_per_step": base["ms_per_step"],
-                   "speedup": value / base["value"], "rel_l1_vs_plain_fp64": rel,
-                   "plain_eos": base.get("eos"), "steps_compared": args.warmup + args.steps}
-        log(f"speed-up {compare['speedup']:.3f}x; rel L1 " +
-            " ".join(f"{k} {v:.2e}" for k, v in rel.items()))
-        if "eq_rho_cc" in pb.profiles:
-            # the WB deviations and the velocity: L1 of the difference over
-            # L1 of the FP64 field itself
-            dev = deviation_errors(res["state"], base["state"], pb)
-            compare["rel_l1_deviations"] = dev
-            log("deviations rel L1 " + " ".join(f"{k} {v:.2e}" for k, v in dev.items()))
-
-    cpu = None
-    if not args.no_cpu_baseline:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        shape = "x".join(str(v) for v in reversed(pb.cfg.shape) if v > 1)
-        log(f"cpu baseline: oracle on {threads} threads, {args.cpu_steps} steps of {shape}")
-        rate, wall = cpu_baseline(pb, args.cpu_steps, threads)
-        cpu = {"value": rate, "unit": "zone-updates/s", "cores": threads, "kind": "port",
-               "sample": f"{args.cpu_steps} full steps of the same {shape} workload "
-                         f"({wall:.1f} s wall), C FP64 restatement with OpenMP"}
-        log(f"cpu baseline {rate:.4e} zone-updates/s")
-
+    value = res["value"]
+    roof, step, models = roofline_of(cfg, res["kern"], value)
+    step["stream_copy_GBps"] = res["copy_gbps"]
+    if args.precision == "fp64" and args.workload == "ot3d" and args.n == 256 and \
+            args.shock_flattening is None and args.diffusion is None:
+        roof["traffic"], roof["traffic_source"] = counter_traffic("ot3d256_fp64_pmc",
+                                                                  roof["kernel"])
     out = {
-        "metric": "zone-updates/s (plain FP64 path; HBM-roofline fraction)",
-        "value": value,
-        "unit": "zone-updates/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else "f32+f64",
-        "data": data,
-        "config": {"workload": desc, "zones": zones, "sim_time": res["scalars"]["time"],
-                   "max_divB": res["divb"]},
-        "roofline": roofline,
-        "step_roofline": step_roof,
-        "kernels": {k: {"avg_ms": v["avg_ms"], "launches": v["launches"]}
-                    for k, v in kern.items()},
-        "cpu_baseline": cpu,
+        "metric": "zone-updates/s (" + ("plain FP64 path" if args.precision == "fp64" else
+                                       "mixed precision") + "; HBM-roofline fraction)",
+        "value": value, "unit": "zone-updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": res["ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32+f64", "data": data,
+        "config": {"workload": desc, "zones": int(np.prod(cfg.shape)),
+                   "sim_time": res["scalars"]["time"], "max_divB": res["divb"]},
+        "roofline": roof, "step_roofline": step, "kernel_models": models,
     }
     if "eos" in res:
         out["eos"] = res["eos"]
-    if "mlp" in res:
-        out["surrogate"] = res["mlp"]
-    if compare:
-        out["compare"] = compare
-    print(json.dumps(out), flush=True)
+    if "surrogate" in res:
+        out["surrogate"] = res["surrogate"]
+    if args.compare:
+        log("plain FP64 warm-start run on the same initial data")
+        base = run_variant(pb, pb.cfg, args, guess="warm", keep_state=True, profile=False,
+                           total_steps=args.warmup + args.steps + (2 if cfg.eos == 1 else 0)
+                           + args.profile_steps)
+        rel = rel_l1_fields(res["state"], base["state"], pb.cfg)
+        out["compare"] = {"plain_value": base["value"], "plain_ms_per_step": base["ms_per_step"],
+                          "speedup": value / base["value"], "rel_l1_vs_plain_fp64": rel,
+                          "plain_eos": base.get("eos")}
+        log(f"speed-up {out['compare']['speedup']:.3f}x; rel L1 " +
+            " ".join(f"{k} {v:.2e}" for k, v in rel.items()))
+        if "eq_rho_cc" in pb.profiles:
+            out["compare"]["rel_l1_deviations"] = deviation_errors(res["state"], base["state"], pb)
+    return out
+
+
+def bench_c5(pb, desc, data, args):
+    """Plain FP64 against the mixed-precision runs with each T(rho, e) start
+    on the combined workload; the first variant is the headline value."""
+    import numpy as np
+    from phlegethon_amd.config import PREC_MIXED
+    variants = [v.strip() for v in args.variants.split(",") if v.strip()]
+    log("plain FP64 (reference algorithm, Newton from the carried T)")
+    plain = run_variant(pb, pb.cfg, args, guess="warm", keep_state=True,
+                        total_steps=args.l1_steps)
+    roof, step, models = roofline_of(pb.cfg, plain["kern"], plain["value"])
+    step["stream_copy_GBps"] = plain["copy_gbps"]
+    roof["path"] = "plain FP64"
+    roof["traffic"], roof["traffic_source"] = counter_traffic("c5_512_fp64_pmc", roof["kernel"])
+    ref_state = plain.pop("state")
+    mixed_cfg = pb.cfg.with_(precision=PREC_MIXED)
+    runs = {}
+    for v in variants:
+        log(f"mixed precision, T(rho, e) start: {v}")
+        r = run_variant(pb, mixed_cfg, args, guess=v, keep_state=True,
+                        total_steps=args.l1_steps)
+        rel = rel_l1_fields(r.pop("state"), ref_state, pb.cfg)
+        log(f"  speed-up {r['value'] / plain['value']:.3f}x; rel L1 after {args.l1_steps} "
+            "steps " + " ".join(f"{k} {e:.2e}" for k, e in rel.items()))
+        mroof, mstep, mmodels = roofline_of(mixed_cfg, r["kern"], r["value"])
+        runs[v] = {"value": r["value"], "ms_per_step": r["ms_per_step"],
+                   "speedup_vs_plain_fp64": r["value"] / plain["value"],
+                   "rel_l1_vs_plain_fp64": rel, "l1_after_steps": args.l1_steps,
+                   "max_rel_l1": max(rel.values()), "eos": r.get("eos"),
+                   "max_divB": r["divb"], "kernels": mmodels, "step_roofline": mstep}
+        if "surrogate" in r:
+            runs[v]["surrogate"] = r["surrogate"]
+    del ref_state
+    head = variants[0]
+    names = {"mlp": "FP32 + NN-EoS guess", "pred": "FP32 + energy-predictor guess",
+             "warm": "FP32 + carried-T guess"}
+    out = {
+        "metric": "zone-updates/s, plain FP64 vs FP32+NN-EoS guess at fixed L1 err; % HBM peak",
+        "value": runs[head]["value"], "unit": "zone-updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": runs[head]["ms_per_step"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32+f64", "data": data,
+        "config": {"workload": desc + f" | value: {names[head]} (mixed precision)",
+                   "zones": int(np.prod(pb.cfg.shape)), "l1_after_steps": args.l1_steps,
+                   "sim_time_plain": plain["scalars"]["time"], "max_divB_plain": plain["divb"]},
+        "roofline": roof, "step_roofline": step,
+        "plain_fp64": {"value": plain["value"], "ms_per_step": plain["ms_per_step"],
+                       "eos": plain.get("eos"), "kernels": models},
+        "mixed": runs,
+    }
+    return out
 
 
 if __name__ == "__main__":

@@ -240,20 +240,24 @@ int phlx_helm_rhoe_given_guess_batch(phlx_ctx *ctx, int64_t n, const double *rho
                                      int32_t guess_mode, int32_t *iters, int32_t *fallback);
 /* Start of the per-zone T(rho, e) Newton inside the Helmholtz update: 0 the
  * carried T (the reference), 1 the carried T corrected by the surrogate
- * (log10 T_prev + s (y_now - y_prev)), 2 bracketed.  count_iterations != 0
+ * (log10 T_prev + s (y_now - y_prev)), 2 bracketed, 3 the energy predictor
+ * T_prev + (e - e_prev) / c_v of the zone's previous solve (its target e and
+ * c_v kept per zone; a new state, a diffusion step or a mode change start
+ * each zone from the carried T once).  The Newton then converges to the
+ * reference's tolerance from that start.  count_iterations != 0
  * accumulates Newton iterations / solves / fallbacks (phlx_eos_stats).
- * Modes other than 0, and counting, need nx1 % 64 == 0 and nx2 % 4 == 0.
+ * Modes 1 and 2, and counting, need nx1 % 64 == 0 and nx2 % 4 == 0.
  * resid_tol is also the surrogate threshold of
  * phlx_helm_rhoe_given_guess_batch; a context of another EoS may set it once
  * a Helmholtz table is loaded. */
 int phlx_set_eos_guess(phlx_ctx *ctx, int32_t mode, double resid_tol, int32_t count_iterations);
-/* Mixed precision with the Helmholtz EoS: the per-zone T(rho, e) solve makes
- * one FP64 evaluation at its start and, when |e(T) - e| / e < lin_tol
- * (default 1e-5; 0 disables the shortcut), applies the Newton step linearly
- * to T and P instead of iterating (error second order in the residual);
- * larger residuals run the reference Newton.  No counterpart in the
- * reference (its hydro_step always iterates helm_rhoe_given,
- * source/source.F90:24982-25019).  lin_tol in [0, 1e-2]. */
+/* Mixed precision with the Helmholtz EoS: by default (lin_tol 0) the
+ * per-zone T(rho, e) solve iterates the reference's Newton to its tolerance
+ * (helm_rhoe_given, source/source.F90:24982-25019), one FP64 evaluation per
+ * iterate.  lin_tol > 0 selects a shortcut with no counterpart in the
+ * reference: one FP64 evaluation at the start and, when |e(T) - e| / e <
+ * lin_tol, the Newton step applied linearly to T and P instead of iterating
+ * (error second order in the residual).  lin_tol in [0, 1e-2]. */
 int phlx_set_eos_lin_tol(phlx_ctx *ctx, double lin_tol);
 int phlx_eos_stats(phlx_ctx *ctx, int64_t *newton_iters, int64_t *solves, int64_t *fallbacks,
                    int32_t reset);

@@ -251,7 +251,9 @@ extern "C" int phlx_create(const phlx_config *cfg, phlx_ctx **out) {
     e |= dalloc(ctx, &g.gam, 2 * N);
   }
   g.eos_guess = GUESS_WARM;
-  g.eos_lin_tol = 1e-5;
+  // the mixed path converges the T(rho, e) Newton to the reference's
+  // tolerance; phlx_set_eos_lin_tol(> 0) selects the linearised shortcut
+  g.eos_lin_tol = 0.0;
   g.mlp.resid_tol = 0.25f;
   g.mlp.box = 1.02f;
   double *prof = nullptr;
@@ -264,6 +266,8 @@ extern "C" int phlx_create(const phlx_config *cfg, phlx_ctx **out) {
   if (helm) {
     e |= dalloc(ctx, &g.eos_part, 3 * (size_t)g.nred);
     e |= dalloc(ctx, &g.mlp_y, N);
+    e |= dalloc(ctx, &g.eprev, N);
+    e |= dalloc(ctx, &g.cvf, N);
   }
   if (e) {
     phlx_destroy(ctx);
@@ -326,6 +330,15 @@ static int copy_out(phlx_ctx *ctx, void *dst, const void *src, size_t bytes) {
 static int check_launch();
 static int materialize(phlx_ctx *ctx);
 
+// The energy predictor's per-zone records (GUESS_PRED) no longer describe
+// the state (a new state, a diffusion step): NaN bytes make each zone start
+// its next solve from the carried T.
+static int forget_predictor(phlx_ctx *ctx) {
+  if (!ctx->g.eprev) return PHLX_OK;
+  HIPCHK(hipMemsetAsync(ctx->g.eprev, 0xFF, sizeof(double) * ctx->g.N, ctx->stream));
+  return PHLX_OK;
+}
+
 extern "C" int phlx_upload_state(phlx_ctx *ctx, const double *prim, const double *eint,
                                  const double *temp, const double *b1, const double *b2,
                                  const double *b3) {
@@ -350,6 +363,7 @@ extern "C" int phlx_upload_state(phlx_ctx *ctx, const double *prim, const double
   // nothing lags it); a state replaced between steps refreshes the sweeps'
   // FP32 inputs (before phlx_init, init does it)
   if (ctx->kt.prims) launch_flags(g.sc, 0, 0, ctx->stream);
+  if ((rc = forget_predictor(ctx))) return rc;
   if (ctx->initialized && ctx->kt.pack) {
     ctx->kt.pack(ctx->g, ctx->stream);
     rc = check_launch();
@@ -549,6 +563,13 @@ static int check_launch() {
   return PHLX_OK;
 }
 
+// The surrogate (its wave-wide inference), the bracketed start and the
+// iteration counters run in the guess variant of the update; the carried-T
+// and predictor starts in the plain one.
+static bool needs_guess_variant(int mode, bool count) {
+  return count || mode == GUESS_MLP || mode == GUESS_BRACKET;
+}
+
 extern "C" int phlx_stage_begin(phlx_ctx *ctx, int32_t irk) {
   // state0 / b0 capture is fused into the first stage's update and face-B
   // kernels; the stage starts with the shock-flattening flags of the
@@ -594,7 +615,7 @@ extern "C" int phlx_rk_update(phlx_ctx *ctx, int32_t irk, double a1, double a2, 
   ctx->tdiff_valid = false;
   ProfRec r;
   int on = kbegin(ctx, KID_UPDATE, r);
-  const bool guess = ctx->kt.update_guess && (ctx->g.eos_guess != GUESS_WARM || ctx->eos_stats);
+  const bool guess = ctx->kt.update_guess && needs_guess_variant(ctx->g.eos_guess, ctx->eos_stats);
   // stage 1 of the mixed path starts from U when the device flags say it is
   // current (the FP64 conserved state of the last step that ran); a last
   // stage that runs sets them, so a step past tmax leaves them unchanged
@@ -1236,18 +1257,22 @@ extern "C" int phlx_helm_rhoe_given_guess_batch(phlx_ctx *ctx, int64_t n, const 
 extern "C" int phlx_set_eos_guess(phlx_ctx *ctx, int32_t mode, double resid_tol,
                                   int32_t count_iterations) {
   if (!ctx) return PHLX_E_ARG;
-  if (mode < GUESS_WARM || mode > GUESS_BRACKET || !(resid_tol > 0.0)) return PHLX_E_ARG;
+  if (mode < GUESS_WARM || mode > GUESS_PRED || !(resid_tol > 0.0)) return PHLX_E_ARG;
   if (ctx->cfg.eos == PHLX_EOS_IDEAL && !ctx->helm_loaded) return PHLX_E_CONFIG;
   // the surrogate is trained on Helmholtz states (abar, zbar inputs)
   if (mode == GUESS_MLP && (ctx->cfg.eos == PHLX_EOS_PIG || ctx->helm.pig)) return PHLX_E_CONFIG;
   if (mode == GUESS_MLP && !ctx->mlp_ready) return PHLX_E_STATE;
   // the guess variant of the update needs every block full of interior cells
-  if (ctx->cfg.eos != PHLX_EOS_IDEAL && (mode != GUESS_WARM || count_iterations) &&
+  if (ctx->cfg.eos != PHLX_EOS_IDEAL && needs_guess_variant(mode, count_iterations != 0) &&
       (ctx->g.n1 % 64 || ctx->g.n2 % 4))
     return PHLX_E_CONFIG;
   if (mode == GUESS_MLP && ctx->g.eos_guess != GUESS_MLP) {
     // no previous network output yet: all-ones bytes are a NaN pattern
     HIPCHK(hipMemsetAsync(ctx->g.mlp_y, 0xFF, sizeof(float) * ctx->g.N, ctx->stream));
+  }
+  if (mode == GUESS_PRED && ctx->g.eos_guess != GUESS_PRED) {
+    int rc = forget_predictor(ctx);
+    if (rc) return rc;
   }
   ctx->g.eos_guess = mode;
   ctx->g.mlp.resid_tol = (float)resid_tol;
@@ -1396,7 +1421,9 @@ static int diffusion_step(phlx_ctx *ctx, int strang) {
     launch_flags(g.sc, ~SC_U_CUR, 0, ctx->stream);
   }
   kend(ctx, on, r);
-  int rc = check_launch();
+  int rc = helm ? forget_predictor(ctx) : PHLX_OK;
+  if (rc) return rc;
+  rc = check_launch();
   if (rc) return rc;
   ctx->tdiff_valid = true;
   ctx->diff_stages = s;

@@ -72,15 +72,19 @@ struct Grid {
   double *temp, *gam;
   const double *eq_ge_cc, *eq_ge_f;
   HelmDev helm;
-  // T(rho, e) start: GUESS_WARM (the reference), GUESS_MLP, GUESS_BRACKET;
+  // T(rho, e) start: GUESS_WARM (the reference), GUESS_MLP, GUESS_BRACKET,
+  // GUESS_PRED;
   // per-block Newton iteration / zone / fallback counters of the guess
   // variant of the update
   MlpDev mlp;
   int eos_guess;
   int eos_count;           // accumulate the counters (the guess variant only)
-  double eos_lin_tol;      // mixed precision: residual below which the EoS step is linear
+  double eos_lin_tol;      // mixed precision: > 0 selects the linearised one-evaluation EoS step
   unsigned int *eos_part;
   float *mlp_y;            // network output at each zone's last solve (NaN: none yet)
+  // GUESS_PRED: each zone's last T(rho, e) target e (NaN: none) and c_v there
+  double *eprev;
+  float *cvf;
   // USE_SHOCK_FLATTENING: detector bytes on cells -2 .. n+1 and flags on
   // cells -1 .. n of every axis (x3 only in 3D); sflat 0 = off
   int sflat;

@@ -74,7 +74,7 @@ extern "C" int phlx_h5_write_grid(const char *dir, const phlx_config *cfg, int64
                                   double time, double dt, const double *prim,
                                   const double *temp, const double *b_cc,
                                   const double *grav_x2, const double *phi_x2,
-                                  const double *edot_x2) {
+                                  const double *edot_x2, const double *kappa) {
   if (!dir || !cfg || !prim || step < 0) return -1;
   const int sd = cfg->sdims;
   const hsize_t n1 = cfg->nx1, n2 = cfg->nx2, n3 = sd == 3 ? cfg->nx3 : 1;
@@ -82,6 +82,10 @@ extern "C" int phlx_h5_write_grid(const char *dir, const phlx_config *cfg, int64
   const int nv = sd + 2 + cfg->nas;
   const bool mhd = cfg->flux == PHLX_HLLD_FLUX || cfg->flux == PHLX_LHLLD_FLUX;
   const bool helm = cfg->eos == PHLX_EOS_HELMHOLTZ;
+  const bool pig = cfg->eos == PHLX_EOS_PIG;
+  // ADVECT_YE_IABAR: the two advected scalars of a tabulated EoS (Helmholtz
+  // always carries them, PIG may)
+  const bool yeiabar = cfg->eos != PHLX_EOS_IDEAL && cfg->nas >= 2;
   char name[64];
   std::snprintf(name, sizeof(name), "/grid_n%05lld.h5", (long long)step);
   const std::string path = std::string(dir) + name;
@@ -112,12 +116,12 @@ extern "C" int phlx_h5_write_grid(const char *dir, const phlx_config *cfg, int64
       scalar_i32(g, "nx1", cfg->nx1, e);
       scalar_i32(g, "nx2", cfg->nx2, e);
       scalar_i32(g, "nx3", cfg->nx3, e);
-      attr_str(g, "advect_yeiabar", helm ? "true" : "false", e);
+      attr_str(g, "advect_yeiabar", yeiabar ? "true" : "false", e);
       attr_str(g, "advect_species", "false", e);
       attr_str(g, "use_prad", "false", e);
       attr_str(g, "use_helmholtz", helm ? "true" : "false", e);
       if (helm) attr_str(g, "use_coulomb_corrections", "false", e);
-      attr_str(g, "use_pig", "false", e);
+      attr_str(g, "use_pig", pig ? "true" : "false", e);
       attr_str(g, "use_gravity_solver", "false", e);
       attr_str(g, "use_monopole_solver", "false", e);
       attr_str(g, "geometry-type", "cartesian", e);
@@ -134,6 +138,11 @@ extern "C" int phlx_h5_write_grid(const char *dir, const phlx_config *cfg, int64
           }
       attr_str(g, "update_kappa", "false", e);
       array_f64(g, "coords", buf, n3, n2, n1, sd, e);
+      if (kappa) {
+        // THERMAL_DIFFUSION_STS without USE_TIMMES_KAPPA (:3070-3076)
+        std::vector<double> s(kappa, kappa + N);
+        array_f64(g, "kappa", s, n3, n2, n1, 0, e);
+      }
       if (cfg->use_gravity && grav_x2) {
         // plane-parallel gravity: grav(2, i, j, k) = g(j), other components 0
         for (hsize_t c = 0; c < N * sd; c++) buf[c] = 0.0;

@@ -193,6 +193,48 @@ __device__ __forceinline__ void helm_sums36(const HelmDev &h, const HelmCell &c,
   }
 }
 
+// Mixed-precision EoS only: the same biquintic sums contracted separably.
+// The cell's 36 table values times the rho basis R_k are first summed over
+// the rho order and corner, G_k[ty][yc] = sum_{rx, xc} q(rx, ty; xc, yc)
+// R_k(rx, xc) (9 entries x 4 corners), then every sum pairing R_k with a T
+// basis is sum_{ty, yc} G_k[ty][yc] T(ty, yc) (6 terms): 36 + 6 fused
+// multiply-adds per sum where helm_sums36 takes 72 operations.  The terms
+// are added in another order than the reference's (round-off level).
+template <int NR>
+__device__ __forceinline__ void helm_contract_rho(const HelmDev &h, const HelmCell &c,
+                                                  const HelmBasis *const (&R)[NR],
+                                                  double (&G)[NR][6]) {
+  const double *n00 = h.tab + (size_t)kHelmRec * ((size_t)c.jh * h.nr + c.ih);
+  const double *n10 = n00 + kHelmRec;
+  const double *n01 = n00 + (size_t)kHelmRec * h.nr;
+  const double *n11 = n01 + kHelmRec;
+  constexpr int RX[9] = {0, 0, 0, 1, 2, 1, 2, 1, 2};
+  constexpr int TY[9] = {0, 1, 2, 0, 0, 1, 1, 2, 2};
+#pragma unroll
+  for (int v = 0; v < 9; v++) {
+    const double q0 = n00[v], q1 = n10[v], q2 = n01[v], q3 = n11[v];
+    const int lo = 2 * TY[v], hi = lo + 1;
+#pragma unroll
+    for (int k = 0; k < NR; k++) {
+      const double a = R[k]->a[RX[v]], am = R[k]->am[RX[v]];
+      // entries 0..2 (rho order 0) are the first of their T order
+      G[k][lo] = (v < 3) ? q0 * a : fma(q0, a, G[k][lo]);
+      G[k][lo] = fma(q1, am, G[k][lo]);
+      G[k][hi] = (v < 3) ? q2 * a : fma(q2, a, G[k][hi]);
+      G[k][hi] = fma(q3, am, G[k][hi]);
+    }
+  }
+}
+__device__ __forceinline__ double helm_contract_T(const double (&G)[6], const HelmBasis &T) {
+  double s = G[0] * T.a[0];
+  s = fma(G[1], T.am[0], s);
+  s = fma(G[2], T.a[1], s);
+  s = fma(G[3], T.am[1], s);
+  s = fma(G[4], T.a[2], s);
+  s = fma(G[5], T.am[2], s);
+  return s;
+}
+
 // Bicubic Hermite of the dP/d(rho*Ye) block (source/source.F90:24078-24125).
 template <bool FMA = false>
 __device__ __forceinline__ double helm_sum16(const HelmDev &h, const HelmCell &c) {
@@ -248,7 +290,16 @@ __device__ __forceinline__ void helm_full(const HelmDev &h, double rho, double T
   helm_dbasis<FMA>(c.y, c.dT, dt);
   helm_ddbasis<FMA>(c.y, c.dT, ddt);
   double sm[5];
-  {
+  if constexpr (FMA) {
+    const HelmBasis *const RR[2] = {&r, &dr};
+    double G[2][6];
+    helm_contract_rho<2>(h, c, RR, G);
+    sm[0] = helm_contract_T(G[0], t);
+    sm[1] = helm_contract_T(G[1], t);
+    sm[2] = helm_contract_T(G[0], dt);
+    sm[3] = helm_contract_T(G[1], dt);
+    sm[4] = helm_contract_T(G[0], ddt);
+  } else {
     const HelmBasis *const RR[5] = {&r, &dr, &r, &dr, &r};
     const HelmBasis *const TT[5] = {&t, &t, &dt, &dt, &ddt};
     helm_sums36<5, FMA>(h, c, RR, TT, sm);
@@ -284,6 +335,133 @@ __device__ __forceinline__ void helm_full(const HelmDev &h, double rho, double T
   cv = dE_dT;
   sound = kHelmClight * sqrt(hdiv<FMA>(gam1, z));
   if (dPdT_out) *dPdT_out = dP_dT;
+}
+
+// ---------------------------------------------------------------------------
+// Mixed-precision EoS: one cell of the table contracted over rho, reused by
+// every Newton iterate of a solve.  Within a cell the interpolant is a
+// polynomial in the T coordinate y (quintic for f, cubic for dP/drho), so
+// once the 36 (16) table values are summed against the rho basis, f and its
+// derivatives at any T are 6 (4) products with the T basis; the evaluation
+// is the same interpolant in FP64, the terms added in another order than the
+// reference's.  A Newton iterate that leaves the cell by more than 1 % of
+// its width relocates; closer iterates keep the polynomial, whose difference
+// from the neighbouring cell's is third order in the distance (the quintic
+// Hermite interpolants share value, slope and curvature at the node).
+struct HelmMx {
+  int ih, jh;
+  double x, drho, dT, Tj;
+  double Ye, rhos, iabar;          // Ye, rho Ye, 1/abar (0: PIG)
+  double Gr[6], Gdr[6], Hc[4];     // f against the rho basis and its slope; dP/drho block
+};
+
+__device__ __forceinline__ void helm_mx_contract(const HelmDev &h, double T, HelmMx &m) {
+  const HelmCell c = helm_locate<true>(h, m.rhos, T);
+  m.ih = c.ih;
+  m.jh = c.jh;
+  m.x = c.x;
+  m.drho = c.drho;
+  m.dT = c.dT;
+  m.Tj = h.T[c.jh];
+  HelmBasis r, dr;
+  helm_basis(c.x, c.drho, r);
+  helm_dbasis<true>(c.x, c.drho, dr);
+  {
+    const HelmBasis *const RR[2] = {&r, &dr};
+    double G[2][6];
+    helm_contract_rho<2>(h, c, RR, G);
+#pragma unroll
+    for (int q = 0; q < 6; q++) {
+      m.Gr[q] = G[0][q];
+      m.Gdr[q] = G[1][q];
+    }
+  }
+  // bicubic dP/d(rho Ye) block: entries c, dc/dT, dc/drho, d2c/drhodT pair
+  // the rho basis of order (0, 0, 1, 1) with the T basis of order (0, 1, 0, 1)
+  const double x = c.x, o = 1.0 - x;
+  const double x2 = x * x, x3 = x2 * x, o2 = o * o, o3 = o2 * o;
+  const double rb[2] = {2.0 * x3 - 3.0 * x2 + 1.0, (x3 - 2.0 * x2 + x) * c.drho};
+  const double rbm[2] = {2.0 * o3 - 3.0 * o2 + 1.0, -(o3 - 2.0 * o2 + o) * c.drho};
+  const double *n00 = h.tab + (size_t)kHelmRec * ((size_t)c.jh * h.nr + c.ih) + 9;
+  const double *n10 = n00 + kHelmRec;
+  const double *n01 = n00 + (size_t)kHelmRec * h.nr;
+  const double *n11 = n01 + kHelmRec;
+  constexpr int RXc[4] = {0, 0, 1, 1}, TYc[4] = {0, 1, 0, 1};
+#pragma unroll
+  for (int v = 0; v < 4; v++) {
+    const int lo = 2 * TYc[v], hi = lo + 1;
+    const double a = rb[RXc[v]], am = rbm[RXc[v]];
+    m.Hc[lo] = (v < 2) ? n00[v] * a : fma(n00[v], a, m.Hc[lo]);
+    m.Hc[lo] = fma(n10[v], am, m.Hc[lo]);
+    m.Hc[hi] = (v < 2) ? n01[v] * a : fma(n01[v], a, m.Hc[hi]);
+    m.Hc[hi] = fma(n11[v], am, m.Hc[hi]);
+  }
+}
+
+__device__ __forceinline__ void helm_mx_setup(const HelmDev &h, double rho, double T, double abar,
+                                              double zbar, HelmMx &m) {
+  m.Ye = h.pig ? 1.0 : hdiv<true>(zbar, abar);
+  m.rhos = rho * m.Ye;
+  m.iabar = h.pig ? 0.0 : hdiv<true>(1.0, abar);
+  helm_mx_contract(h, T, m);
+}
+
+// E and c_v (and with FULL: P, dP/dT, the sound speed) at T from a
+// contracted cell: the operations of helm_full / helm_given2 after their
+// sums (ions, radiation, relativistic sound speed).
+template <bool FULL>
+__device__ __forceinline__ void helm_mx_eval(const HelmDev &h, HelmMx &m, double rho, double T,
+                                             double &E, double &cv, double &P, double &dPdT,
+                                             double &sound) {
+  double y = hdiv<true>(T - m.Tj, m.dT);
+  if (y < -0.01 || y > 1.01) {
+    helm_mx_contract(h, T, m);
+    y = hdiv<true>(T - m.Tj, m.dT);
+  }
+  HelmBasis t, dt, ddt;
+  helm_basis(y, m.dT, t);
+  helm_dbasis<true>(y, m.dT, dt);
+  helm_ddbasis<true>(y, m.dT, ddt);
+  const double f = helm_contract_T(m.Gr, t);
+  const double df_dT = helm_contract_T(m.Gr, dt);
+  const double df_dT2 = helm_contract_T(m.Gr, ddt);
+  const double Ye = m.Ye, tmp = m.iabar;
+  const double s = -Ye * df_dT;
+  const double ds_dT = -Ye * df_dT2;
+  double e = Ye * f + T * s;
+  double dE_dT = T * ds_dT;
+  const double T2 = T * T, T3 = T2 * T, T4 = T3 * T;
+  e = e + 1.5 * kHelmRgas * T * tmp + hdiv<true>(kHelmCrad * T4, rho);
+  dE_dT = dE_dT + 1.5 * kHelmRgas * tmp + hdiv<true>(4.0 * kHelmCrad * T3, rho);
+  E = e;
+  cv = dE_dT;
+  if (!FULL) return;
+  const double df_drho = helm_contract_T(m.Gdr, t);
+  const double df_drhodT = helm_contract_T(m.Gdr, dt);
+  const double oy = 1.0 - y, y2 = y * y, y3 = y2 * y, oy2 = oy * oy, oy3 = oy2 * oy;
+  double c3 = m.Hc[0] * (2.0 * y3 - 3.0 * y2 + 1.0);
+  c3 = fma(m.Hc[1], 2.0 * oy3 - 3.0 * oy2 + 1.0, c3);
+  c3 = fma(m.Hc[2], (y3 - 2.0 * y2 + y) * m.dT, c3);
+  c3 = fma(m.Hc[3], -(oy3 - 2.0 * oy2 + oy) * m.dT, c3);
+  double dP_drho = Ye * c3;
+  const double rr = m.rhos * m.rhos;
+  double p = rr * df_drho;
+  double dP_dT = rr * df_drhodT;
+  const double pion = kHelmRgas * rho * T * tmp;
+  const double dpiondt = kHelmRgas * rho * tmp;
+  const double dpiondd = kHelmRgas * T * tmp;
+  p = p + pion + kHelmCrad * T4 * (1.0 / 3.0);
+  dP_dT = dP_dT + dpiondt + (4.0 / 3.0) * kHelmCrad * T3;
+  dP_drho = dP_drho + dpiondd;
+  const double zz = hdiv<true>(p, rho);
+  const double chiT = hdiv<true>(T, p) * dP_dT;
+  const double chirho = hdiv<true>(dP_drho, zz);
+  const double xx = hdiv<true>(zz * chiT, T * dE_dT);
+  const double gam1 = chiT * xx + chirho;
+  const double z = 1.0 + hdiv<true>(e + kHelmClight2, zz);
+  P = p;
+  dPdT = dP_dT;
+  sound = kHelmClight * sqrt(hdiv<true>(gam1, z));
 }
 
 // helm_rhoT_given1: P and (when ret) dP/dT.
@@ -323,17 +501,27 @@ __device__ __forceinline__ void helm_given1(const HelmDev &h, double rho, double
   dPdT = dP_dT;
 }
 
-// helm_rhoT_given2: E and (when ret) dE/dT.
+// helm_rhoT_given2: E and (when ret) dE/dT.  FMA (the mixed-precision EoS
+// only): the separable contraction, refined reciprocals, FP32 cell lookup.
+template <bool FMA = false>
 __device__ __forceinline__ void helm_given2(const HelmDev &h, double rho, double T, double abar,
                                             double zbar, double &E, double &dEdT, bool ret) {
-  const double Ye = h.pig ? 1.0 : zbar / abar, rhos = rho * Ye;
-  const HelmCell c = helm_locate(h, rhos, T);
+  const double Ye = h.pig ? 1.0 : hdiv<FMA>(zbar, abar), rhos = rho * Ye;
+  const HelmCell c = helm_locate<FMA>(h, rhos, T);
   HelmBasis r, t, dt, ddt;
   helm_basis(c.x, c.drho, r);
   helm_basis(c.y, c.dT, t);
-  helm_dbasis(c.y, c.dT, dt);
+  helm_dbasis<FMA>(c.y, c.dT, dt);
   double f, df_dT, df_dT2 = 0.0;
-  if (ret) {
+  if (FMA) {
+    helm_ddbasis<FMA>(c.y, c.dT, ddt);
+    const HelmBasis *const RR[1] = {&r};
+    double G[1][6];
+    helm_contract_rho<1>(h, c, RR, G);
+    f = helm_contract_T(G[0], t);
+    df_dT = helm_contract_T(G[0], dt);
+    df_dT2 = helm_contract_T(G[0], ddt);
+  } else if (ret) {
     helm_ddbasis(c.y, c.dT, ddt);
     const HelmBasis *const RR[3] = {&r, &r, &r};
     const HelmBasis *const TT[3] = {&t, &dt, &ddt};
@@ -355,9 +543,9 @@ __device__ __forceinline__ void helm_given2(const HelmDev &h, double rho, double
   double e = Ye * f + T * s;
   double dE_dT = T * ds_dT;
   const double T2 = T * T, T3 = T2 * T, T4 = T3 * T;
-  const double tmp = h.pig ? 0.0 : 1.0 / abar;
-  e = e + 1.5 * kHelmRgas * T * tmp + kHelmCrad * T4 / rho;
-  dE_dT = dE_dT + 1.5 * kHelmRgas * tmp + 4.0 * kHelmCrad * T3 / rho;
+  const double tmp = h.pig ? 0.0 : hdiv<FMA>(1.0, abar);
+  e = e + 1.5 * kHelmRgas * T * tmp + hdiv<FMA>(kHelmCrad * T4, rho);
+  dE_dT = dE_dT + 1.5 * kHelmRgas * tmp + hdiv<FMA>(4.0 * kHelmCrad * T3, rho);
   E = e;
   dEdT = dE_dT;
 }
@@ -366,15 +554,19 @@ __device__ __forceinline__ void helm_given2(const HelmDev &h, double rho, double
 // residual of the incoming T is tested after the update, so the returned T is
 // one Newton step past the converged one; a T outside [Tl, Th] resets to the
 // start value and ends the loop.  Returns the iterations taken (<= 100).
+// cv_last (optional): dE/dT of the last Newton evaluation (the energy
+// predictor's slope for the next solve of the zone).
 __device__ __forceinline__ int helm_rhoe_given(const HelmDev &h, double rho, double E0,
                                                double abar, double zbar, double &T, double &P,
-                                               double &sound, bool ret) {
+                                               double &sound, bool ret,
+                                               double *cv_last = nullptr) {
   const double Told = T;
   int iter;
   sound = 0.0;
   for (iter = 1; iter <= kHelmMaxIter; iter++) {
     double e, dEdT;
     helm_given2(h, rho, T, abar, zbar, e, dEdT, true);
+    if (cv_last) *cv_last = dEdT;
     const double res = e - E0;
     T = T - res / dEdT;
     double err = fabs(res / E0);

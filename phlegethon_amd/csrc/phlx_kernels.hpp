@@ -1129,31 +1129,37 @@ __global__ void __launch_bounds__(256)
       const double zbar = ye * abar;
       double pr, sound;
       T = g.temp[c];
+      const double e0 = eint * ir;
+      // GUESS_PRED: start from the energy predictor of the zone's last solve
+      const bool pred = g.eos_guess == GUESS_PRED;
+      double tg = T, cvl = 0.0;
+      if (pred) tg = helm_pred_start(T, e0, g.eprev[c], g.cvf[c]);
       if constexpr (GUESS) {
         __shared__ unsigned int lds[4 * kMlpLdsWaveDw];
         __shared__ unsigned int cnt[3];
         const int mode = g.eos_guess;
-        double tg = T;
         if (mode == GUESS_MLP) {
           // corrected warm start: T_prev * 10^(s (y_now - y_prev)); the
           // previous output is NaN before the first solve of a zone
           float x[4];
           bool ok;
-          mlp_inputs(g.mlp, rho, eint * ir, abar, zbar, x, ok);
+          mlp_inputs(g.mlp, rho, e0, abar, zbar, x, ok);
           const float y = mlp_forward_wave(g.mlp, x, lds + threadIdx.y * kMlpLdsWaveDw);
           const float yp = g.mlp_y[c];
           if (ok && yp == yp) tg = T * exp10((double)(g.mlp.out_scale * (y - yp)));
           g.mlp_y[c] = ok ? y : __int_as_float(0x7fffffff);
         }
-        int fb, it;
+        int fb = 0, it;
         if constexpr (MIX) {
-          if (mode == GUESS_BRACKET) tg = helm_bracket_guess(g.helm, rho, eint * ir, abar, zbar);
+          if (mode == GUESS_BRACKET) tg = helm_bracket_guess(g.helm, rho, e0, abar, zbar);
           if (tg >= g.helm.tl && tg <= g.helm.th) T = tg;
-          it = helm_rhoe_lin(g.helm, g.eos_lin_tol, rho, eint * ir, abar, zbar, T, pr, sound,
-                             fb);
+          if (g.eos_lin_tol > 0.0)
+            it = helm_rhoe_lin(g.helm, g.eos_lin_tol, rho, e0, abar, zbar, T, pr, sound, fb, cvl);
+          else
+            it = helm_rhoe_mixed(g.helm, rho, e0, abar, zbar, T, pr, sound, cvl);
         } else {
-          it = helm_rhoe_guess(g.helm, mode, tg, g.mlp.resid_tol, rho, eint * ir, abar, zbar, T,
-                               pr, sound, g.fasteos != 0, fb);
+          it = helm_rhoe_guess(g.helm, mode, tg, g.mlp.resid_tol, rho, e0, abar, zbar, T, pr,
+                               sound, g.fasteos != 0, fb, &cvl);
         }
         if (g.eos_count) {  // uniform across the grid
           const int tid = threadIdx.y * blockDim.x + threadIdx.x;
@@ -1170,11 +1176,24 @@ __global__ void __launch_bounds__(256)
           }
         }
       } else if constexpr (MIX) {
+        // the reference Newton to its tolerance from the carried T (or the
+        // predictor's start), every iterate one FP64 evaluation; a positive
+        // eos_lin_tol selects the one-evaluation linearised step instead
+        if (tg >= g.helm.tl && tg <= g.helm.th) T = tg;
         int fb;
-        helm_rhoe_lin(g.helm, g.eos_lin_tol, rho, eint * ir, abar, zbar, T, pr, sound, fb);
+        if (g.eos_lin_tol > 0.0)
+          helm_rhoe_lin(g.helm, g.eos_lin_tol, rho, e0, abar, zbar, T, pr, sound, fb, cvl);
+        else
+          helm_rhoe_mixed(g.helm, rho, e0, abar, zbar, T, pr, sound, cvl);
       } else {
         // without FASTEOS the last evaluation is pressure only (:13654-13655)
-        helm_rhoe_given(g.helm, rho, eint * ir, abar, zbar, T, pr, sound, g.fasteos != 0);
+        if (pred && tg >= g.helm.tl && tg <= g.helm.th) T = tg;
+        helm_rhoe_given(g.helm, rho, e0, abar, zbar, T, pr, sound, g.fasteos != 0,
+                        pred ? &cvl : nullptr);
+      }
+      if (pred) {
+        g.eprev[c] = e0;
+        g.cvf[c] = (float)cvl;
       }
       helm_pin(pr);
       helm_pin(sound);
@@ -1609,9 +1628,13 @@ struct Kern {
 template <int SD, int NAS, bool MHD, bool HELM, bool MIX, bool R5,
           int EK = HELM ? EK_FASTEOS : EK_IDEAL>
 static KernelTable make_table() {
+  // the LIM5TH families differ only in their sweeps: their tables carry the
+  // flux launcher alone and take the rest from the 2nd-order family's unit
+  // (so its update, CT and EoS kernels are built once)
   using K = Kern<SD, NAS, MHD, HELM, MIX, R5, EK>;
-  KernelTable t;
+  KernelTable t{};
   t.flux = &K::flux;
+  if constexpr (R5) return t;
   t.corner = &K::corner;
   t.faceb = &K::faceb;
   t.update = &K::update;

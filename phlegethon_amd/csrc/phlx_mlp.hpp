@@ -33,7 +33,11 @@ constexpr int kMlpOffW1 = 0, kMlpOffB1 = 256, kMlpOffW2 = 320, kMlpOffB2 = 4416,
               kMlpOffW3 = 4480, kMlpOffB3 = 4544, kMlpNParam = 4545;
 constexpr int kBracketSteps = 16;  // bisection steps of the fallback guess
 
-enum { GUESS_WARM = 0, GUESS_MLP = 1, GUESS_BRACKET = 2 };
+// T(rho, e) starts: the carried T (the reference), the surrogate's
+// corrected warm start, the bracketed bisection, and the energy predictor
+// T + (e - e_prev) / c_v from the zone's previous solve (its target e_prev
+// and slope c_v stored per zone).
+enum { GUESS_WARM = 0, GUESS_MLP = 1, GUESS_BRACKET = 2, GUESS_PRED = 3 };
 
 typedef short mlp_s4 __attribute__((ext_vector_type(4)));
 typedef float mlp_f4 __attribute__((ext_vector_type(4)));
@@ -203,8 +207,9 @@ __device__ __forceinline__ int helm_rhoe_given_from(const HelmDev &h, double rho
   return iter > kHelmMaxIter ? kHelmMaxIter : iter;
 }
 
-// Mixed-precision T(rho, e): ONE FP64 evaluation (helm_rhoT_given_full
-// plus dP/dT) at the start T.  When its energy residual |e(T) - e0| / e0 is
+// Mixed-precision T(rho, e), linearised (opt-in, phlx_set_eos_lin_tol > 0;
+// an algorithmic shortcut, not the reference's convergence test): ONE FP64
+// evaluation (helm_rhoT_given_full plus dP/dT) at the start T.  When its energy residual |e(T) - e0| / e0 is
 // below lin_tol, T and P take that Newton step linearly, dT = -(e - e0) / c_v,
 // T += dT, P += dP/dT dT, which leaves an error of second order in the
 // residual (~1e-10 relative for a 1e-5 residual); the sound speed stays that
@@ -214,9 +219,11 @@ __device__ __forceinline__ int helm_rhoe_given_from(const HelmDev &h, double rho
 // reference loop ran.
 __device__ __forceinline__ int helm_rhoe_lin(const HelmDev &h, double lin_tol, double rho,
                                              double E0, double abar, double zbar, double &T,
-                                             double &P, double &sound, int &fallback) {
+                                             double &P, double &sound, int &fallback,
+                                             double &cv_out) {
   double E, cv, dPdT;
   helm_full<true>(h, rho, T, abar, zbar, P, E, sound, cv, &dPdT);
+  cv_out = cv;
   const double res = E - E0;
   const double dT = hdiv<true>(-res, cv);
   const double T1 = T + dT;
@@ -232,6 +239,69 @@ __device__ __forceinline__ int helm_rhoe_lin(const HelmDev &h, double lin_tol, d
   return 1 + helm_rhoe_given(h, rho, E0, abar, zbar, T, P, sound, true);
 }
 
+// Mixed-precision T(rho, e) converged to the reference's tolerance: the
+// Newton loop of helm_rhoe_given (T -= (e(T) - e0) / c_v; the loop ends once
+// |e(T) - e0| / e0 < helm_tol, the step of that iteration applied; a T
+// outside the table resets to the start and ends it), every iterate an FP64
+// evaluation of the table interpolant from ONE contraction of the cell over
+// rho (HelmMx; refined reciprocals, FP32 cell lookup).  The reference closes
+// with a helm_full at the final T; here P takes the final step linearly,
+// P(T_k) + dP/dT (T - T_k): that step is below the tolerance, so the
+// linearisation error (~(dT / T)^2 ~ 1e-22 relative) is below FP64
+// resolution, and the sound speed is the last iterate's (relative
+// difference ~dT / T; it feeds the FP32 wave speeds and the CFL).  After a
+// reset the closing evaluation is made at the start T.  Returns the Newton
+// iterations (as helm_rhoe_given); cv_out receives the last iterate's c_v.
+__device__ __forceinline__ int helm_rhoe_mixed(const HelmDev &h, double rho, double E0,
+                                               double abar, double zbar, double &T, double &P,
+                                               double &sound, double &cv_out) {
+  const double Told = T;
+  HelmMx m;
+  helm_mx_setup(h, rho, T, abar, zbar, m);
+  int it = 0;
+#pragma unroll 1
+  for (;;) {
+    double E, cv, dPdT, Pk, sk;
+    helm_mx_eval<true>(h, m, rho, T, E, cv, Pk, dPdT, sk);
+    it++;
+    const double res = E - E0;
+    double Tn = T - hdiv<true>(res, cv);
+    double err = fabs(hdiv<true>(res, E0));
+    bool reset = false;
+    if ((Tn < h.tl) || (Tn > h.th)) {
+      Tn = Told;
+      err = 1e-15;
+      reset = true;
+    }
+    cv_out = cv;
+    if (err < kHelmTol || it >= kHelmMaxIter) {
+      if (reset) {
+        double e2, cv2, d2;
+        helm_mx_eval<true>(h, m, rho, Tn, e2, cv2, P, d2, sound);
+      } else {
+        P = Pk + dPdT * (Tn - T);
+        sound = sk;
+      }
+      T = Tn;
+      return it;
+    }
+    T = Tn;
+  }
+}
+
+// The energy predictor's start for a zone whose previous solve had target
+// e_prev, slope c_v and result T: T + (e0 - e_prev) / c_v (the previous
+// residual is below the tolerance, so the error is second order in the
+// change of e).  A missing record (NaN) or a predicted change above 10 %
+// keeps T.
+__device__ __forceinline__ double helm_pred_start(double T, double e0, double e_prev,
+                                                  float cvf) {
+  const double cv = (double)cvf;
+  if (!(cv > 0.0) || !(e_prev == e_prev)) return T;
+  const double tg = T + hdiv<true>(e0 - e_prev, cv);
+  return (fabs(tg - T) <= 0.1 * T) ? tg : T;
+}
+
 // T(rho, e) with a chosen start (GUESS_WARM: T as given, the reference;
 // GUESS_BRACKET: the bisection guess; GUESS_MLP: tg, the surrogate's guess).
 // A surrogate guess is judged by the first Newton evaluation itself: if it
@@ -241,8 +311,13 @@ __device__ __forceinline__ int helm_rhoe_lin(const HelmDev &h, double lin_tol, d
 __device__ __forceinline__ int helm_rhoe_guess(const HelmDev &h, int mode, double tg,
                                                float resid_tol, double rho, double E0,
                                                double abar, double zbar, double &T, double &P,
-                                               double &sound, bool ret, int &fallback) {
+                                               double &sound, bool ret, int &fallback,
+                                               double *cv_last = nullptr) {
   fallback = 0;
+  if (mode == GUESS_PRED) {
+    if (tg >= h.tl && tg <= h.th) T = tg;
+    return helm_rhoe_given(h, rho, E0, abar, zbar, T, P, sound, ret, cv_last);
+  }
   if (mode == GUESS_MLP) {
     // a guess outside the table's T range is clamped into it (in degenerate
     // matter log T is ill-determined and any T there has a small energy

@@ -5,7 +5,12 @@
 namespace phlx {
 
 bool select_fp64_r5(int sd, int nas, bool mhd, bool helm, bool fasteos, KernelTable &t) {
-  return select_prec<false, true>(sd, nas, mhd, helm, fasteos, t);
+  KernelTable base{}, r5{};
+  if (!select_fp64(sd, nas, mhd, helm, fasteos, base)) return false;
+  if (!select_prec<false, true>(sd, nas, mhd, helm, fasteos, r5)) return false;
+  base.flux = r5.flux;   // LIM5TH sweeps; update, CT and EoS kernels shared
+  t = base;
+  return true;
 }
 
 }  // namespace phlx

@@ -324,12 +324,15 @@ def random_smooth(cfg: PhlxConfig, seed: int = SEED, amp: float = 0.2) -> Proble
 
 def _random_div_free(cfg: PhlxConfig, seed: int, amp: float = 0.3):
     """Face field b = curl A with A on cell edges (discrete div b == 0).
-    Coordinates are normalised by the domain lengths, so the modes are
-    periodic on any box (an exact no-op on the unit box)."""
+    The modes are functions of coordinates normalised by each domain length,
+    so they are periodic on any box; the differences are divided by the cell
+    widths in units of the x1 length (one common scale, so the discrete
+    divergence cancels on boxes of any aspect; an exact no-op on the unit
+    box and on cubes)."""
     nz, ny, nx = cfg.shape
     lo = (cfg.x1l, cfg.x2l, cfg.x3l)
     ln = (cfg.x1u - cfg.x1l, cfg.x2u - cfg.x2l, cfg.x3u - cfg.x3l)
-    dx1, dx2, dx3 = (cfg.dx[d] / ln[d] for d in range(3))
+    dx1, dx2, dx3 = (cfg.dx[d] / ln[0] for d in range(3))
     tp = 2.0 * math.pi
 
     def nodes(cfg, d):  # noqa: F811  (normalised node / centre coordinates)

@@ -123,7 +123,7 @@ def _chk(rc: int, what: str):
 
 
 MLP_NPARAM = 4545
-GUESS_MODES = {"warm": 0, "mlp": 1, "bracket": 2}
+GUESS_MODES = {"warm": 0, "mlp": 1, "bracket": 2, "pred": 3}
 
 KERNELS = {"flux_x1": 0, "flux_x2": 1, "flux_x3": 2, "corner": 3, "faceb": 4,
            "update": 5, "misc": 6, "diffusion": 7}
@@ -251,7 +251,8 @@ class Simulation:
         HDF5 layout (phlegethon_amd.output); ``profiles``: the problem's x2
         profiles for the step-0 statics (grav, phi_cc, edot)."""
         from . import output
-        return output.write_grid(directory, self.cfg, self.scalars(), self.download(), profiles)
+        return output.write_grid(directory, self.cfg, self.scalars(), self.download(), profiles,
+                                 getattr(self, "_kappa_host", None))
 
     def hydro_step(self):
         _chk(self.lib.phlx_hydro_step(self._ctx), "phlx_hydro_step")
@@ -358,14 +359,17 @@ class Simulation:
 
     def set_eos_guess(self, mode: str = "warm", resid_tol: float = 0.25, count: bool = False):
         """T(rho, e) start of the per-zone Newton: "warm" (the carried T, as
-        the reference), "mlp" (surrogate, bracketed fallback) or "bracket"."""
+        the reference), "mlp" (surrogate, bracketed fallback), "bracket" or
+        "pred" (the carried T moved by (e - e_prev) / c_v of the zone's
+        previous solve)."""
         _chk(self.lib.phlx_set_eos_guess(self._ctx, GUESS_MODES[mode], float(resid_tol),
                                          int(count)), "phlx_set_eos_guess")
 
-    def set_eos_lin_tol(self, lin_tol: float = 1e-5):
-        """Mixed precision + Helmholtz: energy residual below which the
-        per-zone T(rho, e) solve takes its Newton step linearly after one
-        FP64 evaluation (0: always the reference Newton)."""
+    def set_eos_lin_tol(self, lin_tol: float = 0.0):
+        """Mixed precision + Helmholtz: 0 (default) converges the per-zone
+        T(rho, e) Newton to the reference's tolerance; a positive value is
+        the energy residual below which the solve instead takes its Newton
+        step linearly after one FP64 evaluation (a shortcut)."""
         _chk(self.lib.phlx_set_eos_lin_tol(self._ctx, float(lin_tol)), "phlx_set_eos_lin_tol")
 
     def set_shock_flattening(self, enable: bool = True, eps_sf: float = 0.25):
@@ -396,6 +400,7 @@ class Simulation:
         if k.shape != shp:
             raise ValueError(f"kappa shape {k.shape}, expected {shp}")
         _chk(self.lib.phlx_set_kappa(self._ctx, _ptr(k)), "phlx_set_kappa")
+        self._kappa_host = k   # for write_output's kappa dataset
 
     def parabolic_dt(self) -> float:
         v = ctypes.c_double()

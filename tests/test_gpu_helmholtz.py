@@ -200,3 +200,22 @@ def test_helm_slab_parity(table, oracle_mod, case):
             assert _rel_l1(got[f"b_x{d+1}"], ref[f"b_x{d+1}"]) <= TOL
         assert got["divb"] <= 1e-13 * max(1.0, float(np.abs(got["b_cc"]).max()) /
                                           pb.cfg.dx[0])
+
+
+def test_full_length_slab_full_table(oracle_mod):
+    """The Helmholtz slab at 32^3 for 100 steps with the full 541 x 271
+    table (log rhoYe -12 .. 15, log T 3 .. 13)."""
+    full = H.default_table()
+    pb = _slab(full, oracle_mod, n=32, sdims=3, recon=LIM2ND_REC, flux=LHLLC_FLUX)
+    nsteps = 100
+    print(f"helm slab 32^3, full table {full.nrho}x{full.nT}, {nsteps} steps", flush=True)
+    got = _gpu_run(pb, nsteps)
+    ref = oracle_mod.run_problem(pb, nsteps, nthreads=16)
+    assert got["step"] == ref["step"] == nsteps
+    assert abs(got["time"] - ref["time"]) <= 1e-14 * abs(ref["time"])
+    for v in range(pb.cfg.nvars):
+        r = _rel_l1(got["prim"][v], ref["prim"][v])
+        print(f"  prim[{v}] rel L1 {r:.2e}")
+        assert r <= TOL, v
+    for name in ("eint", "temp"):
+        assert _rel_l1(got[name], ref[name]) <= TOL, name

@@ -118,3 +118,30 @@ def test_helmholtz_flags(tmp_path):
     assert re.search(r'ATTRIBUTE "use_coulomb_corrections".*?"false"', hdr, re.S)
     t = _read(p0, "/grid/temp", tmp_path).reshape(cfg.shape)
     np.testing.assert_array_equal(t, st["temp"])
+
+
+def test_pig_and_kappa_flags(tmp_path):
+    """PIG runs write use_pig true (advect_yeiabar with two scalars);
+    THERMAL_DIFFUSION_STS writes the interior kappa."""
+    from phlegethon_amd.config import EOS_PIG, PhlxConfig
+    rng = np.random.default_rng(6)
+    for nas, yei in ((0, "false"), (2, "true")):
+        cfg = PhlxConfig(sdims=2, nx1=5, nx2=4, nas=nas, eos=EOS_PIG, fasteos=True,
+                         thermal_diffusion=True).validate()
+        st = {"prim": rng.uniform(1, 2, (cfg.nvars,) + cfg.shape),
+              "temp": rng.uniform(1e4, 1e5, cfg.shape), "b_cc": None}
+        kap = rng.uniform(1.0, 2.0, (1, cfg.nx2 + 2, cfg.nx1 + 2))
+        d = tmp_path / f"nas{nas}"
+        p0 = output.write_grid(str(d), cfg, {"step": 0, "time": 0.0, "dt": 1.0}, st, kappa=kap)
+        hdr = _tool("h5dump", "-A", p0)
+        assert re.search(r'ATTRIBUTE "use_pig".*?"true"', hdr, re.S)
+        assert re.search(r'ATTRIBUTE "use_helmholtz".*?"false"', hdr, re.S)
+        assert re.search(rf'ATTRIBUTE "advect_yeiabar".*?"{yei}"', hdr, re.S)
+        k = _read(p0, "/grid/kappa", tmp_path).reshape(cfg.shape)
+        np.testing.assert_array_equal(k, kap[:, 1:-1, 1:-1])
+    # without thermal diffusion no kappa dataset
+    cfg = PhlxConfig(sdims=2, nx1=5, nx2=4).validate()
+    st = {"prim": rng.uniform(1, 2, (cfg.nvars,) + cfg.shape), "temp": None, "b_cc": None}
+    p0 = output.write_grid(str(tmp_path / "plain"), cfg, {"step": 0, "time": 0.0, "dt": 1.0},
+                           st, kappa=np.ones((1, 6, 7)))
+    assert "kappa" not in _datasets(p0)
