@@ -939,9 +939,17 @@ __device__ __forceinline__ double flux_at(const Grid &g, int d, long long idx) {
 // the EoS state; they are held to 2 waves per SIMD (VGPRs + AGPRs <= 256),
 // which the unconstrained allocation can miss by a few registers and then
 // halve the occupancy.
+// The mixed-precision Helmholtz update (the EoS Newton on a rho-contracted
+// cell) is held to PHLX_HELM_MIX_UPDATE_WAVES waves per SIMD: left free, its
+// allocation lands a few registers above the 2-wave budget (1 wave per SIMD,
+// the table loads and the HBM streaming then overlap poorly).
+#ifndef PHLX_HELM_MIX_UPDATE_WAVES
+#define PHLX_HELM_MIX_UPDATE_WAVES 2
+#endif
 template <int SD, int NAS, bool MHD, bool HELM, bool MIX, bool GUESS = false>
 __global__ void __launch_bounds__(256)
-    __attribute__((amdgpu_waves_per_eu(HELM ? (GUESS ? 2 : PHLX_HELM_UPDATE_WAVES) : 1)))
+    __attribute__((amdgpu_waves_per_eu(
+        HELM ? (GUESS ? 2 : (MIX ? PHLX_HELM_MIX_UPDATE_WAVES : PHLX_HELM_UPDATE_WAVES)) : 1)))
     k_update(Grid g, int first, int last, double a1,
                                                 double a2, double c3, int fin) {
   // first: stage 1, state0 rebuilt from prim + eint (the reference), or in
