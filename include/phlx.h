@@ -301,6 +301,23 @@ int phlx_set_shock_flattening(phlx_ctx *ctx, int32_t enable, double eps_sf);
 
 /* Diagnostics: max over cells of |sum_d (b_d(+) - b_d(-))/dx_d|. */
 int phlx_divb_max(phlx_ctx *ctx, double *out);
+
+/* phlx_compute_dt without a context and without allocating: no counterpart
+ * in the reference; for callers that cannot allocate device memory (code
+ * under HIP graph capture, serving loops that forbid hipMalloc).  The same
+ * signal-speed maximum as phlx_compute_dt (compute_hyperbolic_dt,
+ * source/source.F90:13707-13960), ideal-gas EoS only, over caller-owned
+ * device arrays: prim [nv][nx3][nx2][nx1] (interior SoA), bcc [sdims][...]
+ * (cell-centred B; MHD only, else may be NULL); writes dt = cfl / smax and
+ * smax to dt_smax[0..1] (device memory), asynchronously on `stream`
+ * (a hipStream_t; NULL = the null stream).  Partial maxima go to a
+ * library-owned device array of 2048 doubles per GPU (16 KiB, held until the
+ * process ends), so any grid up to 2^33 cells fits; larger grids are
+ * rejected (PHLX_E_ARG).  Calls that overlap on one GPU race on that array:
+ * serialise them (one stream, or events).  cfg must describe the arrays
+ * (sdims, nx*, domain, gamma, cfl, flux for MHD, eos = PHLX_EOS_IDEAL). */
+int phlx_compute_dt_noalloc(const phlx_config *cfg, const double *prim, const double *bcc,
+                            double *dt_smax, void *stream);
 /* Per-kernel device time accumulated while profiling is on (HIP events on
  * the context stream).  kernel ids: 0 flux x1, 1 flux x2, 2 flux x3,
  * 3 corner EMF, 4 face-B, 5 update, 6 misc (shock flags), 7 thermal

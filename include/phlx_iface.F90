@@ -238,5 +238,14 @@ module phlx_iface
       real(c_double), value :: lr
       real(c_double), intent(out) :: train_loss, val_rms_logt, val_rms_loge
     end function
+    ! compute_hyperbolic_dt on caller-owned device arrays, allocating nothing
+    ! (callers under graph capture or that forbid hipMalloc); dt_smax(2) on
+    ! the device; stream a hipStream_t (c_null_ptr: the null stream)
+    integer(c_int) function phlx_compute_dt_noalloc(cfg, prim, bcc, dt_smax, stream) &
+        bind(c, name="phlx_compute_dt_noalloc")
+      import :: c_int, c_ptr, phlx_config
+      type(phlx_config), intent(in) :: cfg
+      type(c_ptr), value :: prim, bcc, dt_smax, stream
+    end function
   end interface
 end module phlx_iface

@@ -16,7 +16,7 @@ OUT_DIR = os.path.join(HERE, "_lib")
 OUT = os.path.join(OUT_DIR, "libphlx.so")
 SOURCES = ["phlx_sel_fp64.hip", "phlx_sel_mixed.hip", "phlx_sel_fp64_r5.hip",
            "phlx_sel_mixed_r5.hip", "phlx_kernels.hip", "phlx_helm.hip", "phlx_mlp.hip",
-           "phlx_diffusion.hip", "phlx_api.hip"]
+           "phlx_diffusion.hip", "phlx_api.hip", "phlx_noalloc.hip"]
 ARCH = os.environ.get("PHLX_OFFLOAD_ARCH", "gfx950")
 
 
