@@ -443,8 +443,10 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=2)
     ap.add_argument("--l1-steps", type=int, default=100,
                     help="c5: steps after which the mixed runs are compared with plain FP64")
-    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--cpu-n", type=int, default=128,
+                    help="c5: horizontal zones of the CPU baseline's full-height column")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", choices=("c5", "ot3d", "helm_slab", "lowmach", "hotbubble"),
                     default="c5")
@@ -503,19 +505,46 @@ def main():
     print(json.dumps(out), flush=True)
 
 
+def cpu_threads(args) -> int:
+    """Host threads for the CPU baseline: --cpu-threads, else the process's
+    CPU share (OMP_NUM_THREADS where the host sets it, else the affinity
+    mask)."""
+    if args.cpu_threads:
+        return args.cpu_threads
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        return int(env)
+    try:
+        return len(os.sched_getaffinity(0))
+    except Exception:
+        return os.cpu_count() or 1
+
+
 def run_cpu_baseline(pb, args):
-    threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+    threads = cpu_threads(args)
+    what = "the same"
+    if args.workload == "c5" and args.cpu_n and args.cpu_n < args.n:
+        # a full-height column of the slab: the same vertical profile, zone
+        # size, field strength and EoS work per zone, fewer horizontal zones
+        from phlegethon_amd import helmholtz, problems
+        from phlegethon_amd.runtime import HelmEoS
+        table = helmholtz.default_table()
+        eos = HelmEoS(table)
+        pb = problems.helm_slab(eos, table, n=args.cpu_n, ny=args.n, mhd_beta=1e5)
+        eos.close()
+        what = f"a full-height {args.cpu_n}x{args.n}x{args.cpu_n} column of the {args.n}^3"
     shape = "x".join(str(v) for v in reversed(pb.cfg.shape) if v > 1)
     log(f"cpu baseline: oracle on {threads} threads, {args.cpu_steps} steps of {shape}")
     rate, wall = cpu_baseline(pb, args.cpu_steps, threads)
     zones = pb.cfg.nx1 * pb.cfg.nx2 * (pb.cfg.nx3 if pb.cfg.sdims == 3 else 1)
     # wct/cell/cycle/core in microseconds (source/source.F90:6832-6833)
     us = wall / args.cpu_steps / zones * 1e6 * threads
-    log(f"cpu baseline {rate:.4e} zone-updates/s ({us:.3f} us/cell/cycle/core)")
+    log(f"cpu baseline {rate:.4e} zone-updates/s ({us:.3f} us/cell/cycle/core, "
+        f"{wall:.1f} s)")
     return {"value": rate, "unit": "zone-updates/s", "cores": threads, "kind": "port",
             "wct_per_cell_cycle_core_us": us, "host": host_description(threads),
-            "sample": f"{args.cpu_steps} full step(s) of the same {shape} workload "
-                      f"({wall:.1f} s wall) after the oracle's step-0 initialisation; C FP64 "
+            "sample": f"{args.cpu_steps} full step(s) of {what} workload ({shape}, "
+                      f"{wall:.1f} s wall) after the oracle's step-0 initialisation; C FP64 "
                       "restatement with OpenMP (-O3 -march=x86-64-v3 -ffp-contract=off)"}
 
 
@@ -602,9 +631,13 @@ def bench_c5(pb, desc, data, args):
         if "surrogate" in r:
             runs[v]["surrogate"] = r["surrogate"]
     del ref_state
+    if not variants:
+        # plain FP64 only (profiling runs)
+        runs["fp64"] = {"value": plain["value"], "ms_per_step": plain["ms_per_step"]}
+        variants = ["fp64"]
     head = variants[0]
     names = {"mlp": "FP32 + NN-EoS guess", "pred": "FP32 + energy-predictor guess",
-             "warm": "FP32 + carried-T guess"}
+             "warm": "FP32 + carried-T guess", "fp64": "plain FP64"}
     out = {
         "metric": "zone-updates/s, plain FP64 vs FP32+NN-EoS guess at fixed L1 err; % HBM peak",
         "value": runs[head]["value"], "unit": "zone-updates/s", "n_gpus": 1,

@@ -451,11 +451,12 @@ __device__ __forceinline__ void flux_face_mx(const Grid &g, int i, int j, int k,
     else { e0 = Fb[1]; e1v = -Fb[0]; }
     g.emff[DIR][f] = e0;     // FP32 like the induction fluxes they are
     g.emff[DIR][nf + f] = e1v;
+    // |F_rho| / (rho_mean smax) against Mach_ct as a product (den > 0)
     const double den = 0.5 * ((double)qL[0] + (double)qR[0]) * g.sc->smax;
-    const double mach = (double)F[0] / den;
+    const double thr = g.mach_ct * den, f0 = (double)F[0];
     signed char sgn = 0;
-    if (mach > g.mach_ct) sgn = 1;
-    if (mach < -g.mach_ct) sgn = -1;
+    if (f0 > thr) sgn = 1;
+    if (f0 < -thr) sgn = -1;
     g.sgn[DIR][f] = sgn;
   }
   if (g.etot) {
@@ -1154,7 +1155,11 @@ __global__ void __launch_bounds__(256)
           mlp_inputs(g.mlp, rho, e0, abar, zbar, x, ok);
           const float y = mlp_forward_wave(g.mlp, x, lds + threadIdx.y * kMlpLdsWaveDw);
           const float yp = g.mlp_y[c];
-          if (ok && yp == yp) tg = T * exp10((double)(g.mlp.out_scale * (y - yp)));
+          // 10^d with d ~ the change of log10 T since the last solve:
+          // FP32 exp2 (the factor's ~1e-7 error only moves the Newton start)
+          if (ok && yp == yp)
+            tg = T * (double)__builtin_amdgcn_exp2f(g.mlp.out_scale * (y - yp) *
+                                                    3.32192809488736235f);
           g.mlp_y[c] = ok ? y : __int_as_float(0x7fffffff);
         }
         int fb = 0, it;

@@ -77,10 +77,16 @@ __device__ __forceinline__ float fast_tanh(float z) {
   return 1.0f - 2.0f * __builtin_amdgcn_rcpf(e + 1.0f);
 }
 
+// log10 of a positive value in FP32 (hardware v_log_f32; the network's
+// inputs are FP32); 0 stays finite through the clamp to the FP32 minimum.
+__device__ __forceinline__ float mlp_log10f(double v) {
+  const float f = fmaxf((float)v, 1.17549435e-38f);
+  return __builtin_amdgcn_logf(f) * 0.30102999566398120f;
+}
+
 __device__ __forceinline__ void mlp_inputs(const MlpDev &m, double rho, double e, double abar,
                                            double zbar, float x[4], bool &inbox) {
-  const float raw[4] = {(float)log10(rho), (float)log10(fabs(e) > 0.0 ? fabs(e) : 1e-300),
-                        (float)abar, (float)zbar};
+  const float raw[4] = {mlp_log10f(rho), mlp_log10f(fabs(e)), (float)abar, (float)zbar};
   inbox = e > 0.0;
 #pragma unroll
   for (int i = 0; i < 4; i++) {
@@ -93,10 +99,19 @@ __device__ __forceinline__ void mlp_inputs(const MlpDev &m, double rho, double e
   }
 }
 
+// The LDS exchange of mlp_forward_wave is within one wavefront (each wave
+// owns its kMlpLdsWaveDw dwords), so a wave-scope fence orders it: every
+// lane's LDS writes complete before any lane's later reads.  No block
+// barrier, so the waves of a block run their inference independently.
+__device__ __forceinline__ void mlp_wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Normalised network output for this lane's zone.  Every lane of the
 // wavefront must call it (MFMA and LDS exchange are wave-wide); lds points
-// at this wavefront's kMlpLdsWaveDw dwords.  Ends with all lanes having
-// passed a block barrier, so all waves of the block must call it together.
+// at this wavefront's kMlpLdsWaveDw dwords, which no other wave touches.
 __device__ __forceinline__ float mlp_forward_wave(const MlpDev &m, const float x[4],
                                                   unsigned int *lds) {
   const int lane = threadIdx.x & 63;
@@ -116,7 +131,7 @@ __device__ __forceinline__ float mlp_forward_wave(const MlpDev &m, const float x
     const unsigned int lo = bf16_bits(fast_tanh(z0)), hi = bf16_bits(fast_tanh(z1));
     h1[lane * 33 + (n >> 1)] = lo | (hi << 16);
   }
-  __syncthreads();
+  mlp_wave_lds_sync();
   // layer 2 (MFMA 16x16x16 BF16): rows = zones 16rb..16rb+15 of this wave;
   // A lane l holds row l%16, k = 16kb + 4(l/16) .. +3; D lane l holds
   // rows 4(l/16)+r, column l%16
@@ -153,7 +168,7 @@ __device__ __forceinline__ float mlp_forward_wave(const MlpDev &m, const float x
       part[(16 * rb + 4 * kq + r) * 17 + row] = s;
     }
   }
-  __syncthreads();
+  mlp_wave_lds_sync();
   float y = m.theta[kMlpOffB3];
 #pragma unroll
   for (int c = 0; c < 16; c++) y = y + part[lane * 17 + c];
