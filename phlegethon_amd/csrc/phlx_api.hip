@@ -41,7 +41,7 @@ struct phlx_ctx {
   void *mlp_buf = nullptr;
   float *mlp_theta = nullptr, *mlp_m = nullptr, *mlp_v = nullptr, *mlp_w1q = nullptr,
         *mlp_w3q = nullptr, *mlp_gpart = nullptr, *mlp_lpart = nullptr, *mlp_gsum = nullptr;
-  unsigned short *mlp_w2t = nullptr;
+  unsigned short *mlp_wa = nullptr;
   bool mlp_ready = false;
   // training box {log10 rhoYe lo, hi, log10 T lo, hi, abar lo, hi, Ye lo, hi};
   // has_box false: the table's range
@@ -978,7 +978,8 @@ static int mlp_alloc(phlx_ctx *ctx) {
   if (ctx->mlp_buf) return PHLX_OK;
   const size_t P = kMlpNParam;
   const size_t floats = 3 * P + 256 + 64 + (size_t)kMlpMaxGradBlocks * P + kMlpMaxGradBlocks + P;
-  const size_t bytes = floats * sizeof(float) + 16 * 64 * 4 * sizeof(unsigned short) + 4096;
+  const size_t bytes =
+      floats * sizeof(float) + (kMlpW2aShorts + kMlpW1aShorts) * sizeof(unsigned short) + 4096;
   void *buf = nullptr;
   if (hipMalloc(&buf, bytes) != hipSuccess) return PHLX_E_NOMEM;
   HIPCHK(hipMemsetAsync(buf, 0, bytes, ctx->stream));
@@ -992,13 +993,14 @@ static int mlp_alloc(phlx_ctx *ctx) {
   ctx->mlp_lpart = f; f += kMlpMaxGradBlocks;
   ctx->mlp_gsum = f; f += P;
   uintptr_t a = ((uintptr_t)f + 255) & ~(uintptr_t)255;
-  ctx->mlp_w2t = (unsigned short *)a;
+  ctx->mlp_wa = (unsigned short *)a;
   ctx->mlp_buf = buf;
   MlpDev &m = ctx->g.mlp;
   m.theta = ctx->mlp_theta;
   m.w1q = ctx->mlp_w1q;
   m.w3q = ctx->mlp_w3q;
-  m.w2t = (const mlp_s4 *)ctx->mlp_w2t;
+  m.w2a = (const mlp_bf8 *)ctx->mlp_wa;
+  m.w1a = (const mlp_s4 *)(ctx->mlp_wa + kMlpW2aShorts);
   return PHLX_OK;
 }
 
@@ -1021,7 +1023,7 @@ extern "C" int phlx_mlp_set_params(phlx_ctx *ctx, const float *theta, const floa
   if (!(norm[9] > 0.0f)) return PHLX_E_ARG;
   HIPCHK(hipMemcpyAsync(ctx->mlp_theta, theta, sizeof(float) * kMlpNParam, hipMemcpyHostToDevice,
                         ctx->stream));
-  launch_mlp_quantize(ctx->mlp_theta, ctx->mlp_w1q, ctx->mlp_w3q, ctx->mlp_w2t, ctx->stream);
+  launch_mlp_quantize(ctx->mlp_theta, ctx->mlp_w1q, ctx->mlp_w3q, ctx->mlp_wa, ctx->stream);
   rc = check_launch();
   if (rc) return rc;
   HIPCHK(hipStreamSynchronize(ctx->stream));
@@ -1070,7 +1072,7 @@ extern "C" int phlx_mlp_grad(phlx_ctx *ctx, int64_t n, const float *xn, const fl
   const int nblk = launch_mlp_grad(ctx->mlp_theta, ctx->mlp_w1q, ctx->mlp_w3q, xn, tn, wt, 0, n,
                                    ctx->mlp_gpart, ctx->mlp_lpart, ctx->stream);
   launch_mlp_adam(ctx->mlp_theta, ctx->mlp_m, ctx->mlp_v, ctx->mlp_gpart, nblk, 1, 0.0f,
-                  ctx->mlp_w1q, ctx->mlp_w3q, ctx->mlp_w2t, grad, ctx->stream);
+                  ctx->mlp_w1q, ctx->mlp_w3q, ctx->mlp_wa, grad, ctx->stream);
   int rc = check_launch();
   if (rc) return rc;
   if (loss) {
@@ -1178,7 +1180,7 @@ extern "C" int phlx_mlp_train(phlx_ctx *ctx, int64_t nsamples, int32_t epochs, i
     (void)hipFree(data);
     return PHLX_E_HIP;
   }
-  launch_mlp_quantize(ctx->mlp_theta, ctx->mlp_w1q, ctx->mlp_w3q, ctx->mlp_w2t, ctx->stream);
+  launch_mlp_quantize(ctx->mlp_theta, ctx->mlp_w1q, ctx->mlp_w3q, ctx->mlp_wa, ctx->stream);
   const long long spe = ntr / batch;  // steps per epoch
   const long long steps = spe * epochs;
   int nblk = 1;
@@ -1189,7 +1191,7 @@ extern "C" int phlx_mlp_train(phlx_ctx *ctx, int64_t nsamples, int32_t epochs, i
     nblk = launch_mlp_grad(ctx->mlp_theta, ctx->mlp_w1q, ctx->mlp_w3q, xn, tn, wt, n0, batch,
                            ctx->mlp_gpart, ctx->mlp_lpart, ctx->stream);
     launch_mlp_adam(ctx->mlp_theta, ctx->mlp_m, ctx->mlp_v, ctx->mlp_gpart, nblk, (int)t, lr_t,
-                    ctx->mlp_w1q, ctx->mlp_w3q, ctx->mlp_w2t, nullptr, ctx->stream);
+                    ctx->mlp_w1q, ctx->mlp_w3q, ctx->mlp_wa, nullptr, ctx->stream);
   }
   rc = check_launch();
   ctx->mlp_ready = true;

@@ -69,9 +69,9 @@ int launch_mlp_grad(const float *theta, const float *w1q, const float *w3q, cons
                     const float *tn, const float *wt, long long n0, long long nb, float *gpart,
                     float *lpart, hipStream_t st);
 void launch_mlp_adam(float *theta, float *m, float *v, const float *gpart, int nblk, int t,
-                     float lr, float *w1q, float *w3q, unsigned short *w2t, float *gout,
+                     float lr, float *w1q, float *w3q, unsigned short *wa, float *gout,
                      hipStream_t st);
-void launch_mlp_quantize(const float *theta, float *w1q, float *w3q, unsigned short *w2t,
+void launch_mlp_quantize(const float *theta, float *w1q, float *w3q, unsigned short *wa,
                          hipStream_t st);
 void launch_mlp_forward(const MlpDev &m, long long n, const float *raw, float *y, int *inbox,
                         hipStream_t st);

@@ -188,17 +188,24 @@ k_mlp_grad(const float *__restrict__ theta, const float *__restrict__ w1q,
   }
 }
 
+// wa: the inference's MFMA A operands, w2a (4096 BF16) then w1a (1024 BF16,
+// the lane group 3 slots left zero), layouts as documented at MlpDev.
 __device__ __forceinline__ void store_quantized(int p, float w, float *w1q, float *w3q,
-                                                unsigned short *w2t) {
+                                                unsigned short *wa) {
   if (p >= kMlpOffW1 && p < kMlpOffB1) {
     w1q[p - kMlpOffW1] = bf16_round(w);
+    const int e = (p - kMlpOffW1) >> 6, n = (p - kMlpOffW1) & 63;
+    unsigned short *w1a = wa + kMlpW2aShorts;
+#pragma unroll
+    for (int q = 0; q < 3; q++) w1a[((n >> 4) * 64 + 16 * q + (n & 15)) * 4 + e] = bf16_bits(w);
   } else if (p >= kMlpOffW3 && p < kMlpOffB3) {
     w3q[p - kMlpOffW3] = bf16_round(w);
   } else if (p >= kMlpOffW2 && p < kMlpOffB2) {
-    // B tile element: k = 16 kb + 4 (lane / 16) + j, n = 16 nb + lane % 16
-    const int e = p - kMlpOffW2, k = e >> 6, n = e & 63;
-    const int kb = k >> 4, j = k & 3, lane = 16 * ((k & 15) >> 2) + (n & 15), nb = n >> 4;
-    w2t[((kb * 4 + nb) * 64 + lane) * 4 + j] = bf16_bits(w);
+    // w2[k][n]: k-slot e of lane group q in block kb as MlpDev documents
+    const int k = (p - kMlpOffW2) >> 6, n = (p - kMlpOffW2) & 63;
+    const int kb = k >> 5, kk = k & 31;
+    const int e = 4 * (kk >> 4) + (kk & 3), lane = 16 * ((kk & 15) >> 2) + (n & 15);
+    wa[(((n >> 4) * 2 + kb) * 64 + lane) * 8 + e] = bf16_bits(w);
   }
 }
 
@@ -207,7 +214,7 @@ __device__ __forceinline__ void store_quantized(int p, float w, float *w1q, floa
 __global__ void __launch_bounds__(kAdamBlock)
 k_mlp_adam(float *__restrict__ theta, float *__restrict__ mo, float *__restrict__ ve,
            const float *__restrict__ gpart, int nblk, int t, float lr, float *__restrict__ w1q,
-           float *__restrict__ w3q, unsigned short *__restrict__ w2t, float *__restrict__ gout) {
+           float *__restrict__ w3q, unsigned short *__restrict__ wa, float *__restrict__ gout) {
   const int p = blockIdx.x * kAdamBlock + threadIdx.x;
   if (p >= kMlpNParam) return;
   float g = 0.0f;
@@ -225,15 +232,15 @@ k_mlp_adam(float *__restrict__ theta, float *__restrict__ mo, float *__restrict_
   const float vh = v / (1.0f - powf(b2, (float)t));
   const float w = theta[p] - lr * mh / (sqrtf(vh) + eps);
   theta[p] = w;
-  store_quantized(p, w, w1q, w3q, w2t);
+  store_quantized(p, w, w1q, w3q, wa);
 }
 
 __global__ void __launch_bounds__(kAdamBlock)
 k_mlp_quantize(const float *__restrict__ theta, float *__restrict__ w1q, float *__restrict__ w3q,
-               unsigned short *__restrict__ w2t) {
+               unsigned short *__restrict__ wa) {
   const int p = blockIdx.x * kAdamBlock + threadIdx.x;
   if (p >= kMlpNParam) return;
-  store_quantized(p, theta[p], w1q, w3q, w2t);
+  store_quantized(p, theta[p], w1q, w3q, wa);
 }
 
 // Inference on raw inputs (log10 rho, log10 e, abar, zbar as floats):
@@ -315,15 +322,15 @@ int launch_mlp_grad(const float *theta, const float *w1q, const float *w3q, cons
   return nblk;
 }
 void launch_mlp_adam(float *theta, float *m, float *v, const float *gpart, int nblk, int t,
-                     float lr, float *w1q, float *w3q, unsigned short *w2t, float *gout,
+                     float lr, float *w1q, float *w3q, unsigned short *wa, float *gout,
                      hipStream_t st) {
   hipLaunchKernelGGL(k_mlp_adam, dim3(nblocks(kMlpNParam, kAdamBlock)), dim3(kAdamBlock), 0, st,
-                     theta, m, v, gpart, nblk, t, lr, w1q, w3q, w2t, gout);
+                     theta, m, v, gpart, nblk, t, lr, w1q, w3q, wa, gout);
 }
-void launch_mlp_quantize(const float *theta, float *w1q, float *w3q, unsigned short *w2t,
+void launch_mlp_quantize(const float *theta, float *w1q, float *w3q, unsigned short *wa,
                          hipStream_t st) {
   hipLaunchKernelGGL(k_mlp_quantize, dim3(nblocks(kMlpNParam, kAdamBlock)), dim3(kAdamBlock), 0,
-                     st, theta, w1q, w3q, w2t);
+                     st, theta, w1q, w3q, wa);
 }
 void launch_mlp_forward(const MlpDev &m, long long n, const float *raw, float *y, int *inbox,
                         hipStream_t st) {

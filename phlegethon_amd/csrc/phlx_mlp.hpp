@@ -9,11 +9,16 @@
 // inside the hydro update a corrected warm start,
 //   log10 T_g = log10 T_prev + s (y(state now) - y(state at the last solve)),
 // where the network's smooth approximation error cancels in the difference
-// while the change of T with the new internal energy is kept.  Inference runs one wavefront
-// per 64 zones: layer 1 (4 -> 64) and layer 3 (64 -> 1) on the VALU in FP32
-// with BF16-rounded weights, layer 2 (64 x 64) on the matrix cores with
-// v_mfma_f32_16x16x16bf16_1k (BF16 operands, FP32 accumulation): 16 tiles x
-// 4 k-steps = 64 MFMA per wavefront.  The exact-table Newton then runs from
+// while the change of T with the new internal energy is kept.  Inference
+// runs one wavefront per 64 zones, the layers transposed (hidden units as
+// rows, zones as columns) so that each layer's MFMA output is already in the
+// register layout of the next layer's B operand: layer 1 (4 -> 64) on
+// v_mfma_f32_16x16x16_bf16 with the FP32 inputs split into three BF16 parts
+// (hi + mid + lo holds all 24 significand bits, so the products with the
+// BF16 weights are those of the FP32 inputs), layer 2 (64 x 64) on gfx950's
+// v_mfma_f32_16x16x32_bf16 with the BF16-rounded tanh(h1), layer 3 (64 -> 1)
+// as an FP32 dot product finished by a cross-lane butterfly; FP32
+// accumulation, FP32 biases.  The exact-table Newton then runs from
 // the guess to the reference's tolerance, so the surrogate changes the
 // iteration count, never the converged state beyond that tolerance; a guess
 // whose inputs leave the training box, or whose first Newton residual
@@ -31,6 +36,8 @@ constexpr int kMlpIn = 4, kMlpH = 64;
 // flat parameter vector: w1 [4][64], b1 [64], w2 [64][64], b2 [64], w3 [64], b3
 constexpr int kMlpOffW1 = 0, kMlpOffB1 = 256, kMlpOffW2 = 320, kMlpOffB2 = 4416,
               kMlpOffW3 = 4480, kMlpOffB3 = 4544, kMlpNParam = 4545;
+// the inference's BF16 A operands in one buffer: w2a, then w1a
+constexpr int kMlpW2aShorts = 4096, kMlpW1aShorts = 1024;
 constexpr int kBracketSteps = 16;  // bisection steps of the fallback guess
 
 // T(rho, e) starts: the carried T (the reference), the surrogate's
@@ -40,25 +47,32 @@ constexpr int kBracketSteps = 16;  // bisection steps of the fallback guess
 enum { GUESS_WARM = 0, GUESS_MLP = 1, GUESS_BRACKET = 2, GUESS_PRED = 3 };
 
 typedef short mlp_s4 __attribute__((ext_vector_type(4)));
+typedef __bf16 mlp_bf2 __attribute__((ext_vector_type(2)));
+typedef __bf16 mlp_bf8 __attribute__((ext_vector_type(8)));
+typedef float mlp_f2 __attribute__((ext_vector_type(2)));
 typedef float mlp_f4 __attribute__((ext_vector_type(4)));
 
 struct MlpDev {
   const float *theta;   // FP32 master parameters (biases are read from here)
   const float *w1q;     // BF16-rounded w1 as FP32, [4][64]
   const float *w3q;     // BF16-rounded w3 as FP32, [64]
-  const mlp_s4 *w2t;    // w2 as BF16 MFMA B tiles: [kb][nb][lane] x 4 (k = 16kb + 4(lane/16) + j)
+  // MFMA A operands of the transposed layers (lane l: row l % 16, k-slots
+  // of lane group q = l / 16):
+  //   w1a [hb 4][lane 64] x 4 BF16: w1[e][16 hb + l % 16] for q < 3, 0 for q = 3
+  //       (slot group q multiplies the zone's hi / mid / lo input part)
+  //   w2a [nb 4][kb 2][lane 64] x 8 BF16: w2[H][16 nb + l % 16] with
+  //       H = 32 kb + 4 q + e (e < 4), 32 kb + 16 + 4 q + e - 4 (e >= 4)
+  const mlp_s4 *w1a;
+  const mlp_bf8 *w2a;
   float in_off[4], in_scale[4];  // x = (raw - off) * scale
   float out_off, out_scale;      // log10 T = y * out_scale + out_off
   float resid_tol;               // fallback when the first residual exceeds this
   float box;                     // accepted |x| (training box + margin)
 };
 
-// LDS per wavefront for the inference: h1 as BF16 rows of 33 dwords (64
-// values + 1 pad dword against bank conflicts) and the layer-3 partial sums
-// (64 rows x 17 floats).
-constexpr int kMlpLdsH1Dw = 64 * 33;
-constexpr int kMlpLdsPart = 64 * 17;
-constexpr int kMlpLdsWaveDw = kMlpLdsH1Dw + kMlpLdsPart;
+// LDS per wavefront for the inference: the 64 zones' inputs as three BF16
+// parts (hi, mid, lo), two dwords each.
+constexpr int kMlpLdsWaveDw = 64 * 3 * 2;
 
 __device__ __forceinline__ unsigned short bf16_bits(float f) {
   unsigned int u = __float_as_uint(f);
@@ -109,70 +123,108 @@ __device__ __forceinline__ void mlp_wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// (a, b) rounded to BF16 (round to nearest even, v_cvt_pk_bf16_f32), a in
+// the low half
+__device__ __forceinline__ unsigned int mlp_pk_bf16(float a, float b) {
+  const mlp_bf2 h = __builtin_convertvector((mlp_f2){a, b}, mlp_bf2);
+  return __builtin_bit_cast(unsigned int, h);
+}
+__device__ __forceinline__ float mlp_bf16_lo(unsigned int u) { return __uint_as_float(u << 16); }
+__device__ __forceinline__ float mlp_bf16_hi(unsigned int u) {
+  return __uint_as_float(u & 0xFFFF0000u);
+}
+
 // Normalised network output for this lane's zone.  Every lane of the
 // wavefront must call it (MFMA and LDS exchange are wave-wide); lds points
 // at this wavefront's kMlpLdsWaveDw dwords, which no other wave touches.
+// A layer's MFMA output D (lane l: rows 4q + r, r = 0..3, column l % 16;
+// rows = hidden units, columns = zones 16 zb + l % 16) is, after tanh and
+// the BF16 rounding, the next layer's B operand in place.
 __device__ __forceinline__ float mlp_forward_wave(const MlpDev &m, const float x[4],
                                                   unsigned int *lds) {
   const int lane = threadIdx.x & 63;
+  const int q = lane >> 4, col = lane & 15;
+  // this zone's inputs as hi + mid + lo BF16 parts (each residual is exact)
+  {
+    unsigned int part[3][2];
+#pragma unroll
+    for (int pr = 0; pr < 2; pr++) {
+      float a = x[2 * pr], b = x[2 * pr + 1];
+#pragma unroll
+      for (int t = 0; t < 3; t++) {
+        const unsigned int h = mlp_pk_bf16(a, b);
+        part[t][pr] = h;
+        a = a - mlp_bf16_lo(h);
+        b = b - mlp_bf16_hi(h);
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < 3; t++) {
+      lds[(lane * 3 + t) * 2] = part[t][0];
+      lds[(lane * 3 + t) * 2 + 1] = part[t][1];
+    }
+  }
+  mlp_wave_lds_sync();
+  // layer-1 B operands: zone 16 zb + col; lane group q < 3 holds part q of
+  // its four inputs, group 3 zeros
+  mlp_s4 xb[4];
+#pragma unroll
+  for (int zb = 0; zb < 4; zb++) {
+    const unsigned int *src = lds + ((16 * zb + col) * 3 + (q < 3 ? q : 0)) * 2;
+    const unsigned int d0 = src[0], d1 = src[1];
+    const unsigned long long v =
+        q < 3 ? ((unsigned long long)d0 | ((unsigned long long)d1 << 32)) : 0ull;
+    xb[zb] = __builtin_bit_cast(mlp_s4, v);
+  }
+  mlp_wave_lds_sync();
   const float *b1 = m.theta + kMlpOffB1;
   const float *b2 = m.theta + kMlpOffB2;
-  unsigned int *h1 = lds;
-  float *part = (float *)(lds + kMlpLdsH1Dw);
-  // layer 1 (VALU): this lane's 64 hidden units, stored as BF16 pairs
-#pragma unroll 4
-  for (int n = 0; n < kMlpH; n += 2) {
-    float z0 = b1[n], z1 = b1[n + 1];
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-      z0 = z0 + x[i] * m.w1q[i * kMlpH + n];
-      z1 = z1 + x[i] * m.w1q[i * kMlpH + n + 1];
-    }
-    const unsigned int lo = bf16_bits(fast_tanh(z0)), hi = bf16_bits(fast_tanh(z1));
-    h1[lane * 33 + (n >> 1)] = lo | (hi << 16);
-  }
-  mlp_wave_lds_sync();
-  // layer 2 (MFMA 16x16x16 BF16): rows = zones 16rb..16rb+15 of this wave;
-  // A lane l holds row l%16, k = 16kb + 4(l/16) .. +3; D lane l holds
-  // rows 4(l/16)+r, column l%16
-  const int row = lane & 15, kq = lane >> 4;
-  const float *w3 = m.w3q;
+  float s[4];
 #pragma unroll 1
-  for (int rb = 0; rb < 4; rb++) {
-    mlp_f4 acc[4];
+  for (int zb = 0; zb < 4; zb++) {
+    // layer 1: h1^T (hidden 16 hb + 4q + r, zone) = w1^T x^T + b1, then tanh
+    // and BF16 into the k-slots of layer 2 (block kb = hb / 2: slots 0..3
+    // from hb even, 4..7 from hb odd)
+    mlp_bf8 h1[2];
 #pragma unroll
-    for (int nb = 0; nb < 4; nb++) acc[nb] = (mlp_f4){0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-    for (int kb = 0; kb < 4; kb++) {
-      const unsigned int *src = h1 + (16 * rb + row) * 33 + 8 * kb + 2 * kq;
-      const unsigned int d0 = src[0], d1 = src[1];
-      mlp_s4 a;
-      a.x = (short)(d0 & 0xFFFF);
-      a.y = (short)(d0 >> 16);
-      a.z = (short)(d1 & 0xFFFF);
-      a.w = (short)(d1 >> 16);
-#pragma unroll
-      for (int nb = 0; nb < 4; nb++)
-        acc[nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, m.w2t[(kb * 4 + nb) * 64 + lane],
-                                                            acc[nb], 0, 0, 0);
+    for (int hb = 0; hb < 4; hb++) {
+      const float *bb = b1 + 16 * hb + 4 * q;
+      mlp_f4 d = (mlp_f4){bb[0], bb[1], bb[2], bb[3]};
+      d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(m.w1a[hb * 64 + lane], xb[zb], d, 0, 0, 0);
+      const mlp_bf2 v0 =
+          __builtin_bit_cast(mlp_bf2, mlp_pk_bf16(fast_tanh(d[0]), fast_tanh(d[1])));
+      const mlp_bf2 v1 =
+          __builtin_bit_cast(mlp_bf2, mlp_pk_bf16(fast_tanh(d[2]), fast_tanh(d[3])));
+      const int o = (hb & 1) * 4;
+      h1[hb >> 1][o] = v0[0];
+      h1[hb >> 1][o + 1] = v0[1];
+      h1[hb >> 1][o + 2] = v1[0];
+      h1[hb >> 1][o + 3] = v1[1];
     }
-    // epilogue: tanh(z2 + b2) * w3 summed over this lane's four columns
+    // layer 2, and layer 3 over this lane's 16 output rows
+    float acc = 0.0f;
 #pragma unroll
-    for (int r = 0; r < 4; r++) {
-      float s = 0.0f;
+    for (int nb = 0; nb < 4; nb++) {
+      mlp_f4 d = (mlp_f4){0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-      for (int nb = 0; nb < 4; nb++) {
-        const int col = 16 * nb + row;
-        s = s + fast_tanh(acc[nb][r] + b2[col]) * w3[col];
-      }
-      part[(16 * rb + 4 * kq + r) * 17 + row] = s;
+      for (int kb = 0; kb < 2; kb++)
+        d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(m.w2a[(nb * 2 + kb) * 64 + lane], h1[kb], d,
+                                                    0, 0, 0);
+      const int r0 = 16 * nb + 4 * q;
+#pragma unroll
+      for (int r = 0; r < 4; r++) acc = acc + fast_tanh(d[r] + b2[r0 + r]) * m.w3q[r0 + r];
     }
+    s[zb] = acc;
   }
-  mlp_wave_lds_sync();
-  float y = m.theta[kMlpOffB3];
-#pragma unroll
-  for (int c = 0; c < 16; c++) y = y + part[lane * 17 + c];
-  return y;
+  // y(zone 16 zb + col) = b3 + the sum of s[zb] over the four lane groups;
+  // lane l keeps zb = q: halves exchanged across groups q ^ 2, then q ^ 1
+  const bool up = (q & 2) != 0, odd = (q & 1) != 0;
+  const float keep0 = up ? s[2] : s[0], keep1 = up ? s[3] : s[1];
+  const float send0 = up ? s[0] : s[2], send1 = up ? s[1] : s[3];
+  const float t0 = keep0 + __shfl_xor(send0, 32);
+  const float t1 = keep1 + __shfl_xor(send1, 32);
+  const float u = (odd ? t1 : t0) + __shfl_xor(odd ? t0 : t1, 16);
+  return m.theta[kMlpOffB3] + u;
 }
 
 // Fallback: bisection of e(T) - e0 in log10 T over [Tl, Th].
