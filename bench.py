@@ -59,9 +59,10 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def kernel_bytes(cfg, name: str) -> float:
+def kernel_bytes(cfg, name: str, swept: bool = False) -> float:
     """Algorithmic HBM bytes of one launch: the compulsory reads of the
-    kernel's inputs (each value once) and writes of its outputs, FP64."""
+    kernel's inputs (each value once) and writes of its outputs, FP64;
+    swept: the update that finishes a fused-sweep stage."""
     nz, ny, nx = cfg.shape
     N = nx * ny * nz
     nv, sd = cfg.nvars, cfg.sdims
@@ -83,6 +84,23 @@ def kernel_bytes(cfg, name: str) -> float:
     if name == "faceb":    # b, b0, 4 edge EMFs in; b (+b0 at stage 1) out
         w = 4 if cfg.precision == 1 else 8
         return sum(nf[:sd]) * 8 * 3 + 3 * N * w
+    if name == "sweep":
+        # the fused sweep, averaged over the two RK2 stages: the cell inputs
+        # and the normal face B once; stage 1 the prim, eint, b_cc of the
+        # state0 rebuild, stage 2 state0 and the stage state; the new state
+        # (and state0 at stage 1), 2 EMFs + sign per face (MHD) out
+        st = nv + 1 + (sd if mhd else 0)
+        rd = N * cell_in + (sum(nf[:sd]) * 8 if mhd else 0) + 0.5 * N * 8 * (st + 2 * nv)
+        wr = (sum(nf[:sd]) * 17 if mhd else 0) + 1.5 * N * 8 * nv
+        return rd + wr
+    if name == "update" and swept:
+        # cons -> prim after the fused sweep: U and the face B (Helmholtz: T)
+        # in; prim, eint, b_cc, the cell EMFs, gammas (T) out
+        gam = 2 if (helm and cfg.fasteos) else 0
+        ec = sd if mhd else 0
+        rd = nv + (3 if mhd else 0) + (1 if helm else 0)
+        wr = nv + 1 + (sd if mhd else 0) + ec + gam + (1 if helm else 0)
+        return 8.0 * N * (rd + wr)
     if name == "update":
         # compulsory traffic per launch, averaged over the two RK2 stages:
         # face fluxes (FP32 in the mixed path) and face B in; stage 1 reads
@@ -322,6 +340,7 @@ def run_variant(pb, cfg, args, guess="warm", keep_state=False, profile=True, tot
             f"{rmst:.4f}, implied log10 e {rmse:.4f}")
     if helm:
         sim.set_eos_guess(guess, resid_tol=0.25, count=False)
+    sim.set_fused_sweep({"fused": True, "split": False, "auto": None}[args.sweep])
     sim.init()
     log(f"warmup {args.warmup} steps")
     sim.run(args.warmup)
@@ -354,12 +373,13 @@ def run_variant(pb, cfg, args, guess="warm", keep_state=False, profile=True, tot
         sim.synchronize()
         done += args.profile_steps
         kern = {}
-        for name in ("flux_x1", "flux_x2", "flux_x3", "corner", "faceb", "update", "misc",
-                     "diffusion"):
+        swept = sim.profile_read("sweep")[1] > 0
+        for name in ("sweep", "flux_x1", "flux_x2", "flux_x3", "corner", "faceb", "update",
+                     "misc", "diffusion"):
             ms, cnt = sim.profile_read(name)
             if cnt:
                 kern[name] = {"total_ms": ms, "launches": cnt, "avg_ms": ms / cnt,
-                              "alg_bytes": kernel_bytes(cfg, name)}
+                              "alg_bytes": kernel_bytes(cfg, name, swept)}
         if "diffusion" in kern:
             # per step: two diffusion steps and one dt_par pass (rho, kappa)
             s = sim.diffusion_stages()
@@ -454,6 +474,10 @@ def main():
                     help="c5: the mixed-precision T(rho, e) starts to run (comma list of "
                          "mlp, pred, warm; the first is the headline value)")
     ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64")
+    ap.add_argument("--sweep", choices=("auto", "fused", "split"), default="auto",
+                    help="3D runs: the fused one-pass sweep, the three directional sweeps "
+                         "(same results bit for bit), or the library's choice (fused on the "
+                         "mixed-precision path)")
     ap.add_argument("--eos-guess", choices=("warm", "mlp", "pred"), default="warm",
                     help="Helmholtz: Newton start (carried T, the trained surrogate, or the "
                          "energy predictor)")

@@ -153,6 +153,18 @@ int phlx_sweep(phlx_ctx *ctx, int32_t dir);
 int phlx_ct(phlx_ctx *ctx, int32_t irk, double a1, double a2, double a3);
 int phlx_rk_update(phlx_ctx *ctx, int32_t irk, double a1, double a2, double a3,
                    int32_t last_stage);
+/* Whether phlx_hydro_step / phlx_run fuse each stage's three directional
+ * sweeps and flux divergence (the work of sweep(0..2) and the divergence
+ * and RK combine of rk_update) into one pass that stages a pencil tile in
+ * LDS and keeps the face fluxes out of HBM: mode 1 wherever the run admits
+ * it, 0 never, -1 (the default) where it is the faster form, the mixed
+ * precision path.  It applies to 3D runs with LIM2ND or PPH, the ideal or
+ * the FASTEOS Helmholtz state, no shock flattening and nx1, nx2 multiples
+ * of 16 (other runs take the separate sweeps).  Either way the results are
+ * bit for bit the same.  No counterpart in the reference (its hydro_step is
+ * one routine either way): for A/B timing and for callers that want the
+ * per-direction kernels. */
+int phlx_set_fused_sweep(phlx_ctx *ctx, int32_t mode);
 
 /* ---- Helmholtz EoS ----
  * load_helm_table replaces load_helm_table (source/source.F90:23077-23156):

@@ -56,6 +56,9 @@ struct phlx_ctx {
   // prim / eint / b_cc from U (whether it ran, and so whether they lag U,
   // the device flags SC_U_CUR / SC_P_STALE say)
   bool prim_stale = false;
+  // hydro_step's fused sweep: -1 = where it is the faster form (the mixed
+  // path), 0 = never, 1 = wherever the kernel family has one
+  int fused_sweep = -1;
   double dt_par = 0.0;
   int diff_stages = 0;  // stages of the last diffusion step
   // profiling
@@ -609,9 +612,10 @@ extern "C" int phlx_ct(phlx_ctx *ctx, int32_t irk, double a1, double a2, double 
   return check_launch();
 }
 
-extern "C" int phlx_rk_update(phlx_ctx *ctx, int32_t irk, double a1, double a2, double a3,
-                              int32_t last_stage) {
-  if (!ctx) return PHLX_E_ARG;
+// swept: the fused sweep formed the stage's U; finish it (cons -> prim, EoS,
+// CFL) instead of the flux-divergence update.
+static int rk_update(phlx_ctx *ctx, int32_t irk, double a1, double a2, double a3,
+                     int32_t last_stage, bool swept) {
   ctx->tdiff_valid = false;
   ProfRec r;
   int on = kbegin(ctx, KID_UPDATE, r);
@@ -621,13 +625,41 @@ extern "C" int phlx_rk_update(phlx_ctx *ctx, int32_t irk, double a1, double a2, 
   // stage that runs sets them, so a step past tmax leaves them unchanged
   const int first = irk == 0 ? 1 : 0;
   const int fin = irk == ctx->stages - 1;
-  if (guess)
-    ctx->kt.update_guess(ctx->g, first, last_stage != 0, a1, a2, a3, fin, ctx->stream);
+  const int last = last_stage != 0;
+  if (swept && guess)
+    ctx->kt.update_swept_guess(ctx->g, first, last, a1, a2, a3, fin, ctx->stream);
+  else if (swept)
+    ctx->kt.update_swept(ctx->g, first, last, a1, a2, a3, fin, ctx->stream);
+  else if (guess)
+    ctx->kt.update_guess(ctx->g, first, last, a1, a2, a3, fin, ctx->stream);
   else
-    ctx->kt.update(ctx->g, first, last_stage != 0, a1, a2, a3, fin, ctx->stream);
+    ctx->kt.update(ctx->g, first, last, a1, a2, a3, fin, ctx->stream);
   if (fin && ctx->kt.prims) ctx->prim_stale = true;
   kend(ctx, on, r);
   return check_launch();
+}
+
+extern "C" int phlx_rk_update(phlx_ctx *ctx, int32_t irk, double a1, double a2, double a3,
+                              int32_t last_stage) {
+  if (!ctx) return PHLX_E_ARG;
+  return rk_update(ctx, irk, a1, a2, a3, last_stage, false);
+}
+
+// Whether hydro_step takes the fused sweep: a family that has one, no shock
+// flattening (its flagged cells switch scheme per face), and x1 tiles that
+// divide the grid.
+static bool use_fused_sweep(const phlx_ctx *ctx) {
+  const Grid &g = ctx->g;
+  const bool want = ctx->fused_sweep < 0 ? ctx->cfg.precision == PHLX_PREC_MIXED
+                                          : ctx->fused_sweep != 0;
+  return want && ctx->kt.sweep && !g.sflat && g.sd == 3 && g.n1 % kSweepT == 0 &&
+         (!ctx->kt.update_guess || ctx->kt.update_swept_guess);
+}
+
+extern "C" int phlx_set_fused_sweep(phlx_ctx *ctx, int32_t mode) {
+  if (!ctx || mode < -1 || mode > 1) return PHLX_E_ARG;
+  ctx->fused_sweep = mode;
+  return PHLX_OK;
 }
 
 static int diffusion_step(phlx_ctx *ctx, int strang);
@@ -645,17 +677,28 @@ static int materialize(phlx_ctx *ctx) {
 }
 
 static int do_hydro_step(phlx_ctx *ctx, bool with_cfl) {
+  const bool fused = use_fused_sweep(ctx);
   for (int irk = 0; irk < ctx->stages; irk++) {
     int rc = phlx_stage_begin(ctx, irk);
     if (rc) return rc;
-    for (int d = 0; d < ctx->g.sd; d++) {
-      rc = phlx_sweep(ctx, d);
-      if (rc) return rc;
-    }
     const double a1 = ctx->rk[irk][0], a2 = ctx->rk[irk][1], a3 = ctx->rk[irk][2];
+    if (fused) {
+      ProfRec r;
+      int on = kbegin(ctx, KID_SWEEP, r);
+      ctx->kt.sweep(ctx->g, irk == 0 ? 1 : 0, irk == ctx->stages - 1 ? 1 : 0, a1, a2, a3,
+                    ctx->stream);
+      kend(ctx, on, r);
+      rc = check_launch();
+      if (rc) return rc;
+    } else {
+      for (int d = 0; d < ctx->g.sd; d++) {
+        rc = phlx_sweep(ctx, d);
+        if (rc) return rc;
+      }
+    }
     rc = phlx_ct(ctx, irk, a1, a2, a3);
     if (rc) return rc;
-    rc = phlx_rk_update(ctx, irk, a1, a2, a3, with_cfl && irk == ctx->stages - 1);
+    rc = rk_update(ctx, irk, a1, a2, a3, with_cfl && irk == ctx->stages - 1, fused);
     if (rc) return rc;
   }
   return PHLX_OK;

@@ -7,6 +7,9 @@
 
 namespace phlx {
 
+// x1 tile edge of the fused sweep: it runs where it divides the grid
+constexpr int kSweepT = 16;
+
 // Launchers of one template instance (space dims, advected scalars, MHD).
 struct KernelTable {
   void (*flux)(const Grid &, int, hipStream_t);
@@ -20,6 +23,12 @@ struct KernelTable {
   // mixed precision: refresh the FP32 sweep inputs from the FP64 state
   void (*pack)(const Grid &, hipStream_t);
   void (*prims)(const Grid &, hipStream_t);   // mixed: FP64 prim / eint / b_cc from U
+  // fused sweep (3D, 2nd-order reconstructions, ideal / FASTEOS states;
+  // null otherwise): one pass forms the stage's conserved state, the swept
+  // updates finish it after CT
+  void (*sweep)(const Grid &, int, int, double, double, double, hipStream_t);
+  void (*update_swept)(const Grid &, int, int, double, double, double, int, hipStream_t);
+  void (*update_swept_guess)(const Grid &, int, int, double, double, double, int, hipStream_t);
 };
 
 bool select_kernels(int sd, int nas, bool mhd, bool helm, bool fasteos, bool mixed, bool r5,
@@ -82,7 +91,7 @@ void launch_helm_rhoe_guess(const HelmDev &h, const MlpDev &m, int mode, long lo
 
 enum {
   KID_FLUX1 = 0, KID_FLUX2, KID_FLUX3, KID_CORNER, KID_FACEB, KID_UPDATE, KID_MISC, KID_DIFF,
-  KID_COUNT
+  KID_SWEEP, KID_COUNT
 };
 
 }  // namespace phlx

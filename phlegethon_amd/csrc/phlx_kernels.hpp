@@ -79,10 +79,12 @@ __device__ __forceinline__ double face_temp_guess(const Grid &g, int i, int j, i
 // and the FASTEOS gammas reconstructed, the normal B read here): rotation
 // into the face frame, WB add-back, Riemann solve, rotation back, EMF and
 // contact sign (MHD), WB flux correction, ETOT potential flux, store.
-template <int DIR, int SD, int NAS, bool MHD, int EK, bool SFL = false>
+// OUT: the fluxes go to Fo instead of the face array (the fused sweep).
+template <int DIR, int SD, int NAS, bool MHD, int EK, bool SFL = false, bool OUT = false>
 __device__ __forceinline__ void flux_face(const Grid &g, int i, int j, int k, double *qL,
                                           double *qR, double *bL, double *bR, double *gL,
-                                          double *gR, double Tg, bool hll = false) {
+                                          double *gR, double Tg, bool hll = false,
+                                          double *Fo = nullptr) {
   constexpr int NV = SD + 2 + NAS, IE = SD + 1;
   const long long f = fix(g, DIR, i, j, k);
   if (MHD) {
@@ -159,8 +161,13 @@ __device__ __forceinline__ void flux_face(const Grid &g, int i, int j, int k, do
     const double phi = (DIR == 1) ? g.phi_f[jy] : g.phi_cc[jy];
     F[IE] = F[IE] + F[0] * phi;
   }
+  if constexpr (OUT) {
 #pragma unroll
-  for (int v = 0; v < NV; v++) g.F[DIR][v * nf + f] = F[v];
+    for (int v = 0; v < NV; v++) Fo[v] = F[v];
+  } else {
+#pragma unroll
+    for (int v = 0; v < NV; v++) g.F[DIR][v * nf + f] = F[v];
+  }
 }
 
 // One thread per face along DIR.  Loads the four cells f-2..f+1, builds the
@@ -388,10 +395,11 @@ __global__ void __launch_bounds__(256) k_flux_seg(Grid g) {
 // equilibrium is added back in FP64 before the cast; fluxes are stored as
 // FP32 and accumulated in FP64 by k_update.
 // The mixed-precision face solve from its FP32 face states (as flux_face).
-template <int DIR, int SD, int NAS, bool MHD, bool HELM>
+// OUT: the fluxes go to Fo instead of the face array (the fused sweep).
+template <int DIR, int SD, int NAS, bool MHD, bool HELM, bool OUT = false>
 __device__ __forceinline__ void flux_face_mx(const Grid &g, int i, int j, int k, float *qL,
                                              float *qR, float *bL, float *bR, float *gL,
-                                             float *gR) {
+                                             float *gR, float *Fo = nullptr) {
   constexpr int NV = SD + 2 + NAS, IE = SD + 1;
   const long long f = fix(g, DIR, i, j, k);
   if (MHD) {
@@ -463,8 +471,13 @@ __device__ __forceinline__ void flux_face_mx(const Grid &g, int i, int j, int k,
     const double phi = (DIR == 1) ? g.phi_f[jy] : g.phi_cc[jy];
     F[IE] = (float)((double)F[IE] + (double)F[0] * phi);
   }
+  if constexpr (OUT) {
 #pragma unroll
-  for (int v = 0; v < NV; v++) g.Ff[DIR][v * nf + f] = F[v];
+    for (int v = 0; v < NV; v++) Fo[v] = F[v];
+  } else {
+#pragma unroll
+    for (int v = 0; v < NV; v++) g.Ff[DIR][v * nf + f] = F[v];
+  }
 }
 
 // Mixed-precision column segments (as k_flux_seg): cells are loaded and the
@@ -947,7 +960,7 @@ __device__ __forceinline__ double flux_at(const Grid &g, int d, long long idx) {
 #ifndef PHLX_HELM_MIX_UPDATE_WAVES
 #define PHLX_HELM_MIX_UPDATE_WAVES 2
 #endif
-template <int SD, int NAS, bool MHD, bool HELM, bool MIX, bool GUESS = false>
+template <int SD, int NAS, bool MHD, bool HELM, bool MIX, bool GUESS = false, bool SWEPT = false>
 __global__ void __launch_bounds__(256)
     __attribute__((amdgpu_waves_per_eu(
         HELM ? (GUESS ? 2 : (MIX ? PHLX_HELM_MIX_UPDATE_WAVES : PHLX_HELM_UPDATE_WAVES)) : 1)))
@@ -958,6 +971,8 @@ __global__ void __launch_bounds__(256)
   // last: the CFL maximum of the new state.  fin: the step's last stage.
   // The mixed path writes the FP64 primitives, eint, B_cc and the gammas
   // only at fin (between its stages the sweeps read the FP32 copy g.Pf).
+  // SWEPT: the fused sweep (k_sweep) has already formed the stage's new
+  // conserved state in U; this launch is cons -> prim, EoS and CFL only.
   constexpr int NV = SD + 2 + NAS, IE = SD + 1, IA = SD + 2;
   if (!g.sc->active) return;
   const bool from_u = MIX && first && (g.sc->flags & SC_U_CUR);
@@ -983,6 +998,10 @@ __global__ void __launch_bounds__(256)
   if (in) {
     const long long N = g.N, c = cix(g, i, j, k);
     double S[NV], S0[NV], R[NV], p[NV];
+    if constexpr (SWEPT) {
+#pragma unroll
+      for (int v = 0; v < NV; v++) S[v] = out[v * N + c];
+    } else {
     // the stored primitives are read only to rebuild state0 (stage 1)
     if (first == 1) {
 #pragma unroll
@@ -1059,6 +1078,7 @@ __global__ void __launch_bounds__(256)
     const double a3 = c3 * g.sc->dt;
 #pragma unroll
     for (int v = 0; v < NV; v++) S[v] = a1 * S0[v] + a2 * S[v] + a3 * R[v];
+    }  // !SWEPT
     double b[3] = {0.0, 0.0, 0.0};
     if (MHD) {
       // source/source.F90:12439-12489
@@ -1410,6 +1430,343 @@ __global__ void __launch_bounds__(256) k_divb(Grid g) {
   block_max_to(s, g.red);
 }
 
+// ------------------------------------------------------------ fused sweep
+// One RK stage's reconstruction, Riemann fluxes and flux divergence in one
+// pass (3D, the 2nd-order reconstructions, ideal or FASTEOS states; FP64 or
+// the mixed-precision policy).  A block of 256 threads owns a 16 x 15
+// (x1, x2) tile of columns and walks a chunk of kSweepK planes along x3.
+// Per plane the tile plus a 2-cell halo of sweep inputs (WB deviations,
+// cell B, FASTEOS gammas: what load_cell / load_cell_f return) is staged in
+// LDS, SoA.  The tile's 17 x 15 x1 faces, 16 x 16 x2 faces and 16 x 15 x3
+// faces are three rounds of at most one face per thread: the x1 / x2 faces
+// from the LDS stencil, the x3 face (the upper face of the thread's column)
+// from a register window of the cells above, each cell loaded and
+// reconstructed once along x3 as the column-segment sweep does.  The x1 /
+// x2 fluxes go through LDS to the two cells beside each face; the x3 flux
+// of the face below is carried in registers.  The divergence keeps the
+// order of the multi-kernel update (x1 overwrites, x2 then x3 accumulate;
+// the mixed path's FP32 fluxes widened to FP64 first), the face arithmetic
+// is flux_face's / flux_face_mx's and every face state that of the
+// per-direction sweeps, so the result is bit for bit that of the separate
+// sweeps and update.  Face fluxes never reach HBM; the face EMFs and
+// contact signs (MHD) are stored for CT (a face on a tile edge is solved by
+// both tiles beside it and stored twice with the same bits), and the RK
+// combination a1 S0 + a2 S + a3 dt R is written where k_update would write
+// it (stage 1 also rebuilds state0 as k_update does, or, mixed, takes it
+// from U when the device flags say U is current).  k_update<..., SWEPT>
+// then runs cons -> prim, the EoS and the CFL after CT.
+// Requires n1 % 16 == 0 (checked by the host); a last x2 tile past n2
+// masks its missing rows.
+#ifndef PHLX_SWEEP_KC
+#define PHLX_SWEEP_KC 32
+#endif
+constexpr int kSweepK = PHLX_SWEEP_KC, kSweepTy = 15;
+template <int D>
+struct SweepDir {
+  static constexpr int value = D;
+};
+// waves per SIMD the register allocation aims at (FP64; the FP32 sweeps)
+#ifndef PHLX_SWEEP_WAVES
+#define PHLX_SWEEP_WAVES 2
+#endif
+#ifndef PHLX_SWEEP_WAVES_MX
+#define PHLX_SWEEP_WAVES_MX 2
+#endif
+
+template <int NAS, bool MHD, int EK, bool MIX>
+__global__ void __launch_bounds__(256)
+    __attribute__((amdgpu_waves_per_eu(MIX ? PHLX_SWEEP_WAVES_MX : PHLX_SWEEP_WAVES)))
+    k_sweep(Grid g, int first, int fin, double a1, double a2, double c3) {
+  constexpr int SD = 3, NV = SD + 2 + NAS, IE = SD + 1, IA = SD + 2;
+  constexpr bool GAM = EK == EK_FASTEOS;
+  static_assert(EK != EK_HELM_FACE, "per-face Newton states are not staged");
+  using Re = typename std::conditional<MIX, float, double>::type;
+  constexpr int TX = kSweepT, TY = kSweepTy;     // cells per tile
+  constexpr int W = TX + 4, WW = W * (TY + 4);   // LDS row, cells per variable
+  constexpr int OB = NV, OG = NV + 3, NC = NV + 3 + (GAM ? 2 : 0);  // LDS slots per cell
+  constexpr int NF1 = TY * (TX + 1), NF2 = (TY + 1) * TX;           // x1 / x2 faces per plane
+  __shared__ Re tile[NC * WW];
+  __shared__ Re f1[NV * NF1];  // x1 faces: [v][row b][face a = 0..16]
+  __shared__ Re f2[NV * NF2];  // x2 faces: [v][face b = 0..15][column a]
+  if (!g.sc->active) return;
+  const int tid = threadIdx.x;
+  const int ntx = g.n1 / TX;
+  const int i0 = (blockIdx.x % ntx) * TX, j0 = (blockIdx.x / ntx) * TY;
+  const int k0 = blockIdx.y * kSweepK, k1 = min(k0 + kSweepK, g.n3);
+  const long long N = g.N;
+  // thread (a, b): column (i0 + a, j0 + b) for b < TY (the last thread row
+  // has no column; it takes the tile's upper x1 faces)
+  const bool own = (tid >> 4) < TY;
+
+  struct Cell {
+    Re q[NV], b[3], ga[2];
+  };
+  // rows past n2 + 1 (a partial last tile) are read as row n2 + 1: no
+  // face inside the grid reaches them, and the ghost map covers two rows
+  auto load = [&](int ii, int jj, int kk, Cell &c) {
+    jj = min(jj, g.n2 + 1);
+    const bool in = ii >= 0 && ii < g.n1 && jj >= 0 && jj < g.n2 && kk >= 0 && kk < g.n3;
+    if constexpr (MIX) {
+      if (__all(in))
+        load_cell_f<SD, NV, MHD, GAM, true>(g, ii, jj, kk, c.q, c.b, c.ga);
+      else
+        load_cell_f<SD, NV, MHD, GAM, false>(g, ii, jj, kk, c.q, c.b, c.ga);
+    } else {
+      if (__all(in))
+        load_cell<SD, NV, MHD, GAM, true>(g, ii, jj, kk, c.q, c.b, c.ga);
+      else
+        load_cell<SD, NV, MHD, GAM, false>(g, ii, jj, kk, c.q, c.b, c.ga);
+    }
+  };
+  auto put = [&](int cell, const Cell &c) {
+#pragma unroll
+    for (int v = 0; v < NV; v++) tile[v * WW + cell] = c.q[v];
+    if (MHD) {
+#pragma unroll
+      for (int v = 0; v < 3; v++) tile[(OB + v) * WW + cell] = c.b[v];
+    }
+    if (GAM) {
+      tile[OG * WW + cell] = c.ga[0];
+      tile[(OG + 1) * WW + cell] = c.ga[1];
+    }
+  };
+  auto states = [&](bool lim, const Re *w, int st, Re &qL, Re &qR) {
+    if constexpr (MIX)
+      mx::face_states<false>(g.recon, lim, w, st, qL, qR);
+    else
+      face_states<false>(g.recon, lim, w, st, qL, qR);
+  };
+  auto edges = [&](bool lim, Re qm, Re q0, Re qp, Re &l, Re &r) {
+    if constexpr (MIX)
+      cell_edges_mx(g.recon, lim, qm, q0, qp, l, r);
+    else
+      cell_edges(g.recon, lim, qm, q0, qp, l, r);
+  };
+  auto solve = [&](auto dirc, int ii, int jj, int kk, Re *qL, Re *qR, Re *bL, Re *bR, Re *gL,
+                   Re *gR, Re *Fo) {
+    constexpr int DIR = decltype(dirc)::value;
+    if constexpr (MIX)
+      flux_face_mx<DIR, SD, NAS, MHD, GAM, true>(g, ii, jj, kk, qL, qR, bL, bR, gL, gR, Fo);
+    else
+      flux_face<DIR, SD, NAS, MHD, EK, false, true>(g, ii, jj, kk, qL, qR, bL, bR, gL, gR, 0.0,
+                                                    false, Fo);
+  };
+  // the in-plane face (DIR 0: lower x1 face of tile cell (fa, fb); DIR 1:
+  // lower x2 face) from the LDS stencil, its fluxes into the LDS face array
+  auto plane_face = [&](auto dirc, int fa, int fb, int kk) {
+    constexpr int DIR = decltype(dirc)::value;
+    const int base = DIR == 0 ? (fb + 2) * W + fa : fb * W + fa + 2;
+    constexpr int st = DIR == 0 ? 1 : W;
+    Re qL[NV], qR[NV], bL[3] = {0, 0, 0}, bR[3] = {0, 0, 0}, gL[2] = {0, 0}, gR[2] = {0, 0};
+    Re Fo[NV];
+#pragma unroll
+    for (int v = 0; v < NV; v++) states(v > IE, &tile[v * WW + base], st, qL[v], qR[v]);
+    if (MHD) {
+#pragma unroll
+      for (int c = 0; c < SD; c++) {
+        if (c == DIR) continue;
+        states(false, &tile[(OB + c) * WW + base], st, bL[c], bR[c]);
+      }
+    }
+    if (GAM) {
+#pragma unroll
+      for (int c = 0; c < 2; c++) states(false, &tile[(OG + c) * WW + base], st, gL[c], gR[c]);
+    }
+    solve(dirc, i0 + fa, j0 + fb, kk, qL, qR, bL, bR, gL, gR, Fo);
+    Re *dst = DIR == 0 ? f1 + fb * (TX + 1) + fa : f2 + fb * TX + fa;
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+      dst[0] = Fo[v];
+      dst += DIR == 0 ? NF1 : NF2;
+    }
+  };
+
+  // prologue: cells k0-2 .. k0+1 of this column; the right edges of cell k0
+  // and the flux of face k0 (lower x3 face of the first plane).  A column
+  // past n2 (a last, partial tile) only stages its ghost cells.
+  Cell nxt;
+  Re rq[NV], rb[3] = {0, 0, 0}, rg[2] = {0, 0}, F3m[NV];
+  if (own) {
+    const int a = tid & (TX - 1), b = tid >> 4, i = i0 + a, j = j0 + b;
+    Cell cA, cB, cC;
+    load(i, j, k0 - 2, cA);
+    load(i, j, k0 - 1, cB);
+    load(i, j, k0, cC);
+    load(i, j, k0 + 1, nxt);
+    Re l, qL[NV], qR[NV], bL[3] = {0, 0, 0}, bR[3] = {0, 0, 0}, gL[2] = {0, 0}, gR[2] = {0, 0};
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+      edges(v > IE, cA.q[v], cB.q[v], cC.q[v], l, qL[v]);
+      edges(v > IE, cB.q[v], cC.q[v], nxt.q[v], qR[v], rq[v]);
+    }
+    if (MHD) {
+#pragma unroll
+      for (int c = 0; c < 2; c++) {
+        edges(false, cA.b[c], cB.b[c], cC.b[c], l, bL[c]);
+        edges(false, cB.b[c], cC.b[c], nxt.b[c], bR[c], rb[c]);
+      }
+    }
+    if (GAM) {
+#pragma unroll
+      for (int c = 0; c < 2; c++) {
+        edges(false, cA.ga[c], cB.ga[c], cC.ga[c], l, gL[c]);
+        edges(false, cB.ga[c], cC.ga[c], nxt.ga[c], gR[c], rg[c]);
+      }
+    }
+    if (j < g.n2) solve(SweepDir<2>{}, i, j, k0, qL, qR, bL, bR, gL, gR, F3m);
+    put((b + 2) * W + a + 2, cC);
+  }
+
+  // the RK buffers of k_update: the FP64 path saves state0 in U0 and carries
+  // the stage state in U; the mixed path keeps the step's start state in U
+  // and the stage state in U0 until the last stage writes U
+  double *const s0_buf = MIX ? g.U : g.U0;
+  double *const s_buf = MIX ? g.U0 : g.U;
+  double *const out = (MIX && !fin) ? g.U0 : g.U;
+  const bool from_u = MIX && first && (g.sc->flags & SC_U_CUR);
+
+#pragma unroll 1
+  for (int k = k0; k < k1; k++) {
+    // the lane's indices re-materialised per plane: hoisting every
+    // plane-invariant address and profile value of the face solves out of
+    // the loop costs more registers than the kernel has
+    int lt = tid;
+    asm volatile("" : "+v"(lt));
+    const int la = lt & (TX - 1), lb = lt >> 4, li = i0 + la, lj = j0 + lb;
+    const bool col = lb < TY && lj < g.n2;  // a column inside the grid
+    // the halo of plane k (the interior was put by the previous iteration):
+    // x1 halo cells -2, -1, 16, 17 of rows 0..14, x2 halo rows -2, -1, 15, 16
+    if (lt < 128 && (lt >= 64 || lt < 4 * TY)) {
+      int ha, hb;
+      if (lt < 64) {
+        const int s = lt & 3;
+        ha = s < 2 ? s - 2 : TX - 2 + s;
+        hb = lt >> 2;
+      } else {
+        const int s = (lt - 64) >> 4;
+        ha = lt & (TX - 1);
+        hb = s < 2 ? s - 2 : TY - 2 + s;
+      }
+      Cell h;
+      load(i0 + ha, j0 + hb, k, h);
+      put((hb + 2) * W + ha + 2, h);
+    }
+    Cell cp;
+    if (lb < TY) load(li, lj, k + 2, cp);
+    __syncthreads();
+    // x3: face k+1 of this column from cells k (LDS), k+1, k+2
+    double X3[NV];
+    if (lb < TY) {
+      Re qL[NV], qR[NV], bL[3] = {0, 0, 0}, bR[3] = {0, 0, 0}, gL[2] = {0, 0}, gR[2] = {0, 0};
+      Re F3p[NV];
+      const Re *qm = &tile[(lb + 2) * W + la + 2];
+#pragma unroll
+      for (int v = 0; v < NV; v++) {
+        qL[v] = rq[v];
+        edges(v > IE, qm[v * WW], nxt.q[v], cp.q[v], qR[v], rq[v]);
+      }
+      if (MHD) {
+#pragma unroll
+        for (int c = 0; c < 2; c++) {
+          bL[c] = rb[c];
+          edges(false, qm[(OB + c) * WW], nxt.b[c], cp.b[c], bR[c], rb[c]);
+        }
+      }
+      if (GAM) {
+#pragma unroll
+        for (int c = 0; c < 2; c++) {
+          gL[c] = rg[c];
+          edges(false, qm[(OG + c) * WW], nxt.ga[c], cp.ga[c], gR[c], rg[c]);
+        }
+      }
+      if (col) {
+        solve(SweepDir<2>{}, li, lj, k + 1, qL, qR, bL, bR, gL, gR, F3p);
+#pragma unroll
+        for (int v = 0; v < NV; v++) {
+          X3[v] = ((double)F3p[v] - (double)F3m[v]) * g.inv_dx[2];
+          F3m[v] = F3p[v];
+        }
+      }
+    }
+    // in-plane faces, one of each direction per thread: x1 the lower face
+    // of cell (la, lb) (the last thread row: the upper face of row la), x2
+    // the lower face of row lb (16 faces: row 15's is the tile's upper face)
+    {
+      const int fa = lb < TY ? la : TX, fb = lb < TY ? lb : la;
+      if (fb < TY && j0 + fb < g.n2) plane_face(SweepDir<0>{}, fa, fb, k);
+    }
+    if (lj <= g.n2) plane_face(SweepDir<1>{}, la, lb, k);
+    __syncthreads();
+    // divergence (x1 overwrites, x2 and x3 accumulate), sources, RK
+    if (col) {
+      const long long c = cix(g, li, lj, k);
+      double S[NV], S0[NV], R[NV];
+#pragma unroll
+      for (int v = 0; v < NV; v++) {
+        const Re *F1 = f1 + v * NF1 + lb * (TX + 1) + la;
+        const Re *F2 = f2 + v * NF2 + lb * TX + la;
+        R[v] = ((double)F1[1] - (double)F1[0]) * g.inv_dx[0];
+        R[v] = R[v] + ((double)F2[TX] - (double)F2[0]) * g.inv_dx[1];
+        R[v] = R[v] + X3[v];
+      }
+      if (from_u) {
+#pragma unroll
+        for (int v = 0; v < NV; v++) {
+          S0[v] = g.U[v * N + c];
+          S[v] = S0[v];
+        }
+      } else if (first) {
+        // state0 from the primitives (source/source.F90:7057-7117)
+        double p[NV];
+#pragma unroll
+        for (int v = 0; v < NV; v++) p[v] = g.P[v * N + c];
+        const double rho = p[0], v1 = p[1], v2 = p[2], v3 = p[3];
+        S0[0] = rho;
+        S0[1] = rho * v1;
+        S0[2] = rho * v2;
+        S0[3] = rho * v3;
+        double E = g.eint[c] + 0.5 * rho * (v1 * v1 + v2 * v2 + v3 * v3);
+        if (MHD) {
+          const double b1 = g.bcc[c], b2 = g.bcc[N + c], b3 = g.bcc[2 * N + c];
+          E = E + 0.5 * (b1 * b1 + b2 * b2 + b3 * b3);
+        }
+        if (g.etot) E = E + rho * g.phi_cc[lj];
+        S0[IE] = E;
+#pragma unroll
+        for (int v = IA; v < NV; v++) S0[v] = rho * p[v];
+#pragma unroll
+        for (int v = 0; v < NV; v++) {
+          S[v] = S0[v];
+          s0_buf[v * N + c] = S0[v];
+        }
+      } else {
+#pragma unroll
+        for (int v = 0; v < NV; v++) {
+          S0[v] = s0_buf[v * N + c];
+          S[v] = s_buf[v * N + c];
+        }
+      }
+      if (g.grav) {
+        const double r0 = S[0];
+        const double rho = g.wb ? (r0 - g.eq_rho_cc[lj]) : r0;
+        const double gy = g.gravy[lj];
+        R[2] = R[2] - rho * gy;
+        if (!g.etot) R[IE] = R[IE] - S[2] * gy;
+      }
+      if (g.edot) R[IE] = R[IE] - g.edot_cc[lj];
+      const double a3 = c3 * g.sc->dt;
+#pragma unroll
+      for (int v = 0; v < NV; v++) out[v * N + c] = a1 * S0[v] + a2 * S[v] + a3 * R[v];
+    }
+    // plane k+1's interior (every read of plane k's tile is behind the
+    // barrier above; the next reads are behind the next iteration's first)
+    if (lb < TY) {
+      if (k + 1 < k1) put((lb + 2) * W + la + 2, nxt);
+      nxt = cp;
+    }
+  }
+}
+
 // ------------------------------------------------------- shock flattening
 // USE_SHOCK_FLATTENING detector (source/source.F90:8073-8131).  k_sf_detect:
 // one thread per cell of -2 .. n+1 on each axis marks a cell whose summed
@@ -1612,6 +1969,33 @@ struct Kern {
     k_update<SD, NAS, MHD, HELM, MIX, true><<<grid_flat(g.n1, g.n2, g.n3, blk), blk, 0, st>>>(
         g, first, last, a1, a2, c3, fin);
   }
+  // the fused sweep and the update that finishes its stage after CT
+  // (3D FP64 with the 2nd-order reconstructions and ideal / FASTEOS states)
+  static constexpr bool kSwept = SD == 3 && !R5 && EK != EK_HELM_FACE;
+  static void sweep(const Grid &g, int first, int fin, double a1, double a2, double c3,
+                    hipStream_t st) {
+    if constexpr (kSwept) {
+      const dim3 grd((unsigned)((g.n1 / kSweepT) * ((g.n2 + kSweepTy - 1) / kSweepTy)),
+                     (unsigned)((g.n3 + kSweepK - 1) / kSweepK), 1);
+      k_sweep<NAS, MHD, EK, MIX><<<grd, 256, 0, st>>>(g, first, fin, a1, a2, c3);
+    }
+  }
+  static void update_swept(const Grid &g, int first, int last, double a1, double a2, double c3,
+                           int fin, hipStream_t st) {
+    if constexpr (kSwept) {
+      const dim3 blk(64, 4, 1);
+      k_update<SD, NAS, MHD, HELM, MIX, false, true>
+          <<<grid_flat(g.n1, g.n2, g.n3, blk), blk, 0, st>>>(g, first, last, a1, a2, c3, fin);
+    }
+  }
+  static void update_swept_guess(const Grid &g, int first, int last, double a1, double a2,
+                                 double c3, int fin, hipStream_t st) {
+    if constexpr (kSwept && HELM) {
+      const dim3 blk(64, 4, 1);
+      k_update<SD, NAS, MHD, HELM, MIX, true, true>
+          <<<grid_flat(g.n1, g.n2, g.n3, blk), blk, 0, st>>>(g, first, last, a1, a2, c3, fin);
+    }
+  }
   static void cfl(const Grid &g, hipStream_t st) {
     const dim3 blk(64, 4, 1);
     k_cfl<SD, MHD, HELM><<<grid_for(g.n1, g.n2, g.n3, blk), blk, 0, st>>>(g);
@@ -1657,6 +2041,11 @@ static KernelTable make_table() {
   t.divb = &K::divb;
   t.pack = MIX ? &K::pack : nullptr;
   t.prims = MIX ? &K::prims : nullptr;
+  if constexpr (K::kSwept) {
+    t.sweep = &K::sweep;
+    t.update_swept = &K::update_swept;
+    t.update_swept_guess = HELM ? &K::update_swept_guess : nullptr;
+  }
   return t;
 }
 

@@ -77,6 +77,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.phlx_sweep.argtypes = [P, I]
     lib.phlx_ct.argtypes = [P, I, D, D, D]
     lib.phlx_rk_update.argtypes = [P, I, D, D, D, I]
+    lib.phlx_set_fused_sweep.argtypes = [P, I]
     lib.phlx_divb_max.argtypes = [P, DP]
     lib.phlx_profile_enable.argtypes = [P, I]
     lib.phlx_profile_read.argtypes = [P, I, DP, ctypes.POINTER(ctypes.c_int64)]
@@ -126,7 +127,7 @@ MLP_NPARAM = 4545
 GUESS_MODES = {"warm": 0, "mlp": 1, "bracket": 2, "pred": 3}
 
 KERNELS = {"flux_x1": 0, "flux_x2": 1, "flux_x3": 2, "corner": 3, "faceb": 4,
-           "update": 5, "misc": 6, "diffusion": 7}
+           "update": 5, "misc": 6, "diffusion": 7, "sweep": 8}
 
 
 class Simulation:
@@ -297,6 +298,13 @@ class Simulation:
 
     def rk_update(self, irk, a1, a2, a3, last):
         _chk(self.lib.phlx_rk_update(self._ctx, irk, a1, a2, a3, int(last)), "phlx_rk_update")
+
+    def set_fused_sweep(self, on):
+        """hydro_step with the fused one-pass sweep (True), the three
+        directional sweeps (False) or the library's choice (None: fused on
+        the mixed-precision path); the results are the same bit for bit."""
+        mode = -1 if on is None else int(bool(on))
+        _chk(self.lib.phlx_set_fused_sweep(self._ctx, mode), "phlx_set_fused_sweep")
 
     def stream_copy(self, nbytes: int = 1 << 30, reps: int = 10) -> float:
         """Achieved HBM GB/s of a plain device copy (read + write)."""
