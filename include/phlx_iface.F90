@@ -224,6 +224,13 @@ module phlx_iface
       integer(c_int32_t), value :: enable
       real(c_double), value :: eps_sf
     end function
+    ! hydro_step's fused one-pass sweep: 1 wherever admitted, 0 never, -1 default
+    integer(c_int) function phlx_set_fused_sweep(ctx, mode) &
+        bind(c, name="phlx_set_fused_sweep")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: mode
+    end function
     ! box(8): log10(rho*Ye) lo, hi, log10 T lo, hi, abar lo, hi, Ye lo, hi
     integer(c_int) function phlx_mlp_set_train_box(ctx, box) bind(c, name="phlx_mlp_set_train_box")
       import :: c_int, c_ptr

@@ -10,8 +10,8 @@ steps, in FP64 and in the mixed-precision path, since every face state,
 face solve and divergence is the same arithmetic in the same order; and
 the FP64 fused path meets the parity bar of tests/test_gpu_parity.py
 against the oracle (the mixed path's own bar against the oracle is
-tests/test_gpu_mixed.py's, whose 3D cases with x1, x2 multiples of 16 run
-the fused sweep, the default there).  Cases cover hydro and MHD,
+tests/test_gpu_mixed.py's, whose 3D hydro cases with x1 a multiple of 16
+run the fused sweep, the default there).  Cases cover hydro and MHD,
 LIM2ND and PPH, HLLC / LHLLC / HLLD / LHLLD, periodic, reflecting, outflow
 and diode walls, WB + gravity + ETOT + EDOT, RK2 and RK3, the Helmholtz
 FASTEOS states (with and without MHD, with the Newton-counting update), a
@@ -194,3 +194,22 @@ def test_mixed_helmholtz_mhd_slab(oracle_mod, slab_table, guess):
     its = st["eos"]["newton_iters"] / st["eos"]["solves"]
     print(f"[sweep] mixed Helmholtz MHD, {guess} start: Newton iterations/solve {its:.3f}",
           flush=True)
+
+
+def test_default_choice():
+    """The library's choice: fused on the mixed path without MHD, split for
+    MHD and in FP64 (mode -1)."""
+    from phlegethon_amd.runtime import Simulation
+    cases = [(_mixed(P.lowmach_box(n=16, mach=1e-3, heating=1e-9)), True),
+             (_mixed(P.orszag_tang_3d(n=16)), False),
+             (P.lowmach_box(n=16, mach=1e-3, heating=1e-9), False)]
+    for pb, fused in cases:
+        sim = Simulation(pb.cfg)
+        sim.setup(pb)
+        sim.set_fused_sweep(None)
+        sim.profile(True)
+        sim.init()
+        sim.run(1)
+        swept = sim.profile_read("sweep")[1]
+        sim.close()
+        assert (swept > 0) == fused, (pb.name, pb.cfg.precision, swept)
