@@ -476,8 +476,8 @@ def main():
     ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64")
     ap.add_argument("--sweep", choices=("auto", "fused", "split"), default="auto",
                     help="3D runs: the fused one-pass sweep, the three directional sweeps "
-                         "(same results bit for bit), or the library's choice (fused on the "
-                         "mixed-precision path without MHD)")
+                         "(same results bit for bit), or the library's choice (fused in "
+                         "runs without MHD)")
     ap.add_argument("--eos-guess", choices=("warm", "mlp", "pred"), default="warm",
                     help="Helmholtz: Newton start (carried T, the trained surrogate, or the "
                          "energy predictor)")

@@ -11,7 +11,7 @@ face solve and divergence is the same arithmetic in the same order; and
 the FP64 fused path meets the parity bar of tests/test_gpu_parity.py
 against the oracle (the mixed path's own bar against the oracle is
 tests/test_gpu_mixed.py's, whose 3D hydro cases with x1 a multiple of 16
-run the fused sweep, the default there).  Cases cover hydro and MHD,
+run the fused sweep, the default without MHD).  Cases cover hydro and MHD,
 LIM2ND and PPH, HLLC / LHLLC / HLLD / LHLLD, periodic, reflecting, outflow
 and diode walls, WB + gravity + ETOT + EDOT, RK2 and RK3, the Helmholtz
 FASTEOS states (with and without MHD, with the Newton-counting update), a
@@ -197,12 +197,13 @@ def test_mixed_helmholtz_mhd_slab(oracle_mod, slab_table, guess):
 
 
 def test_default_choice():
-    """The library's choice: fused on the mixed path without MHD, split for
-    MHD and in FP64 (mode -1)."""
+    """The library's choice (mode -1): fused without MHD, in FP64 and in
+    mixed precision; split with MHD."""
     from phlegethon_amd.runtime import Simulation
     cases = [(_mixed(P.lowmach_box(n=16, mach=1e-3, heating=1e-9)), True),
              (_mixed(P.orszag_tang_3d(n=16)), False),
-             (P.lowmach_box(n=16, mach=1e-3, heating=1e-9), False)]
+             (P.lowmach_box(n=16, mach=1e-3, heating=1e-9), True),
+             (P.orszag_tang_3d(n=16), False)]
     for pb, fused in cases:
         sim = Simulation(pb.cfg)
         sim.setup(pb)
