@@ -1483,9 +1483,15 @@ __global__ void __launch_bounds__(256)
   using Re = typename std::conditional<MIX, float, double>::type;
   constexpr int TX = kSweepT, TY = kSweepTy;     // cells per tile
   constexpr int W = TX + 4, WW = W * (TY + 4);   // LDS row, cells per variable
-  constexpr int OB = NV, OG = NV + 3, NC = NV + 3 + (GAM ? 2 : 0);  // LDS slots per cell
-  constexpr int NF1 = TY * (TX + 1), NF2 = (TY + 1) * TX;           // x1 / x2 faces per plane
-  __shared__ Re tile[NC * WW];
+  // LDS slots per cell: prim (WB deviations), cell B (MHD), gammas (FASTEOS)
+  constexpr int OB = NV, OG = NV + (MHD ? 3 : 0), NC = OG + (GAM ? 2 : 0);
+  constexpr int NF1 = TY * (TX + 1), NF2 = (TY + 1) * TX;  // x1 / x2 faces per plane
+  // two plane tiles where three blocks per CU still fit: plane k+1's
+  // interior is then staged while plane k is solved, and no cell of the x3
+  // window stays in registers across the in-plane solves
+  constexpr int kTileB = NC * WW * (int)sizeof(Re), kFaceB = NV * (NF1 + NF2) * (int)sizeof(Re);
+  constexpr bool DB = 2 * kTileB + kFaceB <= 53 * 1024;
+  __shared__ Re tile[(DB ? 2 : 1) * NC * WW];
   __shared__ Re f1[NV * NF1];  // x1 faces: [v][row b][face a = 0..16]
   __shared__ Re f2[NV * NF2];  // x2 faces: [v][face b = 0..15][column a]
   if (!g.sc->active) return;
@@ -1497,6 +1503,7 @@ __global__ void __launch_bounds__(256)
   // thread (a, b): column (i0 + a, j0 + b) for b < TY (the last thread row
   // has no column; it takes the tile's upper x1 faces)
   const bool own = (tid >> 4) < TY;
+  auto plane = [&](int kk) { return DB ? tile + (kk & 1) * NC * WW : tile; };
 
   struct Cell {
     Re q[NV], b[3], ga[2];
@@ -1518,16 +1525,16 @@ __global__ void __launch_bounds__(256)
         load_cell<SD, NV, MHD, GAM, false>(g, ii, jj, kk, c.q, c.b, c.ga);
     }
   };
-  auto put = [&](int cell, const Cell &c) {
+  auto put = [&](Re *t, int cell, const Cell &c) {
 #pragma unroll
-    for (int v = 0; v < NV; v++) tile[v * WW + cell] = c.q[v];
+    for (int v = 0; v < NV; v++) t[v * WW + cell] = c.q[v];
     if (MHD) {
 #pragma unroll
-      for (int v = 0; v < 3; v++) tile[(OB + v) * WW + cell] = c.b[v];
+      for (int v = 0; v < 3; v++) t[(OB + v) * WW + cell] = c.b[v];
     }
     if (GAM) {
-      tile[OG * WW + cell] = c.ga[0];
-      tile[(OG + 1) * WW + cell] = c.ga[1];
+      t[OG * WW + cell] = c.ga[0];
+      t[(OG + 1) * WW + cell] = c.ga[1];
     }
   };
   auto states = [&](bool lim, const Re *w, int st, Re &qL, Re &qR) {
@@ -1553,24 +1560,24 @@ __global__ void __launch_bounds__(256)
   };
   // the in-plane face (DIR 0: lower x1 face of tile cell (fa, fb); DIR 1:
   // lower x2 face) from the LDS stencil, its fluxes into the LDS face array
-  auto plane_face = [&](auto dirc, int fa, int fb, int kk) {
+  auto plane_face = [&](auto dirc, const Re *t, int fa, int fb, int kk) {
     constexpr int DIR = decltype(dirc)::value;
     const int base = DIR == 0 ? (fb + 2) * W + fa : fb * W + fa + 2;
     constexpr int st = DIR == 0 ? 1 : W;
     Re qL[NV], qR[NV], bL[3] = {0, 0, 0}, bR[3] = {0, 0, 0}, gL[2] = {0, 0}, gR[2] = {0, 0};
     Re Fo[NV];
 #pragma unroll
-    for (int v = 0; v < NV; v++) states(v > IE, &tile[v * WW + base], st, qL[v], qR[v]);
+    for (int v = 0; v < NV; v++) states(v > IE, &t[v * WW + base], st, qL[v], qR[v]);
     if (MHD) {
 #pragma unroll
       for (int c = 0; c < SD; c++) {
         if (c == DIR) continue;
-        states(false, &tile[(OB + c) * WW + base], st, bL[c], bR[c]);
+        states(false, &t[(OB + c) * WW + base], st, bL[c], bR[c]);
       }
     }
     if (GAM) {
 #pragma unroll
-      for (int c = 0; c < 2; c++) states(false, &tile[(OG + c) * WW + base], st, gL[c], gR[c]);
+      for (int c = 0; c < 2; c++) states(false, &t[(OG + c) * WW + base], st, gL[c], gR[c]);
     }
     solve(dirc, i0 + fa, j0 + fb, kk, qL, qR, bL, bR, gL, gR, Fo);
     Re *dst = DIR == 0 ? f1 + fb * (TX + 1) + fa : f2 + fb * TX + fa;
@@ -1614,7 +1621,7 @@ __global__ void __launch_bounds__(256)
       }
     }
     if (j < g.n2) solve(SweepDir<2>{}, i, j, k0, qL, qR, bL, bR, gL, gR, F3m);
-    put((b + 2) * W + a + 2, cC);
+    put(plane(k0), (b + 2) * W + a + 2, cC);
   }
 
   // the RK buffers of k_update: the FP64 path saves state0 in U0 and carries
@@ -1634,8 +1641,14 @@ __global__ void __launch_bounds__(256)
     asm volatile("" : "+v"(lt));
     const int la = lt & (TX - 1), lb = lt >> 4, li = i0 + la, lj = j0 + lb;
     const bool col = lb < TY && lj < g.n2;  // a column inside the grid
-    // the halo of plane k (the interior was put by the previous iteration):
-    // x1 halo cells -2, -1, 16, 17 of rows 0..14, x2 halo rows -2, -1, 15, 16
+    Re *const tk = plane(k);
+    const int mine = (lb + 2) * W + la + 2;
+    // double tile: cell k+1 of this column into plane k+1's tile (whose
+    // last reads, of plane k-1, are behind the previous iteration's second
+    // barrier); the single tile takes it after this plane's reads
+    if (DB && lb < TY) put(plane(k + 1), mine, nxt);
+    // the halo of plane k (its interior is staged already): x1 halo cells
+    // -2, -1, 16, 17 of rows 0..14, x2 halo rows -2, -1, 15, 16
     if (lt < 128 && (lt >= 64 || lt < 4 * TY)) {
       int ha, hb;
       if (lt < 64) {
@@ -1649,120 +1662,115 @@ __global__ void __launch_bounds__(256)
       }
       Cell h;
       load(i0 + ha, j0 + hb, k, h);
-      put((hb + 2) * W + ha + 2, h);
+      put(tk, (hb + 2) * W + ha + 2, h);
     }
+    __syncthreads();
+    // in-plane faces, one of each direction per thread: x1 the lower face
+    // of cell (la, lb) (the last thread row: the upper face of row la), x2
+    // the lower face of row lb (16 faces: row 15's is the tile's upper face)
+    {
+      const int fa = lb < TY ? la : TX, fb = lb < TY ? lb : la;
+      if (fb < TY && j0 + fb < g.n2) plane_face(SweepDir<0>{}, tk, fa, fb, k);
+    }
+    if (lj <= g.n2) plane_face(SweepDir<1>{}, tk, la, lb, k);
+    // cell k+2 of this column, in flight across the barrier
     Cell cp;
     if (lb < TY) load(li, lj, k + 2, cp);
     __syncthreads();
-    // x3: face k+1 of this column from cells k (LDS), k+1, k+2
-    double X3[NV];
     if (lb < TY) {
+      // x3: face k+1 of this column from cells k (LDS), k+1, k+2
       Re qL[NV], qR[NV], bL[3] = {0, 0, 0}, bR[3] = {0, 0, 0}, gL[2] = {0, 0}, gR[2] = {0, 0};
       Re F3p[NV];
-      const Re *qm = &tile[(lb + 2) * W + la + 2];
+      const Re *qm = tk + mine;
+      const Re *q1 = plane(k + 1) + mine;
 #pragma unroll
       for (int v = 0; v < NV; v++) {
         qL[v] = rq[v];
-        edges(v > IE, qm[v * WW], nxt.q[v], cp.q[v], qR[v], rq[v]);
+        edges(v > IE, qm[v * WW], DB ? q1[v * WW] : nxt.q[v], cp.q[v], qR[v], rq[v]);
       }
       if (MHD) {
 #pragma unroll
         for (int c = 0; c < 2; c++) {
           bL[c] = rb[c];
-          edges(false, qm[(OB + c) * WW], nxt.b[c], cp.b[c], bR[c], rb[c]);
+          edges(false, qm[(OB + c) * WW], DB ? q1[(OB + c) * WW] : nxt.b[c], cp.b[c], bR[c],
+                rb[c]);
         }
       }
       if (GAM) {
 #pragma unroll
         for (int c = 0; c < 2; c++) {
           gL[c] = rg[c];
-          edges(false, qm[(OG + c) * WW], nxt.ga[c], cp.ga[c], gR[c], rg[c]);
+          edges(false, qm[(OG + c) * WW], DB ? q1[(OG + c) * WW] : nxt.ga[c], cp.ga[c], gR[c],
+                rg[c]);
         }
       }
+      // single tile: plane k's reads are done (this thread's own cell k was
+      // the last, just above), so cell k+1 takes its slot
+      if (!DB && k + 1 < k1) put(tk, mine, nxt);
+      nxt = cp;
       if (col) {
         solve(SweepDir<2>{}, li, lj, k + 1, qL, qR, bL, bR, gL, gR, F3p);
+        // divergence (x1 overwrites, x2 and x3 accumulate), sources, RK
+        const long long c = cix(g, li, lj, k);
+        double S[NV], S0[NV], R[NV];
 #pragma unroll
         for (int v = 0; v < NV; v++) {
-          X3[v] = ((double)F3p[v] - (double)F3m[v]) * g.inv_dx[2];
+          const Re *F1 = f1 + v * NF1 + lb * (TX + 1) + la;
+          const Re *F2 = f2 + v * NF2 + lb * TX + la;
+          R[v] = ((double)F1[1] - (double)F1[0]) * g.inv_dx[0];
+          R[v] = R[v] + ((double)F2[TX] - (double)F2[0]) * g.inv_dx[1];
+          R[v] = R[v] + ((double)F3p[v] - (double)F3m[v]) * g.inv_dx[2];
           F3m[v] = F3p[v];
         }
-      }
-    }
-    // in-plane faces, one of each direction per thread: x1 the lower face
-    // of cell (la, lb) (the last thread row: the upper face of row la), x2
-    // the lower face of row lb (16 faces: row 15's is the tile's upper face)
-    {
-      const int fa = lb < TY ? la : TX, fb = lb < TY ? lb : la;
-      if (fb < TY && j0 + fb < g.n2) plane_face(SweepDir<0>{}, fa, fb, k);
-    }
-    if (lj <= g.n2) plane_face(SweepDir<1>{}, la, lb, k);
-    __syncthreads();
-    // divergence (x1 overwrites, x2 and x3 accumulate), sources, RK
-    if (col) {
-      const long long c = cix(g, li, lj, k);
-      double S[NV], S0[NV], R[NV];
+        if (from_u) {
 #pragma unroll
-      for (int v = 0; v < NV; v++) {
-        const Re *F1 = f1 + v * NF1 + lb * (TX + 1) + la;
-        const Re *F2 = f2 + v * NF2 + lb * TX + la;
-        R[v] = ((double)F1[1] - (double)F1[0]) * g.inv_dx[0];
-        R[v] = R[v] + ((double)F2[TX] - (double)F2[0]) * g.inv_dx[1];
-        R[v] = R[v] + X3[v];
-      }
-      if (from_u) {
+          for (int v = 0; v < NV; v++) {
+            S0[v] = g.U[v * N + c];
+            S[v] = S0[v];
+          }
+        } else if (first) {
+          // state0 from the primitives (source/source.F90:7057-7117)
+          double p[NV];
 #pragma unroll
-        for (int v = 0; v < NV; v++) {
-          S0[v] = g.U[v * N + c];
-          S[v] = S0[v];
+          for (int v = 0; v < NV; v++) p[v] = g.P[v * N + c];
+          const double rho = p[0], v1 = p[1], v2 = p[2], v3 = p[3];
+          S0[0] = rho;
+          S0[1] = rho * v1;
+          S0[2] = rho * v2;
+          S0[3] = rho * v3;
+          double E = g.eint[c] + 0.5 * rho * (v1 * v1 + v2 * v2 + v3 * v3);
+          if (MHD) {
+            const double b1 = g.bcc[c], b2 = g.bcc[N + c], b3 = g.bcc[2 * N + c];
+            E = E + 0.5 * (b1 * b1 + b2 * b2 + b3 * b3);
+          }
+          if (g.etot) E = E + rho * g.phi_cc[lj];
+          S0[IE] = E;
+#pragma unroll
+          for (int v = IA; v < NV; v++) S0[v] = rho * p[v];
+#pragma unroll
+          for (int v = 0; v < NV; v++) {
+            S[v] = S0[v];
+            s0_buf[v * N + c] = S0[v];
+          }
+        } else {
+#pragma unroll
+          for (int v = 0; v < NV; v++) {
+            S0[v] = s0_buf[v * N + c];
+            S[v] = s_buf[v * N + c];
+          }
         }
-      } else if (first) {
-        // state0 from the primitives (source/source.F90:7057-7117)
-        double p[NV];
-#pragma unroll
-        for (int v = 0; v < NV; v++) p[v] = g.P[v * N + c];
-        const double rho = p[0], v1 = p[1], v2 = p[2], v3 = p[3];
-        S0[0] = rho;
-        S0[1] = rho * v1;
-        S0[2] = rho * v2;
-        S0[3] = rho * v3;
-        double E = g.eint[c] + 0.5 * rho * (v1 * v1 + v2 * v2 + v3 * v3);
-        if (MHD) {
-          const double b1 = g.bcc[c], b2 = g.bcc[N + c], b3 = g.bcc[2 * N + c];
-          E = E + 0.5 * (b1 * b1 + b2 * b2 + b3 * b3);
+        if (g.grav) {
+          const double r0 = S[0];
+          const double rho = g.wb ? (r0 - g.eq_rho_cc[lj]) : r0;
+          const double gy = g.gravy[lj];
+          R[2] = R[2] - rho * gy;
+          if (!g.etot) R[IE] = R[IE] - S[2] * gy;
         }
-        if (g.etot) E = E + rho * g.phi_cc[lj];
-        S0[IE] = E;
+        if (g.edot) R[IE] = R[IE] - g.edot_cc[lj];
+        const double a3 = c3 * g.sc->dt;
 #pragma unroll
-        for (int v = IA; v < NV; v++) S0[v] = rho * p[v];
-#pragma unroll
-        for (int v = 0; v < NV; v++) {
-          S[v] = S0[v];
-          s0_buf[v * N + c] = S0[v];
-        }
-      } else {
-#pragma unroll
-        for (int v = 0; v < NV; v++) {
-          S0[v] = s0_buf[v * N + c];
-          S[v] = s_buf[v * N + c];
-        }
+        for (int v = 0; v < NV; v++) out[v * N + c] = a1 * S0[v] + a2 * S[v] + a3 * R[v];
       }
-      if (g.grav) {
-        const double r0 = S[0];
-        const double rho = g.wb ? (r0 - g.eq_rho_cc[lj]) : r0;
-        const double gy = g.gravy[lj];
-        R[2] = R[2] - rho * gy;
-        if (!g.etot) R[IE] = R[IE] - S[2] * gy;
-      }
-      if (g.edot) R[IE] = R[IE] - g.edot_cc[lj];
-      const double a3 = c3 * g.sc->dt;
-#pragma unroll
-      for (int v = 0; v < NV; v++) out[v * N + c] = a1 * S0[v] + a2 * S[v] + a3 * R[v];
-    }
-    // plane k+1's interior (every read of plane k's tile is behind the
-    // barrier above; the next reads are behind the next iteration's first)
-    if (lb < TY) {
-      if (k + 1 < k1) put((lb + 2) * W + la + 2, nxt);
-      nxt = cp;
     }
   }
 }
