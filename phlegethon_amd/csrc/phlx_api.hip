@@ -56,8 +56,8 @@ struct phlx_ctx {
   // prim / eint / b_cc from U (whether it ran, and so whether they lag U,
   // the device flags SC_U_CUR / SC_P_STALE say)
   bool prim_stale = false;
-  // hydro_step's fused sweep: -1 = where it is the faster form (runs
-  // without MHD), 0 = never, 1 = wherever the kernel family has one
+  // hydro_step's fused sweep: -1 = where it is the faster form, 0 = never,
+  // 1 = wherever the kernel family has one
   int fused_sweep = -1;
   double dt_par = 0.0;
   int diff_stages = 0;  // stages of the last diffusion step
@@ -651,10 +651,14 @@ extern "C" int phlx_rk_update(phlx_ctx *ctx, int32_t irk, double a1, double a2, 
 static bool use_fused_sweep(const phlx_ctx *ctx) {
   const Grid &g = ctx->g;
   // measured on MI355X: without MHD the fused stage is 5-19 % faster (FP64
-  // and mixed); with MHD the fused kernel's register demand (FP32:
-  // occupancy 3 against 5-7 for the split sweeps; FP64: spills) leaves it
-  // within +-4 % or slower
-  const bool want = ctx->fused_sweep < 0 ? !ctx->g.mhd : ctx->fused_sweep != 0;
+  // and mixed); with MHD it is 1-4 % faster in mixed precision, except
+  // beside the surrogate-start update (whose MFMA / VALU work hides the
+  // split update's flux reads: split 2 % faster), and slower in FP64 (the
+  // FP64 MHD sweep spills)
+  const bool mixed = ctx->cfg.precision == PHLX_PREC_MIXED;
+  const bool want = ctx->fused_sweep < 0
+                        ? !ctx->g.mhd || (mixed && ctx->g.eos_guess != GUESS_MLP)
+                        : ctx->fused_sweep != 0;
   return want && ctx->kt.sweep && !g.sflat && g.sd == 3 && g.n1 % kSweepT == 0 &&
          (!ctx->kt.update_guess || ctx->kt.update_swept_guess);
 }

@@ -1709,19 +1709,10 @@ __global__ void __launch_bounds__(256)
       if (!DB && k + 1 < k1) put(tk, mine, nxt);
       nxt = cp;
       if (col) {
-        solve(SweepDir<2>{}, li, lj, k + 1, qL, qR, bL, bR, gL, gR, F3p);
-        // divergence (x1 overwrites, x2 and x3 accumulate), sources, RK
+        // the stage's state (state0 and stage state, or state0 rebuilt from
+        // the primitives) first: its loads are in flight during the x3 solve
         const long long c = cix(g, li, lj, k);
         double S[NV], S0[NV], R[NV];
-#pragma unroll
-        for (int v = 0; v < NV; v++) {
-          const Re *F1 = f1 + v * NF1 + lb * (TX + 1) + la;
-          const Re *F2 = f2 + v * NF2 + lb * TX + la;
-          R[v] = ((double)F1[1] - (double)F1[0]) * g.inv_dx[0];
-          R[v] = R[v] + ((double)F2[TX] - (double)F2[0]) * g.inv_dx[1];
-          R[v] = R[v] + ((double)F3p[v] - (double)F3m[v]) * g.inv_dx[2];
-          F3m[v] = F3p[v];
-        }
         if (from_u) {
 #pragma unroll
           for (int v = 0; v < NV; v++) {
@@ -1758,6 +1749,17 @@ __global__ void __launch_bounds__(256)
             S0[v] = s0_buf[v * N + c];
             S[v] = s_buf[v * N + c];
           }
+        }
+        solve(SweepDir<2>{}, li, lj, k + 1, qL, qR, bL, bR, gL, gR, F3p);
+        // divergence (x1 overwrites, x2 and x3 accumulate), sources, RK
+#pragma unroll
+        for (int v = 0; v < NV; v++) {
+          const Re *F1 = f1 + v * NF1 + lb * (TX + 1) + la;
+          const Re *F2 = f2 + v * NF2 + lb * TX + la;
+          R[v] = ((double)F1[1] - (double)F1[0]) * g.inv_dx[0];
+          R[v] = R[v] + ((double)F2[TX] - (double)F2[0]) * g.inv_dx[1];
+          R[v] = R[v] + ((double)F3p[v] - (double)F3m[v]) * g.inv_dx[2];
+          F3m[v] = F3p[v];
         }
         if (g.grav) {
           const double r0 = S[0];
