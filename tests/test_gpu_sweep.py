@@ -17,7 +17,9 @@ and diode walls, WB + gravity + ETOT + EDOT, RK2 and RK3, the Helmholtz
 FASTEOS states (with and without MHD, with the Newton-counting update), a
 grid whose x3 extent is not a multiple of the x3 chunk, x2 extents that
 leave a partial 15-row tile (down to 6 rows), and grids the fused path
-does not take (x1 not a multiple of 16), where it must stand aside.
+does not take (x1 not a multiple of 16), where it must stand aside.  The C5-shaped mixed slab runs with the carried-T, the energy-
+predictor and the surrogate's Newton start (the network trained in each
+run: training is deterministic).
 """
 import numpy as np
 import pytest
@@ -59,6 +61,9 @@ def _run(pb, nsteps, fused, count=False, guess="warm"):
     sim = Simulation(pb.cfg)
     sim.setup(pb)
     sim.set_fused_sweep(fused)
+    if guess == "mlp":
+        # the training is deterministic: both runs get the same network
+        sim.mlp_train(nsamples=1 << 18, epochs=2, batch=2048, lr=5e-3, box=P.state_box(pb))
     if count:
         sim.set_eos_guess(guess, count=True)
     sim.profile(True)
@@ -186,7 +191,7 @@ def test_mixed_lowmach_box():
     _compare(_mixed(P.lowmach_box(n=32, mach=1e-3, heating=1e-9)), 8)
 
 
-@pytest.mark.parametrize("guess", ["warm", "pred"])
+@pytest.mark.parametrize("guess", ["warm", "pred", "mlp"])
 def test_mixed_helmholtz_mhd_slab(oracle_mod, slab_table, guess):
     orc = oracle_mod.OracleHelm(slab_table)
     pb = _mixed(P.helm_slab(orc, slab_table, n=64, ny=16, sdims=3, mhd_beta=1e5))
