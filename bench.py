@@ -477,8 +477,7 @@ def main():
     ap.add_argument("--sweep", choices=("auto", "fused", "split"), default="auto",
                     help="3D runs: the fused one-pass sweep, the three directional sweeps "
                          "(same results bit for bit), or the library's choice (fused without "
-                         "MHD, and for mixed-precision MHD unless the surrogate starts the "
-                         "Newton)")
+                         "MHD)")
     ap.add_argument("--eos-guess", choices=("warm", "mlp", "pred"), default="warm",
                     help="Helmholtz: Newton start (carried T, the trained surrogate, or the "
                          "energy predictor)")

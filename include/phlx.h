@@ -158,8 +158,7 @@ int phlx_rk_update(phlx_ctx *ctx, int32_t irk, double a1, double a2, double a3,
  * and RK combine of rk_update) into one pass that stages a pencil tile in
  * LDS and keeps the face fluxes out of HBM: mode 1 wherever the run admits
  * it, 0 never, -1 (the default) where it is the faster form: runs without
- * MHD, and mixed-precision MHD runs except with the surrogate's Newton
- * start.  It applies to 3D runs with LIM2ND or PPH, the ideal or
+ * MHD.  It applies to 3D runs with LIM2ND or PPH, the ideal or
  * the FASTEOS Helmholtz state, no shock flattening and nx1, nx2 multiples
  * of 16 (other runs take the separate sweeps).  Either way the results are
  * bit for bit the same.  No counterpart in the reference (its hydro_step is

@@ -651,14 +651,10 @@ extern "C" int phlx_rk_update(phlx_ctx *ctx, int32_t irk, double a1, double a2, 
 static bool use_fused_sweep(const phlx_ctx *ctx) {
   const Grid &g = ctx->g;
   // measured on MI355X: without MHD the fused stage is 5-19 % faster (FP64
-  // and mixed); with MHD it is 1-4 % faster in mixed precision, except
-  // beside the surrogate-start update (whose MFMA / VALU work hides the
-  // split update's flux reads: split 2 % faster), and slower in FP64 (the
-  // FP64 MHD sweep spills)
-  const bool mixed = ctx->cfg.precision == PHLX_PREC_MIXED;
-  const bool want = ctx->fused_sweep < 0
-                        ? !ctx->g.mhd || (mixed && ctx->g.eos_guess != GUESS_MLP)
-                        : ctx->fused_sweep != 0;
+  // and mixed); with MHD the split sweeps are 1-5 % faster (FP32: the
+  // column segments prefetch a cell ahead at 3-4 waves/SIMD; FP64: the
+  // fused MHD sweep spills)
+  const bool want = ctx->fused_sweep < 0 ? !ctx->g.mhd : ctx->fused_sweep != 0;
   return want && ctx->kt.sweep && !g.sflat && g.sd == 3 && g.n1 % kSweepT == 0 &&
          (!ctx->kt.update_guess || ctx->kt.update_swept_guess);
 }

@@ -529,11 +529,16 @@ __device__ __forceinline__ void seg_walk_mx(const Grid &g, int i, int o, int f0,
   }
   cm = c0;
   c0 = cp;
+  // cell f+1 of the next face is loaded one face ahead, so its latency
+  // overlaps a face solve (the walk reads cells f0-2 .. f1 as before)
+  CellWf cn;
+  load(f0 + 1, cn, i, o);
 #pragma unroll 1
   for (int f = f0; f < f1; f++) {
     int ii = i, oo = o;  // re-materialised per face (see seg_walk)
     asm volatile("" : "+v"(ii), "+v"(oo));
-    load(f + 1, cp, ii, oo);
+    cp = cn;
+    if (f + 1 < f1) load(f + 2, cn, ii, oo);
     float qL[NV], qR[NV], bL[3] = {0.0f, 0.0f, 0.0f}, bR[3] = {0.0f, 0.0f, 0.0f};
     float gL[2] = {0.0f, 0.0f}, gR[2] = {0.0f, 0.0f};
 #pragma unroll

@@ -302,8 +302,7 @@ class Simulation:
     def set_fused_sweep(self, on):
         """hydro_step with the fused one-pass sweep (True), the three
         directional sweeps (False) or the library's choice (None: fused
-        without MHD, and with MHD in mixed precision unless the surrogate
-        starts the Newton); the results are the same bit for bit."""
+        without MHD); the results are the same bit for bit."""
         mode = -1 if on is None else int(bool(on))
         _chk(self.lib.phlx_set_fused_sweep(self._ctx, mode), "phlx_set_fused_sweep")
 

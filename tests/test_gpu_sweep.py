@@ -202,12 +202,11 @@ def test_mixed_helmholtz_mhd_slab(oracle_mod, slab_table, guess):
 
 
 def test_default_choice():
-    """The library's choice (mode -1): fused without MHD (FP64 and mixed)
-    and for mixed-precision MHD; split for FP64 MHD (the surrogate-start
-    case, also split, needs a trained network and is left to the bench)."""
+    """The library's choice (mode -1): fused without MHD (FP64 and mixed),
+    split with MHD."""
     from phlegethon_amd.runtime import Simulation
     cases = [(_mixed(P.lowmach_box(n=16, mach=1e-3, heating=1e-9)), True),
-             (_mixed(P.orszag_tang_3d(n=16)), True),
+             (_mixed(P.orszag_tang_3d(n=16)), False),
              (P.lowmach_box(n=16, mach=1e-3, heating=1e-9), True),
              (P.orszag_tang_3d(n=16), False)]
     for pb, fused in cases:
