@@ -7,6 +7,11 @@
 //   k_faceb                              face-B update (FP64)         (MHD)
 //   k_update                             flux divergence + sources + RK combine
 //                                        + cons->prim (+ CFL max at the last stage)
+// or, fused (k_sweep, 3D with the 2nd-order reconstructions),
+//   k_sweep                              the three sweeps + divergence + sources
+//                                        + RK combine in one pass over LDS tiles
+//   k_corner, k_faceb                    as above (MHD)
+//   k_update<..., SWEPT>                 cons->prim (+ CFL max at the last stage)
 // Ghost zones are never stored: loads outside the interior go through the
 // per-axis boundary map in phlx_device.hpp.  The stencil is not a
 // contraction, so there is no MFMA here; every kernel is HBM/VALU work.
