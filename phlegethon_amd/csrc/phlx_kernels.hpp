@@ -629,6 +629,95 @@ __global__ void __launch_bounds__(256) k_flux_mx(Grid g) {
   flux_face_mx<DIR, SD, NAS, MHD, HELM>(g, i, j, k, qL, qR, bL, bR, gL, gR);
 }
 
+// ------------------------------------------------------- x1, one cell per lane
+// x1 sweep with one reconstruction per cell (the column-segment idea in the
+// direction the lanes run along): lane l of wave w takes cell
+// C = 63 w + l of the plane's row-major sequence of x1 cells with one
+// ghost at each end of a row (n1 + 2 per row), reconstructs both of its x1
+// edges, and solves the face between its cell and the next lane's (its own
+// right edges; the next cell's left edges by a lane shuffle).  Lane 63 and
+// the last cell of a row solve nothing; consecutive waves overlap by one
+// cell.  Each face's states are those of face_states (the same
+// reconstruction calls) and its solve is flux_face / flux_face_mx, so the
+// fluxes equal the per-face kernel's bit for bit.  2nd-order
+// reconstructions, ideal or FASTEOS states.
+#ifndef PHLX_FLUX_X1_CELLS
+#define PHLX_FLUX_X1_CELLS 1
+#endif
+template <int SD, int NAS, bool MHD, int EK, bool MIX>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+    k_flux_x1c(Grid g) {
+  constexpr int NV = SD + 2 + NAS, IE = SD + 1;
+  constexpr bool GAM = EK == EK_FASTEOS;
+  static_assert(EK != EK_HELM_FACE, "the per-face Newton guess is not shuffled");
+  using Re = typename std::conditional<MIX, float, double>::type;
+  if (!g.sc->active) return;
+  const int lane = threadIdx.x & 63;
+  const long long wave = ((long long)blockIdx.x * 256 + threadIdx.x) >> 6;
+  const int rowc = g.n1 + 2;
+  const long long Cn = (long long)rowc * g.n2;
+  const long long C = 63 * wave + lane;
+  const int k = blockIdx.y;
+  // lanes past the plane stand in for its last cell (no face uses them)
+  const long long Cc = C < Cn ? C : Cn - 1;
+  const int j = (int)(Cc / rowc), ic = (int)(Cc - (long long)j * rowc) - 1;
+  Re q[3][NV], b[3][3], ga[3][2];
+  const bool fast = __all(ic - 1 >= 0 && ic + 1 < g.n1);
+#pragma unroll
+  for (int s = 0; s < 3; s++) {
+    b[s][0] = b[s][1] = b[s][2] = 0;
+    ga[s][0] = ga[s][1] = 0;
+    const int ii = ic - 1 + s;
+    if constexpr (MIX) {
+      if (fast)
+        load_cell_f<SD, NV, MHD, GAM, true>(g, ii, j, k, q[s], b[s], ga[s]);
+      else
+        load_cell_f<SD, NV, MHD, GAM>(g, ii, j, k, q[s], b[s], ga[s]);
+    } else {
+      if (fast)
+        load_cell<SD, NV, MHD, GAM, true>(g, ii, j, k, q[s], b[s], ga[s]);
+      else
+        load_cell<SD, NV, MHD, GAM>(g, ii, j, k, q[s], b[s], ga[s]);
+    }
+  }
+  auto edges = [&](bool lim, Re qm, Re q0, Re qp, Re &l, Re &r) {
+    if constexpr (MIX)
+      cell_edges_mx(g.recon, lim, qm, q0, qp, l, r);
+    else
+      cell_edges(g.recon, lim, qm, q0, qp, l, r);
+  };
+  // this cell's edges: right -> the face above (left state), left -> the
+  // face below (right state, shuffled to the lane below)
+  Re qL[NV], qR[NV], bL[3] = {0, 0, 0}, bR[3] = {0, 0, 0}, gL[2] = {0, 0}, gR[2] = {0, 0};
+#pragma unroll
+  for (int v = 0; v < NV; v++) {
+    Re l;
+    edges(v > IE, q[0][v], q[1][v], q[2][v], l, qL[v]);
+    qR[v] = __shfl_down(l, 1);
+  }
+  if (MHD) {
+#pragma unroll
+    for (int c = 1; c < SD; c++) {
+      Re l;
+      edges(false, b[0][c], b[1][c], b[2][c], l, bL[c]);
+      bR[c] = __shfl_down(l, 1);
+    }
+  }
+  if (GAM) {
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+      Re l;
+      edges(false, ga[0][c], ga[1][c], ga[2][c], l, gL[c]);
+      gR[c] = __shfl_down(l, 1);
+    }
+  }
+  if (lane == 63 || C >= Cn || ic >= g.n1) return;
+  if constexpr (MIX)
+    flux_face_mx<0, SD, NAS, MHD, GAM>(g, ic + 1, j, k, qL, qR, bL, bR, gL, gR);
+  else
+    flux_face<0, SD, NAS, MHD, EK>(g, ic + 1, j, k, qL, qR, bL, bR, gL, gR, 0.0);
+}
+
 // ------------------------------------------------------------------ CT
 // Face-centred EMF component `comp` (0..2 = E1..E3) of the face array of
 // direction fd at a possibly-ghost transverse position; ghosts follow the
@@ -1889,6 +1978,12 @@ static inline dim3 grid_plane(int e1, int e2, int e3) {
   return dim3((unsigned)((nplane + 255) / 256), 1, e3);
 }
 static inline dim3 grid_x1_faces(const Grid &g) { return grid_plane(g.n1 + 1, g.n2, g.n3); }
+// k_flux_x1c: waves of 63 faces over the n2 (n1 + 2) cells of a plane, 4 per
+// block, one plane per blockIdx.y
+static inline dim3 grid_x1_cells(const Grid &g) {
+  const long long waves = ((long long)(g.n1 + 2) * g.n2 + 62) / 63;
+  return dim3((unsigned)((waves + 3) / 4), (unsigned)g.n3, 1);
+}
 // flat grid of the blocks of grid_for, for kernels that place their blocks
 // with xcd_cell_block
 static inline dim3 grid_flat(int e1, int e2, int e3, dim3 blk) {
@@ -1900,9 +1995,11 @@ template <int SD, int NAS, bool MHD, int EK, bool MIX, bool R5>
 static void launch_flux(const Grid &g, int dir, hipStream_t st) {
   constexpr bool HELM = EK != EK_IDEAL;
   const dim3 blk(64, 4, 1);
-  if (MIX) {
-    static_assert(!MIX || EK != EK_HELM_FACE, "mixed precision needs FASTEOS");
-    if (dir == 0) {
+  if constexpr (MIX) {
+    static_assert(EK != EK_HELM_FACE, "mixed precision needs FASTEOS");
+    if (dir == 0 && !R5 && PHLX_FLUX_X1_CELLS) {
+      k_flux_x1c<SD, NAS, MHD, EK, true><<<grid_x1_cells(g), 256, 0, st>>>(g);
+    } else if (dir == 0) {
       k_flux_mx<0, SD, NAS, MHD, HELM, R5><<<grid_x1_faces(g), blk, 0, st>>>(g);
     } else if (!R5 && PHLX_FLUX_SEG_MX) {
       const int n = dir == 1 ? g.n2 : g.n3, no = dir == 1 ? g.n3 : g.n2;
@@ -1916,25 +2013,30 @@ static void launch_flux(const Grid &g, int dir, hipStream_t st) {
     } else if (SD == 3) {
       k_flux_mx<2, SD, NAS, MHD, HELM, R5><<<grid_for(g.n1, g.n2, g.n3 + 1, blk), blk, 0, st>>>(g);
     }
-    return;
-  }
-  if (dir == 0) {
-    k_flux<0, SD, NAS, MHD, EK, R5><<<grid_x1_faces(g), blk, 0, st>>>(g);
-  } else if (!R5 && PHLX_FLUX_SEG) {
-    // column segments: (x1 cells, transverse cells, segments of kSegFaces)
-    const int n = dir == 1 ? g.n2 : g.n3, no = dir == 1 ? g.n3 : g.n2;
-    const dim3 grd((g.n1 + 63) / 64, (no + 3) / 4, (n + 1 + kSegFaces - 1) / kSegFaces);
-    if (dir == 1)
-      k_flux_seg<1, SD, NAS, MHD, EK><<<grd, blk, 0, st>>>(g);
-    else if (SD == 3)
-      k_flux_seg<2, SD, NAS, MHD, EK><<<grd, blk, 0, st>>>(g);
-  } else if (dir == 1) {
-    k_flux<1, SD, NAS, MHD, EK, R5><<<grid_for(g.n1, g.n2 + 1, g.n3, blk), blk, 0, st>>>(g);
-  } else if (SD == 3) {
-    k_flux<2, SD, NAS, MHD, EK, R5><<<grid_for(g.n1, g.n2, g.n3 + 1, blk), blk, 0, st>>>(g);
+  } else {
+    if constexpr (!R5 && EK != EK_HELM_FACE) {
+      if (dir == 0 && PHLX_FLUX_X1_CELLS) {
+        k_flux_x1c<SD, NAS, MHD, EK, false><<<grid_x1_cells(g), 256, 0, st>>>(g);
+        return;
+      }
+    }
+    if (dir == 0) {
+      k_flux<0, SD, NAS, MHD, EK, R5><<<grid_x1_faces(g), blk, 0, st>>>(g);
+    } else if (!R5 && PHLX_FLUX_SEG) {
+      // column segments: (x1 cells, transverse cells, segments of kSegFaces)
+      const int n = dir == 1 ? g.n2 : g.n3, no = dir == 1 ? g.n3 : g.n2;
+      const dim3 grd((g.n1 + 63) / 64, (no + 3) / 4, (n + 1 + kSegFaces - 1) / kSegFaces);
+      if (dir == 1)
+        k_flux_seg<1, SD, NAS, MHD, EK><<<grd, blk, 0, st>>>(g);
+      else if (SD == 3)
+        k_flux_seg<2, SD, NAS, MHD, EK><<<grd, blk, 0, st>>>(g);
+    } else if (dir == 1) {
+      k_flux<1, SD, NAS, MHD, EK, R5><<<grid_for(g.n1, g.n2 + 1, g.n3, blk), blk, 0, st>>>(g);
+    } else if (SD == 3) {
+      k_flux<2, SD, NAS, MHD, EK, R5><<<grid_for(g.n1, g.n2, g.n3 + 1, blk), blk, 0, st>>>(g);
+    }
   }
 }
-
 
 // USE_SHOCK_FLATTENING sweeps (FP64): as launch_flux with the flag-reading
 // variants (per-face x1 and LIM5TH sweeps, column segments otherwise).
