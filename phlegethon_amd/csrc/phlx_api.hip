@@ -1023,7 +1023,8 @@ constexpr int kMlpMaxGradBlocks = 1024;  // gradient-check batches up to 131072 
 static int mlp_alloc(phlx_ctx *ctx) {
   if (ctx->mlp_buf) return PHLX_OK;
   const size_t P = kMlpNParam;
-  const size_t floats = 3 * P + 256 + 64 + (size_t)kMlpMaxGradBlocks * P + kMlpMaxGradBlocks + P;
+  const size_t floats =
+      3 * P + 256 + 3 * 64 + (size_t)kMlpMaxGradBlocks * P + kMlpMaxGradBlocks + P;
   const size_t bytes =
       floats * sizeof(float) + (kMlpW2aShorts + kMlpW1aShorts) * sizeof(unsigned short) + 4096;
   void *buf = nullptr;
@@ -1034,7 +1035,7 @@ static int mlp_alloc(phlx_ctx *ctx) {
   ctx->mlp_m = f; f += P;
   ctx->mlp_v = f; f += P;
   ctx->mlp_w1q = f; f += 256;
-  ctx->mlp_w3q = f; f += 64;
+  ctx->mlp_w3q = f; f += 3 * 64;  // w3q, then the inference's scaled b1, b2
   ctx->mlp_gpart = f; f += (size_t)kMlpMaxGradBlocks * P;
   ctx->mlp_lpart = f; f += kMlpMaxGradBlocks;
   ctx->mlp_gsum = f; f += P;
@@ -1045,6 +1046,7 @@ static int mlp_alloc(phlx_ctx *ctx) {
   m.theta = ctx->mlp_theta;
   m.w1q = ctx->mlp_w1q;
   m.w3q = ctx->mlp_w3q;
+  m.bs = ctx->mlp_w3q + kMlpH;
   m.w2a = (const mlp_bf8 *)ctx->mlp_wa;
   m.w1a = (const mlp_s4 *)(ctx->mlp_wa + kMlpW2aShorts);
   return PHLX_OK;

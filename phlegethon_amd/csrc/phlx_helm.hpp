@@ -406,24 +406,30 @@ __device__ __forceinline__ void helm_mx_setup(const HelmDev &h, double rho, doub
   helm_mx_contract(h, T, m);
 }
 
-// E and c_v (and with FULL: P, dP/dT, the sound speed) at T from a
-// contracted cell: the operations of helm_full / helm_given2 after their
-// sums (ions, radiation, relativistic sound speed).
-template <bool FULL>
-__device__ __forceinline__ void helm_mx_eval(const HelmDev &h, HelmMx &m, double rho, double T,
-                                             double &E, double &cv, double &P, double &dPdT,
-                                             double &sound) {
+// E and c_v at T from a contracted cell (helm_mx_energy), and P, dP/dT and
+// the sound speed at the same T (helm_mx_pressure, from the T basis the
+// first kept in a HelmMxAt): the operations of helm_full / helm_given2
+// after their sums (ions, radiation, relativistic sound speed).  A Newton
+// iterate needs only the first; the solve's last iterate takes both.
+struct HelmMxAt {
+  double y;
+  HelmBasis t, dt;
+};
+
+__device__ __forceinline__ void helm_mx_energy(const HelmDev &h, HelmMx &m, double rho, double T,
+                                               HelmMxAt &a, double &E, double &cv) {
   double y = hdiv<true>(T - m.Tj, m.dT);
   if (y < -0.01 || y > 1.01) {
     helm_mx_contract(h, T, m);
     y = hdiv<true>(T - m.Tj, m.dT);
   }
-  HelmBasis t, dt, ddt;
-  helm_basis(y, m.dT, t);
-  helm_dbasis<true>(y, m.dT, dt);
+  a.y = y;
+  HelmBasis ddt;
+  helm_basis(y, m.dT, a.t);
+  helm_dbasis<true>(y, m.dT, a.dt);
   helm_ddbasis<true>(y, m.dT, ddt);
-  const double f = helm_contract_T(m.Gr, t);
-  const double df_dT = helm_contract_T(m.Gr, dt);
+  const double f = helm_contract_T(m.Gr, a.t);
+  const double df_dT = helm_contract_T(m.Gr, a.dt);
   const double df_dT2 = helm_contract_T(m.Gr, ddt);
   const double Ye = m.Ye, tmp = m.iabar;
   const double s = -Ye * df_dT;
@@ -435,9 +441,15 @@ __device__ __forceinline__ void helm_mx_eval(const HelmDev &h, HelmMx &m, double
   dE_dT = dE_dT + 1.5 * kHelmRgas * tmp + hdiv<true>(4.0 * kHelmCrad * T3, rho);
   E = e;
   cv = dE_dT;
-  if (!FULL) return;
-  const double df_drho = helm_contract_T(m.Gdr, t);
-  const double df_drhodT = helm_contract_T(m.Gdr, dt);
+}
+
+__device__ __forceinline__ void helm_mx_pressure(const HelmMx &m, double rho, double T,
+                                                 const HelmMxAt &a, double E, double cv,
+                                                 double &P, double &dPdT, double &sound) {
+  const double y = a.y, Ye = m.Ye, tmp = m.iabar;
+  const double T2 = T * T, T3 = T2 * T, T4 = T3 * T;
+  const double df_drho = helm_contract_T(m.Gdr, a.t);
+  const double df_drhodT = helm_contract_T(m.Gdr, a.dt);
   const double oy = 1.0 - y, y2 = y * y, y3 = y2 * y, oy2 = oy * oy, oy3 = oy2 * oy;
   double c3 = m.Hc[0] * (2.0 * y3 - 3.0 * y2 + 1.0);
   c3 = fma(m.Hc[1], 2.0 * oy3 - 3.0 * oy2 + 1.0, c3);
@@ -456,9 +468,9 @@ __device__ __forceinline__ void helm_mx_eval(const HelmDev &h, HelmMx &m, double
   const double zz = hdiv<true>(p, rho);
   const double chiT = hdiv<true>(T, p) * dP_dT;
   const double chirho = hdiv<true>(dP_drho, zz);
-  const double xx = hdiv<true>(zz * chiT, T * dE_dT);
+  const double xx = hdiv<true>(zz * chiT, T * cv);
   const double gam1 = chiT * xx + chirho;
-  const double z = 1.0 + hdiv<true>(e + kHelmClight2, zz);
+  const double z = 1.0 + hdiv<true>(E + kHelmClight2, zz);
   P = p;
   dPdT = dP_dT;
   sound = kHelmClight * sqrt(hdiv<true>(gam1, z));

@@ -189,7 +189,8 @@ k_mlp_grad(const float *__restrict__ theta, const float *__restrict__ w1q,
 }
 
 // wa: the inference's MFMA A operands, w2a (4096 BF16) then w1a (1024 BF16,
-// the lane group 3 slots left zero), layouts as documented at MlpDev.
+// the lane group 3 slots left zero), layouts as documented at MlpDev; w3q is
+// followed by the inference's scaled biases (MlpDev::bs).
 __device__ __forceinline__ void store_quantized(int p, float w, float *w1q, float *w3q,
                                                 unsigned short *wa) {
   if (p >= kMlpOffW1 && p < kMlpOffB1) {
@@ -198,6 +199,10 @@ __device__ __forceinline__ void store_quantized(int p, float w, float *w1q, floa
     unsigned short *w1a = wa + kMlpW2aShorts;
 #pragma unroll
     for (int q = 0; q < 3; q++) w1a[((n >> 4) * 64 + 16 * q + (n & 15)) * 4 + e] = bf16_bits(w);
+  } else if (p >= kMlpOffB1 && p < kMlpOffW2) {
+    w3q[kMlpH + (p - kMlpOffB1)] = w * kMlpExp2Scale;
+  } else if (p >= kMlpOffB2 && p < kMlpOffW3) {
+    w3q[2 * kMlpH + (p - kMlpOffB2)] = w * kMlpExp2Scale;
   } else if (p >= kMlpOffW3 && p < kMlpOffB3) {
     w3q[p - kMlpOffW3] = bf16_round(w);
   } else if (p >= kMlpOffW2 && p < kMlpOffB2) {

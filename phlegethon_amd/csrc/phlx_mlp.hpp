@@ -39,6 +39,9 @@ constexpr int kMlpOffW1 = 0, kMlpOffB1 = 256, kMlpOffW2 = 320, kMlpOffB2 = 4416,
 // the inference's BF16 A operands in one buffer: w2a, then w1a
 constexpr int kMlpW2aShorts = 4096, kMlpW1aShorts = 1024;
 constexpr int kBracketSteps = 16;  // bisection steps of the fallback guess
+// The inference forms every hidden pre-activation already multiplied by
+// 2 / ln 2, the argument of tanh's exp2 (inputs and biases pre-scaled).
+constexpr float kMlpExp2Scale = 2.8853900817779268f;
 
 // T(rho, e) starts: the carried T (the reference), the surrogate's
 // corrected warm start, the bracketed bisection, and the energy predictor
@@ -56,6 +59,7 @@ struct MlpDev {
   const float *theta;   // FP32 master parameters (biases are read from here)
   const float *w1q;     // BF16-rounded w1 as FP32, [4][64]
   const float *w3q;     // BF16-rounded w3 as FP32, [64]
+  const float *bs;      // b1 [64] then b2 [64], times kMlpExp2Scale
   // MFMA A operands of the transposed layers (lane l: row l % 16, k-slots
   // of lane group q = l / 16):
   //   w1a [hb 4][lane 64] x 4 BF16: w1[e][16 hb + l % 16] for q < 3, 0 for q = 3
@@ -87,8 +91,15 @@ __device__ __forceinline__ float bf16_round(float f) {
 // far below the BF16 resolution of the activations it feeds.
 __device__ __forceinline__ float fast_tanh(float z) {
   z = fminf(fmaxf(z, -15.0f), 15.0f);
-  const float e = __builtin_amdgcn_exp2f(z * 2.8853900817779268f);  // 2 / ln 2
+  const float e = __builtin_amdgcn_exp2f(z * kMlpExp2Scale);  // 2 / ln 2
   return 1.0f - 2.0f * __builtin_amdgcn_rcpf(e + 1.0f);
+}
+// The same from u = z * 2 / ln 2.  No clamp: exp2 overflows to +inf for
+// large u and the reciprocal of +inf is 0 (tanh = 1), and underflows to 0
+// for very negative u (tanh = -1), so the limits are exact as they are.
+// 2 r is exact, so the fused 1 - 2 r rounds as the separate operations do.
+__device__ __forceinline__ float tanh_exp2(float u) {
+  return fmaf(-2.0f, __builtin_amdgcn_rcpf(__builtin_amdgcn_exp2f(u) + 1.0f), 1.0f);
 }
 
 // log10 of a positive value in FP32 (hardware v_log_f32; the network's
@@ -144,12 +155,13 @@ __device__ __forceinline__ float mlp_forward_wave(const MlpDev &m, const float x
                                                   unsigned int *lds) {
   const int lane = threadIdx.x & 63;
   const int q = lane >> 4, col = lane & 15;
-  // this zone's inputs as hi + mid + lo BF16 parts (each residual is exact)
+  // this zone's inputs, times 2 / ln 2, as hi + mid + lo BF16 parts (each
+  // residual is exact)
   {
     unsigned int part[3][2];
 #pragma unroll
     for (int pr = 0; pr < 2; pr++) {
-      float a = x[2 * pr], b = x[2 * pr + 1];
+      float a = x[2 * pr] * kMlpExp2Scale, b = x[2 * pr + 1] * kMlpExp2Scale;
 #pragma unroll
       for (int t = 0; t < 3; t++) {
         const unsigned int h = mlp_pk_bf16(a, b);
@@ -177,24 +189,27 @@ __device__ __forceinline__ float mlp_forward_wave(const MlpDev &m, const float x
     xb[zb] = __builtin_bit_cast(mlp_s4, v);
   }
   mlp_wave_lds_sync();
-  const float *b1 = m.theta + kMlpOffB1;
-  const float *b2 = m.theta + kMlpOffB2;
+  const float *b1 = m.bs;
+  const float *b2 = m.bs + kMlpH;
+  // The zone blocks rotate through xb[0] and s[3] (register moves), so the
+  // rolled loop indexes no register array by its counter (which would
+  // select among the elements lane by lane on every pass).
   float s[4];
 #pragma unroll 1
   for (int zb = 0; zb < 4; zb++) {
-    // layer 1: h1^T (hidden 16 hb + 4q + r, zone) = w1^T x^T + b1, then tanh
-    // and BF16 into the k-slots of layer 2 (block kb = hb / 2: slots 0..3
-    // from hb even, 4..7 from hb odd)
+    // layer 1: h1^T (hidden 16 hb + 4q + r, zone) = w1^T x^T + b1 (times
+    // 2 / ln 2), then tanh and BF16 into the k-slots of layer 2 (block
+    // kb = hb / 2: slots 0..3 from hb even, 4..7 from hb odd)
     mlp_bf8 h1[2];
 #pragma unroll
     for (int hb = 0; hb < 4; hb++) {
       const float *bb = b1 + 16 * hb + 4 * q;
       mlp_f4 d = (mlp_f4){bb[0], bb[1], bb[2], bb[3]};
-      d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(m.w1a[hb * 64 + lane], xb[zb], d, 0, 0, 0);
+      d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(m.w1a[hb * 64 + lane], xb[0], d, 0, 0, 0);
       const mlp_bf2 v0 =
-          __builtin_bit_cast(mlp_bf2, mlp_pk_bf16(fast_tanh(d[0]), fast_tanh(d[1])));
+          __builtin_bit_cast(mlp_bf2, mlp_pk_bf16(tanh_exp2(d[0]), tanh_exp2(d[1])));
       const mlp_bf2 v1 =
-          __builtin_bit_cast(mlp_bf2, mlp_pk_bf16(fast_tanh(d[2]), fast_tanh(d[3])));
+          __builtin_bit_cast(mlp_bf2, mlp_pk_bf16(tanh_exp2(d[2]), tanh_exp2(d[3])));
       const int o = (hb & 1) * 4;
       h1[hb >> 1][o] = v0[0];
       h1[hb >> 1][o + 1] = v0[1];
@@ -212,9 +227,16 @@ __device__ __forceinline__ float mlp_forward_wave(const MlpDev &m, const float x
                                                     0, 0, 0);
       const int r0 = 16 * nb + 4 * q;
 #pragma unroll
-      for (int r = 0; r < 4; r++) acc = acc + fast_tanh(d[r] + b2[r0 + r]) * m.w3q[r0 + r];
+      for (int r = 0; r < 4; r++)
+        acc = fmaf(tanh_exp2(fmaf(d[r], kMlpExp2Scale, b2[r0 + r])), m.w3q[r0 + r], acc);
     }
-    s[zb] = acc;
+    xb[0] = xb[1];
+    xb[1] = xb[2];
+    xb[2] = xb[3];
+    s[0] = s[1];
+    s[1] = s[2];
+    s[2] = s[3];
+    s[3] = acc;  // after the last pass s[b] holds zone block b
   }
   // y(zone 16 zb + col) = b3 + the sum of s[zb] over the four lane groups;
   // lane l keeps zb = q: halves exchanged across groups q ^ 2, then q ^ 1
@@ -328,8 +350,9 @@ __device__ __forceinline__ int helm_rhoe_mixed(const HelmDev &h, double rho, dou
   int it = 0;
 #pragma unroll 1
   for (;;) {
-    double E, cv, dPdT, Pk, sk;
-    helm_mx_eval<true>(h, m, rho, T, E, cv, Pk, dPdT, sk);
+    double E, cv;
+    HelmMxAt at;
+    helm_mx_energy(h, m, rho, T, at, E, cv);
     it++;
     const double res = E - E0;
     double Tn = T - hdiv<true>(res, cv);
@@ -342,12 +365,16 @@ __device__ __forceinline__ int helm_rhoe_mixed(const HelmDev &h, double rho, dou
     }
     cv_out = cv;
     if (err < kHelmTol || it >= kHelmMaxIter) {
+      // P, dP/dT and the sound speed only at the last iterate
       if (reset) {
         double e2, cv2, d2;
-        helm_mx_eval<true>(h, m, rho, Tn, e2, cv2, P, d2, sound);
+        HelmMxAt a2;
+        helm_mx_energy(h, m, rho, Tn, a2, e2, cv2);
+        helm_mx_pressure(m, rho, Tn, a2, e2, cv2, P, d2, sound);
       } else {
+        double Pk, dPdT;
+        helm_mx_pressure(m, rho, T, at, E, cv, Pk, dPdT, sound);
         P = Pk + dPdT * (Tn - T);
-        sound = sk;
       }
       T = Tn;
       return it;
