@@ -147,7 +147,10 @@ int phlx_set_scalars(phlx_ctx *ctx, double time, double dt, double smax, int64_t
  * sweep dir = 0,1,2 (x1 :7960-9105, x2 :9107-10301, x3 :10303-11364);
  * ct() the corner EMFs and face-B update (:11966-12437); rk_update() the
  * sources, RK combine and cons->prim (:12606-13686).  (a1, a2, a3) is the
- * stage's Shu-Osher row; a3 is multiplied by the device-resident dt. */
+ * stage's Shu-Osher row; a3 is multiplied by the device-resident dt.
+ * With RK2 and MHD the two ct() calls of a step come in stage order (the
+ * stage-1 face field is held in the second face buffer between them);
+ * ct(1) before ct(0), or ct(0) twice, returns PHLX_E_STATE. */
 int phlx_stage_begin(phlx_ctx *ctx, int32_t irk);
 int phlx_sweep(phlx_ctx *ctx, int32_t dir);
 int phlx_ct(phlx_ctx *ctx, int32_t irk, double a1, double a2, double a3);

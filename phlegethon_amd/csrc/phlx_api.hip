@@ -59,6 +59,9 @@ struct phlx_ctx {
   // hydro_step's fused sweep: -1 = where it is the faster form, 0 = never,
   // 1 = wherever the kernel family has one
   int fused_sweep = -1;
+  // RK2 with MHD: between a step's two CT calls the stage-1 face field
+  // lives in the b0f buffers and g.bf / g.b0f are swapped (k_faceb)
+  bool b_swapped = false;
   double dt_par = 0.0;
   int diff_stages = 0;  // stages of the last diffusion step
   // profiling
@@ -602,13 +605,27 @@ extern "C" int phlx_sweep(phlx_ctx *ctx, int32_t dir) {
 extern "C" int phlx_ct(phlx_ctx *ctx, int32_t irk, double a1, double a2, double a3) {
   if (!ctx) return PHLX_E_ARG;
   if (!ctx->g.mhd) return PHLX_OK;
+  // RK2: stage 1 writes the new field into the b0f buffers (bf keeps the
+  // step-start field: no copy of it) and the pointer sets swap, so every
+  // later launch reads the stage-1 field through g.bf; stage 2 writes its
+  // field over the step-start one and the sets swap back.  RK3 keeps the
+  // step-start copy.
+  int mode = irk == 0 ? FACEB_FIRST : FACEB_LATER;
+  if (ctx->stages == 2) {
+    if ((irk == 0) == ctx->b_swapped) return PHLX_E_STATE;  // stage order
+    mode = irk == 0 ? FACEB_RK2_1 : FACEB_RK2_2;
+  }
   ProfRec r;
   int on = kbegin(ctx, KID_CORNER, r);
   ctx->kt.corner(ctx->g, ctx->stream);
   kend(ctx, on, r);
   on = kbegin(ctx, KID_FACEB, r);
-  ctx->kt.faceb(ctx->g, irk == 0, a1, a2, a3, ctx->stream);
+  ctx->kt.faceb(ctx->g, mode, a1, a2, a3, ctx->stream);
   kend(ctx, on, r);
+  if (ctx->stages == 2) {
+    for (int d = 0; d < 3; d++) std::swap(ctx->g.bf[d], ctx->g.b0f[d]);
+    ctx->b_swapped = !ctx->b_swapped;
+  }
   return check_launch();
 }
 
@@ -718,6 +735,7 @@ extern "C" int phlx_init(phlx_ctx *ctx) {
   s0.active = 1;
   HIPCHK(hipMemcpyAsync(ctx->g.sc, &s0, sizeof(s0), hipMemcpyHostToDevice, ctx->stream));
   if (const int m = materialize(ctx)) return m;
+  ctx->b_swapped = false;  // g.bf is the current face field whichever buffer it is
   ctx->kt.init(ctx->g, 0, ctx->stream);
   if (ctx->kt.pack) ctx->kt.pack(ctx->g, ctx->stream);
   int rc = check_launch();
@@ -745,6 +763,7 @@ extern "C" int phlx_resume(phlx_ctx *ctx, double time, double dt, double smax, i
   s0.active = time < ctx->cfg.tmax ? 1 : 0;
   HIPCHK(hipMemcpyAsync(ctx->g.sc, &s0, sizeof(s0), hipMemcpyHostToDevice, ctx->stream));
   if (const int m = materialize(ctx)) return m;
+  ctx->b_swapped = false;
   ctx->kt.init(ctx->g, 1, ctx->stream);
   if (ctx->kt.pack) ctx->kt.pack(ctx->g, ctx->stream);
   int rc = check_launch();

@@ -10,6 +10,11 @@ namespace phlx {
 // x1 tile edge of the fused sweep: it runs where it divides the grid
 constexpr int kSweepT = 16;
 
+// Where k_faceb takes b0 and puts the new face field (its comment has the
+// four cases); RK2 steps alternate the face-field buffers instead of
+// copying the step-start field.
+enum { FACEB_LATER = 0, FACEB_FIRST = 1, FACEB_RK2_1 = 2, FACEB_RK2_2 = 3 };
+
 // Launchers of one template instance (space dims, advected scalars, MHD).
 struct KernelTable {
   void (*flux)(const Grid &, int, hipStream_t);

@@ -872,7 +872,17 @@ __global__ void __launch_bounds__(256) k_corner(Grid g) {
 }
 
 // Face-B update b = a1*b0 + a2*b + a3*dt*curl(E) in FP64 (source/source.F90
-// :12208-12435); at the first stage the step-start field is kept in b0.
+// :12208-12435).  The update is pointwise in the face field, so where the
+// new field goes is free:
+//   FACEB_FIRST  b0 = b, kept in b0f; the new field in place of b (RK3);
+//   FACEB_LATER  b0 from b0f, b from bf; the new field in place of b;
+//   FACEB_RK2_1  b0 = b from bf; the new field into b0f, bf keeps the
+//                step-start field (no copy of it); an inactive step copies
+//                bf into b0f instead, so b0f holds the current field either
+//                way (the host then swaps the two pointer sets);
+//   FACEB_RK2_2  b from bf (the stage-1 field, after that swap), b0 from
+//                b0f (the step-start field); the new field in place of b0,
+//                so after the host swaps back bf holds it.
 // F32: the mixed path's FP32 edge EMFs (corf), the update itself in FP64.
 template <bool F32>
 struct EdgeEmf {
@@ -880,13 +890,39 @@ struct EdgeEmf {
   const float *f;
   __device__ double operator[](long long x) const { return F32 ? (double)f[x] : d[x]; }
 };
+__device__ __forceinline__ void faceb_set(const Grid &g, int d, long long f, int mode, double a1,
+                                          double a2, double a3, double t) {
+  const double b = g.bf[d][f];
+  double b0;
+  if (mode == FACEB_FIRST) {
+    b0 = b;
+    g.b0f[d][f] = b;
+  } else if (mode == FACEB_RK2_1) {
+    b0 = b;
+  } else {
+    b0 = g.b0f[d][f];
+  }
+  const double out = a1 * b0 + a2 * b + a3 * t;
+  if (mode == FACEB_RK2_1 || mode == FACEB_RK2_2)
+    g.b0f[d][f] = out;
+  else
+    g.bf[d][f] = out;
+}
 template <int SD, bool F32>
-__global__ void __launch_bounds__(256) k_faceb(Grid g, int first, double a1, double a2,
+__global__ void __launch_bounds__(256) k_faceb(Grid g, int mode, double a1, double a2,
                                                double c3) {
-  if (!g.sc->active) return;
   int i, j;
   const int k = blockIdx.z;
   if (!plane_coords(g.n1 + 1, g.n2 + 1, i, j)) return;
+  if (!g.sc->active) {
+    if (mode == FACEB_RK2_1) {
+      if (j < g.n2 && k < g.n3) g.b0f[0][fix(g, 0, i, j, k)] = g.bf[0][fix(g, 0, i, j, k)];
+      if (i < g.n1 && k < g.n3) g.b0f[1][fix(g, 1, i, j, k)] = g.bf[1][fix(g, 1, i, j, k)];
+      if (SD == 3 && i < g.n1 && j < g.n2)
+        g.b0f[2][fix(g, 2, i, j, k)] = g.bf[2][fix(g, 2, i, j, k)];
+    }
+    return;
+  }
   const double a3 = c3 * g.sc->dt;
   const double i1 = g.inv_dx[0], i2 = g.inv_dx[1], i3 = g.inv_dx[2];
   const EdgeEmf<F32> C1{g.cor[0], g.corf[0]}, C2{g.cor[1], g.corf[1]}, C3{g.cor[2], g.corf[2]};
@@ -897,11 +933,7 @@ __global__ void __launch_bounds__(256) k_faceb(Grid g, int first, double a1, dou
           (C3[kix(g, 2, i, j + 1, k)] - C3[kix(g, 2, i, j, k)]) * i2;
     else
       t = (C3[kix(g, 2, i, j + 1, k)] - C3[kix(g, 2, i, j, k)]) * i2;
-    const long long f = fix(g, 0, i, j, k);
-    const double b = g.bf[0][f];
-    double b0;
-    if (first) { b0 = b; g.b0f[0][f] = b; } else { b0 = g.b0f[0][f]; }
-    g.bf[0][f] = a1 * b0 + a2 * b + a3 * t;
+    faceb_set(g, 0, fix(g, 0, i, j, k), mode, a1, a2, a3, t);
   }
   if (i < g.n1 && k < g.n3) {
     double t;
@@ -910,20 +942,12 @@ __global__ void __launch_bounds__(256) k_faceb(Grid g, int first, double a1, dou
           (C3[kix(g, 2, i, j, k)] - C3[kix(g, 2, i + 1, j, k)]) * i1;
     else
       t = (C3[kix(g, 2, i, j, k)] - C3[kix(g, 2, i + 1, j, k)]) * i1;
-    const long long f = fix(g, 1, i, j, k);
-    const double b = g.bf[1][f];
-    double b0;
-    if (first) { b0 = b; g.b0f[1][f] = b; } else { b0 = g.b0f[1][f]; }
-    g.bf[1][f] = a1 * b0 + a2 * b + a3 * t;
+    faceb_set(g, 1, fix(g, 1, i, j, k), mode, a1, a2, a3, t);
   }
   if (SD == 3 && i < g.n1 && j < g.n2) {
     const double t = (C2[kix(g, 1, i + 1, j, k)] - C2[kix(g, 1, i, j, k)]) * i1 -
                      (C1[kix(g, 0, i, j + 1, k)] - C1[kix(g, 0, i, j, k)]) * i2;
-    const long long f = fix(g, 2, i, j, k);
-    const double b = g.bf[2][f];
-    double b0;
-    if (first) { b0 = b; g.b0f[2][f] = b; } else { b0 = g.b0f[2][f]; }
-    g.bf[2][f] = a1 * b0 + a2 * b + a3 * t;
+    faceb_set(g, 2, fix(g, 2, i, j, k), mode, a1, a2, a3, t);
   }
 }
 
@@ -2074,10 +2098,10 @@ struct Kern {
     const dim3 blk(64, 4, 1);
     k_corner<SD, MIX><<<grid_plane(g.n1 + 1, g.n2 + 1, SD == 3 ? g.n3 + 1 : 1), blk, 0, st>>>(g);
   }
-  static void faceb(const Grid &g, int first, double a1, double a2, double c3, hipStream_t st) {
+  static void faceb(const Grid &g, int mode, double a1, double a2, double c3, hipStream_t st) {
     const dim3 blk(64, 4, 1);
     k_faceb<SD, MIX><<<grid_plane(g.n1 + 1, g.n2 + 1, SD == 3 ? g.n3 + 1 : 1), blk, 0, st>>>(
-        g, first, a1, a2, c3);
+        g, mode, a1, a2, c3);
   }
   static void update(const Grid &g, int first, int last, double a1, double a2, double c3,
                      int fin, hipStream_t st) {

@@ -236,6 +236,44 @@ def test_tmax_stops_the_loop():
     assert rel_l1(st["prim"], ref["prim"]) < 1e-13
 
 
+def test_mhd_tmax_stops_the_loop():
+    """RK2 with MHD alternates the two face-field buffers within a step (the
+    stage-1 field goes to the second buffer, the stage-2 field back over the
+    step-start one); the steps past tmax run nothing, so the face field after
+    them is the one at tmax."""
+    import oracle
+    pb = P.orszag_tang_3d(shape=(16, 12, 8))
+    probe = gpu_run(pb, 5)
+    pb.cfg = pb.cfg.with_(tmax=0.999 * probe["time"])
+    st = gpu_run(pb, 9)
+    ref = oracle.run_problem(pb, 9)
+    assert st["step"] == ref["step"] == 5 and abs(st["time"] / pb.cfg.tmax - 1.0) < 1e-14
+    assert rel_l1(st["prim"], ref["prim"]) < 1e-12
+    for d in range(3):
+        k = f"b_x{d+1}"
+        assert rel_l1(st[k], ref[k]) < 1e-12, k
+
+
+def test_mhd_rk3():
+    """RK3 with CT keeps the step-start copy of the face field."""
+    check_parity(P.orszag_tang_3d(shape=(16, 16, 8), rk_stages=3), 6)
+
+
+def test_stage_api_ct_order():
+    """With RK2 and MHD the two ct() calls of a step come in stage order."""
+    from phlegethon_amd.runtime import PhlxError, Simulation
+    pb = P.orszag_tang_3d(shape=(16, 16, 16))
+    sim = Simulation(pb.cfg)
+    sim.upload(pb.prim, pb.b_faces)
+    sim.init()
+    with pytest.raises(PhlxError):
+        sim.ct(1, 0.5, 0.5, -0.5)
+    sim.ct(0, 1.0, 0.0, -1.0)
+    with pytest.raises(PhlxError):
+        sim.ct(0, 1.0, 0.0, -1.0)
+    sim.close()
+
+
 @pytest.mark.parametrize("sdims,mach", [(3, 1e-2), (2, 1e-3)])
 def test_lowmach_box_edot(sdims, mach):
     """USE_EDOT with WB gravity and EVOLVE_ETOT in FP64: the low-Mach box
