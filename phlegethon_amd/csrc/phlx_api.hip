@@ -481,6 +481,7 @@ extern "C" int phlx_download_gammaf(phlx_ctx *ctx, double *gammaf) {
   if (!ctx || !gammaf) return PHLX_E_ARG;
   if (!ctx->g.gam) return PHLX_E_STATE;
   HIPCHK(hipSetDevice(ctx->device));
+  if (const int m = materialize(ctx)) return m;
   HIPCHK(hipMemcpyAsync(gammaf, ctx->g.gam, sizeof(double) * 2 * ctx->g.N,
                         hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(hipStreamSynchronize(ctx->stream));

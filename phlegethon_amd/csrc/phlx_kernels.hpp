@@ -1110,7 +1110,6 @@ __global__ void __launch_bounds__(256)
   const int i = bx * blockDim.x + threadIdx.x;
   const int j = by * blockDim.y + threadIdx.y;
   const bool in = (i < g.n1) && (j < g.n2);
-  const bool keep = !MIX || fin;
   // The FP64 path saves state0 in U0 and carries the stage state in U.  The
   // mixed path keeps the step's start state in U (no copy at stage 1) and
   // the stage state in U0 until the last stage writes U.
@@ -1353,12 +1352,14 @@ __global__ void __launch_bounds__(256)
       }
       helm_pin(pr);
       helm_pin(sound);
-      if (g.fasteos && keep) {
+      // the mixed path leaves the FP64 pressure and gammas to k_prim_from_u
+      // (an evaluation at the stored T when something reads them)
+      if (g.fasteos && !MIX) {
         g.gam[c] = pr / eint + 1.0;
         g.gam[N + c] = sound * sound * rho / pr;
       }
       g.temp[c] = T;
-      if (keep) g.P[IE * N + c] = pr;
+      if (!MIX) g.P[IE * N + c] = pr;
       p[IE] = pr;
       sound_out = sound;
       if constexpr (MIX) {
@@ -1374,11 +1375,16 @@ __global__ void __launch_bounds__(256)
   if (last) block_max_to(smax_local, g.red);
 }
 
-// The mixed path's FP64 primitives (all but the Helmholtz pressure, which
-// its update stores), eint and b_cc, rebuilt from the conserved state U and
-// the face field by the operations of k_update's cons -> prim, so they equal
-// what that kernel would have stored bit for bit.  The mixed update leaves
-// them to this kernel, which the library runs before anything reads them.
+// The mixed path's FP64 primitives, eint and b_cc, rebuilt from the
+// conserved state U and the face field by the operations of k_update's
+// cons -> prim, so they equal what that kernel would have stored bit for
+// bit; with the Helmholtz EoS the pressure and (FASTEOS) gamma_e / gamma_c
+// come from one evaluation at the zone's stored T, the result of the last
+// stage's Newton solve (source/source.F90:13573-13578; they differ from
+// that solve's closing values by the linearisation of its last step, below
+// FP64 resolution, and in the sound speed by ~dT/T of that step).  The
+// mixed update leaves them to this kernel, which the library runs before
+// anything reads them.
 template <int SD, int NAS, bool MHD, bool HELM>
 __global__ void __launch_bounds__(256) k_prim_from_u(Grid g) {
   constexpr int NV = SD + 2 + NAS, IE = SD + 1, IA = SD + 2;
@@ -1414,6 +1420,20 @@ __global__ void __launch_bounds__(256) k_prim_from_u(Grid g) {
   for (int v = IA; v < NV; v++) g.P[v * N + c] = S[v] * ir;
   if (!HELM) g.P[IE * N + c] = g.gmm1 * eint;
   g.eint[c] = eint;
+  if constexpr (HELM) {
+    double ye = 1.0, abar = 1.0;  // PIG_EOS without scalars ignores both
+    if constexpr (NAS >= 2) {
+      ye = S[IA] * ir;
+      abar = rho / S[IA + 1];
+    }
+    double pr, e, sound, cv;
+    helm_full(g.helm, rho, g.temp[c], abar, ye * abar, pr, e, sound, cv);
+    g.P[IE * N + c] = pr;
+    if (g.fasteos) {
+      g.gam[c] = pr / eint + 1.0;
+      g.gam[N + c] = sound * sound * rho / pr;
+    }
+  }
 }
 
 // Standalone compute_hyperbolic_dt pass over the current prim / b_cc.
