@@ -1092,8 +1092,9 @@ __global__ void __launch_bounds__(256)
   // first: stage 1, state0 rebuilt from prim + eint (the reference), or in
   // the mixed path taken from U when the device flags say U is current.
   // last: the CFL maximum of the new state.  fin: the step's last stage.
-  // The mixed path writes the FP64 primitives, eint, B_cc and the gammas
-  // only at fin (between its stages the sweeps read the FP32 copy g.Pf).
+  // The mixed path writes none of the FP64 primitives, eint, B_cc or the
+  // gammas (the sweeps read the FP32 copy g.Pf; k_prim_from_u rebuilds the
+  // FP64 fields from U and T when something reads them).
   // SWEPT: the fused sweep (k_sweep) has already formed the stage's new
   // conserved state in U; this launch is cons -> prim, EoS and CFL only.
   constexpr int NV = SD + 2 + NAS, IE = SD + 1, IA = SD + 2;
