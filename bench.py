@@ -341,6 +341,10 @@ def run_variant(pb, cfg, args, guess="warm", keep_state=False, profile=True, tot
     if helm:
         sim.set_eos_guess(guess, resid_tol=0.25, count=False)
     sim.set_fused_sweep({"fused": True, "split": False, "auto": None}[args.sweep])
+    # --graph on: phlx_run replays one captured graph per step (captured in
+    # the warmup run; runs with thermal diffusion and the profiled steps
+    # launch eagerly)
+    sim.set_step_graph(args.graph == "on")
     sim.init()
     log(f"warmup {args.warmup} steps")
     sim.run(args.warmup)
@@ -478,6 +482,9 @@ def main():
                     help="3D runs: the fused one-pass sweep, the three directional sweeps "
                          "(same results bit for bit), or the library's choice (fused without "
                          "MHD)")
+    ap.add_argument("--graph", choices=("on", "off"), default="off",
+                    help="run the steps as one captured HIP graph per step (bit for bit the "
+                         "eager launches; measured no faster, even on the 256^2 2D grid)")
     ap.add_argument("--eos-guess", choices=("warm", "mlp", "pred"), default="warm",
                     help="Helmholtz: Newton start (carried T, the trained surrogate, or the "
                          "energy predictor)")
@@ -600,7 +607,8 @@ def bench_single(pb, desc, data, args):
         "scaling": "weak", "vs_baseline": None,
         "dtype": "f64" if args.precision == "fp64" else "f32+f64", "data": data,
         "config": {"workload": desc, "zones": int(np.prod(cfg.shape)),
-                   "sim_time": res["scalars"]["time"], "max_divB": res["divb"]},
+                   "sim_time": res["scalars"]["time"], "max_divB": res["divb"],
+                   "step_graph": args.graph == "on"},
         "roofline": roof, "step_roofline": step, "kernel_models": models,
     }
     if "eos" in res:
@@ -670,7 +678,8 @@ def bench_c5(pb, desc, data, args):
         "dtype": "f32+f64", "data": data,
         "config": {"workload": desc + f" | value: {names[head]} (mixed precision)",
                    "zones": int(np.prod(pb.cfg.shape)), "l1_after_steps": args.l1_steps,
-                   "sim_time_plain": plain["scalars"]["time"], "max_divB_plain": plain["divb"]},
+                   "sim_time_plain": plain["scalars"]["time"], "max_divB_plain": plain["divb"],
+                   "step_graph": args.graph == "on"},
         "roofline": roof, "step_roofline": step,
         "plain_fp64": {"value": plain["value"], "ms_per_step": plain["ms_per_step"],
                        "eos": plain.get("eos"), "kernels": models},

@@ -168,6 +168,16 @@ int phlx_rk_update(phlx_ctx *ctx, int32_t irk, double a1, double a2, double a3,
  * one routine either way): for A/B timing and for callers that want the
  * per-direction kernels. */
 int phlx_set_fused_sweep(phlx_ctx *ctx, int32_t mode);
+/* Whether phlx_run replays one captured HIP graph per step (on = 1) instead
+ * of launching the step's kernels one by one (0, the default).  The step is
+ * captured on the first phlx_run that needs it and again whenever a setting
+ * that changes its kernels or their arguments has changed; runs with
+ * thermal diffusion or profiling launch eagerly.  The results are bit for
+ * bit the same.  Measured on MI355X it is not faster (2D 256^2, where
+ * each step is a dozen short kernels: 0.090 against 0.085 ms per step), so
+ * it is off by default; it is for callers that embed the step in their own
+ * graphs or want one launch per step.  No counterpart in the reference. */
+int phlx_set_step_graph(phlx_ctx *ctx, int32_t on);
 
 /* ---- Helmholtz EoS ----
  * load_helm_table replaces load_helm_table (source/source.F90:23077-23156):

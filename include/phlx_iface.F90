@@ -231,6 +231,12 @@ module phlx_iface
       type(c_ptr), value :: ctx
       integer(c_int32_t), value :: mode
     end function
+    ! phlx_run replays a captured HIP graph per step: 1 on, 0 off (default)
+    integer(c_int) function phlx_set_step_graph(ctx, on) bind(c, name="phlx_set_step_graph")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: on
+    end function
     ! box(8): log10(rho*Ye) lo, hi, log10 T lo, hi, abar lo, hi, Ye lo, hi
     integer(c_int) function phlx_mlp_set_train_box(ctx, box) bind(c, name="phlx_mlp_set_train_box")
       import :: c_int, c_ptr

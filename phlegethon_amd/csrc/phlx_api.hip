@@ -62,6 +62,14 @@ struct phlx_ctx {
   // RK2 with MHD: between a step's two CT calls the stage-1 face field
   // lives in the b0f buffers and g.bf / g.b0f are swapped (k_faceb)
   bool b_swapped = false;
+  // phlx_run's captured step (phlx_set_step_graph) and what it was
+  // captured with: the grid descriptor (every kernel argument but the
+  // per-stage constants), the sweep form and the Newton counting
+  bool step_graph = false;
+  hipGraph_t sg_graph = nullptr;
+  hipGraphExec_t sg_exec = nullptr;
+  Grid sg_g{};
+  bool sg_fused = false, sg_stats = false;
   double dt_par = 0.0;
   int diff_stages = 0;  // stages of the last diffusion step
   // profiling
@@ -307,6 +315,8 @@ extern "C" int phlx_destroy(phlx_ctx *ctx) {
   if (!ctx) return PHLX_E_ARG;
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->sg_exec) (void)hipGraphExecDestroy(ctx->sg_exec);
+  if (ctx->sg_graph) (void)hipGraphDestroy(ctx->sg_graph);
   for (auto &r : ctx->recs) {
     (void)hipEventDestroy(r.a);
     (void)hipEventDestroy(r.b);
@@ -799,11 +809,66 @@ extern "C" int phlx_hydro_step(phlx_ctx *ctx) {
   return do_hydro_step(ctx, false);
 }
 
+extern "C" int phlx_set_step_graph(phlx_ctx *ctx, int32_t on) {
+  if (!ctx || on < 0 || on > 1) return PHLX_E_ARG;
+  ctx->step_graph = on != 0;
+  return PHLX_OK;
+}
+
+static void drop_step_graph(phlx_ctx *ctx) {
+  if (ctx->sg_exec) (void)hipGraphExecDestroy(ctx->sg_exec);
+  if (ctx->sg_graph) (void)hipGraphDestroy(ctx->sg_graph);
+  ctx->sg_exec = nullptr;
+  ctx->sg_graph = nullptr;
+}
+
+// The executable graph of one step (step begin, hydro_step with the CFL,
+// step end), captured from the same launch sequence as the eager loop.  It
+// holds only kernel nodes.  Captured again when the grid descriptor, the
+// sweep form or the Newton counting differ from the capture's.
+static int step_graph_ready(phlx_ctx *ctx) {
+  const bool fused = use_fused_sweep(ctx);
+  if (ctx->sg_exec && ctx->sg_fused == fused && ctx->sg_stats == ctx->eos_stats &&
+      std::memcmp(&ctx->sg_g, &ctx->g, sizeof(Grid)) == 0)
+    return PHLX_OK;
+  drop_step_graph(ctx);
+  HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+  launch_step_begin(ctx->g.sc, ctx->stream);
+  const int rc = do_hydro_step(ctx, true);
+  launch_step_end(ctx->g, 1, ctx->stream);
+  hipGraph_t gr = nullptr;
+  const hipError_t e = hipStreamEndCapture(ctx->stream, &gr);
+  if (rc || e != hipSuccess) {
+    if (gr) (void)hipGraphDestroy(gr);
+    (void)hipGetLastError();
+    return rc ? rc : PHLX_E_HIP;
+  }
+  ctx->sg_graph = gr;
+  if (hipGraphInstantiate(&ctx->sg_exec, gr, nullptr, nullptr, 0) != hipSuccess) {
+    ctx->sg_exec = nullptr;
+    drop_step_graph(ctx);
+    return PHLX_E_HIP;
+  }
+  ctx->sg_g = ctx->g;
+  ctx->sg_fused = fused;
+  ctx->sg_stats = ctx->eos_stats;
+  return PHLX_OK;
+}
+
 extern "C" int phlx_run(phlx_ctx *ctx, int32_t nsteps) {
   if (!ctx || nsteps < 0) return PHLX_E_ARG;
   if (!ctx->initialized) return PHLX_E_STATE;
   HIPCHK(hipSetDevice(ctx->device));
   const bool diff = ctx->diff_on;
+  if (ctx->step_graph && nsteps > 0 && !diff && !ctx->prof && !ctx->b_swapped) {
+    const int rc = step_graph_ready(ctx);
+    if (rc) return rc;
+    for (int s = 0; s < nsteps; s++) HIPCHK(hipGraphLaunch(ctx->sg_exec, ctx->stream));
+    // the host-side records an eager step leaves
+    ctx->tdiff_valid = false;
+    if (ctx->kt.prims) ctx->prim_stale = true;
+    return PHLX_OK;
+  }
   for (int s = 0; s < nsteps; s++) {
     launch_step_begin(ctx->g.sc, ctx->stream);
     int rc;

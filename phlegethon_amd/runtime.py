@@ -78,6 +78,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.phlx_ct.argtypes = [P, I, D, D, D]
     lib.phlx_rk_update.argtypes = [P, I, D, D, D, I]
     lib.phlx_set_fused_sweep.argtypes = [P, I]
+    lib.phlx_set_step_graph.argtypes = [P, I]
     lib.phlx_divb_max.argtypes = [P, DP]
     lib.phlx_profile_enable.argtypes = [P, I]
     lib.phlx_profile_read.argtypes = [P, I, DP, ctypes.POINTER(ctypes.c_int64)]
@@ -305,6 +306,12 @@ class Simulation:
         without MHD); the results are the same bit for bit."""
         mode = -1 if on is None else int(bool(on))
         _chk(self.lib.phlx_set_fused_sweep(self._ctx, mode), "phlx_set_fused_sweep")
+
+    def set_step_graph(self, on: bool):
+        """run() replays one captured HIP graph per step (True) or launches
+        each kernel (False, the default); the results are the same bit for
+        bit."""
+        _chk(self.lib.phlx_set_step_graph(self._ctx, int(bool(on))), "phlx_set_step_graph")
 
     def stream_copy(self, nbytes: int = 1 << 30, reps: int = 10) -> float:
         """Achieved HBM GB/s of a plain device copy (read + write)."""
