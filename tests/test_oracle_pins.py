@@ -360,7 +360,7 @@ def _loop_run(n, lib=None, nthreads=4, every=20, **kw):
     return np.array(em), err
 
 
-def field_loop_verdict(lib=None):
+def field_loop_outcome(lib=None):
     """(ok, detail): magnetic energy finite and never increasing over the
     period at 64 x 32 and 128 x 64 cells (HLLD + PPH), and the loop error
     falling by at least 1.3x under refinement."""
@@ -382,12 +382,12 @@ def test_ct_field_loop(oracle_mod):
     """Gardiner & Stone's field loop: with the contact-upwinded corner EMFs
     (source/source.F90:11966-12027) the loop's magnetic energy decays
     monotonically and the loop error converges."""
-    ok, detail = field_loop_verdict()
+    ok, detail = field_loop_outcome()
     assert ok, detail
 
 
 def test_ct_field_loop_catches_downwind(oracle_mod):
-    ok, detail = field_loop_verdict(mutant("ct_downwind"))
+    ok, detail = field_loop_outcome(mutant("ct_downwind"))
     assert not ok, detail
 
 
@@ -406,7 +406,7 @@ def _sign_run(mach_ct, lib=None, n=24, steps=12):
     return st, smax0
 
 
-def sign_threshold_verdict(lib=None):
+def sign_threshold_outcome(lib=None):
     """The contact sign of a face is +-1 when |F_rho| / (rho_mean smax)
     exceeds Mach_ct (source/source.F90:8936-8947), with rho_mean the mean of
     the two face densities and smax the CFL signal-speed maximum.  For
@@ -424,15 +424,15 @@ def sign_threshold_verdict(lib=None):
 
 
 def test_contact_sign_threshold(oracle_mod):
-    assert sign_threshold_verdict()
+    assert sign_threshold_outcome()
 
 
 def test_contact_sign_threshold_catches(oracle_mod):
-    assert not sign_threshold_verdict(mutant("sign_density"))
+    assert not sign_threshold_outcome(mutant("sign_density"))
 
 
 # ---------------------------------------------------------------- USE_EDOT
-def edot_verdict(lib=None):
+def edot_outcome(lib=None):
     """A uniform gas at rest heated in one row j0 only (USE_EDOT,
     source/source.F90:12646-12676): after one RK2 step the total energy has
     grown by exactly dt * edot per heated cell (the flux divergences
@@ -466,9 +466,9 @@ def edot_verdict(lib=None):
 
 
 def test_edot_heating(oracle_mod):
-    assert edot_verdict()
+    assert edot_outcome()
 
 
 @pytest.mark.parametrize("name", ["edot_sign", "edot_row"])
 def test_edot_catches(oracle_mod, name):
-    assert not edot_verdict(mutant(name))
+    assert not edot_outcome(mutant(name))
