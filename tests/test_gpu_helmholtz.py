@@ -219,3 +219,68 @@ def test_full_length_slab_full_table(oracle_mod):
         assert r <= TOL, v
     for name in ("eint", "temp"):
         assert _rel_l1(got[name], ref[name]) <= TOL, name
+
+
+def _conserved(st, cfg):
+    p = st["prim"]
+    rho = p[0]
+    out = {"rho": rho}
+    ke = np.zeros_like(rho)
+    for d in range(cfg.sdims):
+        out[f"m{d+1}"] = rho * p[1 + d]
+        ke = ke + 0.5 * rho * p[1 + d] ** 2
+    out["E"] = st["eint"] + ke + 0.5 * (st["b_cc"] ** 2).sum(axis=0)
+    for s in range(cfg.sdims + 2, cfg.nvars):
+        out[f"s{s}"] = rho * p[s]
+    return out
+
+
+def test_full_length_c5_slab_full_table(oracle_mod):
+    """The configuration of the headline workload (MHD + Helmholtz slab,
+    beta 1e5: LHLLD + CT + LIM2ND, FASTEOS + WB + ETOT + Ye / abar, RK2) on
+    64 x 16 x 64 zones (x1 a multiple of 64, as the surrogate's wave-wide
+    update needs) for 100 steps with the full 541 x 271 table, against one oracle
+    run: the plain FP64 path at 1e-12 (face B and div B included), and the
+    mixed path with the surrogate's Newton start (the network trained in the
+    run) at the 1e-6 bar per conserved field and per face-B component
+    (relative to the L1 of the whole face field)."""
+    from phlegethon_amd import problems as P
+    from phlegethon_amd.config import PREC_MIXED
+    from phlegethon_amd.runtime import Simulation
+    full = H.default_table()
+    pb = _slab(full, oracle_mod, n=64, ny=16, sdims=3, recon=LIM2ND_REC, mhd_beta=1e5)
+    nsteps = 100
+    print(f"C5-shaped slab {pb.cfg.shape}, full table, {nsteps} steps", flush=True)
+    ref = oracle_mod.run_problem(pb, nsteps, nthreads=16)
+    got = _gpu_run(pb, nsteps)
+    assert got["step"] == ref["step"] == nsteps
+    assert abs(got["time"] - ref["time"]) <= 1e-14 * abs(ref["time"])
+    for v in range(pb.cfg.nvars):
+        assert _rel_l1(got["prim"][v], ref["prim"][v]) <= TOL, v
+    for name in ("eint", "temp"):
+        assert _rel_l1(got[name], ref[name]) <= TOL, name
+    for d in range(3):
+        assert _rel_l1(got[f"b_x{d+1}"], ref[f"b_x{d+1}"]) <= TOL, d
+    assert got["divb"] <= 1e-13 * max(1.0, float(np.abs(got["b_cc"]).max()) / pb.cfg.dx[0])
+
+    mixed = pb.copy()
+    mixed.cfg = pb.cfg.with_(precision=PREC_MIXED)
+    sim = Simulation(mixed.cfg)
+    sim.setup(mixed)
+    sim.mlp_train(nsamples=1 << 18, epochs=2, batch=2048, lr=5e-3, box=P.state_box(mixed))
+    sim.set_eos_guess("mlp")
+    sim.init()
+    sim.run(nsteps)
+    st = sim.download()
+    st.update(sim.scalars())
+    sim.close()
+    assert st["step"] == nsteps
+    cg, cr = _conserved(st, pb.cfg), _conserved(ref, pb.cfg)
+    errs = {k: _rel_l1(cg[k], cr[k]) for k in cr}
+    bn = sum(np.abs(ref[f"b_x{d+1}"]).sum() for d in range(3))
+    for d in range(3):
+        k = f"b_x{d+1}"
+        errs[k] = float(np.abs(st[k] - ref[k]).sum() / bn)
+    print("  mixed + surrogate:", "  ".join(f"{k} {v:.2e}" for k, v in errs.items()), flush=True)
+    for k, v in errs.items():
+        assert v <= 1e-6, (k, v)
